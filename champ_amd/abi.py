@@ -1,0 +1,133 @@
+"""ctypes mirror of ``include/champ_amd.h`` and the loader of libchamp_amd.so.
+
+The layout of :class:`QmcwSystem` must match ``struct qmcw_system`` field by
+field; ``tests/test_abi.py`` checks the exported symbols and the struct size
+against the compiled library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_here = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_here, "libchamp_amd.so")
+
+QMCW_OK = 0
+QMCW_ERR_ARG = 1
+QMCW_ERR_HIP = 2
+QMCW_ERR_SINGULAR = 3
+QMCW_ERR_NAN = 4
+QMCW_ERR_NOMEM = 5
+
+QMCW_FP64 = 0
+QMCW_MIXED = 1
+QMCW_MIXED_BF16 = 2
+
+QMCW_BASIS_SLATER = 1
+QMCW_BASIS_GAUSSIAN = 2
+QMCW_MAX_L = 4
+QMCW_MAX_ORD = 8
+QMCW_MAX_LPOT = 5
+QMCW_MAX_NQUAD = 12
+
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int32)
+
+
+class QmcwSystem(C.Structure):
+    _fields_ = [
+        ("nelec", C.c_int32), ("nup", C.c_int32),
+        ("ncent", C.c_int32), ("nctype", C.c_int32),
+        ("cent", _pd), ("iwctype", _pi), ("znuc", _pd),
+        ("basis_type", C.c_int32), ("nrad", C.c_int32),
+        ("rad_n", _pi), ("rad_prim_off", _pi),
+        ("prim_zeta", _pd), ("prim_coef", _pd),
+        ("ctype_bas_off", _pi), ("bas_rad", _pi), ("bas_l", _pi), ("bas_m", _pi),
+        ("nbasis", C.c_int32),
+        ("norb", C.c_int32), ("coef", _pd),
+        ("ndet", C.c_int32), ("det_up", _pi), ("det_dn", _pi),
+        ("ncsf", C.c_int32), ("csf_coef", _pd), ("csf_det_off", _pi),
+        ("csf_det", _pi), ("csf_det_coef", _pd),
+        ("isc", C.c_int32), ("nspin2b", C.c_int32), ("nocuspb", C.c_int32),
+        ("norda", C.c_int32), ("nordb", C.c_int32), ("nordc", C.c_int32),
+        ("scalek", C.c_double),
+        ("a4", _pd), ("b", _pd), ("c", _pd),
+        ("nloc", C.c_int32), ("npotd", _pi), ("lpotp1", _pi), ("ecp_off", _pi),
+        ("ecp_coef", _pd), ("ecp_power", _pi), ("ecp_expo", _pd),
+        ("nquad", C.c_int32),
+        ("nparma_tot", C.c_int32), ("jas_a_type", _pi), ("jas_a_idx", _pi),
+        ("nparmb_tot", C.c_int32), ("jas_b_spin", _pi), ("jas_b_idx", _pi),
+        ("nparmc_tot", C.c_int32), ("jas_c_type", _pi), ("jas_c_idx", _pi),
+        ("nparmcsf", C.c_int32), ("csf_parm", _pi),
+        ("nparmorb", C.c_int32), ("orb_parm_i", _pi), ("orb_parm_j", _pi),
+        ("orb_parm_rev", _pi),
+        ("hb", C.c_double),
+    ]
+
+
+def dptr(a):
+    """Pointer to a contiguous float64 numpy array (or NULL)."""
+    if a is None:
+        return _pd()
+    assert a.dtype.name == "float64" and a.flags["C_CONTIGUOUS"], a.dtype
+    return a.ctypes.data_as(_pd)
+
+
+def iptr(a):
+    """Pointer to a contiguous int32 numpy array (or NULL)."""
+    if a is None:
+        return _pi()
+    assert a.dtype.name == "int32" and a.flags["C_CONTIGUOUS"], a.dtype
+    return a.ctypes.data_as(_pi)
+
+
+# exported C-ABI entry points and their ctypes signatures
+_vp = C.c_void_p
+SIGNATURES = {
+    "qmcw_nterms4": (C.c_int32, [C.c_int32]),
+    "qmcw_create": (C.c_int, [C.POINTER(QmcwSystem), C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "qmcw_destroy": (C.c_int, [_vp]),
+    "qmcw_set_params": (C.c_int, [_vp, _pd, _pd, _pd, _pd, _pd]),
+    "qmcw_set_walkers": (C.c_int, [_vp, _pd]),
+    "qmcw_get_walkers": (C.c_int, [_vp, _pd]),
+    "qmcw_eval_full": (C.c_int, [_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "qmcw_ratio_1e": (C.c_int, [_vp, C.c_int32, _pd, _pd, _pd, _pd]),
+    "qmcw_accept_1e": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_uint8)]),
+    "qmcw_grad_1e": (C.c_int, [_vp, C.c_int32, _pd, _pd]),
+    "qmcw_nparm": (C.c_int32, [_vp]),
+    "qmcw_derivs": (C.c_int, [_vp, _pd, _pd]),
+    "qmcw_set_rng": (C.c_int, [_vp, _pi]),
+    "qmcw_get_rng": (C.c_int, [_vp, _pi]),
+    "qmcw_vmc_run": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _pd,
+                               C.POINTER(C.c_int64)]),
+    "qmcw_vmc_run_device": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double]),
+    "qmcw_get_eloc": (C.c_int, [_vp, _pd]),
+    "qmcw_sync": (C.c_int, [_vp]),
+    "qmcw_last_error": (C.c_char_p, [_vp]),
+}
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libchamp_amd.so (built by __graft_entry__.build()).
+
+    Raises ``RuntimeError`` when the library is missing: there is no CPU
+    fallback for the product path.
+    """
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib

@@ -1,0 +1,212 @@
+/*
+ * champ_amd.h -- C ABI of the MI355X walker-batched wavefunction engine.
+ *
+ * The engine replaces CHAMP's per-walker "psi / eloc / deriv" call surface
+ * (qmc/psi_utils_mod.f90, qmc/vmc/hpsi.f90, qmc/vmc/hpsie.f90,
+ * qmc/deriv_mod.f90) with batched entry points over W walkers that keep
+ * their state resident in GPU memory.  Every entry point returns an int
+ * status (QMCW_OK = 0).  Host pointers passed in are borrowed and never
+ * freed; device state is owned by the handle.  One handle per HIP stream;
+ * a handle is not thread-safe.
+ *
+ * Index conventions: all indices are 0-based.  Electrons 0..nup-1 are spin
+ * up, nup..nelec-1 spin down.  Walker-major layouts: x[w][i][k].
+ */
+#ifndef CHAMP_AMD_H
+#define CHAMP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMCW_OK 0
+#define QMCW_ERR_ARG 1       /* invalid argument / unsupported system */
+#define QMCW_ERR_HIP 2       /* HIP runtime error */
+#define QMCW_ERR_SINGULAR 3  /* a Slater matrix was singular (det = 0) */
+#define QMCW_ERR_NAN 4       /* non-finite value produced */
+#define QMCW_ERR_NOMEM 5     /* device allocation failed */
+
+#define QMCW_FP64 0          /* orbitals, Jastrow and determinants in FP64 */
+#define QMCW_MIXED 1         /* orbitals + Jastrow FP32, determinants/energies FP64 */
+#define QMCW_MIXED_BF16 2    /* as MIXED with BF16 orbital-contraction inputs */
+
+#define QMCW_BASIS_SLATER 1
+#define QMCW_BASIS_GAUSSIAN 2
+
+#define QMCW_MAX_L 4         /* explicit real solid harmonics up to g */
+#define QMCW_MAX_ORD 8       /* maximum Jastrow polynomial order */
+#define QMCW_MAX_LPOT 5      /* maximum pseudopotential l components */
+#define QMCW_MAX_NQUAD 12
+
+/*
+ * System description.  It mirrors the data CHAMP keeps in its common
+ * modules (qmc/commons/atom_mod.f90, basis1_mod, coefs_mod, dets_mod,
+ * jaspar*_mod, pseudo_mod, gauss_ecp_mod, qua_mod, optim_mod) as flat
+ * arrays.  The normalisation of the basis functions is folded into `coef`
+ * as CHAMP does (qmc/vmc/basis_norm.f90:84-88).
+ */
+typedef struct qmcw_system {
+  /* electrons */
+  int32_t nelec, nup;
+  /* nuclei */
+  int32_t ncent, nctype;
+  const double *cent;       /* [ncent][3] */
+  const int32_t *iwctype;   /* [ncent] centre type */
+  const double *znuc;       /* [nctype] (effective) nuclear charge */
+  /* basis: radial functions r^(n-1) sum_p d_p exp(-zeta_p r) (Slater) or
+     r^(n-1) sum_p d_p exp(-zeta_p r^2) (Gaussian), times a real solid
+     harmonic (l, m) as in qmc/vmc/basis_fns.f90:200-350 */
+  int32_t basis_type;
+  int32_t nrad;             /* radial functions over all centre types */
+  const int32_t *rad_n;     /* [nrad] */
+  const int32_t *rad_prim_off; /* [nrad+1] */
+  const double *prim_zeta;  /* [nprim] */
+  const double *prim_coef;  /* [nprim] */
+  const int32_t *ctype_bas_off; /* [nctype+1] basis functions of one centre of each type */
+  const int32_t *bas_rad;   /* [nbas_ctype_total] radial function (global index) */
+  const int32_t *bas_l;     /* [nbas_ctype_total] */
+  const int32_t *bas_m;     /* [nbas_ctype_total] m in -l..l; cos-like m>=0, sin-like m<0 */
+  int32_t nbasis;           /* sum over centres of the per-type counts */
+  /* orbitals: orb(i,o) = sum_b coef[o*nbasis + b] phi_b(r_i) */
+  int32_t norb;
+  const double *coef;       /* [norb][nbasis] */
+  /* determinants and CSFs (qmc/csfs_mod.f90 after normalisation) */
+  int32_t ndet;
+  const int32_t *det_up;    /* [ndet][nup]   orbital labels */
+  const int32_t *det_dn;    /* [ndet][nelec-nup] */
+  int32_t ncsf;
+  const double *csf_coef;   /* [ncsf] */
+  const int32_t *csf_det_off; /* [ncsf+1] */
+  const int32_t *csf_det;   /* dets of each csf */
+  const double *csf_det_coef;
+  /* Jastrow, ijas=4 (qmc/vmc/jastrow4.f90) */
+  int32_t isc;              /* 2 or 4 */
+  int32_t nspin2b;          /* 1: b shared by spin pairs; 2: separate parallel-spin b */
+  int32_t nocuspb;          /* 0: parallel-spin e-e cusp is half (sspinn=1/2) */
+  int32_t norda, nordb, nordc;
+  double scalek;
+  const double *a4;         /* [nctype][norda+1] */
+  const double *b;          /* [nspin2b][nordb+1] */
+  const double *c;          /* [nctype][nterms4(nordc)] */
+  /* pseudopotential (nloc = 6: Gaussian, qmc/vmc/readps_gauss.f90); 0 = none */
+  int32_t nloc;
+  const int32_t *npotd;     /* [nctype] number of l components */
+  const int32_t *lpotp1;    /* [nctype] 1-based component that is local */
+  const int32_t *ecp_off;   /* [nctype*QMCW_MAX_LPOT + 1] term offsets of (type, l) */
+  const double *ecp_coef;   /* [nterm] */
+  const int32_t *ecp_power; /* [nterm] */
+  const double *ecp_expo;   /* [nterm] */
+  int32_t nquad;            /* 12 (icosahedron) */
+  /* parameters whose derivatives are wanted (qmc/optim_mod, optimization menu) */
+  int32_t nparma_tot;       /* Jastrow a's varied, over types */
+  const int32_t *jas_a_type;/* [nparma_tot] centre type */
+  const int32_t *jas_a_idx; /* [nparma_tot] 1-based a4 index (iwjasa) */
+  int32_t nparmb_tot;
+  const int32_t *jas_b_spin;/* [nparmb_tot] */
+  const int32_t *jas_b_idx; /* [nparmb_tot] 1-based b index (iwjasb) */
+  int32_t nparmc_tot;
+  const int32_t *jas_c_type;/* [nparmc_tot] */
+  const int32_t *jas_c_idx; /* [nparmc_tot] 1-based c term index (iwjasc), independent only */
+  int32_t nparmcsf;
+  const int32_t *csf_parm;  /* [nparmcsf] CSF index varied (iwcsf) */
+  int32_t nparmorb;
+  const int32_t *orb_parm_i;/* [nparmorb] first orbital label of the rotation */
+  const int32_t *orb_parm_j;/* [nparmorb] second orbital label */
+  const int32_t *orb_parm_rev; /* [nparmorb] 1: include the reverse excitation */
+  double hb;                /* hbar^2/2m = 0.5 */
+} qmcw_system;
+
+/* number of e-e-n terms for order nordc (qmc/vmc/jastrow4.f90:495-516) */
+int32_t qmcw_nterms4(int32_t nordc);
+
+typedef struct qmcw_handle qmcw_handle;
+
+/* Create a handle for W walkers (replaces the psi_t allocation in
+   qmc/psi_type_mod.f90:5-39 and walker_t, qmc/walker_type_mod.f90:9-18).
+   precision: QMCW_FP64 / QMCW_MIXED / QMCW_MIXED_BF16. */
+int qmcw_create(const qmcw_system *sys, int32_t nwalk, int32_t precision,
+                qmcw_handle **out);
+int qmcw_destroy(qmcw_handle *h);
+
+/* Replace the wavefunction parameters (a4, b, c, csf_coef, coef) in place;
+   the arrays have the shapes given in qmcw_system.  Any pointer may be NULL
+   to keep the current values.  (Replaces the parameter update after the
+   linear-method step, qmc/optimization_mod.f90.) */
+int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b,
+                    const double *c, const double *csf_coef,
+                    const double *coef);
+
+/* Copy walker positions in/out (host arrays [W][nelec][3]). */
+int qmcw_set_walkers(qmcw_handle *h, const double *x);
+int qmcw_get_walkers(const qmcw_handle *h, double *x);
+
+/* Full evaluation of every walker at its current position, refreshing the
+   resident state (inverses, pair Jastrow arrays, orbitals).  Replaces
+   psi_at (qmc/psi_utils_mod.f90:13-42) -> hpsi (qmc/vmc/hpsi.f90:2-381).
+   Outputs (host, any may be NULL):
+     logpsid[W]  log|Psi_D|, sgn[W] sign of Psi_D, psij[W] ln J,
+     grad[W][N][3] grad ln Psi, lap[W][N] laplacian of ln Psi per electron,
+     eloc[W], epot[W] (total potential incl. nonlocal). */
+int qmcw_eval_full(qmcw_handle *h, double *logpsid, double *sgn,
+                   double *psij, double *grad, double *lap, double *eloc,
+                   double *epot);
+
+/* Propose moving electron ie of every walker to xnew[W][3] and return the
+   ratio Psi(new)/Psi(old) (determinant ratio times exp(Delta ln J)), the new
+   grad ln Psi of electron ie and its laplacian.  The proposal is held
+   pending until qmcw_accept_1e.  Replaces move_1e
+   (qmc/psi_utils_mod.f90:45-90) -> hpsie (qmc/vmc/hpsie.f90:1-71). */
+int qmcw_ratio_1e(qmcw_handle *h, int32_t ie, const double *xnew,
+                  double *ratio, double *grad_new, double *lap_new);
+
+/* Commit (accept[w] != 0) or drop the pending move of electron ie.
+   Replaces copy_psi_1e (qmc/psi_utils_mod.f90:299-387) and the VMC commit
+   jassav + detsav + dete_save (qmc/vmc/metrop_slat_mov1.f90:544-546). */
+int qmcw_accept_1e(qmcw_handle *h, int32_t ie, const uint8_t *accept);
+
+/* grad ln Psi and laplacian of electron ie in the current state.
+   Replaces calc_1e_derivatives (qmc/psi_utils_mod.f90:457-483). */
+int qmcw_grad_1e(qmcw_handle *h, int32_t ie, double *grad, double *lap);
+
+/* Number of parameters P and, per walker, d ln Psi/dp and dE_L/dp at the
+   last full evaluation, in CHAMP's order CSF, Jastrow, orbitals
+   (qmc/deriv_mod.f90:219-239).  Replaces dpsi_bld / deloc_bld
+   (qmc/deriv_mod.f90:248-556).  Arrays are [W][P]. */
+int32_t qmcw_nparm(const qmcw_handle *h);
+int qmcw_derivs(qmcw_handle *h, double *dpsi, double *deloc);
+
+/* Per-walker random streams (CHAMP's rannyu, qmc/vmc/rannyu.f90:53-74):
+   seed[W][4] 12-bit limbs as in setrn (qmc/vmc/rannyu.f90:11-21). */
+int qmcw_set_rng(qmcw_handle *h, const int32_t *seed);
+int qmcw_get_rng(const qmcw_handle *h, int32_t *seed);
+
+/* Run nstep VMC steps on device.  One step = a Metropolis sweep over all
+   electrons with the proposal of qmc/vmc/metrop_mov1.f90:63-99 (box step
+   delta, force bias fbias) followed by a full evaluation, as in
+   qmc/vmc/metrop_mov1.f90:283-300.  Per walker and step the accumulated
+   local energy is written to eloc_trace[step][W] when non-NULL (host).
+   naccept (host, may be NULL) receives the accepted move count per walker. */
+int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double fbias,
+                 double *eloc_trace, int64_t *naccept);
+
+/* Device-resident variant of qmcw_vmc_run for timing: no host copies;
+   the local energies of the last step stay on the device. */
+int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta,
+                        double fbias);
+
+/* Read back per-walker local energies of the last evaluation. */
+int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
+
+/* Wait for all work queued on the handle's stream. */
+int qmcw_sync(qmcw_handle *h);
+
+/* Last error message of the handle (or of the library when h is NULL). */
+const char *qmcw_last_error(const qmcw_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHAMP_AMD_H */

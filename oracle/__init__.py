@@ -1,0 +1,252 @@
+"""ctypes bindings of the FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+Parity status of each oracle function is listed in the module docstring of
+tests/test_oracle.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_here = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_here, "liboracle.so")
+REF_LIB = os.path.join(_here, "_ref", "libchamp_ref.so")
+
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+def build(ref: bool = True):
+    """Compile liboracle.so (and _ref/libchamp_ref.so when the reference
+    sources are present)."""
+    subprocess.run(["make", "-s", "-C", _here, "liboracle.so"], check=True)
+    if ref and os.path.isdir("/root/reference/qmc"):
+        subprocess.run(["make", "-s", "-C", _here, "ref"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(ref=False)
+        L = C.CDLL(LIB)
+        sig = {
+            "orc_create": (_vp, [_vp]),
+            "orc_destroy": (None, [_vp]),
+            "orc_error": (C.c_char_p, []),
+            "orc_nterms": (C.c_int32, [_vp]),
+            "orc_nparm": (C.c_int32, [_vp]),
+            "orc_asymp": (None, [_vp, _pd, _pd]),
+            "orc_cusp": (C.c_int32, [_vp, _pd, _pi]),
+            "orc_cuspexact": (None, [_vp, _pd]),
+            "orc_set_params": (None, [_vp, _pd, _pd, _pd, _pd, _pd]),
+            "orc_det_info": (None, [_vp, _pi]),
+            "orc_setrn": (None, [_pi, _pi]),
+            "orc_rannyu": (C.c_double, [_pi]),
+            "orc_gauss": (C.c_double, [_pi]),
+            "orc_quadrature": (None, [_vp, _pd, _pd, _pd, _pd]),
+            "orc_rotqua": (None, [_pi, _pd, _pd, _pd, C.c_int32, _pd, _pd, _pd]),
+            "orc_matinv": (C.c_double, [_pd, C.c_int32]),
+            "orc_basis": (None, [_vp, _pd, _pd, _pd, _pd]),
+            "orc_orbitals": (None, [_vp, _pd, _pd, _pd, _pd]),
+            "orc_walker_create": (_vp, [_vp]),
+            "orc_walker_destroy": (None, [_vp]),
+            "orc_walker_copy": (None, [_vp, _vp, _vp]),
+            "orc_walker_set_quad": (None, [_vp, _pd, _pd, _pd, C.c_int]),
+            "orc_eval_full": (C.c_int, [_vp, _vp, _pd, _pd, _pd, _pd]),
+            "orc_jastrow_full": (None, [_vp, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+            "orc_determinant_full": (C.c_int, [_vp, _pd, _pd, _pd, _pd, _pd]),
+            "orc_ratio_1e": (C.c_int, [_vp, _vp, C.c_int32, _pd, _pd, _pd, _pd]),
+            "orc_accept_1e": (None, [_vp, _vp, C.c_int32, C.c_int]),
+            "orc_grad_1e": (None, [_vp, _vp, C.c_int32, _pd, _pd]),
+            "orc_derivs": (C.c_int, [_vp, _vp, _pd, _pd]),
+            "orc_walker_get": (None, [_vp, _vp, _pd, _pd]),
+            "orc_vmc_run": (C.c_int, [_vp, C.c_int32, _pd, _pi, C.c_int32, C.c_double,
+                                      C.c_double, _pd, C.POINTER(C.c_int64), C.c_int32]),
+            "orc_eval_batch": (C.c_int, [_vp, C.c_int32, _pd, _pd, _pd, _pd, _pd, _pd,
+                                         C.c_int32]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_pd)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_pi)
+
+
+class Oracle:
+    """One wavefunction (an orc_sys) plus a scratch walker."""
+
+    def __init__(self, system):
+        self.system = system
+        self._struct = system.struct()
+        self.L = lib()
+        self.h = self.L.orc_create(C.byref(self._struct))
+        if not self.h:
+            raise RuntimeError(self.L.orc_error().decode())
+        self.N = system.nelec
+        self.P = self.L.orc_nparm(self.h)
+        self.w = self.L.orc_walker_create(self.h)
+
+    def __del__(self):
+        try:
+            self.L.orc_walker_destroy(self.w)
+            self.L.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    # ---- setup info ----
+    def asymp(self):
+        aa = np.zeros(self.system.nctype)
+        ab = np.zeros(2)
+        self.L.orc_asymp(self.h, _d(aa), _d(ab))
+        return aa, ab
+
+    def cusp(self):
+        nt = self.L.orc_nterms(self.h)
+        d = np.zeros((64, max(nt, 1)))
+        iw = np.zeros(64, np.int32)
+        n = self.L.orc_cusp(self.h, _d(d), _i(iw))
+        return d[:n, :nt].copy(), iw[:n].copy()
+
+    def det_info(self):
+        info = np.zeros(5, np.int32)
+        self.L.orc_det_info(self.h, _i(info))
+        return dict(ndetup=info[0], ndetdn=info[1], noccup=info[2], noccdn=info[3], kmax=info[4])
+
+    def set_params(self, a4=None, b=None, c=None, csf_coef=None, coef=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
+        arrs = [conv(a) for a in (a4, b, c, csf_coef, coef)]
+        self.L.orc_set_params(self.h, *[_d(a) for a in arrs])
+
+    # ---- pieces ----
+    def basis(self, xe):
+        nb = self.system.nbasis
+        xe = np.ascontiguousarray(xe, np.float64)
+        phi, dphi, d2 = np.zeros(nb), np.zeros((nb, 3)), np.zeros(nb)
+        self.L.orc_basis(self.h, _d(xe), _d(phi), _d(dphi), _d(d2))
+        return phi, dphi, d2
+
+    def orbitals(self, xe):
+        no = self.system.norb
+        xe = np.ascontiguousarray(xe, np.float64)
+        o, do, d2 = np.zeros(no), np.zeros((no, 3)), np.zeros(no)
+        self.L.orc_orbitals(self.h, _d(xe), _d(o), _d(do), _d(d2))
+        return o, do, d2
+
+    def jastrow(self, x, derivs=False):
+        N = self.N
+        x = np.ascontiguousarray(x, np.float64)
+        val, d2 = np.zeros(1), np.zeros(1)
+        v, dv = np.zeros((N, 3)), np.zeros(N)
+        npj = len(self.system.jas_a) + len(self.system.jas_b) + len(self.system.jas_c)
+        gv = g = d2g = None
+        if derivs:
+            gv, g, d2g = np.zeros(npj), np.zeros((npj, N, 3)), np.zeros(npj)
+        self.L.orc_jastrow_full(self.h, _d(x), _d(val), _d(v), _d(d2), _d(dv),
+                                _d(gv), _d(g), _d(d2g))
+        out = dict(value=val[0], v=v, d2=d2[0], divvj=dv)
+        if derivs:
+            out.update(gvalue=gv, g=g, d2g=d2g)
+        return out
+
+    def determinant(self, x):
+        N = self.N
+        x = np.ascontiguousarray(x, np.float64)
+        ld, sg = np.zeros(1), np.zeros(1)
+        vd, dv = np.zeros((N, 3)), np.zeros(N)
+        st = self.L.orc_determinant_full(self.h, _d(x), _d(ld), _d(sg), _d(vd), _d(dv))
+        return dict(logdet=ld[0], sign=sg[0], vd=vd, divvd=dv, status=st)
+
+    def eval_full(self, x, walker=None):
+        N = self.N
+        x = np.ascontiguousarray(x, np.float64)
+        out9, grad, divv = np.zeros(9), np.zeros((N, 3)), np.zeros(N)
+        st = self.L.orc_eval_full(self.h, walker or self.w, _d(x), _d(out9), _d(grad), _d(divv))
+        return dict(logpsid=out9[0], sign=out9[1], psij=out9[2], eloc=out9[3],
+                    pe=out9[4], d2lnpsi=out9[5], vpsp=out9[6], pe_ee=out9[7],
+                    pe_loc=out9[8], grad=grad, divv=divv, status=st)
+
+    def derivs(self, walker=None):
+        dp, de = np.zeros(self.P), np.zeros(self.P)
+        st = self.L.orc_derivs(self.h, walker or self.w, _d(dp), _d(de))
+        return dp, de
+
+    def ratio_1e(self, ie, xnew, walker=None):
+        xnew = np.ascontiguousarray(xnew, np.float64)
+        o4, v, lap = np.zeros(4), np.zeros(3), np.zeros(1)
+        st = self.L.orc_ratio_1e(self.h, walker or self.w, ie, _d(xnew), _d(o4), _d(v), _d(lap))
+        return dict(ratio=o4[0], logpsid=o4[1], sign=o4[2], psij=o4[3], grad=v,
+                    lap=lap[0], status=st)
+
+    def accept_1e(self, ie, accept, walker=None):
+        self.L.orc_accept_1e(self.h, walker or self.w, ie, int(bool(accept)))
+
+    def grad_1e(self, ie, walker=None):
+        g, lap = np.zeros(3), np.zeros(1)
+        self.L.orc_grad_1e(self.h, walker or self.w, ie, _d(g), _d(lap))
+        return g, lap[0]
+
+    def set_quad(self, xq, yq, zq, walker=None):
+        xq, yq, zq = (np.ascontiguousarray(a, np.float64) for a in (xq, yq, zq))
+        self.L.orc_walker_set_quad(walker or self.w, _d(xq), _d(yq), _d(zq), len(xq))
+
+    def quadrature(self):
+        q = [np.zeros(12) for _ in range(4)]
+        self.L.orc_quadrature(self.h, *[_d(a) for a in q])
+        return q
+
+    # ---- batched ----
+    def eval_batch(self, x, derivs=False, nthreads=0):
+        x = np.ascontiguousarray(x, np.float64)
+        nw, N = x.shape[0], self.N
+        out9, grad, divv = np.zeros((nw, 9)), np.zeros((nw, N, 3)), np.zeros((nw, N))
+        dp = de = None
+        if derivs and self.P:
+            dp, de = np.zeros((nw, self.P)), np.zeros((nw, self.P))
+        st = self.L.orc_eval_batch(self.h, nw, _d(x), _d(out9), _d(grad), _d(divv),
+                                   _d(dp), _d(de), nthreads)
+        return dict(out9=out9, grad=grad, divv=divv, dpsi=dp, deloc=de, status=st)
+
+    def vmc_run(self, x, rng, nstep, delta, fbias=1.0, trace=True, nthreads=0):
+        x = np.ascontiguousarray(x, np.float64).copy()
+        rng = np.ascontiguousarray(rng, np.int32).copy()
+        nw = x.shape[0]
+        tr = np.zeros((nstep, nw)) if trace else None
+        nacc = np.zeros(nw, np.int64)
+        st = self.L.orc_vmc_run(self.h, nw, _d(x), _i(rng), nstep, delta, fbias, _d(tr),
+                                nacc.ctypes.data_as(C.POINTER(C.c_int64)), nthreads)
+        return dict(x=x, rng=rng, eloc=tr, naccept=nacc, status=st)
+
+
+def rng_stream(seed4, n):
+    L = lib()
+    st = np.zeros(4, np.int32)
+    s = np.ascontiguousarray(seed4, np.int32)
+    L.orc_setrn(_i(st), _i(s))
+    return np.array([L.orc_rannyu(_i(st)) for _ in range(n)]), st
+
+
+def matinv(a):
+    a = np.asfortranarray(np.array(a, np.float64))
+    n = a.shape[0]
+    flat = np.ascontiguousarray(a.T.reshape(-1))   # column-major storage
+    det = lib().orc_matinv(_d(flat), n)
+    return flat.reshape(n, n).T.copy(), det
