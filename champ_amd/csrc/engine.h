@@ -1,0 +1,119 @@
+// engine.h -- device-side data layout of the walker-batched wavefunction
+// engine (MI355X / gfx950).  Shared by the kernels (engine_kernels.hip)
+// and the host side of the C ABI (engine_api.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/champ_amd.h"
+
+namespace qmcw {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kMaxShellM = 9;      // 2l+1 for l <= 4
+constexpr int kMaxCtype = 16;
+constexpr int kMaxTerms = 64;
+constexpr int kMaxKmax = 8;        // excitation order of an AMF alpha block
+constexpr int kMaxNs = 512;        // electrons of one spin
+
+// Flattened system in device memory.  Pointers are device pointers.
+struct DevSys {
+  int N, nup, ndn, M, nctype, NB, NO;
+  int basis_type;
+  double pecent, hb;
+  const double *cent;        // [M][3]
+  const int *iwctype;        // [M]
+  const double *znuc;        // [nctype]
+  // basis shells: one radial function r^a sum_p d_p exp(-zeta_p r^q) and the
+  // real solid harmonics S_lm (m listed in sh_m) that multiply it
+  int nshell;                // over all centres
+  const int *cent_sh_off;    // [M+1]
+  const int *sh_cent, *sh_l, *sh_a, *sh_nm, *sh_prim_off;
+  const int *sh_m;           // [nshell][9]
+  const int *sh_bas;         // [nshell][9] basis index of each component
+  const double *prim_zeta, *prim_coef;
+  int maxshell_per_cent;
+  const double *coefT;       // [NB][NO] (normalisation folded in)
+  // determinants (spin s = 0 up, 1 down)
+  int ns[2], nocc[2], ndetu[2], ndet, kmax, noccmax;
+  const int *ref[2];         // [ns] reference orbital labels
+  const int *occ[2];         // [nocc] occupied orbital labels (sorted union)
+  const int *iref[2];        // [ns] position in occ of ref_k
+  const int *ivir[2];        // [nocc-ns] occ positions not in ref
+  const int *ord[2];         // [ndetu] excitation order k
+  const int *dsgn[2];        // [ndetu] permutation sign
+  const int *rf[2];          // [ndetu][ns] replaced reference positions
+  const int *exl[2];         // [ndetu][ns] excited orbital labels
+  const int *exo[2];         // [ndetu][ns] excited orbital positions in occ
+  const int *iwdet[2];       // [ndet] unique det of each spin
+  const double *cdet;        // [ndet] cdet_in_wf
+  int kmax2;                 // kmax*kmax (>= 1)
+  // Jastrow ijas=4
+  int isc, nspin2b, nocuspb, norda, nordb, nordc, nterms;
+  double scalek, asymp_r;
+  const double *asymp_jasa;  // [nctype]
+  double asymp_jasb0, asymp_jasb1;
+  const double *a4;          // [nctype][norda+1]
+  const double *b;           // [nspin2b][nordb+1]
+  const double *c;           // [nctype][nterms]
+  const int *term_klm;       // [nterms][3]
+  // pseudopotential
+  int nloc, nquad;
+  const int *npotd, *lpotp1, *ecp_off, *ecp_power;
+  const double *ecp_coef, *ecp_expo;
+  const double *q0;          // [4][nquad] x0,y0,z0,w
+  // derivative parameters
+  int npcsf, npa, npb, npc, nporb, npj, nparm;
+  const int *pcsf;           // CSF index of each CSF parameter
+  const int *csf_det_off, *csf_det;
+  const double *csf_det_coef, *csf_coef;
+  int ncsf;
+  const int *pa_type, *pa_idx, *pb_spin, *pb_idx, *pc_type, *pc_idx;
+  const int *c_dep_n;        // [nterms*nctype] number of params a term feeds
+  const int *c_dep_p;        // [nterms*nctype][maxdep] parameter index
+  const double *c_dep_w;     // [nterms*nctype][maxdep] weight
+  int c_maxdep;
+  const int *porb_i, *porb_j, *porb_rev;
+  const int *iocc_lab[2];    // [NO] position in occ or -1
+};
+
+// Offsets (in doubles) of one walker's persistent state.
+struct WLayout {
+  int64_t x, rvec_en, r_en, rvec_ee, r_ee;
+  int64_t fso, fijo, lapj, fjo, fsumo;
+  int64_t oc;                // [N][noccmax][5] occupied-orbital values
+  int64_t ai[2], tocc[2], inv[2], detv[2], y[2], deta[2], chi;
+  int64_t psi2o, logpsid, sgn, psij, eloc, epot;
+  int64_t quad;              // [3][nquad] rotated quadrature grid
+  int64_t stride;
+};
+
+// Offsets (in doubles) of one walker's scratch.
+struct SLayout {
+  int64_t phi;               // [max(NB, points)][5] basis values
+  int64_t orb;               // [N][NO][5] all orbitals (full evaluation)
+  int64_t g[2];              // [nocc][ns] G matrices
+  int64_t tfull[2];          // [ns][NO]
+  int64_t vd, divv, vj;      // [N][3], [N], [N][3]
+  int64_t pnl;               // [N][NO] psp_nonloc_orb
+  int64_t pnl_c;             // [N*M][NO] per-centre partials
+  int64_t red;               // reduction scratch
+  int64_t cup;               // [ndetu0 + ndetu1] det-coefficient sums
+  int64_t qd;                // [G][4M] quadrature-point distances
+  int64_t mv;                // pending one-electron move buffers
+  int64_t gvalue, gjas, d2g, dpe; // Jastrow parameter derivatives
+  int64_t dpsi, deloc;       // [nparm]
+  int64_t stride;
+};
+
+// Pending one-electron move buffers inside SLayout::mv
+struct MLayout {
+  int64_t xn, ren, rven, ree, rvee;       // new distances
+  int64_t fsn, fijn_i, fijn_j, lapn_i;    // pair rows [N]...
+  int64_t fjn;                            // [N][3]
+  int64_t orbe;                           // [noccmax][5]
+  int64_t ain, tn, invn, detn, yn, misc;  // candidate det state
+  int64_t stride;
+};
+
+}  // namespace qmcw

@@ -1,0 +1,819 @@
+// engine_api.hip -- host side of the C ABI (include/champ_amd.h).
+//
+// qmcw_create flattens the system into device arrays (basis shells, the
+// determinant/excitation bookkeeping of CHAMP's AMF formulas, Jastrow
+// constants, quadrature grid), lays out the per-walker state and scratch,
+// and picks the lane-group width G.  The entry points launch the kernels of
+// engine_kernels.h on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_kernels.h"
+
+using namespace qmcw;
+
+static thread_local std::string g_lib_err;
+
+struct qmcw_handle {
+  int W = 0, Wp = 0, G = 16, precision = QMCW_FP64, chunk = 1;
+  hipStream_t stream = nullptr;
+  KArgs A{};
+  std::vector<void *> allocs;
+  double *io = nullptr;
+  size_t io_doubles = 0;
+  uint8_t *acc_buf = nullptr;
+  long long *nacc = nullptr;
+  double *trace = nullptr;
+  int trace_steps = 0;
+  // host copies of the parameter arrays that set_params can replace
+  std::vector<double> a4, b, c, csf_coef, coef, cdet, asymp_jasa;
+  double *d_a4 = nullptr, *d_b = nullptr, *d_c = nullptr, *d_cdet = nullptr,
+         *d_coefT = nullptr, *d_asymp = nullptr, *d_csf_coef = nullptr;
+  // determinant bookkeeping (host) used to rebuild cdet
+  int ncsf = 0, ndet = 0;
+  std::vector<int> csf_det_off, csf_det;
+  std::vector<double> csf_det_coef;
+  int nparm = 0;
+  std::string err;
+};
+
+#define HIPCHK(h, call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      if (h) (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);     \
+      else g_lib_err = std::string(#call) + ": " + hipGetErrorString(e_);      \
+      return QMCW_ERR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+static int fail(qmcw_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  else g_lib_err = msg;
+  return code;
+}
+
+template <typename V>
+static int upload(qmcw_handle *h, const V &vec, const typename V::value_type **out) {
+  using T = typename V::value_type;
+  size_t n = std::max<size_t>(vec.size(), 1);
+  void *p = nullptr;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return fail(h, QMCW_ERR_NOMEM, "hipMalloc failed");
+  h->allocs.push_back(p);
+  if (!vec.empty() && hipMemcpy(p, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(h, QMCW_ERR_HIP, "hipMemcpy failed");
+  *out = (const T *)p;
+  return QMCW_OK;
+}
+
+extern "C" int32_t qmcw_nterms4(int32_t nord) {
+  int i = 0;
+  for (int n = 2; n <= nord; n++)
+    for (int k = n - 1; k >= 0; k--) {
+      int l_hi = (k == 0) ? n - k - 2 : n - k;
+      for (int l = l_hi; l >= 0; l--)
+        if ((n - k - l) % 2 == 0) i++;
+    }
+  return i;
+}
+
+// -------------------------------------------------------------------------
+// host-side preparation
+namespace {
+
+struct DetBook {
+  int ns[2] = {0, 0}, nocc[2] = {0, 0}, ndetu[2] = {0, 0}, kmax = 0;
+  std::vector<int> ref[2], occ[2], iref[2], ivir[2], ord[2], sgn[2], rf[2], exl[2], exo[2],
+      iwdet[2], iocc_lab[2];
+};
+
+// unique determinants of each spin (qmc/vmc/determinant_up_dn.f90), the
+// sorted union of occupied orbitals and the excitation of each unique det
+// relative to the first one (qmc/deriv_fast_mod.f90:211-365)
+DetBook build_dets(const qmcw_system *s) {
+  DetBook D;
+  int nsz[2] = {s->nup, s->nelec - s->nup};
+  const int *lists[2] = {s->det_up, s->det_dn};
+  for (int sp = 0; sp < 2; sp++) {
+    int ns = nsz[sp];
+    D.ns[sp] = ns;
+    std::vector<std::vector<int>> uniq;
+    D.iwdet[sp].resize(s->ndet);
+    for (int d = 0; d < s->ndet; d++) {
+      std::vector<int> v(lists[sp] + (size_t)d * ns, lists[sp] + (size_t)d * ns + ns);
+      int f = -1;
+      for (size_t u = 0; u < uniq.size(); u++)
+        if (uniq[u] == v) { f = (int)u; break; }
+      if (f < 0) { uniq.push_back(v); f = (int)uniq.size() - 1; }
+      D.iwdet[sp][d] = f;
+    }
+    if (ns == 0) { uniq.assign(1, std::vector<int>()); std::fill(D.iwdet[sp].begin(), D.iwdet[sp].end(), 0); }
+    D.ndetu[sp] = (int)uniq.size();
+    std::vector<int> occ;
+    for (auto &v : uniq)
+      for (int o : v)
+        if (std::find(occ.begin(), occ.end(), o) == occ.end()) occ.push_back(o);
+    std::sort(occ.begin(), occ.end());
+    D.occ[sp] = occ;
+    D.nocc[sp] = (int)occ.size();
+    D.iocc_lab[sp].assign(s->norb, -1);
+    for (size_t j = 0; j < occ.size(); j++) D.iocc_lab[sp][occ[j]] = (int)j;
+    const std::vector<int> &ref = uniq[0];
+    D.ref[sp] = ref;
+    for (int k = 0; k < ns; k++) D.iref[sp].push_back(D.iocc_lab[sp][ref[k]]);
+    for (int j = 0; j < D.nocc[sp]; j++)
+      if (std::find(D.iref[sp].begin(), D.iref[sp].end(), j) == D.iref[sp].end()) D.ivir[sp].push_back(j);
+    int nu = D.ndetu[sp];
+    D.ord[sp].assign(nu, 0);
+    D.sgn[sp].assign(nu, 1);
+    D.rf[sp].assign((size_t)nu * std::max(ns, 1), 0);
+    D.exl[sp].assign((size_t)nu * std::max(ns, 1), 0);
+    D.exo[sp].assign((size_t)nu * std::max(ns, 1), 0);
+    for (int u = 0; u < nu; u++) {
+      const std::vector<int> &det = uniq[u];
+      std::vector<int> out_ref, in_det;
+      for (int o : ref) if (std::find(det.begin(), det.end(), o) == det.end()) out_ref.push_back(o);
+      for (int o : det) if (std::find(ref.begin(), ref.end(), o) == ref.end()) in_det.push_back(o);
+      int k = (int)out_ref.size();
+      D.ord[sp][u] = k;
+      D.kmax = std::max(D.kmax, k);
+      for (int q = 0; q < k; q++) {
+        D.rf[sp][u * ns + q] = (int)(std::find(ref.begin(), ref.end(), out_ref[q]) - ref.begin());
+        D.exl[sp][u * ns + q] = in_det[q];
+        D.exo[sp][u * ns + q] = D.iocc_lab[sp][in_det[q]];
+      }
+      // parity of the permutation bringing det into reference order: orbitals
+      // shared with the reference are moved to their reference positions
+      int n = 1, nsw = 0;
+      for (int i = 1; i <= ns; i++) {
+        auto it = std::find(ref.begin(), ref.end(), det[i - 1]);
+        if (it != ref.end()) {
+          int ind = (int)(it - ref.begin()) + 1;
+          nsw += (i - n) + (ind - n);
+          n++;
+        }
+      }
+      D.sgn[sp][u] = 1 - 2 * (nsw % 2);
+    }
+  }
+  return D;
+}
+
+std::vector<double> build_cdet(int ndet, int ncsf, const double *csf_coef, const int *off,
+                               const int *dets, const double *cdc) {
+  std::vector<double> c(ndet, 0.0);
+  for (int k = 0; k < ncsf; k++)
+    for (int q = off[k]; q < off[k + 1]; q++) c[dets[q]] += cdc[q] * csf_coef[k];
+  return c;
+}
+
+}  // namespace
+
+static int set_asymp(qmcw_handle *h, const qmcw_system *s, DevSys &S) {
+  double k = s->scalek, ar = 1.0 / k;
+  S.asymp_r = ar;
+  h->asymp_jasa.assign(s->nctype, 0.0);
+  for (int it = 0; it < s->nctype; it++) {
+    const double *a = h->a4.data() + it * (s->norda + 1);
+    double v = a[0] * ar / (1.0 + a[1] * ar);
+    for (int iord = 2; iord <= s->norda; iord++) v += a[iord] * std::pow(ar, iord);
+    h->asymp_jasa[it] = v;
+  }
+  for (int i = 0; i < 2; i++) {
+    double sspinn = 1.0;
+    int isp = 0;
+    if (i == 1) {
+      sspinn = (s->nspin2b == 1 && s->nocuspb == 0) ? 0.5 : 1.0;
+      isp = s->nspin2b - 1;
+    }
+    const double *b = h->b.data() + isp * (s->nordb + 1);
+    double v = sspinn * b[0] * ar / (1.0 + b[1] * ar);
+    for (int iord = 2; iord <= s->nordb; iord++) v += b[iord] * std::pow(ar, iord);
+    if (i == 0) S.asymp_jasb0 = v; else S.asymp_jasb1 = v;
+  }
+  return QMCW_OK;
+}
+
+// 12-point icosahedral rule with CHAMP's value of pi (qmc/vmc/rotqua.f90:300-500)
+static void gesqua12(double *q) {
+  const double pi = 3.14159265359;
+  double *x = q, *y = q + 12, *z = q + 24, *w = q + 36;
+  for (int j = 0; j < 12; j++) { x[j] = y[j] = z[j] = 0; w[j] = 1.0 / 12.0; }
+  z[0] = 1.0;
+  z[1] = -1.0;
+  double fi0 = pi / 5.0, s5 = std::sqrt(5.0), c = 1.0 / s5, sn = 2.0 / s5;
+  for (int j = 1; j <= 5; j++) {
+    double rk2 = 2.0 * (j - 1), rk21 = rk2 + 1.0;
+    x[j + 1] = sn * std::cos(rk2 * fi0);
+    y[j + 1] = sn * std::sin(rk2 * fi0);
+    z[j + 1] = c;
+    x[j + 6] = sn * std::cos(rk21 * fi0);
+    y[j + 6] = sn * std::sin(rk21 * fi0);
+    z[j + 6] = -c;
+  }
+}
+
+// -------------------------------------------------------------------------
+// kernels local to the API
+__global__ void k_init_walkers(KArgs A, int W) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  double *ws = A.st + (int64_t)w * A.L.stride;
+  int nq = A.S.nquad;
+  for (int q = 0; q < 3 * nq; q++) ws[A.L.quad + q] = A.S.q0[q];
+  if (w >= W) {  // padded walkers copy walker 0
+    const double *w0 = A.st + A.L.x;
+    for (int q = 0; q < 3 * A.S.N; q++) ws[A.L.x + q] = w0[q];
+    for (int q = 0; q < 4; q++) A.rng[4 * w + q] = A.rng[q];
+  }
+}
+
+__global__ void k_gather_full(KArgs A, double *out, int W) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const int N = A.S.N, stride = 8 + 4 * N;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  const double *ss = A.sc + (int64_t)w * A.SL.stride;
+  double *o = out + (int64_t)w * stride;
+  o[0] = ws[A.L.logpsid]; o[1] = ws[A.L.sgn]; o[2] = ws[A.L.psij]; o[3] = ws[A.L.eloc];
+  o[4] = ws[A.L.epot]; o[5] = ss[A.SL.red + 60]; o[6] = ss[A.SL.red + 61]; o[7] = ss[A.SL.red + 62];
+  for (int q = 0; q < 3 * N; q++) o[8 + q] = ss[A.SL.vj + q] + ss[A.SL.vd + q];
+  for (int i = 0; i < N; i++) o[8 + 3 * N + i] = ss[A.SL.divv + i];
+}
+
+__global__ void k_trace_eloc(KArgs A, double *trace, int step, int W) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  trace[(int64_t)step * W + w] = A.st[(int64_t)w * A.L.stride + A.L.eloc];
+}
+
+__global__ void k_copy_strided(double *dst, int64_t dstride, const double *src, int64_t sstride,
+                               int n, int rows) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  for (int q = 0; q < n; q++) dst[(int64_t)r * dstride + q] = src[(int64_t)r * sstride + q];
+}
+
+// -------------------------------------------------------------------------
+template <int G>
+static void launch_eval(qmcw_handle *h) {
+  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  if (h->precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_eval_full<double, G>), grid, block, 0, h->stream, h->A);
+  else
+    hipLaunchKernelGGL((k_eval_full<float, G>), grid, block, 0, h->stream, h->A);
+}
+template <int G>
+static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
+                         double fbias, double *out) {
+  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  if (h->precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_ratio_1e<double, G>), grid, block, 0, h->stream, h->A, ie, mode, xn,
+                       delta, fbias, out);
+  else
+    hipLaunchKernelGGL((k_ratio_1e<float, G>), grid, block, 0, h->stream, h->A, ie, mode, xn,
+                       delta, fbias, out);
+}
+template <int G>
+static void launch_accept(qmcw_handle *h, int ie, int mode, const uint8_t *acc, double delta,
+                          double fbias, long long *nacc) {
+  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  hipLaunchKernelGGL((k_accept_1e<G>), grid, block, 0, h->stream, h->A, ie, mode, acc, delta,
+                     fbias, nacc);
+}
+template <int G>
+static void launch_grad(qmcw_handle *h, int ie, double *out) {
+  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  hipLaunchKernelGGL((k_grad_1e<G>), grid, block, 0, h->stream, h->A, ie, out);
+}
+
+#define DISPATCH_G(h, fn, ...)                    \
+  switch ((h)->G) {                               \
+    case 8: fn<8>(h, ##__VA_ARGS__); break;       \
+    case 16: fn<16>(h, ##__VA_ARGS__); break;     \
+    case 32: fn<32>(h, ##__VA_ARGS__); break;     \
+    default: fn<64>(h, ##__VA_ARGS__); break;     \
+  }
+
+// -------------------------------------------------------------------------
+extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precision,
+                           qmcw_handle **out) {
+  if (!s || !out || nwalk <= 0) return fail(nullptr, QMCW_ERR_ARG, "invalid arguments");
+  if (s->basis_type != QMCW_BASIS_SLATER && s->basis_type != QMCW_BASIS_GAUSSIAN)
+    return fail(nullptr, QMCW_ERR_ARG, "basis_type must be Slater or Gaussian");
+  if (s->isc != 2 && s->isc != 4) return fail(nullptr, QMCW_ERR_ARG, "isc must be 2 or 4");
+  if (s->nordc > 12 || s->norda > QMCW_MAX_ORD || s->nordb > QMCW_MAX_ORD)
+    return fail(nullptr, QMCW_ERR_ARG, "Jastrow order too large");
+  if (s->nloc != 0 && (s->nloc != 6 || s->nquad != 12))
+    return fail(nullptr, QMCW_ERR_ARG, "pseudopotential must be nloc=6 with nquad=12");
+  if (precision < QMCW_FP64 || precision > QMCW_MIXED_BF16)
+    return fail(nullptr, QMCW_ERR_ARG, "unknown precision mode");
+  int nup = s->nup, ndn = s->nelec - s->nup;
+  if (nup > 128 || ndn > 128) return fail(nullptr, QMCW_ERR_ARG, "more than 128 electrons of one spin");
+  qmcw_handle *h = new qmcw_handle();
+  h->W = nwalk;
+  h->precision = precision;
+  DevSys &S = h->A.S;
+  S.N = s->nelec; S.nup = nup; S.ndn = ndn; S.M = s->ncent; S.nctype = s->nctype;
+  S.NO = s->norb; S.basis_type = s->basis_type; S.hb = s->hb > 0 ? s->hb : 0.5;
+  S.pecent = 0.0;
+  for (int i = 1; i < s->ncent; i++)
+    for (int j = 0; j < i; j++) {
+      double d2 = 0;
+      for (int k = 0; k < 3; k++) {
+        double d = s->cent[3 * i + k] - s->cent[3 * j + k];
+        d2 += d * d;
+      }
+      S.pecent += s->znuc[s->iwctype[i]] * s->znuc[s->iwctype[j]] / std::sqrt(d2);
+    }
+  int st = 0;
+#define UP(vec, dst)                                   \
+  do {                                                 \
+    if ((st = upload(h, vec, &(dst))) != QMCW_OK) {    \
+      *out = nullptr;                                  \
+      g_lib_err = h->err;                              \
+      qmcw_destroy(h);                                 \
+      return st;                                       \
+    }                                                  \
+  } while (0)
+  std::vector<double> cent(s->cent, s->cent + 3 * s->ncent);
+  std::vector<int> iwctype(s->iwctype, s->iwctype + s->ncent);
+  std::vector<double> znuc(s->znuc, s->znuc + s->nctype);
+  UP(cent, S.cent); UP(iwctype, S.iwctype); UP(znuc, S.znuc);
+  // ---- basis shells: group each type's functions by (radial function
+  // content, l); the components of one shell share the radial part
+  std::vector<int> sh_cent, sh_l, sh_a, sh_nm, sh_prim_off{0}, sh_m, sh_bas, cent_sh_off{0};
+  std::vector<double> pz, pc;
+  int nbas = 0, maxsh = 0;
+  for (int ic = 0; ic < s->ncent; ic++) {
+    int t = s->iwctype[ic];
+    int b0 = s->ctype_bas_off[t], b1 = s->ctype_bas_off[t + 1];
+    int first_shell = (int)sh_l.size();
+    for (int q = b0; q < b1; q++) {
+      int r = s->bas_rad[q], l = s->bas_l[q], m = s->bas_m[q];
+      int n = s->rad_n[r];
+      if (l < 0 || l > QMCW_MAX_L || m < -l || m > l) {
+        delete h;
+        return fail(nullptr, QMCW_ERR_ARG, "basis function with l > 4 or |m| > l");
+      }
+      if (n - 1 - l < 0) {
+        delete h;
+        return fail(nullptr, QMCW_ERR_ARG, "basis function with n < l + 1");
+      }
+      int p0 = s->rad_prim_off[r], p1 = s->rad_prim_off[r + 1];
+      int found = -1;
+      for (int sh = first_shell; sh < (int)sh_l.size() && found < 0; sh++) {
+        if (sh_l[sh] != l || sh_a[sh] != n - 1 - l || sh_nm[sh] >= 2 * l + 1) continue;
+        int np = sh_prim_off[sh + 1] - sh_prim_off[sh];
+        if (np != p1 - p0) continue;
+        bool same = true;
+        for (int p = 0; p < np; p++)
+          if (pz[sh_prim_off[sh] + p] != s->prim_zeta[p0 + p] ||
+              pc[sh_prim_off[sh] + p] != s->prim_coef[p0 + p]) same = false;
+        bool mdup = false;
+        for (int qq = 0; qq < sh_nm[sh]; qq++) if (sh_m[sh * kMaxShellM + qq] == m) mdup = true;
+        if (same && !mdup) found = sh;
+      }
+      if (found < 0) {
+        sh_cent.push_back(ic); sh_l.push_back(l); sh_a.push_back(n - 1 - l); sh_nm.push_back(0);
+        for (int p = p0; p < p1; p++) { pz.push_back(s->prim_zeta[p]); pc.push_back(s->prim_coef[p]); }
+        sh_prim_off.push_back((int)pz.size());
+        for (int qq = 0; qq < kMaxShellM; qq++) { sh_m.push_back(0); sh_bas.push_back(0); }
+        found = (int)sh_l.size() - 1;
+      }
+      sh_m[found * kMaxShellM + sh_nm[found]] = m;
+      sh_bas[found * kMaxShellM + sh_nm[found]] = nbas + (q - b0);
+      sh_nm[found]++;
+    }
+    nbas += b1 - b0;
+    cent_sh_off.push_back((int)sh_l.size());
+    maxsh = std::max(maxsh, (int)sh_l.size() - first_shell);
+  }
+  if (nbas != s->nbasis) {
+    delete h;
+    return fail(nullptr, QMCW_ERR_ARG, "nbasis does not match the per-type basis lists");
+  }
+  S.NB = nbas; S.nshell = (int)sh_l.size(); S.maxshell_per_cent = maxsh;
+  UP(sh_cent, S.sh_cent); UP(sh_l, S.sh_l); UP(sh_a, S.sh_a); UP(sh_nm, S.sh_nm);
+  UP(sh_prim_off, S.sh_prim_off); UP(sh_m, S.sh_m); UP(sh_bas, S.sh_bas);
+  UP(cent_sh_off, S.cent_sh_off); UP(pz, S.prim_zeta); UP(pc, S.prim_coef);
+  h->coef.assign(s->coef, s->coef + (size_t)s->norb * s->nbasis);
+  std::vector<double> coefT((size_t)nbas * s->norb);
+  for (int o = 0; o < s->norb; o++)
+    for (int b = 0; b < nbas; b++) coefT[(size_t)b * s->norb + o] = h->coef[(size_t)o * nbas + b];
+  UP(coefT, S.coefT);
+  h->d_coefT = const_cast<double *>(S.coefT);
+  // ---- determinants
+  DetBook D = build_dets(s);
+  if (D.kmax > kMaxKmax) {
+    delete h;
+    return fail(nullptr, QMCW_ERR_ARG, "excitation order above 8");
+  }
+  S.ndet = s->ndet; S.kmax = D.kmax; S.kmax2 = std::max(1, D.kmax * D.kmax);
+  S.noccmax = std::max(D.nocc[0], D.nocc[1]);
+  for (int sp = 0; sp < 2; sp++) {
+    S.ns[sp] = D.ns[sp]; S.nocc[sp] = D.nocc[sp]; S.ndetu[sp] = D.ndetu[sp];
+    UP(D.ref[sp], S.ref[sp]); UP(D.occ[sp], S.occ[sp]); UP(D.iref[sp], S.iref[sp]);
+    UP(D.ivir[sp], S.ivir[sp]); UP(D.ord[sp], S.ord[sp]); UP(D.sgn[sp], S.dsgn[sp]);
+    UP(D.rf[sp], S.rf[sp]); UP(D.exl[sp], S.exl[sp]); UP(D.exo[sp], S.exo[sp]);
+    UP(D.iwdet[sp], S.iwdet[sp]); UP(D.iocc_lab[sp], S.iocc_lab[sp]);
+  }
+  h->ncsf = s->ncsf; h->ndet = s->ndet;
+  h->csf_det_off.assign(s->csf_det_off, s->csf_det_off + s->ncsf + 1);
+  h->csf_det.assign(s->csf_det, s->csf_det + s->csf_det_off[s->ncsf]);
+  h->csf_det_coef.assign(s->csf_det_coef, s->csf_det_coef + s->csf_det_off[s->ncsf]);
+  h->csf_coef.assign(s->csf_coef, s->csf_coef + s->ncsf);
+  h->cdet = build_cdet(s->ndet, s->ncsf, s->csf_coef, s->csf_det_off, s->csf_det, s->csf_det_coef);
+  UP(h->cdet, S.cdet);
+  h->d_cdet = const_cast<double *>(S.cdet);
+  S.ncsf = s->ncsf;
+  UP(h->csf_det_off, S.csf_det_off); UP(h->csf_det, S.csf_det);
+  UP(h->csf_det_coef, S.csf_det_coef); UP(h->csf_coef, S.csf_coef);
+  h->d_csf_coef = const_cast<double *>(S.csf_coef);
+  // ---- Jastrow
+  S.isc = s->isc; S.nspin2b = s->nspin2b; S.nocuspb = s->nocuspb;
+  S.norda = s->norda; S.nordb = s->nordb; S.nordc = s->nordc; S.scalek = s->scalek;
+  S.nterms = qmcw_nterms4(s->nordc);
+  h->a4.assign(s->a4, s->a4 + s->nctype * (s->norda + 1));
+  h->b.assign(s->b, s->b + s->nspin2b * (s->nordb + 1));
+  h->c.assign(s->c, s->c + std::max(1, s->nctype * S.nterms));
+  set_asymp(h, s, S);
+  UP(h->a4, S.a4); UP(h->b, S.b); UP(h->c, S.c); UP(h->asymp_jasa, S.asymp_jasa);
+  h->d_a4 = const_cast<double *>(S.a4); h->d_b = const_cast<double *>(S.b);
+  h->d_c = const_cast<double *>(S.c); h->d_asymp = const_cast<double *>(S.asymp_jasa);
+  std::vector<int> klm;
+  for (int n = 2; n <= s->nordc; n++)
+    for (int k = n - 1; k >= 0; k--) {
+      int l_hi = (k == 0) ? n - k - 2 : n - k;
+      for (int l = l_hi; l >= 0; l--) {
+        int m = (n - k - l) / 2;
+        if (2 * m == n - k - l) { klm.push_back(k); klm.push_back(l); klm.push_back(m); }
+      }
+    }
+  UP(klm, S.term_klm);
+  // ---- pseudopotential
+  S.nloc = s->nloc;
+  S.nquad = 12;
+  std::vector<double> q0(48);
+  gesqua12(q0.data());
+  UP(q0, S.q0);
+  if (s->nloc > 0) {
+    std::vector<int> npotd(s->npotd, s->npotd + s->nctype), lpotp1(s->lpotp1, s->lpotp1 + s->nctype);
+    int nt = s->ecp_off[s->nctype * QMCW_MAX_LPOT];
+    std::vector<int> off(s->ecp_off, s->ecp_off + s->nctype * QMCW_MAX_LPOT + 1);
+    std::vector<int> pw(s->ecp_power, s->ecp_power + nt);
+    std::vector<double> cf(s->ecp_coef, s->ecp_coef + nt), ex(s->ecp_expo, s->ecp_expo + nt);
+    UP(npotd, S.npotd); UP(lpotp1, S.lpotp1); UP(off, S.ecp_off); UP(pw, S.ecp_power);
+    UP(cf, S.ecp_coef); UP(ex, S.ecp_expo);
+  }
+  // ---- parameter counts (derivatives)
+  S.npcsf = s->nparmcsf; S.npa = s->nparma_tot; S.npb = s->nparmb_tot; S.npc = s->nparmc_tot;
+  S.nporb = s->nparmorb; S.npj = S.npa + S.npb + S.npc;
+  S.nparm = S.npcsf + S.npj + S.nporb;
+  h->nparm = S.nparm;
+  // ---- lane-group width and layouts
+  int N = S.N, M = S.M, NO = S.NO, NB = S.NB;
+  int G = N <= 4 ? 8 : (N <= 16 ? 16 : (N <= 40 ? 32 : 64));
+  if (const char *e = getenv("QMCW_GROUP")) {
+    int g = atoi(e);
+    if (g == 8 || g == 16 || g == 32 || g == 64) G = g;
+  }
+  h->G = G;
+  int wpb = kWave / G;
+  h->Wp = ((nwalk + wpb - 1) / wpb) * wpb;
+  h->chunk = std::max(1, std::min(N, 32768 / (NB * 5)));
+  h->A.chunk = h->chunk;
+  h->A.full_orb = 0;
+  h->A.derivs = 0;
+  int nsm = std::max(nup, ndn), noccm = S.noccmax;
+  int ndum = std::max(D.ndetu[0], D.ndetu[1]);
+  WLayout &L = h->A.L;
+  int64_t o = 0;
+  auto take = [&o](int64_t n) { int64_t r = o; o += std::max<int64_t>(n, 1); return r; };
+  L.x = take(3 * N); L.rvec_en = take(3 * N * M); L.r_en = take(N * M);
+  L.rvec_ee = take(3 * N * N); L.r_ee = take(N * N);
+  L.fso = take(N * N); L.fijo = take(3 * N * N); L.lapj = take(N * N); L.fjo = take(3 * N);
+  L.fsumo = take(1); L.oc = take((int64_t)N * noccm * 5);
+  for (int sp = 0; sp < 2; sp++) {
+    L.ai[sp] = take(D.ns[sp] * D.ns[sp]); L.tocc[sp] = take(D.ns[sp] * D.nocc[sp]);
+    L.inv[sp] = take(D.ndetu[sp] * S.kmax2); L.detv[sp] = take(D.ndetu[sp]);
+    L.y[sp] = take(D.nocc[sp] * D.ns[sp]); L.deta[sp] = take(1);
+  }
+  L.chi = take(1); L.psi2o = take(1); L.logpsid = take(1); L.sgn = take(1); L.psij = take(1);
+  L.eloc = take(1); L.epot = take(1); L.quad = take(36);
+  L.stride = (o + 7) / 8 * 8;
+  MLayout &ML = h->A.ML;
+  o = 0;
+  ML.xn = take(3); ML.ren = take(M); ML.rven = take(3 * M); ML.ree = take(N); ML.rvee = take(3 * N);
+  ML.fsn = take(N); ML.fijn_i = take(3 * N); ML.fijn_j = take(3 * N); ML.lapn_i = take(2 * N);
+  ML.fjn = take(3 * N); ML.orbe = take(noccm * 5); ML.ain = take(nsm * nsm);
+  ML.tn = take(nsm * noccm); ML.invn = take(ndum * S.kmax2); ML.detn = take(ndum);
+  ML.yn = take(D.nocc[0] * D.ns[0] + D.nocc[1] * D.ns[1]); ML.misc = take(16 + nsm + noccm);
+  ML.stride = o;
+  SLayout &SL = h->A.SL;
+  o = 0;
+  SL.phi = take((int64_t)std::max(h->chunk, G) * NB * 5);
+  SL.orb = take(h->A.full_orb ? (int64_t)N * NO * 5 : 1);
+  SL.g[0] = take(D.nocc[0] * D.ns[0]); SL.g[1] = take(D.nocc[1] * D.ns[1]);
+  SL.tfull[0] = take(1); SL.tfull[1] = take(1);
+  SL.vd = take(3 * N); SL.divv = take(N); SL.vj = take(3 * N);
+  SL.pnl = take((int64_t)N * noccm + (h->A.full_orb ? (int64_t)N * NO : 0));
+  SL.pnl_c = take(1);
+  SL.red = take(std::max(std::max(64, nsm * nsm), noccm * nsm) + 64);
+  SL.cup = take(D.ndetu[0] + D.ndetu[1]);
+  SL.qd = take((int64_t)G * 4 * M);
+  SL.mv = take(ML.stride);
+  SL.gvalue = take(1); SL.gjas = take(1); SL.d2g = take(1); SL.dpe = take(1);
+  SL.dpsi = take(1); SL.deloc = take(1);
+  SL.stride = (o + 7) / 8 * 8;
+#undef UP
+  // ---- device allocations
+  if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail(nullptr, QMCW_ERR_HIP, "hipStreamCreate"); }
+  size_t stb = (size_t)h->Wp * L.stride * sizeof(double), scb = (size_t)h->Wp * SL.stride * sizeof(double);
+  void *p;
+  if (hipMalloc(&p, stb) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "state alloc"); }
+  h->allocs.push_back(p); h->A.st = (double *)p;
+  if (hipMalloc(&p, scb) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "scratch alloc"); }
+  h->allocs.push_back(p); h->A.sc = (double *)p;
+  if (hipMalloc(&p, (size_t)h->Wp * 4 * sizeof(int)) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "rng alloc"); }
+  h->allocs.push_back(p); h->A.rng = (int *)p;
+  if (hipMalloc(&p, (size_t)h->Wp * sizeof(int)) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "flag alloc"); }
+  h->allocs.push_back(p); h->A.flags = (int *)p;
+  if (hipMalloc(&p, (size_t)h->Wp * sizeof(long long)) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "counter alloc"); }
+  h->allocs.push_back(p); h->nacc = (long long *)p;
+  if (hipMalloc(&p, (size_t)h->Wp) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "flag alloc"); }
+  h->allocs.push_back(p); h->acc_buf = (uint8_t *)p;
+  h->io_doubles = (size_t)h->Wp * std::max(8 + 4 * N, 8);
+  if (hipMalloc(&p, h->io_doubles * sizeof(double)) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "io alloc"); }
+  h->allocs.push_back(p); h->io = (double *)p;
+  (void)hipMemsetAsync(h->A.st, 0, stb, h->stream);
+  (void)hipMemsetAsync(h->A.sc, 0, scb, h->stream);
+  (void)hipMemsetAsync(h->A.rng, 0, (size_t)h->Wp * 4 * sizeof(int), h->stream);
+  (void)hipMemsetAsync(h->nacc, 0, (size_t)h->Wp * sizeof(long long), h->stream);
+  h->A.Wp = h->Wp;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_HIP, "init sync"); }
+  *out = h;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_destroy(qmcw_handle *h) {
+  if (!h) return QMCW_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->trace) (void)hipFree(h->trace);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return QMCW_OK;
+}
+
+extern "C" const char *qmcw_last_error(const qmcw_handle *h) {
+  return h ? h->err.c_str() : g_lib_err.c_str();
+}
+
+extern "C" int32_t qmcw_nparm(const qmcw_handle *h) { return h ? h->nparm : 0; }
+
+extern "C" int qmcw_sync(qmcw_handle *h) {
+  if (!h) return QMCW_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b, const double *c,
+                               const double *csf_coef, const double *coef) {
+  if (!h) return QMCW_ERR_ARG;
+  DevSys &S = h->A.S;
+  if (a4) h->a4.assign(a4, a4 + h->a4.size());
+  if (b) h->b.assign(b, b + h->b.size());
+  if (c) h->c.assign(c, c + h->c.size());
+  if (a4 || b) {
+    // asymptotic constants depend on a4 and b (set_scale_dist)
+    double ar = S.asymp_r;
+    for (int it = 0; it < S.nctype; it++) {
+      const double *a = h->a4.data() + it * (S.norda + 1);
+      double v = a[0] * ar / (1.0 + a[1] * ar);
+      for (int iord = 2; iord <= S.norda; iord++) v += a[iord] * std::pow(ar, iord);
+      h->asymp_jasa[it] = v;
+    }
+    for (int i = 0; i < 2; i++) {
+      double sspinn = 1.0;
+      int isp = 0;
+      if (i == 1) { sspinn = (S.nspin2b == 1 && S.nocuspb == 0) ? 0.5 : 1.0; isp = S.nspin2b - 1; }
+      const double *bb = h->b.data() + isp * (S.nordb + 1);
+      double v = sspinn * bb[0] * ar / (1.0 + bb[1] * ar);
+      for (int iord = 2; iord <= S.nordb; iord++) v += bb[iord] * std::pow(ar, iord);
+      if (i == 0) S.asymp_jasb0 = v; else S.asymp_jasb1 = v;
+    }
+    HIPCHK(h, hipMemcpy(h->d_asymp, h->asymp_jasa.data(), h->asymp_jasa.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_a4, h->a4.data(), h->a4.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_b, h->b.data(), h->b.size() * 8, hipMemcpyHostToDevice));
+  }
+  if (c) HIPCHK(h, hipMemcpy(h->d_c, h->c.data(), h->c.size() * 8, hipMemcpyHostToDevice));
+  if (csf_coef) {
+    h->csf_coef.assign(csf_coef, csf_coef + h->ncsf);
+    h->cdet = build_cdet(h->ndet, h->ncsf, h->csf_coef.data(), h->csf_det_off.data(),
+                         h->csf_det.data(), h->csf_det_coef.data());
+    HIPCHK(h, hipMemcpy(h->d_cdet, h->cdet.data(), h->cdet.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_csf_coef, h->csf_coef.data(), h->csf_coef.size() * 8, hipMemcpyHostToDevice));
+  }
+  if (coef) {
+    int NB = S.NB, NO = S.NO;
+    h->coef.assign(coef, coef + (size_t)NO * NB);
+    std::vector<double> coefT((size_t)NB * NO);
+    for (int o = 0; o < NO; o++)
+      for (int bb = 0; bb < NB; bb++) coefT[(size_t)bb * NO + o] = h->coef[(size_t)o * NB + bb];
+    HIPCHK(h, hipMemcpy(h->d_coefT, coefT.data(), coefT.size() * 8, hipMemcpyHostToDevice));
+  }
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_set_walkers(qmcw_handle *h, const double *x) {
+  if (!h || !x) return QMCW_ERR_ARG;
+  int N = h->A.S.N;
+  HIPCHK(h, hipMemcpyAsync(h->io, x, (size_t)h->W * 3 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_copy_strided, dim3((h->W + 63) / 64), dim3(64), 0, h->stream,
+                     h->A.st + h->A.L.x, (int64_t)h->A.L.stride, (const double *)h->io,
+                     (int64_t)(3 * N), 3 * N, h->W);
+  hipLaunchKernelGGL(k_init_walkers, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, h->A, h->W);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_get_walkers(const qmcw_handle *hc, double *x) {
+  qmcw_handle *h = const_cast<qmcw_handle *>(hc);
+  if (!h || !x) return QMCW_ERR_ARG;
+  int N = h->A.S.N;
+  hipLaunchKernelGGL(k_copy_strided, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->io,
+                     (int64_t)(3 * N), (const double *)(h->A.st + h->A.L.x), (int64_t)h->A.L.stride,
+                     3 * N, h->W);
+  HIPCHK(h, hipMemcpyAsync(x, h->io, (size_t)h->W * 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_set_rng(qmcw_handle *h, const int32_t *seed) {
+  if (!h || !seed) return QMCW_ERR_ARG;
+  std::vector<int> s4((size_t)h->Wp * 4);
+  for (int w = 0; w < h->Wp; w++)
+    for (int q = 0; q < 4; q++) s4[4 * w + q] = seed[4 * std::min(w, h->W - 1) + q];
+  HIPCHK(h, hipMemcpy(h->A.rng, s4.data(), s4.size() * sizeof(int), hipMemcpyHostToDevice));
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_get_rng(const qmcw_handle *hc, int32_t *seed) {
+  qmcw_handle *h = const_cast<qmcw_handle *>(hc);
+  if (!h || !seed) return QMCW_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(seed, h->A.rng, (size_t)h->W * 4 * sizeof(int), hipMemcpyDeviceToHost));
+  return QMCW_OK;
+}
+
+static int check_flags(qmcw_handle *h) {
+  std::vector<int> fl(h->W);
+  HIPCHK(h, hipMemcpyAsync(fl.data(), h->A.flags, h->W * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int any = 0;
+  for (int f : fl) any |= f;
+  if (any & 1) return fail(h, QMCW_ERR_SINGULAR, "a Slater matrix is singular");
+  if (any & 2) return fail(h, QMCW_ERR_NAN, "non-finite local energy");
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_eval_full(qmcw_handle *h, double *logpsid, double *sgn, double *psij,
+                              double *grad, double *lap, double *eloc, double *epot) {
+  if (!h) return QMCW_ERR_ARG;
+  DISPATCH_G(h, launch_eval);
+  HIPCHK(h, hipGetLastError());
+  int N = h->A.S.N, stride = 8 + 4 * N;
+  hipLaunchKernelGGL(k_gather_full, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->A, h->io, h->W);
+  std::vector<double> o((size_t)h->W * stride);
+  HIPCHK(h, hipMemcpyAsync(o.data(), h->io, o.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int w = 0; w < h->W; w++) {
+    const double *r = o.data() + (size_t)w * stride;
+    if (logpsid) logpsid[w] = r[0];
+    if (sgn) sgn[w] = r[1];
+    if (psij) psij[w] = r[2];
+    if (eloc) eloc[w] = r[3];
+    if (epot) epot[w] = r[4];
+    if (grad) memcpy(grad + (size_t)w * 3 * N, r + 8, 3 * N * sizeof(double));
+    if (lap) memcpy(lap + (size_t)w * N, r + 8 + 3 * N, N * sizeof(double));
+  }
+  return check_flags(h);
+}
+
+extern "C" int qmcw_ratio_1e(qmcw_handle *h, int32_t ie, const double *xnew, double *ratio,
+                             double *grad_new, double *lap_new) {
+  if (!h || !xnew || ie < 0 || ie >= h->A.S.N) return QMCW_ERR_ARG;
+  // xnew staged after the 8*Wp output block
+  double *xin = h->io + (size_t)h->Wp * 8;
+  std::vector<double> xp((size_t)h->Wp * 3);
+  for (int w = 0; w < h->Wp; w++)
+    for (int q = 0; q < 3; q++) xp[3 * w + q] = xnew[3 * std::min(w, h->W - 1) + q];
+  HIPCHK(h, hipMemcpyAsync(xin, xp.data(), xp.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  DISPATCH_G(h, launch_ratio, ie, 0, (const double *)xin, 1.0, 1.0, h->io);
+  HIPCHK(h, hipGetLastError());
+  std::vector<double> o((size_t)h->W * 8);
+  HIPCHK(h, hipMemcpyAsync(o.data(), h->io, o.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int w = 0; w < h->W; w++) {
+    if (ratio) ratio[w] = o[8 * w];
+    if (grad_new) for (int q = 0; q < 3; q++) grad_new[3 * w + q] = o[8 * w + 4 + q];
+    if (lap_new) lap_new[w] = o[8 * w + 7];
+  }
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_accept_1e(qmcw_handle *h, int32_t ie, const uint8_t *accept) {
+  if (!h || !accept || ie < 0 || ie >= h->A.S.N) return QMCW_ERR_ARG;
+  std::vector<uint8_t> a(h->Wp, 0);
+  for (int w = 0; w < h->W; w++) a[w] = accept[w];
+  HIPCHK(h, hipMemcpyAsync(h->acc_buf, a.data(), h->Wp, hipMemcpyHostToDevice, h->stream));
+  DISPATCH_G(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_grad_1e(qmcw_handle *h, int32_t ie, double *grad, double *lap) {
+  if (!h || ie < 0 || ie >= h->A.S.N) return QMCW_ERR_ARG;
+  DISPATCH_G(h, launch_grad, ie, h->io);
+  HIPCHK(h, hipGetLastError());
+  std::vector<double> o((size_t)h->W * 4);
+  HIPCHK(h, hipMemcpyAsync(o.data(), h->io, o.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int w = 0; w < h->W; w++) {
+    if (grad) for (int q = 0; q < 3; q++) grad[3 * w + q] = o[4 * w + q];
+    if (lap) lap[w] = o[4 * w + 3];
+  }
+  return QMCW_OK;
+}
+
+static int vmc_steps(qmcw_handle *h, int nstep, double delta, double fbias, bool trace) {
+  int N = h->A.S.N;
+  DISPATCH_G(h, launch_eval);
+  for (int is = 0; is < nstep; is++) {
+    if (h->A.S.nloc > 0)
+      hipLaunchKernelGGL(k_rotqua, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, h->A);
+    for (int ie = 0; ie < N; ie++) {
+      DISPATCH_G(h, launch_ratio, ie, 1, (const double *)nullptr, delta, fbias, (double *)nullptr);
+      DISPATCH_G(h, launch_accept, ie, 1, (const uint8_t *)nullptr, delta, fbias, h->nacc);
+    }
+    DISPATCH_G(h, launch_eval);
+    if (trace)
+      hipLaunchKernelGGL(k_trace_eloc, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->A,
+                         h->trace, is, h->W);
+  }
+  HIPCHK(h, hipGetLastError());
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double fbias,
+                            double *eloc_trace, int64_t *naccept) {
+  if (!h || nstep < 0 || delta <= 0) return QMCW_ERR_ARG;
+  if (eloc_trace && nstep > h->trace_steps) {
+    if (h->trace) (void)hipFree(h->trace);
+    HIPCHK(h, hipMalloc(&h->trace, (size_t)nstep * h->W * sizeof(double)));
+    h->trace_steps = nstep;
+  }
+  HIPCHK(h, hipMemsetAsync(h->nacc, 0, (size_t)h->Wp * sizeof(long long), h->stream));
+  int st = vmc_steps(h, nstep, delta, fbias, eloc_trace != nullptr);
+  if (st) return st;
+  if (eloc_trace)
+    HIPCHK(h, hipMemcpyAsync(eloc_trace, h->trace, (size_t)nstep * h->W * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+  if (naccept)
+    HIPCHK(h, hipMemcpyAsync(naccept, h->nacc, (size_t)h->W * sizeof(long long), hipMemcpyDeviceToHost,
+                             h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_flags(h);
+}
+
+extern "C" int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta, double fbias) {
+  if (!h || nstep < 0 || delta <= 0) return QMCW_ERR_ARG;
+  return vmc_steps(h, nstep, delta, fbias, false);
+}
+
+extern "C" int qmcw_get_eloc(const qmcw_handle *hc, double *eloc) {
+  qmcw_handle *h = const_cast<qmcw_handle *>(hc);
+  if (!h || !eloc) return QMCW_ERR_ARG;
+  hipLaunchKernelGGL(k_copy_strided, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->io, (int64_t)1,
+                     (const double *)(h->A.st + h->A.L.eloc), (int64_t)h->A.L.stride, 1, h->W);
+  HIPCHK(h, hipMemcpyAsync(eloc, h->io, (size_t)h->W * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
+  (void)dpsi;
+  (void)deloc;
+  if (!h) return QMCW_ERR_ARG;
+  return fail(h, QMCW_ERR_ARG, "parameter derivatives are not built into this library yet");
+}

@@ -1,0 +1,459 @@
+// engine_device.h -- device functions of the wavefunction engine.
+//
+// A walker is handled by a group of G lanes of one wavefront (G in
+// {8,16,32,64}); a 64-lane workgroup holds 64/G walkers.  Group-parallel
+// loops stride over lanes, reductions use wavefront shuffles of width G.
+// Orbital and Jastrow arithmetic is templated on the real type T (double for
+// the FP64 engine, float for the mixed engine); determinants, inverses and
+// energies are always double.
+#pragma once
+#include "engine.h"
+
+namespace qmcw {
+
+template <int G>
+__device__ __forceinline__ double gsum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+template <int G>
+__device__ __forceinline__ float gsumf(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+template <int G>
+__device__ __forceinline__ int gsumi(int v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+// argmax of |v| over the group (first index wins among equal values, as a
+// sequential scan from the lowest index would)
+template <int G>
+__device__ __forceinline__ void gargmax(double v, int idx, double &bv, int &bi) {
+  double a = fabs(v);
+  bi = idx;
+  bv = v;
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    double a2 = __shfl_xor(a, o, G);
+    double v2 = __shfl_xor(bv, o, G);
+    int i2 = __shfl_xor(bi, o, G);
+    bool take = (a2 > a) || (a2 == a && i2 < bi);
+    if (take) { a = a2; bv = v2; bi = i2; }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scaled distances (isc = 2: (1-exp(-k r))/k ; isc = 4: r/(1+k r)) with first
+// and second derivative; the e-e-n variant is 1 - k * rr.
+template <typename T>
+__device__ __forceinline__ void scaled(int isc, T k, T r, T &rr, T &d1, T &d2) {
+  if (isc == 2) {
+    T e = exp(-k * r);
+    rr = (T(1) - e) / k;
+    d1 = e;
+    d2 = -k * e;
+  } else {
+    T deni = T(1) / (T(1) + k * r);
+    rr = r * deni;
+    d1 = deni * deni;
+    d2 = T(-2) * k * deni * d1;
+  }
+}
+
+// Pair term of electrons (i, j), i > j: e-e Pade + polynomial and e-e-n
+// polynomial.  Returns value; gi/gj gradient wrt r_i and r_j; li/lj the
+// laplacian wrt r_i and r_j.
+template <typename T>
+struct PairOut {
+  T f, gi[3], gj[3], li, lj;
+};
+
+template <typename T>
+__device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
+                         const T *rvij, const double *ri_v, const double *rj_v,
+                         const double *rvi, const double *rvj, PairOut<T> &o) {
+  const T k = T(S.scalek);
+  T sspinn = T(1);
+  int isb = 0;
+  double asymp_b = S.asymp_jasb0;
+  if (parallel_spin) {
+    if (S.nspin2b == 2) isb = 1;
+    else if (S.nocuspb == 0) sspinn = T(0.5);
+    asymp_b = S.asymp_jasb1;
+  }
+  const double *b = S.b + isb * (S.nordb + 1);
+  T u, d1, d2;
+  scaled<T>(S.isc, k, rij, u, d1, d2);
+  T bot = T(1) + T(b[1]) * u;
+  T f = sspinn * T(b[0]) * u / bot;
+  T fu = sspinn * T(b[0]) / (bot * bot);
+  T fuu = T(-2) * fu * T(b[1]) / bot;
+  f -= T(asymp_b);
+  T up = u;  // u^(iord-1)
+  T upm = T(1);
+  for (int iord = 2; iord <= S.nordb; iord++) {
+    T bb = T(b[iord]);
+    fuu += bb * T(iord * (iord - 1)) * upm;
+    fu += bb * T(iord) * up;
+    upm = up;
+    up *= u;
+    f += bb * up;
+  }
+  fuu = fuu * d1 * d1 + fu * d2;
+  fu = fu * d1 / rij;
+  o.f = f;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    o.gi[q] = fu * rvij[q];
+    o.gj[q] = -fu * rvij[q];
+  }
+  o.li = fuu + T(2) * fu;
+  o.lj = o.li;
+  if (S.nordc <= 1) return;
+  // e-e-n: variables 1 - k rr of r_ij, r_i, r_j
+  T U, Ud1, Ud2;
+  scaled<T>(S.isc, k, rij, U, Ud1, Ud2);
+  U = T(1) - k * U;
+  Ud1 = -k * Ud1;
+  Ud2 = -k * Ud2;
+  T upow[kMaxTerms / 4 + 4];
+  upow[0] = T(1);
+  for (int q = 1; q <= S.nordc; q++) upow[q] = upow[q - 1] * U;
+  for (int ic = 0; ic < S.M; ic++) {
+    int it = S.iwctype[ic];
+    T ri = (T)ri_v[ic], rj = (T)rj_v[ic];
+    T A, Ad1, Ad2, B, Bd1, Bd2;
+    scaled<T>(S.isc, k, ri, A, Ad1, Ad2);
+    scaled<T>(S.isc, k, rj, B, Bd1, Bd2);
+    A = T(1) - k * A; Ad1 = -k * Ad1; Ad2 = -k * Ad2;
+    B = T(1) - k * B; Bd1 = -k * Bd1; Bd2 = -k * Bd2;
+    T ap[kMaxTerms / 4 + 4], bp[kMaxTerms / 4 + 4];
+    ap[0] = bp[0] = T(1);
+    for (int q = 1; q <= S.nordc; q++) { ap[q] = ap[q - 1] * A; bp[q] = bp[q - 1] * B; }
+    T fc = 0, fU = 0, fUU = 0, fA = 0, fAA = 0, fB = 0, fBB = 0, fUA = 0, fUB = 0;
+    const double *cc = S.c + it * S.nterms;
+    for (int t = 0; t < S.nterms; t++) {
+      int kk = S.term_klm[3 * t], l = S.term_klm[3 * t + 1], m = S.term_klm[3 * t + 2];
+      T cl = T(cc[t]);
+      if (cl == T(0)) continue;
+      // P = A^(l+m) B^m + A^m B^(l+m)
+      int lm = l + m;
+      T P = ap[lm] * bp[m] + ap[m] * bp[lm];
+      T PA = (lm ? T(lm) * ap[lm - 1] * bp[m] : T(0)) + (m ? T(m) * ap[m - 1] * bp[lm] : T(0));
+      T PB = (lm ? T(lm) * bp[lm - 1] * ap[m] : T(0)) + (m ? T(m) * bp[m - 1] * ap[lm] : T(0));
+      T PAA = (lm > 1 ? T(lm * (lm - 1)) * ap[lm - 2] * bp[m] : T(0)) +
+              (m > 1 ? T(m * (m - 1)) * ap[m - 2] * bp[lm] : T(0));
+      T PBB = (lm > 1 ? T(lm * (lm - 1)) * bp[lm - 2] * ap[m] : T(0)) +
+              (m > 1 ? T(m * (m - 1)) * bp[m - 2] * ap[lm] : T(0));
+      T Uk = upow[kk];
+      T Uk1 = kk ? T(kk) * upow[kk - 1] : T(0);
+      T Uk2 = kk > 1 ? T(kk * (kk - 1)) * upow[kk - 2] : T(0);
+      fc += cl * Uk * P;
+      fU += cl * Uk1 * P;
+      fUU += cl * Uk2 * P;
+      fA += cl * Uk * PA;
+      fAA += cl * Uk * PAA;
+      fB += cl * Uk * PB;
+      fBB += cl * Uk * PBB;
+      fUA += cl * Uk1 * PA;
+      fUB += cl * Uk1 * PB;
+    }
+    // chain rule to unscaled distances
+    T gU = fU * Ud1 / rij;
+    T gUU = fUU * Ud1 * Ud1 + fU * Ud2;
+    T gA = fA * Ad1 / ri, gAA = fAA * Ad1 * Ad1 + fA * Ad2;
+    T gB = fB * Bd1 / rj, gBB = fBB * Bd1 * Bd1 + fB * Bd2;
+    T gUA = fUA * Ud1 * Ad1, gUB = fUB * Ud1 * Bd1;
+    o.f += fc;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      o.gi[q] += gA * (T)rvi[3 * ic + q] + gU * rvij[q];
+      o.gj[q] += gB * (T)rvj[3 * ic + q] - gU * rvij[q];
+    }
+    // cos of the angle between r_ij and r_i: (rij^2 + ri^2 - rj^2)/(2 rij ri)
+    T cI = (rij * rij + ri * ri - rj * rj) / (rij * ri);
+    T cJ = (rij * rij + rj * rj - ri * ri) / (rij * rj);
+    o.li += T(2) * (gU + gA) + gUU + gAA + gUA * cI;
+    o.lj += T(2) * (gU + gB) + gUU + gBB + gUB * cJ;
+  }
+}
+
+// e-n Pade + polynomial of one electron summed over centres
+template <typename T>
+__device__ void jas_en(const DevSys &S, const double *ri_v, const double *rvi, T &f,
+                       T *g, T &lap) {
+  const T k = T(S.scalek);
+  f = 0;
+  g[0] = g[1] = g[2] = 0;
+  lap = 0;
+  for (int ic = 0; ic < S.M; ic++) {
+    int it = S.iwctype[ic];
+    const double *a = S.a4 + it * (S.norda + 1);
+    T r = (T)ri_v[ic], u, d1, d2;
+    scaled<T>(S.isc, k, r, u, d1, d2);
+    T bot = T(1) + T(a[1]) * u;
+    T fe = T(a[0]) * u / bot - T(S.asymp_jasa[it]);
+    T fu = T(a[0]) / (bot * bot);
+    T fuu = T(-2) * fu * T(a[1]) / bot;
+    T up = u, upm = T(1);
+    for (int iord = 2; iord <= S.norda; iord++) {
+      T aa = T(a[iord]);
+      fuu += aa * T(iord * (iord - 1)) * upm;
+      fu += aa * T(iord) * up;
+      upm = up;
+      up *= u;
+      fe += aa * up;
+    }
+    fuu = fuu * d1 * d1 + fu * d2;
+    fu = fu * d1 / r;
+    f += fe;
+#pragma unroll
+    for (int q = 0; q < 3; q++) g[q] += fu * (T)rvi[3 * ic + q];
+    lap += fuu + T(2) * fu;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// basis shell: r^a sum_p d_p exp(-z_p r^q) times real solid harmonics S_lm.
+// Writes phi[b*5 + {val, dx, dy, dz, lap}] (lap only when want_d).
+template <typename T>
+__device__ __forceinline__ void solid_harm(int l, T x, T y, T z, T *Sv, T (*Sg)[3]) {
+  // real solid harmonics with <Y^2> = 4 pi/(2l+1) normalisation, m order
+  // 0, 1, -1, 2, -2, ... ; stored at index (m>0 ? 2m-1 : (m<0 ? -2m : 0))
+  const T x2 = x * x, y2 = y * y, z2 = z * z;
+  switch (l) {
+    case 0:
+      Sv[0] = T(1); Sg[0][0] = Sg[0][1] = Sg[0][2] = T(0);
+      break;
+    case 1:
+      Sv[0] = z; Sg[0][0] = 0; Sg[0][1] = 0; Sg[0][2] = 1;
+      Sv[1] = x; Sg[1][0] = 1; Sg[1][1] = 0; Sg[1][2] = 0;
+      Sv[2] = y; Sg[2][0] = 0; Sg[2][1] = 1; Sg[2][2] = 0;
+      break;
+    case 2: {
+      const T c1 = T(1.7320508075688772), c2 = T(0.8660254037844386);
+      Sv[0] = T(0.5) * (T(2) * z2 - x2 - y2);
+      Sg[0][0] = -x; Sg[0][1] = -y; Sg[0][2] = T(2) * z;
+      Sv[1] = c1 * x * z; Sg[1][0] = c1 * z; Sg[1][1] = 0; Sg[1][2] = c1 * x;
+      Sv[2] = c1 * y * z; Sg[2][0] = 0; Sg[2][1] = c1 * z; Sg[2][2] = c1 * y;
+      Sv[3] = c2 * (x2 - y2); Sg[3][0] = T(2) * c2 * x; Sg[3][1] = T(-2) * c2 * y; Sg[3][2] = 0;
+      Sv[4] = T(2) * c2 * x * y; Sg[4][0] = T(2) * c2 * y; Sg[4][1] = T(2) * c2 * x; Sg[4][2] = 0;
+      break;
+    }
+    case 3: {
+      const T c0 = T(0.5), c1 = T(0.6123724356957945), c2 = T(1.9364916731037085),
+              c3 = T(0.7905694150420949);
+      T rho2 = x2 + y2;
+      Sv[0] = c0 * z * (T(2) * z2 - T(3) * rho2);
+      Sg[0][0] = T(-6) * c0 * z * x; Sg[0][1] = T(-6) * c0 * z * y;
+      Sg[0][2] = c0 * (T(6) * z2 - T(3) * rho2);
+      T w = T(4) * z2 - rho2;
+      Sv[1] = c1 * w * x;
+      Sg[1][0] = c1 * (w - T(2) * x2); Sg[1][1] = T(-2) * c1 * x * y; Sg[1][2] = T(8) * c1 * z * x;
+      Sv[2] = c1 * w * y;
+      Sg[2][0] = T(-2) * c1 * x * y; Sg[2][1] = c1 * (w - T(2) * y2); Sg[2][2] = T(8) * c1 * z * y;
+      Sv[3] = c2 * z * (x2 - y2);
+      Sg[3][0] = T(2) * c2 * z * x; Sg[3][1] = T(-2) * c2 * z * y; Sg[3][2] = c2 * (x2 - y2);
+      Sv[4] = T(2) * c2 * z * x * y;
+      Sg[4][0] = T(2) * c2 * z * y; Sg[4][1] = T(2) * c2 * z * x; Sg[4][2] = T(2) * c2 * x * y;
+      Sv[5] = c3 * (x2 * x - T(3) * x * y2);
+      Sg[5][0] = c3 * (T(3) * x2 - T(3) * y2); Sg[5][1] = T(-6) * c3 * x * y; Sg[5][2] = 0;
+      Sv[6] = c3 * (T(3) * x2 * y - y2 * y);
+      Sg[6][0] = T(6) * c3 * x * y; Sg[6][1] = c3 * (T(3) * x2 - T(3) * y2); Sg[6][2] = 0;
+      break;
+    }
+    default: {  // l = 4
+      const T c0 = T(0.125), c1 = T(0.7905694150420949), c2 = T(0.5590169943749475),
+              c3 = T(2.0916500663351889), c4 = T(0.7395099728874520);
+      T r2 = x2 + y2 + z2;
+      Sv[0] = c0 * (T(35) * z2 * z2 - T(30) * r2 * z2 + T(3) * r2 * r2);
+      // d/dx: -60 x z2 + 12 r2 x ; d/dz: 140 z^3 - 60 z r2 - 60 z^3 + 12 r2 z
+      Sg[0][0] = c0 * (T(-60) * x * z2 + T(12) * r2 * x);
+      Sg[0][1] = c0 * (T(-60) * y * z2 + T(12) * r2 * y);
+      Sg[0][2] = c0 * (T(80) * z2 * z - T(48) * r2 * z);
+      T w1 = T(7) * z2 - T(3) * r2;   // d/dx = -6x, d/dz = 8z
+      Sv[1] = c1 * z * w1 * x;
+      Sg[1][0] = c1 * z * (w1 - T(6) * x2); Sg[1][1] = c1 * z * (T(-6) * x * y);
+      Sg[1][2] = c1 * x * (w1 + T(8) * z2);
+      Sv[2] = c1 * z * w1 * y;
+      Sg[2][0] = c1 * z * (T(-6) * x * y); Sg[2][1] = c1 * z * (w1 - T(6) * y2);
+      Sg[2][2] = c1 * y * (w1 + T(8) * z2);
+      T w2 = T(7) * z2 - r2;          // d/dx = -2x, d/dz = 12z
+      T p2 = x2 - y2, q2 = T(2) * x * y;
+      Sv[3] = c2 * w2 * p2;
+      Sg[3][0] = c2 * (T(-2) * x * p2 + w2 * T(2) * x);
+      Sg[3][1] = c2 * (T(-2) * y * p2 - w2 * T(2) * y);
+      Sg[3][2] = c2 * T(12) * z * p2;
+      Sv[4] = c2 * w2 * q2;
+      Sg[4][0] = c2 * (T(-2) * x * q2 + w2 * T(2) * y);
+      Sg[4][1] = c2 * (T(-2) * y * q2 + w2 * T(2) * x);
+      Sg[4][2] = c2 * T(12) * z * q2;
+      T p3 = x2 * x - T(3) * x * y2, q3 = T(3) * x2 * y - y2 * y;
+      Sv[5] = c3 * z * p3;
+      Sg[5][0] = c3 * z * (T(3) * x2 - T(3) * y2); Sg[5][1] = c3 * z * (T(-6) * x * y); Sg[5][2] = c3 * p3;
+      Sv[6] = c3 * z * q3;
+      Sg[6][0] = c3 * z * (T(6) * x * y); Sg[6][1] = c3 * z * (T(3) * x2 - T(3) * y2); Sg[6][2] = c3 * q3;
+      T p4 = x2 * x2 - T(6) * x2 * y2 + y2 * y2, q4 = T(4) * x * y * (x2 - y2);
+      Sv[7] = c4 * p4;
+      Sg[7][0] = c4 * (T(4) * x2 * x - T(12) * x * y2); Sg[7][1] = c4 * (T(-12) * x2 * y + T(4) * y2 * y);
+      Sg[7][2] = 0;
+      Sv[8] = c4 * q4;
+      Sg[8][0] = c4 * (T(12) * x2 * y - T(4) * y2 * y); Sg[8][1] = c4 * (T(4) * x2 * x - T(12) * x * y2);
+      Sg[8][2] = 0;
+      break;
+    }
+  }
+}
+
+__device__ __forceinline__ int m_slot(int m) { return m > 0 ? 2 * m - 1 : (m < 0 ? -2 * m : 0); }
+
+// Evaluate shell s at displacement (x, y, z) from its centre, distance r.
+// want: 0 value only, 1 value + gradient + laplacian.
+template <typename T>
+__device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
+                           T *phi) {
+  int l = S.sh_l[s], a = S.sh_a[s];
+  if (r == T(0)) { r = T(1e-30); x = T(1e-30); }
+  // radial f = r^a g(r), g = sum d exp(-z r^q); need f, f'/r, f''
+  T g = 0, g1 = 0, g2 = 0;  // g, dg/dr, d2g/dr2
+  const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
+  for (int p = S.sh_prim_off[s]; p < S.sh_prim_off[s + 1]; p++) {
+    T zeta = T(S.prim_zeta[p]), d = T(S.prim_coef[p]);
+    if (gauss) {
+      T e = d * exp(-zeta * r * r);
+      g += e;
+      g1 += T(-2) * zeta * r * e;
+      g2 += (T(4) * zeta * zeta * r * r - T(2) * zeta) * e;
+    } else {
+      T e = d * exp(-zeta * r);
+      g += e;
+      g1 += -zeta * e;
+      g2 += zeta * zeta * e;
+    }
+  }
+  T f, fr, f2;  // f, f'/r, f''
+  if (a == 0) {
+    f = g;
+    fr = g1 / r;
+    f2 = g2;
+  } else {
+    T ra1 = T(1);
+    for (int q = 0; q < a - 1; q++) ra1 *= r;   // r^(a-1)
+    T ra = ra1 * r;
+    T ra2 = (a >= 2) ? ra1 / r : T(0);          // r^(a-2) (only used when a >= 2)
+    f = ra * g;
+    fr = (T(a) * ra1 * g + ra * g1) / r;
+    f2 = (a >= 2 ? T(a * (a - 1)) * ra2 * g : T(0)) + T(2 * a) * ra1 * g1 + ra * g2;
+  }
+  T Sv[kMaxShellM], Sg[kMaxShellM][3];
+  solid_harm<T>(l, x, y, z, Sv, Sg);
+  // laplacian of S f(r): S (f'' + 2 (l+1) f'/r)
+  T lapfac = f2 + T(2 * (l + 1)) * fr;
+  int nm = S.sh_nm[s];
+  for (int q = 0; q < nm; q++) {
+    int sl = m_slot(S.sh_m[s * kMaxShellM + q]);
+    int bidx = S.sh_bas[s * kMaxShellM + q];
+    T *o = phi + 5 * bidx;
+    o[0] = Sv[sl] * f;
+    if (want) {
+      o[1] = f * Sg[sl][0] + Sv[sl] * fr * x;
+      o[2] = f * Sg[sl][1] + Sv[sl] * fr * y;
+      o[3] = f * Sg[sl][2] + Sv[sl] * fr * z;
+      o[4] = Sv[sl] * lapfac;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CHAMP's pivoted Gauss-Jordan (qmc/vmc/matinv.f90) on a column-major k x k
+// block held by one lane.  Singular -> det = 0, inverse zeroed.
+__device__ inline double small_matinv(double *a, int n) {
+  if (n == 0) return 1.0;
+  int ipiv[kMaxKmax];
+  double atemp[kMaxKmax];
+  for (int i = 0; i < n; i++) ipiv[i] = i;
+  double det = 1.0;
+  for (int i = 0; i < n; i++) {
+    int col = i * n;
+    double adiag = a[ipiv[i] + col];
+    int idiag = i;
+    for (int k = i; k < n; k++)
+      if (fabs(a[ipiv[k] + col]) > fabs(adiag)) { adiag = a[ipiv[k] + col]; idiag = k; }
+    if (idiag != i) {
+      det = -det;
+      int t = ipiv[i]; ipiv[i] = ipiv[idiag]; ipiv[idiag] = t;
+    }
+    det *= adiag;
+    int p = ipiv[i];
+    a[p + col] = 1.0;
+    if (adiag == 0.0) {
+      for (int q = 0; q < n * n; q++) a[q] = 0.0;
+      return 0.0;
+    }
+    double inv = 1.0 / adiag;
+    for (int q = 0; q < n; q++) a[p + q * n] *= inv;
+    for (int j = 0; j < n; j++)
+      if (j != p) {
+        double t = -a[j + col];
+        a[j + col] = 0.0;
+        for (int q = 0; q < n; q++) a[j + q * n] += t * a[p + q * n];
+      }
+  }
+  for (int j = 0; j < n; j++) {
+    for (int i = 0; i < n; i++) atemp[i] = a[i + j * n];
+    for (int i = 0; i < n; i++) a[i + j * n] = atemp[ipiv[i]];
+  }
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) atemp[j] = a[i + j * n];
+    for (int j = 0; j < n; j++) a[i + ipiv[j] * n] = atemp[j];
+  }
+  return det;
+}
+
+// (2l+1) P_l(cos) with l given as l+1 (qmc/vmc/nonloc.f90:249-272)
+__device__ __forceinline__ double legendre_w(int l1, double c) {
+  switch (l1) {
+    case 1: return 1.0;
+    case 2: return 3.0 * c;
+    case 3: return 2.5 * (3 * c * c - 1);
+    case 4: return 3.5 * c * (5 * c * c - 3);
+    default: return 1.125 * (35 * c * c * c * c - 30 * c * c + 3);
+  }
+}
+
+__device__ __forceinline__ double ecp_radial(const DevSys &S, double r, int l0, int ict) {
+  double v = 0, rsq = r * r;
+  int b = S.ecp_off[ict * QMCW_MAX_LPOT + l0], e = S.ecp_off[ict * QMCW_MAX_LPOT + l0 + 1];
+  for (int q = b; q < e; q++) {
+    int pw = S.ecp_power[q];
+    double p = 1.0;
+    if (pw != 2) p = pow(r, (double)(pw - 2));
+    v += p * S.ecp_coef[q] * exp(-S.ecp_expo[q] * rsq);
+  }
+  return v;
+}
+
+// CHAMP's rannyu on a 4 x 12-bit limb state (qmc/vmc/rannyu.f90:53-74)
+__device__ __forceinline__ double rannyu(int *l) {
+  const int m1 = 502, m2 = 1521, m3 = 4071, m4 = 2107;
+  const double two12i = 2.44140625e-4;
+  int l1 = l[0], l2 = l[1], l3 = l[2], l4 = l[3];
+  int i1 = l1 * m4 + l2 * m3 + l3 * m2 + l4 * m1;
+  int i2 = l2 * m4 + l3 * m3 + l4 * m2;
+  int i3 = l3 * m4 + l4 * m3;
+  int i4 = l4 * m4;
+  l4 = i4 & 4095;
+  i3 = i3 + (i4 >> 12);
+  l3 = i3 & 4095;
+  i2 = i2 + (i3 >> 12);
+  l2 = i2 & 4095;
+  l1 = (i1 + (i2 >> 12)) & 4095;
+  l[0] = l1; l[1] = l2; l[2] = l3; l[3] = l4;
+  return two12i * ((double)l1 + two12i * ((double)l2 + two12i * ((double)l3 + two12i * (double)l4)));
+}
+
+}  // namespace qmcw

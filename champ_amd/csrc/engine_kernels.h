@@ -1,0 +1,961 @@
+// engine_kernels.h -- kernels of the walker-batched wavefunction engine.
+// Every kernel runs 64-lane workgroups; group g of G lanes owns walker
+// blockIdx.x*(64/G)+g.  The walker count is padded to a multiple of 64/G
+// on the host, so every group has a valid walker and all groups reach the
+// same barriers.
+#pragma once
+#include "engine_device.h"
+
+namespace qmcw {
+
+struct KArgs {
+  DevSys S;
+  WLayout L;
+  SLayout SL;
+  MLayout ML;
+  double *st;        // walker state  [Wp][L.stride]
+  double *sc;        // walker scratch [Wp][SL.stride]
+  int *rng;          // [Wp][4]
+  int *flags;        // [Wp] status bits
+  int Wp;
+  int derivs;        // compute Jastrow/NLPP parameter derivative pieces
+  int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
+  int chunk;         // electrons per basis chunk in the full evaluation
+};
+
+#define QMCW_GROUP_SETUP                                         \
+  const int lane = threadIdx.x % G;                              \
+  const int grp = threadIdx.x / G;                               \
+  const int w = blockIdx.x * (kWave / G) + grp;                  \
+  const DevSys &S = A.S;                                         \
+  double *ws = A.st + (int64_t)w * A.L.stride;                   \
+  double *ss = A.sc + (int64_t)w * A.SL.stride;                  \
+  (void)ss;
+
+// ---------------------------------------------------------------------------
+// In-place inverse of the column-major n x n block `a` by a group (CHAMP's
+// matinv pivoting, rows in parallel).  tmp: n*n scratch.  Returns det (all
+// lanes).  ipiv: LDS array of the group, >= n entries.
+template <int G>
+__device__ double group_matinv(double *a, double *tmp, int n, int lane, int *ipiv) {
+  for (int i = lane; i < n; i += G) ipiv[i] = i;
+  __syncthreads();
+  double det = 1.0;
+  bool singular = false;
+  for (int i = 0; i < n; i++) {
+    const int col = i * n;
+    // matinv scans k = i..n-1 keeping the first strictly larger |a(ipiv(k), i)|
+    double am = -1.0, vm = 0.0;
+    int km = n;
+    for (int k = i + lane; k < n; k += G) {
+      double v = a[ipiv[k] + col];
+      if (fabs(v) > am) { am = fabs(v); vm = v; km = k; }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      double a2 = __shfl_xor(am, o, G);
+      int k2 = __shfl_xor(km, o, G);
+      double v2 = __shfl_xor(vm, o, G);
+      if (a2 > am || (a2 == am && k2 < km)) { am = a2; km = k2; vm = v2; }
+    }
+    __syncthreads();
+    if (!singular) {
+      if (lane == 0 && km != i) { int t = ipiv[i]; ipiv[i] = ipiv[km]; ipiv[km] = t; }
+      if (km != i) det = -det;
+      det *= vm;
+      if (vm == 0.0) singular = true;
+    }
+    __syncthreads();
+    const int p = ipiv[i];
+    const double inv = singular ? 0.0 : 1.0 / vm;
+    if (!singular && lane == 0) a[p + col] = 1.0;
+    __syncthreads();
+    if (!singular)
+      for (int q = lane; q < n; q += G) a[p + q * n] *= inv;
+    __syncthreads();
+    if (!singular)
+      for (int j = lane; j < n; j += G) {
+        if (j == p) continue;
+        double t = -a[j + col];
+        a[j + col] = 0.0;
+        for (int q = 0; q < n; q++) a[j + q * n] += t * a[p + q * n];
+      }
+    __syncthreads();
+  }
+  // row permutation: new a(r, c) = a(ipiv[r], c); then columns: new
+  // a(r, ipiv[c]) = a(r, c).  A singular matrix returns det 0 and zeros.
+  for (int q = lane; q < n * n; q += G) tmp[q] = a[q];
+  __syncthreads();
+  for (int q = lane; q < n * n; q += G) {
+    int r = q % n, c = q / n;
+    a[r + c * n] = singular ? 0.0 : tmp[ipiv[r] + c * n];
+  }
+  __syncthreads();
+  for (int q = lane; q < n * n; q += G) tmp[q] = a[q];
+  __syncthreads();
+  for (int q = lane; q < n * n; q += G) {
+    int r = q % n, c = q / n;
+    a[r + ipiv[c] * n] = tmp[r + c * n];
+  }
+  __syncthreads();
+  return singular ? 0.0 : det;
+}
+
+// chi and Y of one walker from the alpha inverses/dets of both spins
+// (qmc/deriv_fast_mod.f90:545-598).  inv/detv per spin, y out per spin.
+template <int G>
+__device__ double group_eval_y(const DevSys &S, const double *const inv[2],
+                               const double *const detv[2], double *const y[2],
+                               double *cup, int lane) {
+  // chi = sum_d cdet sgn detup detdn ; cup_s(u) = sum over dets with spin-s part u
+  double chi = 0.0;
+  for (int d = lane; d < S.ndet; d += G) {
+    int iu = S.iwdet[0][d], id = S.iwdet[1][d];
+    chi += S.cdet[d] * S.dsgn[0][iu] * S.dsgn[1][id] * detv[0][iu] * detv[1][id];
+  }
+  chi = gsum<G>(chi);
+  const double chii = 1.0 / chi;
+  for (int t = lane; t < S.ndetu[0] + S.ndetu[1]; t += G) {
+    int s = t < S.ndetu[0] ? 0 : 1, u = s ? t - S.ndetu[0] : t;
+    double cu = 0.0;
+    for (int d = 0; d < S.ndet; d++) {
+      if (S.iwdet[s][d] != u) continue;
+      int iu = S.iwdet[0][d], id = S.iwdet[1][d];
+      cu += S.cdet[d] * S.dsgn[0][iu] * S.dsgn[1][id] * detv[0][iu] * detv[1][id];
+    }
+    cup[t] = cu;
+  }
+  __syncthreads();
+  for (int s = 0; s < 2; s++) {
+    int ns = S.ns[s], nocc = S.nocc[s];
+    const double *cus = cup + (s ? S.ndetu[0] : 0);
+    for (int e = lane; e < nocc * ns; e += G) {
+      int jo = e / ns, ir = e % ns;
+      double acc = 0.0;
+      for (int u = 1; u < S.ndetu[s]; u++) {   // u = 0 is the reference, k = 0
+        int k = S.ord[s][u];
+        int jj = -1, ii = -1;
+        for (int q = 0; q < k; q++) {
+          if (S.exo[s][u * ns + q] == jo) jj = q;
+          if (S.rf[s][u * ns + q] == ir) ii = q;
+        }
+        if (jj < 0 || ii < 0) continue;
+        acc += cus[u] * inv[s][u * S.kmax2 + jj + k * ii];
+      }
+      y[s][e] = acc * chii;
+    }
+  }
+  __syncthreads();
+  return chi;
+}
+
+// G = A^-1 - T Y A^-1 on reference rows, Y A^-1 on the others
+// (qmc/deriv_fast_mod.f90:600-625).  ai col-major ns x ns, tocc ns x nocc,
+// y nocc x ns, g out nocc x ns; tmp >= nocc*ns.
+template <int G>
+__device__ void group_g(const DevSys &S, int s, const double *ai,
+                        const double *tocc, const double *y, double *g,
+                        double *tmp, int lane) {
+  int ns = S.ns[s], nocc = S.nocc[s];
+  if (ns == 0) return;
+  for (int e = lane; e < nocc * ns; e += G) {
+    int j = e / ns, i = e % ns;
+    double acc = 0.0;
+    for (int k = 0; k < ns; k++) acc += y[j * ns + k] * ai[k + ns * i];
+    tmp[e] = acc;   // yai
+  }
+  __syncthreads();
+  for (int e = lane; e < nocc * ns; e += G) {
+    int j = e / ns, i = e % ns;
+    g[e] = tmp[e];
+    (void)j;
+  }
+  __syncthreads();
+  for (int e = lane; e < ns * ns; e += G) {
+    int k = e / ns, i = e % ns;
+    double acc = 0.0;
+    for (int j = 0; j < nocc; j++) acc += tocc[k * nocc + j] * tmp[j * ns + i];
+    g[S.iref[s][k] * ns + i] = ai[k + ns * i] - acc;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Full evaluation (hpsi): distances, Jastrow, orbitals, determinants (AMF),
+// pseudopotential, local energy.  Refreshes the walker's resident state.
+template <typename T, int G>
+__global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
+  QMCW_GROUP_SETUP
+  __shared__ int s_ipiv[kWave / G][128];
+  const int N = S.N, M = S.M, NO = S.NO;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const double *x = ws + L.x;
+  double *rven = ws + L.rvec_en, *ren = ws + L.r_en, *rvee = ws + L.rvec_ee, *ree = ws + L.r_ee;
+  // ---- distances (qmc/vmc/distances.f90:1-233)
+  double pe_en = 0.0, pe_ee = 0.0;
+  for (int p = lane; p < N * M; p += G) {
+    int i = p / M, ic = p % M;
+    double s2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      double d = x[3 * i + q] - S.cent[3 * ic + q];
+      rven[3 * p + q] = d;
+      s2 += d * d;
+    }
+    double r = sqrt(s2);
+    ren[p] = r;
+    if (S.nloc == 0) pe_en -= S.znuc[S.iwctype[ic]] / r;
+  }
+  for (int p = lane; p < N * N; p += G) {
+    int i = p / N, j = p % N;
+    if (j >= i) continue;
+    double s2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      double d = x[3 * i + q] - x[3 * j + q];
+      rvee[3 * p + q] = d;
+      s2 += d * d;
+    }
+    double r = sqrt(s2);
+    ree[p] = r;
+    pe_ee += 1.0 / r;
+  }
+  pe_en = gsum<G>(pe_en);
+  pe_ee = gsum<G>(pe_ee);
+  __syncthreads();
+  // ---- Jastrow pair arrays (qmc/vmc/jastrow4.f90)
+  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double fsum = 0.0;
+  for (int p = lane; p < N * N; p += G) {
+    int i = p / N, j = p % N;
+    if (j > i) continue;
+    if (j == i) {
+      T f, g[3], lp;
+      jas_en<T>(S, ren + i * M, rven + i * M * 3, f, g, lp);
+      fso[p] = f;
+      for (int q = 0; q < 3; q++) fijo[3 * p + q] = g[q];
+      lapj[p] = lp;
+      fsum += f;
+    } else {
+      PairOut<T> o;
+      T rvij[3];
+      for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * p + q];
+      bool par = (i < S.nup) || (j >= S.nup);
+      jas_pair<T>(S, par, (T)ree[p], rvij, ren + i * M, ren + j * M, rven + i * M * 3,
+                  rven + j * M * 3, o);
+      fso[p] = o.f;
+      for (int q = 0; q < 3; q++) {
+        fijo[3 * (i * N + j) + q] = o.gi[q];
+        fijo[3 * (j * N + i) + q] = o.gj[q];
+      }
+      lapj[i * N + j] = o.li;
+      lapj[j * N + i] = o.lj;
+      fsum += o.f;
+    }
+  }
+  fsum = gsum<G>(fsum);
+  __syncthreads();
+  double *vj = ss + SL.vj, *divv = ss + SL.divv;
+  double d2j = 0.0;
+  for (int i = lane; i < N; i += G) {
+    double gx = 0, gy = 0, gz = 0, dv = lapj[i * N + i], lt = lapj[i * N + i];
+    for (int j = 0; j < N; j++) {
+      gx += fijo[3 * (i * N + j)];
+      gy += fijo[3 * (i * N + j) + 1];
+      gz += fijo[3 * (i * N + j) + 2];
+      if (j != i) {
+        dv += 0.5 * (lapj[i * N + j] + lapj[j * N + i]);
+        lt += lapj[i * N + j];
+      }
+    }
+    fjo[3 * i] = gx; fjo[3 * i + 1] = gy; fjo[3 * i + 2] = gz;
+    vj[3 * i] = gx; vj[3 * i + 1] = gy; vj[3 * i + 2] = gz;
+    divv[i] = dv;    // CHAMP's div_vj (pair terms split evenly)
+    d2j += lt;
+  }
+  d2j = gsum<G>(d2j);
+  if (lane == 0) ws[L.fsumo] = fsum;
+  // ---- orbitals (basis_fns + orbitals_loc_ana), chunks of electrons
+  double *phi = ss + SL.phi;
+  double *oc = ws + L.oc;
+  double *orbf = ss + SL.orb;
+  const int noccmax = S.noccmax;
+  for (int i0 = 0; i0 < N; i0 += A.chunk) {
+    int nch = min(A.chunk, N - i0);
+    __syncthreads();
+    for (int t = lane; t < nch * S.nshell; t += G) {
+      int ii = t / S.nshell, s = t % S.nshell;
+      int i = i0 + ii, ic = S.sh_cent[s];
+      const double *rv = rven + (i * M + ic) * 3;
+      eval_shell<double>(S, s, rv[0], rv[1], rv[2], ren[i * M + ic], 1,
+                         phi + (int64_t)ii * S.NB * 5);
+    }
+    __syncthreads();
+    // occupied orbitals of each electron's spin into the resident state
+    for (int t = lane; t < nch * noccmax; t += G) {
+      int ii = t / noccmax, j = t % noccmax;
+      int i = i0 + ii, s = (i < S.nup) ? 0 : 1;
+      double *o = oc + ((int64_t)i * noccmax + j) * 5;
+      if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
+      int lab = S.occ[s][j];
+      double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+      const double *ph = phi + (int64_t)ii * S.NB * 5;
+      for (int b = 0; b < S.NB; b++) {
+        double cf = S.coefT[(int64_t)b * NO + lab];
+        a0 += cf * ph[5 * b]; a1 += cf * ph[5 * b + 1]; a2 += cf * ph[5 * b + 2];
+        a3 += cf * ph[5 * b + 3]; a4 += cf * ph[5 * b + 4];
+      }
+      o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
+    }
+    if (A.full_orb) {
+      for (int t = lane; t < nch * NO; t += G) {
+        int ii = t / NO, lab = t % NO;
+        int i = i0 + ii;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+        const double *ph = phi + (int64_t)ii * S.NB * 5;
+        for (int b = 0; b < S.NB; b++) {
+          double cf = S.coefT[(int64_t)b * NO + lab];
+          a0 += cf * ph[5 * b]; a1 += cf * ph[5 * b + 1]; a2 += cf * ph[5 * b + 2];
+          a3 += cf * ph[5 * b + 3]; a4 += cf * ph[5 * b + 4];
+        }
+        double *o = orbf + ((int64_t)i * NO + lab) * 5;
+        o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
+      }
+    }
+  }
+  __syncthreads();
+  // ---- determinants (t_bld, alpha_bld, y_bld, g_bld)
+  double *tmp = ss + SL.red;  // >= max(ns^2, nocc*ns) doubles
+  for (int s = 0; s < 2; s++) {
+    int ns = S.ns[s], nocc = S.nocc[s], off = s ? S.nup : 0;
+    double *ai = ws + L.ai[s];
+    if (ns == 0) {
+      if (lane == 0) { ws[L.deta[s]] = 1.0; ws[L.detv[s]] = 1.0; }
+      __syncthreads();
+      continue;
+    }
+    // A(i, k) = orb(i, ref_k): position of ref_k in occ is iref[k]
+    for (int e = lane; e < ns * ns; e += G) {
+      int i = e % ns, k = e / ns;
+      ai[i + ns * k] = oc[((int64_t)(off + i) * noccmax + S.iref[s][k]) * 5];
+    }
+    __syncthreads();
+    double det = group_matinv<G>(ai, tmp, ns, lane, s_ipiv[grp]);
+    if (lane == 0) ws[L.deta[s]] = det;
+    // T(k, occ_j) = sum_i Ainv(k, i) orb(i, occ_j)
+    double *tocc = ws + L.tocc[s];
+    for (int e = lane; e < ns * nocc; e += G) {
+      int k = e / nocc, j = e % nocc;
+      double acc = 0.0;
+      for (int i = 0; i < ns; i++)
+        acc += ai[k + ns * i] * oc[((int64_t)(off + i) * noccmax + j) * 5];
+      tocc[e] = acc;
+    }
+    __syncthreads();
+    // alpha blocks of the unique dets
+    double *inv = ws + L.inv[s], *dv = ws + L.detv[s];
+    for (int u = lane; u < S.ndetu[s]; u += G) {
+      int k = S.ord[s][u];
+      double *blk = inv + u * S.kmax2;
+      for (int ii = 0; ii < k; ii++)
+        for (int jj = 0; jj < k; jj++)
+          blk[jj + k * ii] = tocc[S.rf[s][u * ns + jj] * nocc + S.exo[s][u * ns + ii]];
+      dv[u] = small_matinv(blk, k);
+    }
+    __syncthreads();
+  }
+  {
+    const double *invs[2] = {ws + L.inv[0], ws + L.inv[1]};
+    const double *dvs[2] = {ws + L.detv[0], ws + L.detv[1]};
+    double *ys[2] = {ws + L.y[0], ws + L.y[1]};
+    double chi = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
+    if (lane == 0) ws[L.chi] = chi;
+    __syncthreads();
+    for (int s = 0; s < 2; s++)
+      group_g<G>(S, s, ws + L.ai[s], ws + L.tocc[s], ws + L.y[s], ss + SL.g[s], tmp, lane);
+  }
+  // ---- grad / laplacian of ln D (qmc/vmc/determinant.f90:99-152)
+  double *vd = ss + SL.vd;
+  double d2d = 0.0;
+  for (int i = lane; i < N; i += G) {
+    int s = i < S.nup ? 0 : 1, ie = s ? i - S.nup : i;
+    int ns = S.ns[s], nocc = S.nocc[s];
+    const double *g = ss + SL.g[s];
+    double dx = 0, dy = 0, dz = 0, ek = 0;
+    for (int j = 0; j < nocc; j++) {
+      double gj = g[j * ns + ie];
+      const double *o = oc + ((int64_t)i * noccmax + j) * 5;
+      dx += gj * o[1]; dy += gj * o[2]; dz += gj * o[3]; ek += gj * o[4];
+    }
+    vd[3 * i] = dx; vd[3 * i + 1] = dy; vd[3 * i + 2] = dz;
+    double dvd = ek - (dx * dx + dy * dy + dz * dz);
+    divv[i] += dvd;
+    d2d += dvd;
+  }
+  d2d = gsum<G>(d2d);
+  __syncthreads();
+  // ---- pseudopotential (qmc/vmc/nonloc_pot.f90, nonloc.f90)
+  double pe = S.pecent + pe_en + pe_ee;
+  double vpsp = 0.0;
+  if (S.nloc > 0) {
+    double ploc = 0.0;
+    for (int p = lane; p < N * M; p += G) {
+      int ic = p % M, ict = S.iwctype[ic];
+      double r = fmax(1e-10, ren[p]);
+      ploc += ecp_radial(S, r, S.lpotp1[ict] - 1, ict) - S.znuc[ict] / r;
+    }
+    ploc = gsum<G>(ploc);
+    pe += ploc;
+    // psp_nonloc_orb restricted to the occupied orbitals of each spin
+    double *pnl = ss + SL.pnl;   // [N][noccmax] (+ [N][NO] when full_orb)
+    const double *quad = ws + L.quad;
+    const int nq = S.nquad;
+    for (int i = lane; i < N; i += G) {
+      int s = i < S.nup ? 0 : 1, nocc = S.nocc[s];
+      double *prow = pnl + (int64_t)i * noccmax;
+      for (int j = 0; j < noccmax; j++) prow[j] = 0.0;
+      double *prowf = pnl + (int64_t)N * noccmax + (int64_t)i * NO;
+      if (A.full_orb) for (int j = 0; j < NO; j++) prowf[j] = 0.0;
+      double *phq = phi + (int64_t)lane * S.NB * 5;  // lane-private basis values
+      for (int ic = 0; ic < M; ic++) {
+        int ict = S.iwctype[ic];
+        double rei = ren[i * M + ic];
+        double r = fmax(1e-10, rei);
+        double vps[QMCW_MAX_LPOT];
+        bool skip = true;
+        for (int l = 1; l <= S.npotd[ict]; l++) {
+          vps[l - 1] = ecp_radial(S, r, l - 1, ict);
+          if (l == S.lpotp1[ict]) vps[l - 1] -= S.znuc[ict] / r;
+          if (l != S.lpotp1[ict] && fabs(vps[l - 1]) > 1e-4) skip = false;
+        }
+        if (skip) continue;
+        const double *rv0 = rven + (i * M + ic) * 3;
+        for (int iq = 0; iq < nq; iq++) {
+          double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
+          double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
+          double xn[3] = {rei * qx + S.cent[3 * ic], rei * qy + S.cent[3 * ic + 1],
+                          rei * qz + S.cent[3 * ic + 2]};
+          // basis values at the quadrature point
+          for (int sh = 0; sh < S.nshell; sh++) {
+            int jc = S.sh_cent[sh];
+            double dxv = xn[0] - S.cent[3 * jc], dyv = xn[1] - S.cent[3 * jc + 1],
+                   dzv = xn[2] - S.cent[3 * jc + 2];
+            double rr = (jc == ic) ? rei : sqrt(dxv * dxv + dyv * dyv + dzv * dzv);
+            eval_shell<double>(S, sh, dxv, dyv, dzv, rr, 0, phq);
+          }
+          // Jastrow value change of moving electron i to xn
+          double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
+          for (int jc = 0; jc < M; jc++) {
+            double dxv = xn[0] - S.cent[3 * jc], dyv = xn[1] - S.cent[3 * jc + 1],
+                   dzv = xn[2] - S.cent[3 * jc + 2];
+            rvnew[3 * jc] = dxv; rvnew[3 * jc + 1] = dyv; rvnew[3 * jc + 2] = dzv;
+            rnew[jc] = (jc == ic) ? rei : sqrt(dxv * dxv + dyv * dyv + dzv * dzv);
+          }
+          T delta = 0;
+          {
+            T fe, ge[3], le;
+            jas_en<T>(S, rnew, rvnew, fe, ge, le);
+            delta += fe - (T)fso[i * N + i];
+          }
+          for (int jj = 0; jj < N; jj++) {
+            if (jj == i) continue;
+            int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
+            T rv[3];
+            double r2 = 0.0;
+            for (int q = 0; q < 3; q++) {
+              double xi = (hi == i) ? xn[q] : x[3 * hi + q];
+              double xj = (lo == i) ? xn[q] : x[3 * lo + q];
+              rv[q] = (T)(xi - xj);
+              r2 += (xi - xj) * (xi - xj);
+            }
+            const double *rj_ = ren + jj * M, *rvj_ = rven + jj * M * 3;
+            PairOut<T> po;
+            bool par = (hi < S.nup) || (lo >= S.nup);
+            if (hi == i)
+              jas_pair<T>(S, par, (T)sqrt(r2), rv, rnew, rj_, rvnew, rvj_, po);
+            else
+              jas_pair<T>(S, par, (T)sqrt(r2), rv, rj_, rnew, rvj_, rvnew, po);
+            delta += po.f - (T)fso[hi * N + lo];
+          }
+          double vpot = 0.0;
+          for (int l = 1; l <= S.npotd[ict]; l++)
+            if (l != S.lpotp1[ict]) vpot += vps[l - 1] * legendre_w(l, costh);
+          double factor = S.q0[3 * nq + iq] * exp((double)delta) * vpot;
+          for (int j = 0; j < nocc; j++) {
+            int lab = S.occ[s][j];
+            double acc = 0.0;
+            for (int b = 0; b < S.NB; b++) acc += S.coefT[(int64_t)b * NO + lab] * phq[5 * b];
+            prow[j] += factor * acc;
+          }
+          if (A.full_orb)
+            for (int lab = 0; lab < NO; lab++) {
+              double acc = 0.0;
+              for (int b = 0; b < S.NB; b++) acc += S.coefT[(int64_t)b * NO + lab] * phq[5 * b];
+              prowf[lab] += factor * acc;
+            }
+        }
+      }
+      // V_nl contribution of electron i: sum_j G(j, i) pnl(i, j)
+      int ie = s ? i - S.nup : i, ns = S.ns[s];
+      const double *g = ss + SL.g[s];
+      for (int j = 0; j < nocc; j++) vpsp += g[j * ns + ie] * prow[j];
+    }
+    vpsp = gsum<G>(vpsp);
+    pe += vpsp;
+  }
+  // ---- local energy (qmc/vmc/hpsi.f90:336-344)
+  double v2 = 0.0;
+  for (int i = lane; i < N; i += G)
+    for (int q = 0; q < 3; q++) {
+      double v = vj[3 * i + q] + vd[3 * i + q];
+      v2 += v * v;
+    }
+  v2 = gsum<G>(v2);
+  double d2lnpsi = d2d + d2j;
+  double eloc = pe - S.hb * d2lnpsi - S.hb * v2;
+  if (lane == 0) {
+    double chi = ws[L.chi], du = ws[L.deta[0]], dd = ws[L.deta[1]];
+    double psid = chi * du * dd;
+    double lp = log(fabs(chi)) + log(fabs(du)) + log(fabs(dd));
+    ws[L.logpsid] = lp;
+    ws[L.sgn] = psid < 0 ? -1.0 : 1.0;
+    ws[L.psij] = fsum;
+    ws[L.psi2o] = 2.0 * (lp + fsum);
+    ws[L.eloc] = eloc;
+    ws[L.epot] = pe;
+    ss[SL.red + 60] = d2lnpsi;
+    ss[SL.red + 61] = vpsp;
+    ss[SL.red + 62] = pe_ee;
+    int fl = 0;
+    if (psid == 0.0) fl |= 1;
+    if (!isfinite(eloc)) fl |= 2;
+    A.flags[w] = fl;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// gradient of ln Psi of electron ie in the current state (eval_grad +
+// Jastrow), qmc/dete_mod.f90:249-292 and qmc/psi_utils_mod.f90:457-483.
+// Also the laplacian (lap Psi)/Psi when lap != nullptr.  All lanes return.
+template <int G>
+__device__ void group_grad_1e(const KArgs &A, const double *ws, int ie,
+                              const double *orbv, const double *ai,
+                              const double *tocc, const double *y,
+                              const double *fj, double lapj, double *grad,
+                              double *lapout, int lane) {
+  const DevSys &S = A.S;
+  int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  int e = s ? ie - S.nup : ie;
+  // grad(c) = sum_k ai(k,e) d_c(ref_k) + sum_i sum_j y(j,i) ai(i,e) v_c(j)
+  //   v_c(j) = d_c(occ_j) - sum_k d_c(ref_k) T(k, occ_j)
+  double acc[4] = {0, 0, 0, 0};
+  for (int k = lane; k < ns; k += G) {
+    double a = ai[k + ns * e];
+    const double *o = orbv + S.iref[s][k] * 5;
+    for (int c = 0; c < 4; c++) acc[c] += a * o[1 + c];
+  }
+  // w_i = ai(i, e) ; z_j = sum_i y(j,i) w_i
+  for (int j = lane; j < nocc; j += G) {
+    double z = 0.0;
+    for (int i = 0; i < ns; i++) z += y[j * ns + i] * ai[i + ns * e];
+    if (z == 0.0) continue;
+    for (int c = 0; c < 4; c++) {
+      double v = orbv[j * 5 + 1 + c];
+      for (int k = 0; k < ns; k++) v -= orbv[S.iref[s][k] * 5 + 1 + c] * tocc[k * nocc + j];
+      acc[c] += z * v;
+    }
+  }
+  for (int c = 0; c < 4; c++) acc[c] = gsum<G>(acc[c]);
+  for (int c = 0; c < 3; c++) grad[c] = acc[c] + fj[c];
+  if (lapout)
+    *lapout = acc[3] + 2.0 * (fj[0] * acc[0] + fj[1] * acc[1] + fj[2] * acc[2]) + lapj;
+}
+
+// ---------------------------------------------------------------------------
+// One-electron move of electron ie: propose (mode 1, metrop_mov1 with the
+// walker's rannyu stream) or take xnew (mode 0), then distancese +
+// jastrow4e + orbitals + dete_update + eval_grad into the pending buffers.
+// out[w][8]: ratio, ln|psid_new|, sign, psij_new, grad(3), lap
+template <typename T, int G>
+__global__ __launch_bounds__(64) void k_ratio_1e(KArgs A, int ie, int mode,
+                                                 const double *xnew_in,
+                                                 double delta, double fbias,
+                                                 double *out) {
+  QMCW_GROUP_SETUP
+  const int N = S.N, M = S.M, noccmax = S.noccmax;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const MLayout &ML = A.ML;
+  double *mv = ss + SL.mv;
+  const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  const int e = s ? ie - S.nup : ie;
+  const double *x = ws + L.x;
+  const double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double *xn = mv + ML.xn;
+  // ---- proposal
+  if (mode == 1) {
+    double lapo_dummy;
+    double vold[3];
+    double lapjo = 0.0;  // not needed for the proposal
+    group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, ws + L.ai[s],
+                     ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lapjo, vold, nullptr, lane);
+    (void)lapo_dummy;
+    if (lane == 0) {
+      int *rs = A.rng + 4 * w;
+      int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+      double deltfi = 2.0 * fbias / delta;
+      double fxop = 1.0;
+      for (int k = 0; k < 3; k++) {
+        double fx = fmin(fabs(vold[k]), deltfi);
+        double ftest = fx * delta * 0.5;
+        fx = copysign(fx, vold[k]);
+        double dx;
+        if (ftest <= rannyu(st4)) {
+          dx = (rannyu(st4) - 0.5) * delta;
+        } else {
+          double r1 = rannyu(st4), r2 = rannyu(st4);
+          dx = (fmax(r1, r2) - 0.5) * delta;
+          if (vold[k] < 0) dx = -dx;
+        }
+        fxop *= (1.0 + fx * dx);
+        xn[k] = x[3 * ie + k] + dx;
+        mv[ML.misc + 2 + k] = dx;
+      }
+      mv[ML.misc + 1] = fxop;
+      rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+    }
+  } else {
+    if (lane < 3) xn[lane] = xnew_in[3 * w + lane];
+  }
+  __syncthreads();
+  // ---- new distances of electron ie
+  double *ren = mv + ML.ren, *rven = mv + ML.rven, *ree = mv + ML.ree, *rvee = mv + ML.rvee;
+  for (int ic = lane; ic < M; ic += G) {
+    double s2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      double d = xn[q] - S.cent[3 * ic + q];
+      rven[3 * ic + q] = d;
+      s2 += d * d;
+    }
+    ren[ic] = sqrt(s2);
+  }
+  for (int j = lane; j < N; j += G) {
+    if (j == ie) continue;
+    int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
+    double s2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      double xi = (hi == ie) ? xn[q] : x[3 * hi + q];
+      double xj = (lo == ie) ? xn[q] : x[3 * lo + q];
+      rvee[3 * j + q] = xi - xj;
+      s2 += (xi - xj) * (xi - xj);
+    }
+    ree[j] = sqrt(s2);
+  }
+  __syncthreads();
+  // ---- jastrow4e (+ per-electron laplacian)
+  double *fsn = mv + ML.fsn, *fijn_i = mv + ML.fijn_i, *fijn_j = mv + ML.fijn_j,
+         *lapn = mv + ML.lapn_i, *fjn = mv + ML.fjn;
+  const double *ren_o = ws + L.r_en, *rven_o = ws + L.rvec_en;
+  double dsum = 0.0, gix = 0, giy = 0, giz = 0, lapi = 0;
+  for (int j = lane; j <= N; j += G) {
+    if (j == ie) continue;
+    if (j == N) {  // e-n diagonal term
+      T f, g[3], lp;
+      jas_en<T>(S, ren, rven, f, g, lp);
+      fsn[ie] = f;
+      for (int q = 0; q < 3; q++) fijn_i[3 * ie + q] = g[q];
+      lapn[ie] = lp;
+      dsum += (double)f - fso[ie * N + ie];
+      gix += (double)g[0] - fijo[3 * (ie * N + ie)];
+      giy += (double)g[1] - fijo[3 * (ie * N + ie) + 1];
+      giz += (double)g[2] - fijo[3 * (ie * N + ie) + 2];
+      lapi += lp;
+      continue;
+    }
+    int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
+    T rvij[3];
+    for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * j + q];
+    const double *rI = ren, *rvI = rven, *rJ = ren_o + j * M, *rvJ = rven_o + j * M * 3;
+    PairOut<T> po;
+    bool par = (hi < S.nup) || (lo >= S.nup);
+    // moved electron's arrays go to whichever slot (hi or lo) it occupies
+    if (hi == ie) jas_pair<T>(S, par, (T)ree[j], rvij, rI, rJ, rvI, rvJ, po);
+    else jas_pair<T>(S, par, (T)ree[j], rvij, rJ, rI, rvJ, rvI, po);
+    const T *g_ie = (hi == ie) ? po.gi : po.gj;
+    const T *g_j = (hi == ie) ? po.gj : po.gi;
+    T l_ie = (hi == ie) ? po.li : po.lj;
+    T l_j = (hi == ie) ? po.lj : po.li;
+    fsn[j] = po.f;
+    for (int q = 0; q < 3; q++) {
+      fijn_i[3 * j + q] = g_ie[q];
+      fijn_j[3 * j + q] = g_j[q];
+    }
+    lapn[j] = l_ie;
+    mv[ML.lapn_i + N + j] = l_j;
+    dsum += (double)po.f - fso[hi * N + lo];
+    gix += (double)g_ie[0] - fijo[3 * (ie * N + j)];
+    giy += (double)g_ie[1] - fijo[3 * (ie * N + j) + 1];
+    giz += (double)g_ie[2] - fijo[3 * (ie * N + j) + 2];
+    lapi += l_ie;
+    // other electron's new gradient
+    for (int q = 0; q < 3; q++)
+      fjn[3 * j + q] = fjo[3 * j + q] + (double)g_j[q] - fijo[3 * (j * N + ie) + q];
+  }
+  dsum = gsum<G>(dsum);
+  gix = gsum<G>(gix); giy = gsum<G>(giy); giz = gsum<G>(giz);
+  lapi = gsum<G>(lapi);
+  const double fsumn = ws[L.fsumo] + dsum;
+  double fji[3] = {fjo[3 * ie] + gix, fjo[3 * ie + 1] + giy, fjo[3 * ie + 2] + giz};
+  if (lane == 0) { fjn[3 * ie] = fji[0]; fjn[3 * ie + 1] = fji[1]; fjn[3 * ie + 2] = fji[2]; }
+  double lapjn = lapi + fji[0] * fji[0] + fji[1] * fji[1] + fji[2] * fji[2];
+  // ---- orbitals of the new position (occupied set of the spin)
+  double *phi = ss + SL.phi;
+  for (int sh = lane; sh < S.nshell; sh += G) {
+    int ic = S.sh_cent[sh];
+    eval_shell<double>(S, sh, rven[3 * ic], rven[3 * ic + 1], rven[3 * ic + 2], ren[ic], 1, phi);
+  }
+  __syncthreads();
+  double *orbe = mv + ML.orbe;
+  for (int j = lane; j < nocc; j += G) {
+    int lab = S.occ[s][j];
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < S.NB; b++) {
+      double cf = S.coefT[(int64_t)b * S.NO + lab];
+      for (int c = 0; c < 5; c++) a[c] += cf * phi[5 * b + c];
+    }
+    for (int c = 0; c < 5; c++) orbe[5 * j + c] = a[c];
+  }
+  __syncthreads();
+  // ---- dete_update (qmc/dete_mod.f90:114-214)
+  const double *ai = ws + L.ai[s];
+  const double *tocc = ws + L.tocc[s];
+  double *ain = mv + ML.ain, *tn = mv + ML.tn;
+  double rat = 0.0;
+  for (int k = lane; k < ns; k += G) rat += orbe[5 * S.iref[s][k]] * ai[k + ns * e];
+  rat = gsum<G>(rat);
+  // vi1(i) = sum_k orbe(ref_k) ai(k, i) - delta(i, e); ui(k) = -ai(k,e)/rat
+  double *vi1 = mv + ML.misc + 8;            // [ns]
+  double *vi2 = vi1 + ns;                    // [nocc]
+  for (int i = lane; i < ns; i += G) {
+    double acc = 0.0;
+    for (int k = 0; k < ns; k++) acc += orbe[5 * S.iref[s][k]] * ai[k + ns * i];
+    vi1[i] = acc - (i == e ? 1.0 : 0.0);
+  }
+  for (int j = lane; j < nocc; j += G) {
+    double acc = 0.0;
+    for (int k = 0; k < ns; k++) acc += orbe[5 * S.iref[s][k]] * tocc[k * nocc + j];
+    vi2[j] = acc - orbe[5 * j];
+  }
+  __syncthreads();
+  for (int q = lane; q < ns * ns; q += G) {
+    int k = q % ns, i = q / ns;
+    ain[q] = ai[q] + (-ai[k + ns * e] / rat) * vi1[i];
+  }
+  for (int q = lane; q < ns * nocc; q += G) {
+    int k = q / nocc, j = q % nocc;
+    tn[q] = tocc[q] + (-ai[k + ns * e] / rat) * vi2[j];
+  }
+  __syncthreads();
+  double *invn = mv + ML.invn, *detn = mv + ML.detn;
+  for (int u = lane; u < S.ndetu[s]; u += G) {
+    int k = S.ord[s][u];
+    double *blk = invn + u * S.kmax2;
+    for (int ii = 0; ii < k; ii++)
+      for (int jj = 0; jj < k; jj++)
+        blk[jj + k * ii] = tn[S.rf[s][u * ns + jj] * nocc + S.exo[s][u * ns + ii]];
+    detn[u] = small_matinv(blk, k);
+  }
+  __syncthreads();
+  double chin;
+  {
+    const double *invs[2], *dvs[2];
+    double *ys[2];
+    invs[s] = invn; dvs[s] = detn; ys[s] = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
+    invs[1 - s] = ws + L.inv[1 - s]; dvs[1 - s] = ws + L.detv[1 - s];
+    ys[1 - s] = mv + ML.yn + (s ? 0 : S.nocc[0] * S.ns[0]);
+    chin = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
+  }
+  __syncthreads();
+  const double *yn = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
+  double deta_n = ws[L.deta[s]] * rat;
+  double gnew[3], lapn_tot;
+  group_grad_1e<G>(A, ws, ie, orbe, ain, tn, yn, fji, lapjn, gnew, &lapn_tot, lane);
+  if (lane == 0) {
+    double psio = ws[L.chi] * ws[L.deta[0]] * ws[L.deta[1]];
+    double psin = chin * deta_n * ws[L.deta[1 - s]];
+    double logn = log(fabs(chin)) + log(fabs(deta_n)) + log(fabs(ws[L.deta[1 - s]]));
+    mv[ML.misc + 0] = rat;
+    mv[ML.misc + 5] = chin;
+    mv[ML.misc + 6] = deta_n;
+    mv[ML.misc + 7] = fsumn;
+    mv[ML.misc + 8 + ns + nocc] = logn;
+    double *o = out ? out + 8 * w : nullptr;
+    double ratio = (psin / psio) * exp(fsumn - ws[L.fsumo]);
+    mv[ML.misc + 9 + ns + nocc] = ratio;
+    for (int q = 0; q < 3; q++) mv[ML.misc + 10 + ns + nocc + q] = gnew[q];
+    if (o) {
+      o[0] = ratio; o[1] = logn; o[2] = psin < 0 ? -1.0 : 1.0; o[3] = fsumn;
+      o[4] = gnew[0]; o[5] = gnew[1]; o[6] = gnew[2]; o[7] = lapn_tot;
+    }
+  }
+}
+
+// Commit or drop the pending move of electron ie.  mode 1: Metropolis test
+// of metrop_mov1 (qmc/vmc/metrop_mov1.f90:90-99, 226-250) with the walker's
+// rannyu; mode 0: accept[w] given.  naccept[w] incremented on acceptance.
+template <int G>
+__global__ __launch_bounds__(64) void k_accept_1e(KArgs A, int ie, int mode,
+                                                  const uint8_t *accept_in,
+                                                  double delta, double fbias,
+                                                  long long *naccept) {
+  QMCW_GROUP_SETUP
+  const int N = S.N, M = S.M, noccmax = S.noccmax;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const MLayout &ML = A.ML;
+  double *mv = ss + SL.mv;
+  const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  __shared__ int s_acc[kWave / G];
+  if (lane == 0) {
+    int acc;
+    if (mode == 1) {
+      int *rs = A.rng + 4 * w;
+      int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+      double deltfi = 2.0 * fbias / delta;
+      double logn = mv[ML.misc + 8 + ns + nocc];
+      double psi2n = 2.0 * (logn + mv[ML.misc + 7]);
+      double fxnp = 1.0;
+      for (int k = 0; k < 3; k++) {
+        double vn = mv[ML.misc + 10 + ns + nocc + k];
+        fxnp *= (1.0 - mv[ML.misc + 2 + k] * copysign(fmin(fabs(vn), deltfi), vn));
+      }
+      double p = exp(psi2n - ws[L.psi2o]) * fxnp / mv[ML.misc + 1];
+      p = fmin(1.0, p);
+      acc = rannyu(st4) < p;
+      rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+    } else {
+      acc = accept_in[w] != 0;
+    }
+    s_acc[grp] = acc;
+    if (acc && naccept) naccept[w] += 1;
+  }
+  __syncthreads();
+  if (!s_acc[grp]) return;   // uniform within the group; no barriers follow
+  double *x = ws + L.x;
+  const double *xn = mv + ML.xn;
+  if (lane < 3) x[3 * ie + lane] = xn[lane];
+  for (int ic = lane; ic < M; ic += G) {
+    ws[L.r_en + ie * M + ic] = mv[ML.ren + ic];
+    for (int q = 0; q < 3; q++) ws[L.rvec_en + (ie * M + ic) * 3 + q] = mv[ML.rven + 3 * ic + q];
+  }
+  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  for (int j = lane; j < N; j += G) {
+    if (j == ie) {
+      fso[ie * N + ie] = mv[ML.fsn + ie];
+      for (int q = 0; q < 3; q++) fijo[3 * (ie * N + ie) + q] = mv[ML.fijn_i + 3 * ie + q];
+      lapj[ie * N + ie] = mv[ML.lapn_i + ie];
+      continue;
+    }
+    int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
+    ws[L.r_ee + hi * N + lo] = mv[ML.ree + j];
+    for (int q = 0; q < 3; q++) ws[L.rvec_ee + 3 * (hi * N + lo) + q] = mv[ML.rvee + 3 * j + q];
+    fso[hi * N + lo] = mv[ML.fsn + j];
+    for (int q = 0; q < 3; q++) {
+      fijo[3 * (ie * N + j) + q] = mv[ML.fijn_i + 3 * j + q];
+      fijo[3 * (j * N + ie) + q] = mv[ML.fijn_j + 3 * j + q];
+    }
+    lapj[ie * N + j] = mv[ML.lapn_i + j];
+    lapj[j * N + ie] = mv[ML.lapn_i + N + j];
+  }
+  for (int q = lane; q < 3 * N; q += G) fjo[q] = mv[ML.fjn + q];
+  double *oc = ws + L.oc + (int64_t)ie * noccmax * 5;
+  for (int q = lane; q < nocc * 5; q += G) oc[q] = mv[ML.orbe + q];
+  double *ai = ws + L.ai[s], *tocc = ws + L.tocc[s];
+  for (int q = lane; q < ns * ns; q += G) ai[q] = mv[ML.ain + q];
+  for (int q = lane; q < ns * nocc; q += G) tocc[q] = mv[ML.tn + q];
+  double *inv = ws + L.inv[s], *dv = ws + L.detv[s];
+  for (int q = lane; q < S.ndetu[s] * S.kmax2; q += G) inv[q] = mv[ML.invn + q];
+  for (int q = lane; q < S.ndetu[s]; q += G) dv[q] = mv[ML.detn + q];
+  for (int t = 0; t < 2; t++) {
+    double *y = ws + L.y[t];
+    const double *yn = mv + ML.yn + (t ? S.nocc[0] * S.ns[0] : 0);
+    for (int q = lane; q < S.nocc[t] * S.ns[t]; q += G) y[q] = yn[q];
+  }
+  if (lane == 0) {
+    ws[L.chi] = mv[ML.misc + 5];
+    ws[L.deta[s]] = mv[ML.misc + 6];
+    ws[L.fsumo] = mv[ML.misc + 7];
+    ws[L.psij] = mv[ML.misc + 7];
+    double logn = mv[ML.misc + 8 + ns + nocc];
+    ws[L.logpsid] = logn;
+    ws[L.psi2o] = 2.0 * (logn + mv[ML.misc + 7]);
+  }
+}
+
+// grad/lap of electron ie in the current state: out[w][4]
+template <int G>
+__global__ __launch_bounds__(64) void k_grad_1e(KArgs A, int ie, double *out) {
+  QMCW_GROUP_SETUP
+  const WLayout &L = A.L;
+  const int N = S.N, s = ie < S.nup ? 0 : 1;
+  const double *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double lj = 0.0;
+  for (int j = lane; j < N; j += G) lj += lapj[ie * N + j];
+  lj = gsum<G>(lj);
+  lj += fjo[3 * ie] * fjo[3 * ie] + fjo[3 * ie + 1] * fjo[3 * ie + 1] + fjo[3 * ie + 2] * fjo[3 * ie + 2];
+  double g[3], lap;
+  group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * S.noccmax * 5, ws + L.ai[s],
+                   ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lj, g, &lap, lane);
+  if (lane == 0) {
+    out[4 * w] = g[0]; out[4 * w + 1] = g[1]; out[4 * w + 2] = g[2]; out[4 * w + 3] = lap;
+  }
+}
+
+// random rotation of each walker's quadrature grid (qmc/vmc/rotqua.f90:621-682)
+__global__ void k_rotqua(KArgs A) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const DevSys &S = A.S;
+  int *rs = A.rng + 4 * w;
+  int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+  double x1, x2, x3, xsum;
+  do {
+    x1 = 1.0 - 2.0 * rannyu(st4);
+    x2 = 1.0 - 2.0 * rannyu(st4);
+    xsum = x1 * x1 + x2 * x2;
+  } while (xsum >= 1.0);
+  double xsum2 = 2.0 * sqrt(fabs(1.0 - xsum));
+  x1 *= xsum2;
+  x2 *= xsum2;
+  x3 = 1.0 - 2.0 * xsum;
+  double theta = acos(x3), sthet = sin(theta), cfi, sfi;
+  if (sthet < 1e-5) {
+    cfi = 1; sfi = 0; x1 = 0; x2 = 0; x3 = 1; sthet = 0;
+  } else {
+    cfi = x1 / sthet;
+    sfi = x2 / sthet;
+  }
+  double yy1 = x3 * cfi, yy2 = x3 * sfi, yy3 = -sthet;
+  double zz1 = -sfi, zz2 = cfi, zz3 = 0;
+  double u1, u2, usum;
+  do {
+    u1 = rannyu(st4) * 2.0 - 1.0;
+    u2 = rannyu(st4) * 2.0 - 1.0;
+    usum = u1 * u1 + u2 * u2;
+  } while (usum >= 1.0);
+  double uu = sqrt(usum);
+  u1 /= uu;
+  u2 /= uu;
+  double y1 = yy1 * u1 + zz1 * u2, y2 = yy2 * u1 + zz2 * u2, y3 = yy3 * u1 + zz3 * u2;
+  double z1 = yy1 * u2 - zz1 * u1, z2 = yy2 * u2 - zz2 * u1, z3 = yy3 * u2 - zz3 * u1;
+  int nq = S.nquad;
+  double *quad = A.st + (int64_t)w * A.L.stride + A.L.quad;
+  for (int iq = 0; iq < nq; iq++) {
+    double a = S.q0[iq], b = S.q0[nq + iq], c = S.q0[2 * nq + iq];
+    quad[iq] = a * x1 + b * y1 + c * z1;
+    quad[nq + iq] = a * x2 + b * y2 + c * z2;
+    quad[2 * nq + iq] = a * x3 + b * y3 + c * z3;
+  }
+  rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+}
+
+}  // namespace qmcw

@@ -1,0 +1,152 @@
+"""Host-side mirror of CHAMP's per-walker psi call surface over the C ABI.
+
+:class:`Engine` owns one ``qmcw_handle`` (W walkers resident in GPU memory).
+Its methods keep the names of the reference routines they replace:
+
+=====================  =====================================================
+``psi_at``             qmc/psi_utils_mod.f90:13-42 (full evaluation, hpsi)
+``move_1e``            qmc/psi_utils_mod.f90:45-90 (one-electron move, hpsie)
+``copy_psi_1e``        qmc/psi_utils_mod.f90:299-387 (commit a move)
+``calc_1e_derivatives`` qmc/psi_utils_mod.f90:457-483
+``derivs``             qmc/deriv_mod.f90:248-556 (dpsi, deloc)
+``vmc``                qmc/vmc/vmc.f90:250-331 with metrop_mov1's proposal
+=====================  =====================================================
+
+Errors from the library raise :class:`QmcwError` carrying the status code
+and the library's message (the reference stops the program instead).
+There is no CPU fallback: the HIP library must be built and a GPU present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+_PREC = {"fp64": abi.QMCW_FP64, "mixed": abi.QMCW_MIXED, "mixed_bf16": abi.QMCW_MIXED_BF16}
+
+
+class QmcwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qmcw error {code}: {msg}")
+        self.code = code
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Engine:
+    def __init__(self, system, nwalk: int, precision: str = "fp64"):
+        self.lib = abi.load_library()
+        self.system = system
+        self.N = system.nelec
+        self.W = int(nwalk)
+        self._struct = system.struct()
+        h = C.c_void_p()
+        st = self.lib.qmcw_create(C.byref(self._struct), self.W, _PREC[precision], C.byref(h))
+        if st != 0:
+            raise QmcwError(st, self.lib.qmcw_last_error(None).decode())
+        self.h = h
+        self.P = self.lib.qmcw_nparm(h)
+
+    def _chk(self, st):
+        if st != 0:
+            raise QmcwError(st, self.lib.qmcw_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.qmcw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- walkers and streams
+    def set_walkers(self, x):
+        x = np.ascontiguousarray(x, np.float64).reshape(self.W, self.N, 3)
+        self._chk(self.lib.qmcw_set_walkers(self.h, _d(x)))
+
+    def get_walkers(self):
+        x = np.zeros((self.W, self.N, 3))
+        self._chk(self.lib.qmcw_get_walkers(self.h, _d(x)))
+        return x
+
+    def set_rng(self, seeds):
+        s = np.ascontiguousarray(seeds, np.int32).reshape(self.W, 4)
+        self._chk(self.lib.qmcw_set_rng(self.h, s.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def get_rng(self):
+        s = np.zeros((self.W, 4), np.int32)
+        self._chk(self.lib.qmcw_get_rng(self.h, s.ctypes.data_as(C.POINTER(C.c_int32))))
+        return s
+
+    def set_params(self, a4=None, b=None, c=None, csf_coef=None, coef=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)
+        arrs = [conv(a) for a in (a4, b, c, csf_coef, coef)]
+        self._keep = arrs
+        self._chk(self.lib.qmcw_set_params(self.h, *[_d(a) for a in arrs]))
+
+    # ---- psi call surface
+    def psi_at(self):
+        W, N = self.W, self.N
+        out = dict(logpsid=np.zeros(W), sign=np.zeros(W), psij=np.zeros(W),
+                   grad=np.zeros((W, N, 3)), divv=np.zeros((W, N)), eloc=np.zeros(W),
+                   epot=np.zeros(W))
+        self._chk(self.lib.qmcw_eval_full(self.h, _d(out["logpsid"]), _d(out["sign"]),
+                                          _d(out["psij"]), _d(out["grad"]), _d(out["divv"]),
+                                          _d(out["eloc"]), _d(out["epot"])))
+        return out
+
+    eval_full = psi_at
+
+    def move_1e(self, ie: int, xnew):
+        W = self.W
+        xn = np.ascontiguousarray(xnew, np.float64).reshape(W, 3)
+        ratio, grad, lap = np.zeros(W), np.zeros((W, 3)), np.zeros(W)
+        self._chk(self.lib.qmcw_ratio_1e(self.h, int(ie), _d(xn), _d(ratio), _d(grad), _d(lap)))
+        return dict(ratio=ratio, grad=grad, lap=lap)
+
+    ratio_1e = move_1e
+
+    def copy_psi_1e(self, ie: int, accept):
+        a = np.ascontiguousarray(np.asarray(accept).astype(np.uint8)).reshape(self.W)
+        self._chk(self.lib.qmcw_accept_1e(self.h, int(ie),
+                                          a.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    accept_1e = copy_psi_1e
+
+    def calc_1e_derivatives(self, ie: int):
+        grad, lap = np.zeros((self.W, 3)), np.zeros(self.W)
+        self._chk(self.lib.qmcw_grad_1e(self.h, int(ie), _d(grad), _d(lap)))
+        return grad, lap
+
+    grad_1e = calc_1e_derivatives
+
+    def derivs(self):
+        dp, de = np.zeros((self.W, self.P)), np.zeros((self.W, self.P))
+        self._chk(self.lib.qmcw_derivs(self.h, _d(dp), _d(de)))
+        return dp, de
+
+    # ---- drivers
+    def vmc(self, nstep: int, delta: float, fbias: float = 1.0, trace: bool = True):
+        tr = np.zeros((nstep, self.W)) if trace else None
+        nacc = np.zeros(self.W, np.int64)
+        self._chk(self.lib.qmcw_vmc_run(self.h, int(nstep), float(delta), float(fbias), _d(tr),
+                                        nacc.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(eloc=tr, naccept=nacc)
+
+    def vmc_device(self, nstep: int, delta: float, fbias: float = 1.0):
+        self._chk(self.lib.qmcw_vmc_run_device(self.h, int(nstep), float(delta), float(fbias)))
+
+    def get_eloc(self):
+        e = np.zeros(self.W)
+        self._chk(self.lib.qmcw_get_eloc(self.h, _d(e)))
+        return e
+
+    def sync(self):
+        self._chk(self.lib.qmcw_sync(self.h))

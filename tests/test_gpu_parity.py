@@ -1,0 +1,111 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 oracle on
+identical walker configurations and identical rannyu streams.
+
+Tolerances (FP64 engine): log|Psi_D|, ln J, E_L, gradients and laplacians
+agree to 1e-10 relative (1e-9 absolute on E_L, whose terms cancel); VMC
+accept/reject decisions are bit-exact, so acceptance counts and positions
+match exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from champ_amd import rng, system
+from champ_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def _systems():
+    import json, os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "be_transcript.json")))
+    be = system.from_champ_input(fx["input"], "be")
+    p = [r for r in fx["runs"] if r["iteration"] == 4][0]["params"]
+    be.a4 = np.array([p["a"]]); be.b = np.array([p["b"]])
+    be.c = system.cusp_exact(np.array([p["c"]]), be.nordc)
+    return {"be": be, "be_atom": system.be_atom(1), "c2": system.c2_dimer(2),
+            "chain4": system.hchain(4, ncarbon=4)}
+
+
+SYS = _systems()
+
+
+def _close(a, b, rtol=RTOL, atol=1e-10):
+    a, b = np.asarray(a), np.asarray(b)
+    err = np.abs(a - b) / (np.abs(b) + atol / rtol)
+    return float(err.max())
+
+
+@pytest.mark.parametrize("name", list(SYS))
+def test_psi_at_matches_oracle(name):
+    s = SYS[name]
+    W = 37          # not a multiple of the walkers per wavefront
+    x = system.random_walkers(s, W, seed=11)
+    e = Engine(s, W)
+    e.set_walkers(x)
+    g = e.psi_at()
+    r = oracle.Oracle(s).eval_batch(x)
+    o9 = r["out9"]
+    assert _close(g["logpsid"], o9[:, 0]) < RTOL
+    assert np.array_equal(g["sign"], o9[:, 1])
+    assert _close(g["psij"], o9[:, 2]) < RTOL
+    assert _close(g["epot"], o9[:, 4]) < RTOL
+    assert _close(g["eloc"], o9[:, 3], atol=1e-9) < 1e-9
+    assert _close(g["grad"], r["grad"]) < RTOL
+    assert _close(g["divv"], r["divv"], atol=1e-9) < 1e-9
+
+
+@pytest.mark.parametrize("name", list(SYS))
+def test_move_accept_sequence_matches_oracle(name):
+    s = SYS[name]
+    W = 19
+    x = system.random_walkers(s, W, seed=12)
+    e = Engine(s, W)
+    e.set_walkers(x)
+    e.psi_at()
+    O = oracle.Oracle(s)
+    ws = [O.L.orc_walker_create(O.h) for _ in range(W)]
+    for w in range(W):
+        O.eval_full(x[w], walker=ws[w])
+    rs = np.random.default_rng(5)
+    for step in range(2 * s.nelec):
+        ie = step % s.nelec
+        xn = x[:, ie, :] + rs.normal(0, 0.4, (W, 3))
+        g = e.move_1e(ie, xn)
+        ref = [O.ratio_1e(ie, xn[w], walker=ws[w]) for w in range(W)]
+        assert _close(g["ratio"], [r["ratio"] for r in ref]) < 1e-9
+        assert _close(g["grad"], [r["grad"] for r in ref]) < 1e-9
+        assert _close(g["lap"], [r["lap"] for r in ref], atol=1e-9) < 1e-8
+        acc = rs.random(W) < 0.6
+        e.copy_psi_1e(ie, acc)
+        for w in range(W):
+            O.accept_1e(ie, bool(acc[w]), walker=ws[w])
+            if acc[w]:
+                x[w, ie] = xn[w]
+        je = (ie + 1) % s.nelec
+        gg, ll = e.calc_1e_derivatives(je)
+        rr = [O.grad_1e(je, walker=ws[w]) for w in range(W)]
+        assert _close(gg, [r[0] for r in rr]) < 1e-9
+    assert np.allclose(e.get_walkers(), x, rtol=0, atol=0)
+    for w in ws:
+        O.L.orc_walker_destroy(w)
+
+
+@pytest.mark.parametrize("name", ["be", "c2", "chain4"])
+def test_vmc_bit_exact_decisions(name):
+    s = SYS[name]
+    W = 24
+    x = system.random_walkers(s, W, seed=13)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    e = Engine(s, W)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    nstep = 4
+    g = e.vmc(nstep, delta=0.6)
+    o = oracle.Oracle(s).vmc_run(x, seeds, nstep, 0.6)
+    assert np.array_equal(g["naccept"], o["naccept"])
+    assert np.array_equal(e.get_walkers(), o["x"])
+    assert np.array_equal(e.get_rng(), o["rng"])
+    assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
