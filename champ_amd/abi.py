@@ -85,6 +85,7 @@ def iptr(a):
 _vp = C.c_void_p
 SIGNATURES = {
     "qmcw_nterms4": (C.c_int32, [C.c_int32]),
+    "qmcw_sizeof_system": (C.c_int64, []),
     "qmcw_create": (C.c_int, [C.POINTER(QmcwSystem), C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "qmcw_destroy": (C.c_int, [_vp]),
     "qmcw_set_params": (C.c_int, [_vp, _pd, _pd, _pd, _pd, _pd]),
@@ -103,6 +104,9 @@ SIGNATURES = {
     "qmcw_vmc_run_device": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double]),
     "qmcw_get_eloc": (C.c_int, [_vp, _pd]),
     "qmcw_sync": (C.c_int, [_vp]),
+    "qmcw_autotune": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "qmcw_profile": (C.c_int, [_vp, C.c_int32]),
+    "qmcw_profile_get": (C.c_int, [_vp, _pd, C.POINTER(C.c_int64)]),
     "qmcw_last_error": (C.c_char_p, [_vp]),
 }
 

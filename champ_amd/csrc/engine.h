@@ -97,9 +97,12 @@ struct SLayout {
   int64_t vd, divv, vj;      // [N][3], [N], [N][3]
   int64_t pnl;               // [N][NO] psp_nonloc_orb
   int64_t pnl_c;             // [N*M][NO] per-centre partials
-  int64_t red;               // reduction scratch
+  int64_t red;               // reduction scratch (matrix temporaries)
+  int64_t phs;               // [64] scalars passed between the phase kernels
   int64_t cup;               // [ndetu0 + ndetu1] det-coefficient sums
   int64_t qd;                // [G][4M] quadrature-point distances
+  int64_t vnl;               // [N*M][QMCW_MAX_LPOT+1] nonlocal channel values
+  int64_t gproj;             // [N][NB] G(:,i)^T coef of the occupied orbitals
   int64_t mv;                // pending one-electron move buffers
   int64_t gvalue, gjas, d2g, dpe; // Jastrow parameter derivatives
   int64_t dpsi, deloc;       // [nparm]

@@ -41,8 +41,38 @@ struct qmcw_handle {
   std::vector<int> csf_det_off, csf_det;
   std::vector<double> csf_det_coef;
   int nparm = 0;
+  // event timing of the device driver
+  int prof = 0;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_phase;
+  size_t ev_used = 0;
   std::string err;
 };
+
+// record a start/stop event pair around one phase (profiling only)
+static hipEvent_t prof_event(qmcw_handle *h) {
+  if (h->ev_used >= h->ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->ev.push_back(e);
+    h->ev_phase.push_back(-1);
+  }
+  return h->ev[h->ev_used++];
+}
+static void prof_begin(qmcw_handle *h, int phase) {
+  if (!h->prof) return;
+  hipEvent_t e = prof_event(h);
+  if (!e) return;
+  h->ev_phase[h->ev_used - 1] = phase;
+  (void)hipEventRecord(e, h->stream);
+}
+static void prof_end(qmcw_handle *h) {
+  if (!h->prof) return;
+  hipEvent_t e = prof_event(h);
+  if (!e) return;
+  h->ev_phase[h->ev_used - 1] = -1;
+  (void)hipEventRecord(e, h->stream);
+}
 
 #define HIPCHK(h, call)                                                        \
   do {                                                                         \
@@ -72,6 +102,8 @@ static int upload(qmcw_handle *h, const V &vec, const typename V::value_type **o
   *out = (const T *)p;
   return QMCW_OK;
 }
+
+extern "C" int64_t qmcw_sizeof_system(void) { return (int64_t)sizeof(qmcw_system); }
 
 extern "C" int32_t qmcw_nterms4(int32_t nord) {
   int i = 0;
@@ -176,6 +208,33 @@ std::vector<double> build_cdet(int ndet, int ncsf, const double *csf_coef, const
 
 }  // namespace
 
+// e-e-n coefficients from CHAMP's term order (jastrow4.f90:225-244) to the
+// kernels' compile-time order (k, then m, then l)
+static std::vector<double> reorder_een(const std::vector<double> &c, int nctype, int nordc) {
+  std::vector<int> champ_k, champ_l, champ_m;
+  for (int n = 2; n <= nordc; n++)
+    for (int k = n - 1; k >= 0; k--) {
+      int l_hi = (k == 0) ? n - k - 2 : n - k;
+      for (int l = l_hi; l >= 0; l--) {
+        int m = (n - k - l) / 2;
+        if (2 * m == n - k - l) { champ_k.push_back(k); champ_l.push_back(l); champ_m.push_back(m); }
+      }
+    }
+  int nt = (int)champ_k.size();
+  std::vector<double> out(std::max<size_t>(c.size(), 1), 0.0);
+  int idx = 0;
+  for (int k = 0; k <= nordc; k++)
+    for (int m = 0; 2 * m <= nordc - k; m++)
+      for (int l = 0; l <= nordc - k - 2 * m; l++) {
+        if (!((k >= 1 || m >= 1) && (l + 2 * m >= 1) && (k + l + 2 * m >= 2))) continue;
+        int t = 0;
+        while (t < nt && !(champ_k[t] == k && champ_l[t] == l && champ_m[t] == m)) t++;
+        for (int it = 0; it < nctype; it++) out[it * nt + idx] = c[it * nt + t];
+        idx++;
+      }
+  return out;
+}
+
 static int set_asymp(qmcw_handle *h, const qmcw_system *s, DevSys &S) {
   double k = s->scalek, ar = 1.0 / k;
   S.asymp_r = ar;
@@ -243,7 +302,7 @@ __global__ void k_gather_full(KArgs A, double *out, int W) {
   const double *ss = A.sc + (int64_t)w * A.SL.stride;
   double *o = out + (int64_t)w * stride;
   o[0] = ws[A.L.logpsid]; o[1] = ws[A.L.sgn]; o[2] = ws[A.L.psij]; o[3] = ws[A.L.eloc];
-  o[4] = ws[A.L.epot]; o[5] = ss[A.SL.red + 60]; o[6] = ss[A.SL.red + 61]; o[7] = ss[A.SL.red + 62];
+  o[4] = ws[A.L.epot]; o[5] = ss[A.SL.phs + 60]; o[6] = ss[A.SL.phs + 61]; o[7] = ss[A.SL.phs + 62];
   for (int q = 0; q < 3 * N; q++) o[8 + q] = ss[A.SL.vj + q] + ss[A.SL.vd + q];
   for (int i = 0; i < N; i++) o[8 + 3 * N + i] = ss[A.SL.divv + i];
 }
@@ -266,9 +325,17 @@ template <int G>
 static void launch_eval(qmcw_handle *h) {
   dim3 grid(h->Wp / (kWave / G)), block(kWave);
   if (h->precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_eval_full<double, G>), grid, block, 0, h->stream, h->A);
+    hipLaunchKernelGGL((k_full_jas<double, G>), grid, block, 0, h->stream, h->A);
   else
-    hipLaunchKernelGGL((k_eval_full<float, G>), grid, block, 0, h->stream, h->A);
+    hipLaunchKernelGGL((k_full_jas<float, G>), grid, block, 0, h->stream, h->A);
+  hipLaunchKernelGGL((k_full_det<G>), grid, block, 0, h->stream, h->A);
+  if (h->A.S.nloc > 0) {
+    if (h->precision == QMCW_FP64)
+      hipLaunchKernelGGL((k_full_nlpp<double, G>), grid, block, 0, h->stream, h->A);
+    else
+      hipLaunchKernelGGL((k_full_nlpp<float, G>), grid, block, 0, h->stream, h->A);
+  }
+  hipLaunchKernelGGL((k_full_fin<G>), grid, block, 0, h->stream, h->A);
 }
 template <int G>
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
@@ -309,8 +376,8 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (s->basis_type != QMCW_BASIS_SLATER && s->basis_type != QMCW_BASIS_GAUSSIAN)
     return fail(nullptr, QMCW_ERR_ARG, "basis_type must be Slater or Gaussian");
   if (s->isc != 2 && s->isc != 4) return fail(nullptr, QMCW_ERR_ARG, "isc must be 2 or 4");
-  if (s->nordc > 12 || s->norda > QMCW_MAX_ORD || s->nordb > QMCW_MAX_ORD)
-    return fail(nullptr, QMCW_ERR_ARG, "Jastrow order too large");
+  if (s->nordc > 7 || s->norda > QMCW_MAX_ORD || s->nordb > QMCW_MAX_ORD)
+    return fail(nullptr, QMCW_ERR_ARG, "Jastrow order too large (nordc <= 7)");
   if (s->nloc != 0 && (s->nloc != 6 || s->nquad != 12))
     return fail(nullptr, QMCW_ERR_ARG, "pseudopotential must be nloc=6 with nquad=12");
   if (precision < QMCW_FP64 || precision > QMCW_MIXED_BF16)
@@ -368,28 +435,29 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
         return fail(nullptr, QMCW_ERR_ARG, "basis function with n < l + 1");
       }
       int p0 = s->rad_prim_off[r], p1 = s->rad_prim_off[r + 1];
+      // slot of m in a complete shell: m = 0, 1, -1, 2, -2, ...
+      const int slot = m > 0 ? 2 * m - 1 : (m < 0 ? -2 * m : 0);
       int found = -1;
       for (int sh = first_shell; sh < (int)sh_l.size() && found < 0; sh++) {
-        if (sh_l[sh] != l || sh_a[sh] != n - 1 - l || sh_nm[sh] >= 2 * l + 1) continue;
+        if (sh_l[sh] != l || sh_a[sh] != n - 1 - l) continue;
+        if (sh_bas[sh * kMaxShellM + slot] >= 0) continue;
         int np = sh_prim_off[sh + 1] - sh_prim_off[sh];
         if (np != p1 - p0) continue;
         bool same = true;
         for (int p = 0; p < np; p++)
           if (pz[sh_prim_off[sh] + p] != s->prim_zeta[p0 + p] ||
               pc[sh_prim_off[sh] + p] != s->prim_coef[p0 + p]) same = false;
-        bool mdup = false;
-        for (int qq = 0; qq < sh_nm[sh]; qq++) if (sh_m[sh * kMaxShellM + qq] == m) mdup = true;
-        if (same && !mdup) found = sh;
+        if (same) found = sh;
       }
       if (found < 0) {
         sh_cent.push_back(ic); sh_l.push_back(l); sh_a.push_back(n - 1 - l); sh_nm.push_back(0);
         for (int p = p0; p < p1; p++) { pz.push_back(s->prim_zeta[p]); pc.push_back(s->prim_coef[p]); }
         sh_prim_off.push_back((int)pz.size());
-        for (int qq = 0; qq < kMaxShellM; qq++) { sh_m.push_back(0); sh_bas.push_back(0); }
+        for (int qq = 0; qq < kMaxShellM; qq++) { sh_m.push_back(0); sh_bas.push_back(-1); }
         found = (int)sh_l.size() - 1;
       }
-      sh_m[found * kMaxShellM + sh_nm[found]] = m;
-      sh_bas[found * kMaxShellM + sh_nm[found]] = nbas + (q - b0);
+      sh_m[found * kMaxShellM + slot] = m;
+      sh_bas[found * kMaxShellM + slot] = nbas + (q - b0);
       sh_nm[found]++;
     }
     nbas += b1 - b0;
@@ -445,7 +513,8 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   h->b.assign(s->b, s->b + s->nspin2b * (s->nordb + 1));
   h->c.assign(s->c, s->c + std::max(1, s->nctype * S.nterms));
   set_asymp(h, s, S);
-  UP(h->a4, S.a4); UP(h->b, S.b); UP(h->c, S.c); UP(h->asymp_jasa, S.asymp_jasa);
+  std::vector<double> c_ord = reorder_een(h->c, s->nctype, s->nordc);
+  UP(h->a4, S.a4); UP(h->b, S.b); UP(c_ord, S.c); UP(h->asymp_jasa, S.asymp_jasa);
   h->d_a4 = const_cast<double *>(S.a4); h->d_b = const_cast<double *>(S.b);
   h->d_c = const_cast<double *>(S.c); h->d_asymp = const_cast<double *>(S.asymp_jasa);
   std::vector<int> klm;
@@ -486,8 +555,9 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     if (g == 8 || g == 16 || g == 32 || g == 64) G = g;
   }
   h->G = G;
-  int wpb = kWave / G;
-  h->Wp = ((nwalk + wpb - 1) / wpb) * wpb;
+  // padded walker count: a multiple of the walkers per wavefront of every
+  // group width (8 for G = 8), so that the autotuner can switch G freely
+  h->Wp = ((nwalk + 7) / 8) * 8;
   h->chunk = std::max(1, std::min(N, 32768 / (NB * 5)));
   h->A.chunk = h->chunk;
   h->A.full_orb = 0;
@@ -519,7 +589,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   ML.stride = o;
   SLayout &SL = h->A.SL;
   o = 0;
-  SL.phi = take((int64_t)std::max(h->chunk, G) * NB * 5);
+  SL.phi = take((int64_t)h->chunk * NB * 5);
   SL.orb = take(h->A.full_orb ? (int64_t)N * NO * 5 : 1);
   SL.g[0] = take(D.nocc[0] * D.ns[0]); SL.g[1] = take(D.nocc[1] * D.ns[1]);
   SL.tfull[0] = take(1); SL.tfull[1] = take(1);
@@ -528,7 +598,10 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SL.pnl_c = take(1);
   SL.red = take(std::max(std::max(64, nsm * nsm), noccm * nsm) + 64);
   SL.cup = take(D.ndetu[0] + D.ndetu[1]);
-  SL.qd = take((int64_t)G * 4 * M);
+  SL.phs = take(64);
+  SL.qd = take((int64_t)kWave * 4 * M);
+  SL.vnl = take((int64_t)N * M * (QMCW_MAX_LPOT + 1));
+  SL.gproj = take((int64_t)N * NB);
   SL.mv = take(ML.stride);
   SL.gvalue = take(1); SL.gjas = take(1); SL.d2g = take(1); SL.dpe = take(1);
   SL.dpsi = take(1); SL.deloc = take(1);
@@ -566,6 +639,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
 extern "C" int qmcw_destroy(qmcw_handle *h) {
   if (!h) return QMCW_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->trace) (void)hipFree(h->trace);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -614,7 +688,10 @@ extern "C" int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b
     HIPCHK(h, hipMemcpy(h->d_a4, h->a4.data(), h->a4.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_b, h->b.data(), h->b.size() * 8, hipMemcpyHostToDevice));
   }
-  if (c) HIPCHK(h, hipMemcpy(h->d_c, h->c.data(), h->c.size() * 8, hipMemcpyHostToDevice));
+  if (c) {
+    std::vector<double> c_ord = reorder_een(h->c, S.nctype, S.nordc);
+    HIPCHK(h, hipMemcpy(h->d_c, c_ord.data(), c_ord.size() * 8, hipMemcpyHostToDevice));
+  }
   if (csf_coef) {
     h->csf_coef.assign(csf_coef, csf_coef + h->ncsf);
     h->cdet = build_cdet(h->ndet, h->ncsf, h->csf_coef.data(), h->csf_det_off.data(),
@@ -760,13 +837,20 @@ static int vmc_steps(qmcw_handle *h, int nstep, double delta, double fbias, bool
   int N = h->A.S.N;
   DISPATCH_G(h, launch_eval);
   for (int is = 0; is < nstep; is++) {
-    if (h->A.S.nloc > 0)
+    if (h->A.S.nloc > 0) {
+      prof_begin(h, 2);
       hipLaunchKernelGGL(k_rotqua, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, h->A);
+      prof_end(h);
+    }
+    prof_begin(h, 1);
     for (int ie = 0; ie < N; ie++) {
       DISPATCH_G(h, launch_ratio, ie, 1, (const double *)nullptr, delta, fbias, (double *)nullptr);
       DISPATCH_G(h, launch_accept, ie, 1, (const uint8_t *)nullptr, delta, fbias, h->nacc);
     }
+    prof_end(h);
+    prof_begin(h, 0);
     DISPATCH_G(h, launch_eval);
+    prof_end(h);
     if (trace)
       hipLaunchKernelGGL(k_trace_eloc, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->A,
                          h->trace, is, h->W);
@@ -799,6 +883,76 @@ extern "C" int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double 
 extern "C" int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta, double fbias) {
   if (!h || nstep < 0 || delta <= 0) return QMCW_ERR_ARG;
   return vmc_steps(h, nstep, delta, fbias, false);
+}
+
+extern "C" int qmcw_profile(qmcw_handle *h, int32_t enable) {
+  if (!h) return QMCW_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->prof = enable != 0;
+  h->ev_used = 0;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count) {
+  if (!h || !ms || !count) return QMCW_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int p = 0; p < 3; p++) { ms[p] = 0.0; count[p] = 0; }
+  for (size_t i = 0; i + 1 < h->ev_used; i += 2) {
+    int ph = h->ev_phase[i];
+    if (ph < 0 || ph > 2) continue;
+    float t = 0.0f;
+    HIPCHK(h, hipEventElapsedTime(&t, h->ev[i], h->ev[i + 1]));
+    ms[ph] += t;
+    count[ph] += 1;
+  }
+  return QMCW_OK;
+}
+
+// Group-width autotuner: times the full evaluation plus one ratio launch per
+// electron (mode 0 at the current positions; neither changes the walkers or
+// their random streams) for each admissible G and keeps the fastest.
+extern "C" int qmcw_autotune(qmcw_handle *h, int32_t *chosen) {
+  if (!h) return QMCW_ERR_ARG;
+  const int N = h->A.S.N;
+  int cand[4] = {8, 16, 32, 64};
+  double best = 1e30;
+  int bestG = h->G;
+  double *xin = h->io + (size_t)h->Wp * 8;
+  // current positions of electron ie as the proposal
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  for (int c = 0; c < 4; c++) {
+    int G = cand[c];
+    if (G > 16 && N <= 8) continue;      // lanes would mostly idle
+    if (G < 16 && N > 48) continue;
+    h->G = G;
+    double tot = 0.0;
+    for (int rep = 0; rep < 2; rep++) {
+      HIPCHK(h, hipEventRecord(e0, h->stream));
+      DISPATCH_G(h, launch_eval);
+      for (int ie = 0; ie < N; ie++) {
+        hipLaunchKernelGGL(k_copy_strided, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, xin,
+                           (int64_t)3, (const double *)(h->A.st + h->A.L.x + 3 * ie),
+                           (int64_t)h->A.L.stride, 3, h->Wp);
+        DISPATCH_G(h, launch_ratio, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
+      }
+      HIPCHK(h, hipEventRecord(e1, h->stream));
+      HIPCHK(h, hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+      if (rep == 1) tot = ms;   // first repetition warms up
+    }
+    if (tot < best) { best = tot; bestG = G; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->G = bestG;
+  // restore a consistent resident state with the chosen width
+  DISPATCH_G(h, launch_eval);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (chosen) *chosen = bestG;
+  return QMCW_OK;
 }
 
 extern "C" int qmcw_get_eloc(const qmcw_handle *hc, double *eloc) {

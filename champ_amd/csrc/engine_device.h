@@ -64,6 +64,103 @@ __device__ __forceinline__ void scaled(int isc, T k, T r, T &rr, T &d1, T &d2) {
   }
 }
 
+// e-e-n polynomial sum_t c_t U^k (A^(l+m) B^m + A^m B^(l+m)) with the terms
+// visited in a compile-time order (k, then m, then l); the coefficients are
+// stored in that order (DevSys::c).  Every power index is a constant after
+// unrolling, so the power tables live in registers.
+#define QMCW_EEN_TERM(k, l, m) (((k) >= 1 || (m) >= 1) && ((l) + 2 * (m) >= 1) && ((k) + (l) + 2 * (m) >= 2))
+
+template <typename T, int ORD>
+__device__ __forceinline__ T een_value(T U, T A, T B, const double *__restrict__ c) {
+  T up[ORD + 1], ap[ORD + 1], bp[ORD + 1];
+  up[0] = ap[0] = bp[0] = T(1);
+#pragma unroll
+  for (int q = 1; q <= ORD; q++) { up[q] = up[q - 1] * U; ap[q] = ap[q - 1] * A; bp[q] = bp[q - 1] * B; }
+  T f = 0;
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k <= ORD; k++)
+#pragma unroll
+    for (int m = 0; 2 * m <= ORD - k; m++)
+#pragma unroll
+      for (int l = 0; l <= ORD - k - 2 * m; l++)
+        if (QMCW_EEN_TERM(k, l, m)) {
+          f += T(c[idx]) * up[k] * (ap[l + m] * bp[m] + ap[m] * bp[l + m]);
+          idx++;
+        }
+  return f;
+}
+
+// value and the partial derivatives wrt U, A, B (first and second, and the
+// mixed U-A, U-B ones) of the e-e-n polynomial
+template <typename T>
+struct EenD { T f, fU, fUU, fA, fAA, fB, fBB, fUA, fUB; };
+
+template <typename T, int ORD>
+__device__ __forceinline__ void een_derivs(T U, T A, T B, const double *__restrict__ c, EenD<T> &o) {
+  T up[ORD + 1], ap[ORD + 1], bp[ORD + 1];
+  up[0] = ap[0] = bp[0] = T(1);
+#pragma unroll
+  for (int q = 1; q <= ORD; q++) { up[q] = up[q - 1] * U; ap[q] = ap[q - 1] * A; bp[q] = bp[q - 1] * B; }
+  o.f = o.fU = o.fUU = o.fA = o.fAA = o.fB = o.fBB = o.fUA = o.fUB = T(0);
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k <= ORD; k++)
+#pragma unroll
+    for (int m = 0; 2 * m <= ORD - k; m++)
+#pragma unroll
+      for (int l = 0; l <= ORD - k - 2 * m; l++)
+        if (QMCW_EEN_TERM(k, l, m)) {
+          const int lm = l + m;
+          T cl = T(c[idx]);
+          idx++;
+          T P = ap[lm] * bp[m] + ap[m] * bp[lm];
+          T PA = (lm ? T(lm) * ap[lm > 0 ? lm - 1 : 0] * bp[m] : T(0)) +
+                 (m ? T(m) * ap[m > 0 ? m - 1 : 0] * bp[lm] : T(0));
+          T PB = (lm ? T(lm) * bp[lm > 0 ? lm - 1 : 0] * ap[m] : T(0)) +
+                 (m ? T(m) * bp[m > 0 ? m - 1 : 0] * ap[lm] : T(0));
+          T PAA = (lm > 1 ? T(lm * (lm - 1)) * ap[lm > 1 ? lm - 2 : 0] * bp[m] : T(0)) +
+                  (m > 1 ? T(m * (m - 1)) * ap[m > 1 ? m - 2 : 0] * bp[lm] : T(0));
+          T PBB = (lm > 1 ? T(lm * (lm - 1)) * bp[lm > 1 ? lm - 2 : 0] * ap[m] : T(0)) +
+                  (m > 1 ? T(m * (m - 1)) * bp[m > 1 ? m - 2 : 0] * ap[lm] : T(0));
+          T Uk = up[k];
+          T Uk1 = k ? T(k) * up[k > 0 ? k - 1 : 0] : T(0);
+          T Uk2 = k > 1 ? T(k * (k - 1)) * up[k > 1 ? k - 2 : 0] : T(0);
+          o.f += cl * Uk * P;
+          o.fU += cl * Uk1 * P;
+          o.fUU += cl * Uk2 * P;
+          o.fA += cl * Uk * PA;
+          o.fAA += cl * Uk * PAA;
+          o.fB += cl * Uk * PB;
+          o.fBB += cl * Uk * PBB;
+          o.fUA += cl * Uk1 * PA;
+          o.fUB += cl * Uk1 * PB;
+        }
+}
+
+template <typename T>
+__device__ __forceinline__ T een_value_rt(int ord, T U, T A, T B, const double *c) {
+  switch (ord) {
+    case 2: return een_value<T, 2>(U, A, B, c);
+    case 3: return een_value<T, 3>(U, A, B, c);
+    case 4: return een_value<T, 4>(U, A, B, c);
+    case 5: return een_value<T, 5>(U, A, B, c);
+    case 6: return een_value<T, 6>(U, A, B, c);
+    default: return een_value<T, 7>(U, A, B, c);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void een_derivs_rt(int ord, T U, T A, T B, const double *c, EenD<T> &o) {
+  switch (ord) {
+    case 2: een_derivs<T, 2>(U, A, B, c, o); break;
+    case 3: een_derivs<T, 3>(U, A, B, c, o); break;
+    case 4: een_derivs<T, 4>(U, A, B, c, o); break;
+    case 5: een_derivs<T, 5>(U, A, B, c, o); break;
+    case 6: een_derivs<T, 6>(U, A, B, c, o); break;
+    default: een_derivs<T, 7>(U, A, B, c, o); break;
+  }
+}
+
 // Pair term of electrons (i, j), i > j: e-e Pade + polynomial and e-e-n
 // polynomial.  Returns value; gi/gj gradient wrt r_i and r_j; li/lj the
 // laplacian wrt r_i and r_j.
@@ -120,9 +217,6 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
   U = T(1) - k * U;
   Ud1 = -k * Ud1;
   Ud2 = -k * Ud2;
-  T upow[kMaxTerms / 4 + 4];
-  upow[0] = T(1);
-  for (int q = 1; q <= S.nordc; q++) upow[q] = upow[q - 1] * U;
   for (int ic = 0; ic < S.M; ic++) {
     int it = S.iwctype[ic];
     T ri = (T)ri_v[ic], rj = (T)rj_v[ic];
@@ -131,37 +225,10 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
     scaled<T>(S.isc, k, rj, B, Bd1, Bd2);
     A = T(1) - k * A; Ad1 = -k * Ad1; Ad2 = -k * Ad2;
     B = T(1) - k * B; Bd1 = -k * Bd1; Bd2 = -k * Bd2;
-    T ap[kMaxTerms / 4 + 4], bp[kMaxTerms / 4 + 4];
-    ap[0] = bp[0] = T(1);
-    for (int q = 1; q <= S.nordc; q++) { ap[q] = ap[q - 1] * A; bp[q] = bp[q - 1] * B; }
-    T fc = 0, fU = 0, fUU = 0, fA = 0, fAA = 0, fB = 0, fBB = 0, fUA = 0, fUB = 0;
-    const double *cc = S.c + it * S.nterms;
-    for (int t = 0; t < S.nterms; t++) {
-      int kk = S.term_klm[3 * t], l = S.term_klm[3 * t + 1], m = S.term_klm[3 * t + 2];
-      T cl = T(cc[t]);
-      if (cl == T(0)) continue;
-      // P = A^(l+m) B^m + A^m B^(l+m)
-      int lm = l + m;
-      T P = ap[lm] * bp[m] + ap[m] * bp[lm];
-      T PA = (lm ? T(lm) * ap[lm - 1] * bp[m] : T(0)) + (m ? T(m) * ap[m - 1] * bp[lm] : T(0));
-      T PB = (lm ? T(lm) * bp[lm - 1] * ap[m] : T(0)) + (m ? T(m) * bp[m - 1] * ap[lm] : T(0));
-      T PAA = (lm > 1 ? T(lm * (lm - 1)) * ap[lm - 2] * bp[m] : T(0)) +
-              (m > 1 ? T(m * (m - 1)) * ap[m - 2] * bp[lm] : T(0));
-      T PBB = (lm > 1 ? T(lm * (lm - 1)) * bp[lm - 2] * ap[m] : T(0)) +
-              (m > 1 ? T(m * (m - 1)) * bp[m - 2] * ap[lm] : T(0));
-      T Uk = upow[kk];
-      T Uk1 = kk ? T(kk) * upow[kk - 1] : T(0);
-      T Uk2 = kk > 1 ? T(kk * (kk - 1)) * upow[kk - 2] : T(0);
-      fc += cl * Uk * P;
-      fU += cl * Uk1 * P;
-      fUU += cl * Uk2 * P;
-      fA += cl * Uk * PA;
-      fAA += cl * Uk * PAA;
-      fB += cl * Uk * PB;
-      fBB += cl * Uk * PBB;
-      fUA += cl * Uk1 * PA;
-      fUB += cl * Uk1 * PB;
-    }
+    EenD<T> e;
+    een_derivs_rt<T>(S.nordc, U, A, B, S.c + it * S.nterms, e);
+    T fc = e.f, fU = e.fU, fUU = e.fUU, fA = e.fA, fAA = e.fAA, fB = e.fB, fBB = e.fBB,
+      fUA = e.fUA, fUB = e.fUB;
     // chain rule to unscaled distances
     T gU = fU * Ud1 / rij;
     T gUU = fUU * Ud1 * Ud1 + fU * Ud2;
@@ -180,6 +247,60 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
     o.li += T(2) * (gU + gA) + gUU + gAA + gUA * cI;
     o.lj += T(2) * (gU + gB) + gUU + gBB + gUB * cJ;
   }
+}
+
+// value only of the pair term (pseudopotential quadrature)
+template <typename T>
+__device__ T jas_pair_val(const DevSys &S, bool parallel_spin, T rij,
+                          const double *ri_v, const double *rj_v) {
+  const T k = T(S.scalek);
+  T sspinn = T(1);
+  int isb = 0;
+  double asymp_b = S.asymp_jasb0;
+  if (parallel_spin) {
+    if (S.nspin2b == 2) isb = 1;
+    else if (S.nocuspb == 0) sspinn = T(0.5);
+    asymp_b = S.asymp_jasb1;
+  }
+  const double *b = S.b + isb * (S.nordb + 1);
+  T u, d1, d2;
+  scaled<T>(S.isc, k, rij, u, d1, d2);
+  T f = sspinn * T(b[0]) * u / (T(1) + T(b[1]) * u) - T(asymp_b);
+  T up = u;
+  for (int iord = 2; iord <= S.nordb; iord++) {
+    up *= u;
+    f += T(b[iord]) * up;
+  }
+  if (S.nordc <= 1) return f;
+  T U = T(1) - k * u;   // e-e-n variable (same scaling as the e-e term)
+  for (int ic = 0; ic < S.M; ic++) {
+    int it = S.iwctype[ic];
+    T A, B, t1, t2;
+    scaled<T>(S.isc, k, (T)ri_v[ic], A, t1, t2);
+    scaled<T>(S.isc, k, (T)rj_v[ic], B, t1, t2);
+    f += een_value_rt<T>(S.nordc, U, T(1) - k * A, T(1) - k * B, S.c + it * S.nterms);
+  }
+  return f;
+}
+
+template <typename T>
+__device__ T jas_en_val(const DevSys &S, const double *ri_v) {
+  const T k = T(S.scalek);
+  T f = 0;
+  for (int ic = 0; ic < S.M; ic++) {
+    int it = S.iwctype[ic];
+    const double *a = S.a4 + it * (S.norda + 1);
+    T u, d1, d2;
+    scaled<T>(S.isc, k, (T)ri_v[ic], u, d1, d2);
+    T fe = T(a[0]) * u / (T(1) + T(a[1]) * u) - T(S.asymp_jasa[it]);
+    T up = u;
+    for (int iord = 2; iord <= S.norda; iord++) {
+      up *= u;
+      fe += T(a[iord]) * up;
+    }
+    f += fe;
+  }
+  return f;
 }
 
 // e-n Pade + polynomial of one electron summed over centres
@@ -311,6 +432,26 @@ __device__ __forceinline__ void solid_harm(int l, T x, T y, T z, T *Sv, T (*Sg)[
 
 __device__ __forceinline__ int m_slot(int m) { return m > 0 ? 2 * m - 1 : (m < 0 ? -2 * m : 0); }
 
+template <typename T, int L>
+__device__ __forceinline__ void shell_write(T x, T y, T z, T f, T fr, T lapfac, int want,
+                                            const int *bas, T *phi) {
+  T Sv[2 * L + 1], Sg[2 * L + 1][3];
+  solid_harm<T>(L, x, y, z, Sv, Sg);
+#pragma unroll
+  for (int q = 0; q < 2 * L + 1; q++) {
+    int b = bas[q];
+    if (b < 0) continue;
+    T *o = phi + 5 * b;
+    o[0] = Sv[q] * f;
+    if (want) {
+      o[1] = f * Sg[q][0] + Sv[q] * fr * x;
+      o[2] = f * Sg[q][1] + Sv[q] * fr * y;
+      o[3] = f * Sg[q][2] + Sv[q] * fr * z;
+      o[4] = Sv[q] * lapfac;
+    }
+  }
+}
+
 // Evaluate shell s at displacement (x, y, z) from its centre, distance r.
 // want: 0 value only, 1 value + gradient + laplacian.
 template <typename T>
@@ -349,23 +490,53 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
     fr = (T(a) * ra1 * g + ra * g1) / r;
     f2 = (a >= 2 ? T(a * (a - 1)) * ra2 * g : T(0)) + T(2 * a) * ra1 * g1 + ra * g2;
   }
-  T Sv[kMaxShellM], Sg[kMaxShellM][3];
-  solid_harm<T>(l, x, y, z, Sv, Sg);
   // laplacian of S f(r): S (f'' + 2 (l+1) f'/r)
   T lapfac = f2 + T(2 * (l + 1)) * fr;
-  int nm = S.sh_nm[s];
-  for (int q = 0; q < nm; q++) {
-    int sl = m_slot(S.sh_m[s * kMaxShellM + q]);
-    int bidx = S.sh_bas[s * kMaxShellM + q];
-    T *o = phi + 5 * bidx;
-    o[0] = Sv[sl] * f;
-    if (want) {
-      o[1] = f * Sg[sl][0] + Sv[sl] * fr * x;
-      o[2] = f * Sg[sl][1] + Sv[sl] * fr * y;
-      o[3] = f * Sg[sl][2] + Sv[sl] * fr * z;
-      o[4] = Sv[sl] * lapfac;
-    }
+  const int *bas = S.sh_bas + s * kMaxShellM;
+  switch (l) {
+    case 0: shell_write<T, 0>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 1: shell_write<T, 1>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 2: shell_write<T, 2>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 3: shell_write<T, 3>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    default: shell_write<T, 4>(x, y, z, f, fr, lapfac, want, bas, phi); break;
   }
+}
+
+// sum over the shell's components of gp[basis] * phi_basis at (x, y, z).
+// Shells are complete: slot q holds m = 0, 1, -1, 2, -2, ...; an absent
+// component has basis index -1.
+template <typename T, int L>
+__device__ __forceinline__ T shell_dot_l(T x, T y, T z, const int *bas, const T *gp) {
+  T Sv[2 * L + 1], Sg[2 * L + 1][3];
+  solid_harm<T>(L, x, y, z, Sv, Sg);
+  T acc = 0;
+#pragma unroll
+  for (int q = 0; q < 2 * L + 1; q++) {
+    int b = bas[q];
+    if (b >= 0) acc += gp[b] * Sv[q];
+  }
+  return acc;
+}
+template <typename T>
+__device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const T *gp) {
+  int l = S.sh_l[s], a = S.sh_a[s];
+  const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
+  T g = 0;
+  for (int p = S.sh_prim_off[s]; p < S.sh_prim_off[s + 1]; p++) {
+    T zeta = T(S.prim_zeta[p]), d = T(S.prim_coef[p]);
+    g += d * exp(gauss ? -zeta * r * r : -zeta * r);
+  }
+  for (int q = 0; q < a; q++) g *= r;
+  const int *bas = S.sh_bas + s * kMaxShellM;
+  T acc;
+  switch (l) {
+    case 0: acc = shell_dot_l<T, 0>(x, y, z, bas, gp); break;
+    case 1: acc = shell_dot_l<T, 1>(x, y, z, bas, gp); break;
+    case 2: acc = shell_dot_l<T, 2>(x, y, z, bas, gp); break;
+    case 3: acc = shell_dot_l<T, 3>(x, y, z, bas, gp); break;
+    default: acc = shell_dot_l<T, 4>(x, y, z, bas, gp); break;
+  }
+  return acc * g;
 }
 
 // ---------------------------------------------------------------------------

@@ -23,6 +23,12 @@ struct KArgs {
   int chunk;         // electrons per basis chunk in the full evaluation
 };
 
+// occupancy hint of the heavy per-walker kernels (waves per SIMD)
+#ifndef QMCW_WPE
+#define QMCW_WPE 2
+#endif
+#define QMCW_HEAVY __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(QMCW_WPE, 8)))
+
 #define QMCW_GROUP_SETUP                                         \
   const int lane = threadIdx.x % G;                              \
   const int grp = threadIdx.x / G;                               \
@@ -183,15 +189,24 @@ __device__ void group_g(const DevSys &S, int s, const double *ai,
 // ---------------------------------------------------------------------------
 // Full evaluation (hpsi): distances, Jastrow, orbitals, determinants (AMF),
 // pseudopotential, local energy.  Refreshes the walker's resident state.
+// The full evaluation runs as four kernels (distances + Jastrow; orbitals +
+// determinants; pseudopotential; local energy) so that each carries fewer
+// live registers; scalars pass through the walker's reduction scratch:
+// phs[50] pe_en, [51] pe_ee, [52] ln J, [53] lap ln J, [54] lap ln D,
+// [55] local pseudopotential, [56] nonlocal pseudopotential (SLayout::phs).
 template <typename T, int G>
-__global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
+__global__ QMCW_HEAVY void k_full_jas(KArgs A) {
   QMCW_GROUP_SETUP
-  __shared__ int s_ipiv[kWave / G][128];
   const int N = S.N, M = S.M, NO = S.NO;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
   const double *x = ws + L.x;
   double *rven = ws + L.rvec_en, *ren = ws + L.r_en, *rvee = ws + L.rvec_ee, *ree = ws + L.r_ee;
+  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double *vj = ss + SL.vj, *divv = ss + SL.divv, *vd = ss + SL.vd;
+  double *red = ss + SL.phs;
+  (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
+  (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
   // ---- distances (qmc/vmc/distances.f90:1-233)
   double pe_en = 0.0, pe_ee = 0.0;
   for (int p = lane; p < N * M; p += G) {
@@ -223,7 +238,6 @@ __global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
   pe_ee = gsum<G>(pe_ee);
   __syncthreads();
   // ---- Jastrow pair arrays (qmc/vmc/jastrow4.f90)
-  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
   double fsum = 0.0;
   for (int p = lane; p < N * N; p += G) {
     int i = p / N, j = p % N;
@@ -254,7 +268,6 @@ __global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
   }
   fsum = gsum<G>(fsum);
   __syncthreads();
-  double *vj = ss + SL.vj, *divv = ss + SL.divv;
   double d2j = 0.0;
   for (int i = lane; i < N; i += G) {
     double gx = 0, gy = 0, gz = 0, dv = lapj[i * N + i], lt = lapj[i * N + i];
@@ -273,7 +286,26 @@ __global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
     d2j += lt;
   }
   d2j = gsum<G>(d2j);
-  if (lane == 0) ws[L.fsumo] = fsum;
+  if (lane == 0) {
+    ws[L.fsumo] = fsum;
+    red[50] = pe_en; red[51] = pe_ee; red[52] = fsum; red[53] = d2j;
+  }
+}
+
+template <int G>
+__global__ QMCW_HEAVY void k_full_det(KArgs A) {
+  QMCW_GROUP_SETUP
+  const int N = S.N, M = S.M, NO = S.NO;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const double *x = ws + L.x;
+  double *rven = ws + L.rvec_en, *ren = ws + L.r_en, *rvee = ws + L.rvec_ee, *ree = ws + L.r_ee;
+  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double *vj = ss + SL.vj, *divv = ss + SL.divv, *vd = ss + SL.vd;
+  double *red = ss + SL.phs;
+  (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
+  (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
+  __shared__ int s_ipiv[kWave / G][128];
   // ---- orbitals (basis_fns + orbitals_loc_ana), chunks of electrons
   double *phi = ss + SL.phi;
   double *oc = ws + L.oc;
@@ -374,7 +406,6 @@ __global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
       group_g<G>(S, s, ws + L.ai[s], ws + L.tocc[s], ws + L.y[s], ss + SL.g[s], tmp, lane);
   }
   // ---- grad / laplacian of ln D (qmc/vmc/determinant.f90:99-152)
-  double *vd = ss + SL.vd;
   double d2d = 0.0;
   for (int i = lane; i < N; i += G) {
     int s = i < S.nup ? 0 : 1, ie = s ? i - S.nup : i;
@@ -392,117 +423,120 @@ __global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
     d2d += dvd;
   }
   d2d = gsum<G>(d2d);
-  __syncthreads();
-  // ---- pseudopotential (qmc/vmc/nonloc_pot.f90, nonloc.f90)
-  double pe = S.pecent + pe_en + pe_ee;
-  double vpsp = 0.0;
+  if (lane == 0) red[54] = d2d;
+  // projected coefficients gp_i = G(:,i)^T coef_occ of the pseudopotential
   if (S.nloc > 0) {
-    double ploc = 0.0;
+    double *gp = ss + SL.gproj;       // [N][NB]
+    for (int t = lane; t < N * S.NB; t += G) {
+      int i = t / S.NB, b = t % S.NB;
+      int s = i < S.nup ? 0 : 1, ie = s ? i - S.nup : i, ns = S.ns[s];
+      const double *g = ss + SL.g[s];
+      double acc = 0.0;
+      for (int j = 0; j < S.nocc[s]; j++) acc += g[j * ns + ie] * S.coefT[(int64_t)b * NO + S.occ[s][j]];
+      gp[t] = acc;
+    }
+  }
+}
+
+template <typename T, int G>
+__global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
+  QMCW_GROUP_SETUP
+  const int N = S.N, M = S.M, NO = S.NO;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const double *x = ws + L.x;
+  double *rven = ws + L.rvec_en, *ren = ws + L.r_en, *rvee = ws + L.rvec_ee, *ree = ws + L.r_ee;
+  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double *vj = ss + SL.vj, *divv = ss + SL.divv, *vd = ss + SL.vd;
+  double *red = ss + SL.phs;
+  (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
+  (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
+  // ---- pseudopotential (qmc/vmc/nonloc_pot.f90, nonloc.f90).
+  // V_nl = sum_G psp_nonloc_orb is linear in the orbitals, so per quadrature
+  // point only the determinant ratio sum_j G(j,i) phi_occj = sum_b gp_i[b]
+  // phi_b is needed, with gp_i = G(:,i)^T coef_occ formed once per electron.
+  double vpsp = 0.0, ploc = 0.0;
+  if (S.nloc > 0) {
+    double *vnl = ss + SL.vnl;        // [N*M][QMCW_MAX_LPOT + 1] (last: in range)
     for (int p = lane; p < N * M; p += G) {
       int ic = p % M, ict = S.iwctype[ic];
       double r = fmax(1e-10, ren[p]);
-      ploc += ecp_radial(S, r, S.lpotp1[ict] - 1, ict) - S.znuc[ict] / r;
+      double *v = vnl + p * (QMCW_MAX_LPOT + 1);
+      bool inr = false;
+      for (int l = 1; l <= S.npotd[ict]; l++) {
+        double vl = ecp_radial(S, r, l - 1, ict);
+        if (l == S.lpotp1[ict]) { ploc += vl - S.znuc[ict] / r; vl = 0.0; }
+        else if (fabs(vl) > 1e-4) inr = true;
+        v[l - 1] = vl;
+      }
+      v[QMCW_MAX_LPOT] = inr ? 1.0 : 0.0;
     }
     ploc = gsum<G>(ploc);
-    pe += ploc;
-    // psp_nonloc_orb restricted to the occupied orbitals of each spin
-    double *pnl = ss + SL.pnl;   // [N][noccmax] (+ [N][NO] when full_orb)
+    __syncthreads();
+    const double *gp = ss + SL.gproj;
     const double *quad = ws + L.quad;
     const int nq = S.nquad;
-    for (int i = lane; i < N; i += G) {
-      int s = i < S.nup ? 0 : 1, nocc = S.nocc[s];
-      double *prow = pnl + (int64_t)i * noccmax;
-      for (int j = 0; j < noccmax; j++) prow[j] = 0.0;
-      double *prowf = pnl + (int64_t)N * noccmax + (int64_t)i * NO;
-      if (A.full_orb) for (int j = 0; j < NO; j++) prowf[j] = 0.0;
-      double *phq = phi + (int64_t)lane * S.NB * 5;  // lane-private basis values
-      for (int ic = 0; ic < M; ic++) {
-        int ict = S.iwctype[ic];
-        double rei = ren[i * M + ic];
-        double r = fmax(1e-10, rei);
-        double vps[QMCW_MAX_LPOT];
-        bool skip = true;
-        for (int l = 1; l <= S.npotd[ict]; l++) {
-          vps[l - 1] = ecp_radial(S, r, l - 1, ict);
-          if (l == S.lpotp1[ict]) vps[l - 1] -= S.znuc[ict] / r;
-          if (l != S.lpotp1[ict] && fabs(vps[l - 1]) > 1e-4) skip = false;
-        }
-        if (skip) continue;
-        const double *rv0 = rven + (i * M + ic) * 3;
-        for (int iq = 0; iq < nq; iq++) {
-          double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
-          double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
-          double xn[3] = {rei * qx + S.cent[3 * ic], rei * qy + S.cent[3 * ic + 1],
-                          rei * qz + S.cent[3 * ic + 2]};
-          // basis values at the quadrature point
-          for (int sh = 0; sh < S.nshell; sh++) {
-            int jc = S.sh_cent[sh];
-            double dxv = xn[0] - S.cent[3 * jc], dyv = xn[1] - S.cent[3 * jc + 1],
-                   dzv = xn[2] - S.cent[3 * jc + 2];
-            double rr = (jc == ic) ? rei : sqrt(dxv * dxv + dyv * dyv + dzv * dzv);
-            eval_shell<double>(S, sh, dxv, dyv, dzv, rr, 0, phq);
-          }
-          // Jastrow value change of moving electron i to xn
-          double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
-          for (int jc = 0; jc < M; jc++) {
-            double dxv = xn[0] - S.cent[3 * jc], dyv = xn[1] - S.cent[3 * jc + 1],
-                   dzv = xn[2] - S.cent[3 * jc + 2];
-            rvnew[3 * jc] = dxv; rvnew[3 * jc + 1] = dyv; rvnew[3 * jc + 2] = dzv;
-            rnew[jc] = (jc == ic) ? rei : sqrt(dxv * dxv + dyv * dyv + dzv * dzv);
-          }
-          T delta = 0;
-          {
-            T fe, ge[3], le;
-            jas_en<T>(S, rnew, rvnew, fe, ge, le);
-            delta += fe - (T)fso[i * N + i];
-          }
-          for (int jj = 0; jj < N; jj++) {
-            if (jj == i) continue;
-            int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
-            T rv[3];
-            double r2 = 0.0;
-            for (int q = 0; q < 3; q++) {
-              double xi = (hi == i) ? xn[q] : x[3 * hi + q];
-              double xj = (lo == i) ? xn[q] : x[3 * lo + q];
-              rv[q] = (T)(xi - xj);
-              r2 += (xi - xj) * (xi - xj);
-            }
-            const double *rj_ = ren + jj * M, *rvj_ = rven + jj * M * 3;
-            PairOut<T> po;
-            bool par = (hi < S.nup) || (lo >= S.nup);
-            if (hi == i)
-              jas_pair<T>(S, par, (T)sqrt(r2), rv, rnew, rj_, rvnew, rvj_, po);
-            else
-              jas_pair<T>(S, par, (T)sqrt(r2), rv, rj_, rnew, rvj_, rvnew, po);
-            delta += po.f - (T)fso[hi * N + lo];
-          }
-          double vpot = 0.0;
-          for (int l = 1; l <= S.npotd[ict]; l++)
-            if (l != S.lpotp1[ict]) vpot += vps[l - 1] * legendre_w(l, costh);
-          double factor = S.q0[3 * nq + iq] * exp((double)delta) * vpot;
-          for (int j = 0; j < nocc; j++) {
-            int lab = S.occ[s][j];
-            double acc = 0.0;
-            for (int b = 0; b < S.NB; b++) acc += S.coefT[(int64_t)b * NO + lab] * phq[5 * b];
-            prow[j] += factor * acc;
-          }
-          if (A.full_orb)
-            for (int lab = 0; lab < NO; lab++) {
-              double acc = 0.0;
-              for (int b = 0; b < S.NB; b++) acc += S.coefT[(int64_t)b * NO + lab] * phq[5 * b];
-              prowf[lab] += factor * acc;
-            }
-        }
+    for (int t = lane; t < N * M * nq; t += G) {
+      int iq = t % nq, p = t / nq, ic = p % M, i = p / M;
+      const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
+      if (v[QMCW_MAX_LPOT] == 0.0) continue;
+      int ict = S.iwctype[ic];
+      double rei = ren[p];
+      double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
+      const double *rv0 = rven + 3 * p;
+      double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
+      double xn[3] = {rei * qx + S.cent[3 * ic], rei * qy + S.cent[3 * ic + 1],
+                      rei * qz + S.cent[3 * ic + 2]};
+      double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
+      for (int jc = 0; jc < M; jc++) {
+        double dx = xn[0] - S.cent[3 * jc], dy = xn[1] - S.cent[3 * jc + 1], dz = xn[2] - S.cent[3 * jc + 2];
+        rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
+        rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
       }
-      // V_nl contribution of electron i: sum_j G(j, i) pnl(i, j)
-      int ie = s ? i - S.nup : i, ns = S.ns[s];
-      const double *g = ss + SL.g[s];
-      for (int j = 0; j < nocc; j++) vpsp += g[j * ns + ie] * prow[j];
+      // determinant ratio of moving electron i to the point
+      double rd = 0.0;
+      const double *gpi = gp + (int64_t)i * S.NB;
+      for (int sh = 0; sh < S.nshell; sh++) {
+        int jc = S.sh_cent[sh];
+        rd += shell_dot<double>(S, sh, rvnew[3 * jc], rvnew[3 * jc + 1], rvnew[3 * jc + 2], rnew[jc], gpi);
+      }
+      // Jastrow value change
+      T delta = jas_en_val<T>(S, rnew) - (T)fso[i * N + i];
+      for (int jj = 0; jj < N; jj++) {
+        if (jj == i) continue;
+        int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
+        double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
+        double rij = sqrt(dx * dx + dy * dy + dz * dz);
+        bool par = (hi < S.nup) || (lo >= S.nup);
+        delta += jas_pair_val<T>(S, par, (T)rij, rnew, ren + jj * M) - (T)fso[hi * N + lo];
+      }
+      double vpot = 0.0;
+      for (int l = 1; l <= S.npotd[ict]; l++)
+        if (l != S.lpotp1[ict]) vpot += v[l - 1] * legendre_w(l, costh);
+      vpsp += S.q0[3 * nq + iq] * exp((double)delta) * vpot * rd;
     }
     vpsp = gsum<G>(vpsp);
-    pe += vpsp;
   }
+  if (lane == 0) { red[55] = ploc; red[56] = vpsp; }
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void k_full_fin(KArgs A) {
+  QMCW_GROUP_SETUP
+  const int N = S.N, M = S.M, NO = S.NO;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const double *x = ws + L.x;
+  double *rven = ws + L.rvec_en, *ren = ws + L.r_en, *rvee = ws + L.rvec_ee, *ree = ws + L.r_ee;
+  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double *vj = ss + SL.vj, *divv = ss + SL.divv, *vd = ss + SL.vd;
+  double *red = ss + SL.phs;
+  (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
+  (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
   // ---- local energy (qmc/vmc/hpsi.f90:336-344)
+  const double pe_ee = red[51], fsum = red[52], d2j = red[53], d2d = red[54];
+  const double vpsp = S.nloc > 0 ? red[56] : 0.0;
+  const double pe = S.pecent + red[50] + pe_ee + (S.nloc > 0 ? red[55] + vpsp : 0.0);
   double v2 = 0.0;
   for (int i = lane; i < N; i += G)
     for (int q = 0; q < 3; q++) {
@@ -522,9 +556,9 @@ __global__ __launch_bounds__(64) void k_eval_full(KArgs A) {
     ws[L.psi2o] = 2.0 * (lp + fsum);
     ws[L.eloc] = eloc;
     ws[L.epot] = pe;
-    ss[SL.red + 60] = d2lnpsi;
-    ss[SL.red + 61] = vpsp;
-    ss[SL.red + 62] = pe_ee;
+    red[60] = d2lnpsi;
+    red[61] = vpsp;
+    red[62] = pe_ee;
     int fl = 0;
     if (psid == 0.0) fl |= 1;
     if (!isfinite(eloc)) fl |= 2;
@@ -576,7 +610,7 @@ __device__ void group_grad_1e(const KArgs &A, const double *ws, int ie,
 // jastrow4e + orbitals + dete_update + eval_grad into the pending buffers.
 // out[w][8]: ratio, ln|psid_new|, sign, psij_new, grad(3), lap
 template <typename T, int G>
-__global__ __launch_bounds__(64) void k_ratio_1e(KArgs A, int ie, int mode,
+__global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
                                                  const double *xnew_in,
                                                  double delta, double fbias,
                                                  double *out) {

@@ -148,5 +148,19 @@ class Engine:
         self._chk(self.lib.qmcw_get_eloc(self.h, _d(e)))
         return e
 
+    def autotune(self) -> int:
+        g = C.c_int32(0)
+        self._chk(self.lib.qmcw_autotune(self.h, C.byref(g)))
+        return g.value
+
+    def profile(self, enable: bool = True):
+        self._chk(self.lib.qmcw_profile(self.h, int(enable)))
+
+    def profile_get(self):
+        ms, cnt = np.zeros(3), np.zeros(3, np.int64)
+        self._chk(self.lib.qmcw_profile_get(self.h, _d(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(eval_ms=ms[0], eval_n=int(cnt[0]), sweep_ms=ms[1], sweep_n=int(cnt[1]),
+                    rot_ms=ms[2], rot_n=int(cnt[2]))
+
     def sync(self):
         self._chk(self.lib.qmcw_sync(self.h))

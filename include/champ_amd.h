@@ -118,6 +118,9 @@ typedef struct qmcw_system {
   double hb;                /* hbar^2/2m = 0.5 */
 } qmcw_system;
 
+/* sizeof(qmcw_system) as compiled into the library (ABI check) */
+int64_t qmcw_sizeof_system(void);
+
 /* number of e-e-n terms for order nordc (qmc/vmc/jastrow4.f90:495-516) */
 int32_t qmcw_nterms4(int32_t nordc);
 
@@ -198,6 +201,20 @@ int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta,
 
 /* Read back per-walker local energies of the last evaluation. */
 int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
+
+/* Kernel timing of the device VMC driver with HIP events recorded on the
+   handle's stream (no counterpart in the reference; for measurement).
+   enable != 0 clears and starts accumulation.  qmcw_profile_get returns, per
+   phase (0 full evaluation, 1 the one-electron sweep of ratio + accept
+   kernels, 2 quadrature rotation), the summed milliseconds and the number
+   of timed launches of that phase. */
+int qmcw_profile(qmcw_handle *h, int32_t enable);
+int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count);
+
+/* Choose the lane-group width (walkers per wavefront) by timing the
+   candidates on the resident walkers with side-effect-free launches (no
+   counterpart in the reference).  chosen (may be NULL) receives the width. */
+int qmcw_autotune(qmcw_handle *h, int32_t *chosen);
 
 /* Wait for all work queued on the handle's stream. */
 int qmcw_sync(qmcw_handle *h);
