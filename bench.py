@@ -10,7 +10,11 @@ electrons (one-electron ratio + accept) and the full evaluation (local energy
 including the pseudopotential quadrature).  Inputs are resident in HBM before
 timing; nothing is copied to the host inside the timed region.
 
-Prints one JSON line (see README/DESIGN for the fields).
+The timed run records no events.  A second, shorter run of the same engine
+records HIP events around every kernel launch on the engine's stream
+(qmcw_profile) and gives the per-kernel averages the roofline entry uses.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -26,68 +30,110 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
-PEAK_FP32_TFLOPS = 157.3    # MI355X FP32
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic.json")
 
 
-def eval_full_flops(s, x):
-    """Algorithmic FP operations of one full evaluation of every walker in x
-    (counted per term of the formulas the kernel evaluates):
-      e-e pair Pade+polynomial 12 + 6 nordb; e-e-n per (pair, centre)
-      12 nterms + 60; e-n per (electron, centre) 12 + 6 norda;
-      basis per (point, shell) 30 + 12 nprim + 10 (2l+1) (value, gradient,
-      laplacian) or 8 + 8 nprim + 3 (2l+1) (value only);
-      orbital contraction 2 NB per component and orbital;
-      pseudopotential: per quadrature point of an electron inside a nonlocal
-      channel's range, value-only basis + 2 NB nocc + the Jastrow value change
-      of one electron ((N-1) pair terms + one e-n term)."""
+# ---------------------------------------------------------------- FLOP models
+def _det_sizes(s, sp):
+    """(N_sigma, N_occ, [excitation order of each unique determinant])."""
+    dets = np.asarray(s.det_up if sp == 0 else s.det_dn)
+    if dets.shape[1] == 0:
+        return 0, 0, []
+    ref = set(dets[0].tolist())
+    occ = set(dets.ravel().tolist())
+    uniq = {tuple(sorted(d.tolist())) for d in dets}
+    return dets.shape[1], len(occ), [len(set(u) - ref) for u in uniq]
+
+
+def _unit_costs(s):
+    """Algorithmic FP operations of the building blocks (per evaluation):
+      e-e pair (Pade + polynomial): value+grad+lap 12 + 6 nordb, value 6 + 2 nordb
+      e-e-n per (pair, centre): value+grad+lap 12 nterms + 60, value 8 nterms + 20
+      e-n per (electron, centre): value+grad+lap 12 + 6 norda, value 6 + 2 norda
+      basis per point and shell: value+grad+lap 30 + 12 nprim + 10 (2l+1),
+                                 value 8 + 8 nprim + 3 (2l+1)."""
     from champ_amd.system import nterms4
-    N, M = int(s.nelec), int(s.ncent)
-    nt = nterms4(s.nordc)
-    npair = N * (N - 1) // 2
-    pair = 12 + 6 * s.nordb + M * (12 * nt + 60)
-    en = M * (12 + 6 * s.norda)
-    # shells per centre type (grouped like the library does: one per radial fn)
+    nt, M = nterms4(s.nordc), int(s.ncent)
     shells = []
     for ic in range(M):
         t = s.iwctype[ic]
         seen = {}
         for q in range(s.ctype_bas_off[t], s.ctype_bas_off[t + 1]):
-            r = s.bas_rad[q]
-            key = (r, s.bas_l[q])
+            key = (s.bas_rad[q], s.bas_l[q])
             seen[key] = seen.get(key, 0) + 1
         for (r, l), nm in seen.items():
-            npr = int(s.rad_prim_off[r + 1] - s.rad_prim_off[r])
-            shells.append((npr, nm))
-    basis_full = sum(30 + 12 * npr + 10 * nm for npr, nm in shells)
-    basis_val = sum(8 + 8 * npr + 3 * nm for npr, nm in shells)
-    nocc = len(set(np.asarray(s.det_up).ravel().tolist()))
-    NB = int(s.nbasis)
-    per_walker = npair * pair + N * en + N * (basis_full + 5 * 2 * NB * nocc)
-    flops = float(per_walker) * x.shape[0]
-    if s.nloc:
-        # electrons within range of a nonlocal channel (|v_l| > 1e-4)
-        cent = np.asarray(s.cent)
-        r = np.linalg.norm(x[:, :, None, :] - cent[None, None, :, :], axis=-1)
-        inr = np.zeros_like(r, dtype=bool)
-        for ic in range(M):
-            t = s.iwctype[ic]
-            if s.npotd[t] <= 1:
+            shells.append((int(s.rad_prim_off[r + 1] - s.rad_prim_off[r]), nm))
+    return dict(
+        pair_full=12 + 6 * s.nordb + M * (12 * nt + 60),
+        pair_val=6 + 2 * s.nordb + M * (8 * nt + 20),
+        en_full=M * (12 + 6 * s.norda),
+        en_val=M * (6 + 2 * s.norda),
+        basis_full=sum(30 + 12 * npr + 10 * nm for npr, nm in shells),
+        basis_val=sum(8 + 8 * npr + 3 * nm for npr, nm in shells),
+    )
+
+
+def _nlpp_points(s, x):
+    """Quadrature points evaluated by the pseudopotential: electrons within
+    range of a nonlocal channel (|v_l| > 1e-4) times nquad."""
+    M = int(s.ncent)
+    cent = np.asarray(s.cent)
+    r = np.linalg.norm(x[:, :, None, :] - cent[None, None, :, :], axis=-1)
+    inr = np.zeros_like(r, dtype=bool)
+    for ic in range(M):
+        t = s.iwctype[ic]
+        if s.npotd[t] <= 1:
+            continue
+        for l in range(s.npotd[t]):
+            if l + 1 == s.lpotp1[t]:
                 continue
-            for l in range(s.npotd[t]):
-                if l + 1 == s.lpotp1[t]:
-                    continue
-                b, e = s.ecp_off[t * 5 + l], s.ecp_off[t * 5 + l + 1]
-                v = np.zeros_like(r[:, :, ic])
-                rr = np.maximum(r[:, :, ic], 1e-10)
-                for q in range(b, e):
-                    p = rr ** (s.ecp_power[q] - 2) if s.ecp_power[q] != 2 else 1.0
-                    v += p * s.ecp_coef[q] * np.exp(-s.ecp_expo[q] * rr * rr)
-                inr[:, :, ic] |= np.abs(v) > 1e-4
-        npts = int(inr.sum()) * s.nquad
-        flops += float(npts) * (basis_val + 2 * NB * nocc + (N - 1) * pair + en)
-    return float(flops)
+            b, e = s.ecp_off[t * 5 + l], s.ecp_off[t * 5 + l + 1]
+            v = np.zeros_like(r[:, :, ic])
+            rr = np.maximum(r[:, :, ic], 1e-10)
+            for q in range(b, e):
+                p = rr ** (s.ecp_power[q] - 2) if s.ecp_power[q] != 2 else 1.0
+                v += p * s.ecp_coef[q] * np.exp(-s.ecp_expo[q] * rr * rr)
+            inr[:, :, ic] |= np.abs(v) > 1e-4
+    return inr.sum(axis=2), int(s.nquad)     # [W, N] centres in range, Q
 
 
+def kernel_flops(s, name, x):
+    """Algorithmic FP operations of one launch of kernel `name` over the W
+    walkers in x (the per-unit figures of the scope table, times the units of
+    one launch).  k_ratio_1e is averaged over the electrons of a sweep."""
+    N, M, NB = int(s.nelec), int(s.ncent), int(s.nbasis)
+    W = x.shape[0]
+    u = _unit_costs(s)
+    sp = [_det_sizes(s, 0), _det_sizes(s, 1)]
+    npair = N * (N - 1) // 2
+    if name == "k_full_jas":
+        return float(W) * (npair * u["pair_full"] + N * u["en_full"])
+    if name == "k_full_det":
+        f = N * (u["basis_full"] + 2 * NB * 5 * max(sp[0][1], sp[1][1]))
+        for ns, nocc, ks in sp:
+            f += 2 * ns ** 3 + 2 * ns * ns * nocc + sum(k ** 3 for k in ks) + 4 * ns * nocc * ns
+        return float(W) * f
+    if name == "k_full_nlpp":
+        cnt, Q = _nlpp_points(s, x)
+        f = 0.0
+        for i in range(N):
+            nocc = sp[0][1] if i < s.nup else sp[1][1]
+            pts = float(cnt[:, i].sum()) * Q
+            f += pts * (u["basis_val"] + 2 * NB * nocc + (N - 1) * u["pair_val"] + u["en_val"])
+        return f
+    if name == "k_ratio_1e":
+        f = 0.0
+        for i in range(N):
+            ns, nocc, ks = sp[0] if i < s.nup else sp[1]
+            f += (u["basis_full"] + 2 * NB * 5 * nocc
+                  + (N - 1) * u["pair_full"] + u["en_full"]
+                  + 2 * ns * ns + 2 * ns * nocc + sum(k ** 3 for k in ks))
+        return float(W) * f / N
+    return None
+
+
+# ---------------------------------------------------------------- CPU baseline
 def cpu_baseline(s, seconds=12.0, nthreads=None):
     """The FP64 oracle's VMC (OpenMP over walkers) on the host cores, run on
     a bounded sample: batches of 128 walkers x 2 steps until `seconds`."""
@@ -109,7 +155,8 @@ def cpu_baseline(s, seconds=12.0, nthreads=None):
                 sample=f"{k} batches of {nw} walkers x {nst} VMC steps ({done} walker-steps, {dt:.1f} s)")
 
 
-def run_engine(s, W, steps, warmup, precision, delta, x, seeds):
+# ---------------------------------------------------------------- GPU runs
+def run_engine(s, W, steps, warmup, precision, delta, x, seeds, prof_steps):
     from champ_amd.engine import Engine
     e = Engine(s, W, precision)
     e.set_walkers(x)
@@ -118,14 +165,30 @@ def run_engine(s, W, steps, warmup, precision, delta, x, seeds):
     print(f"[bench] {precision}: autotuned lane-group width G={g}", file=sys.stderr, flush=True)
     e.vmc_device(max(1, warmup), delta)
     e.sync()
-    e.profile(True)
     t0 = time.perf_counter()
     e.vmc_device(steps, delta)
     e.sync()
     t1 = time.perf_counter()
+    # per-kernel HIP-event timing in a separate run (events add host work)
+    e.profile(True)
+    e.vmc_device(prof_steps, delta)
+    e.sync()
     prof = e.profile_get()
     e.profile(False)
-    return e, t1 - t0, prof
+    return e, t1 - t0, prof, g
+
+
+def copy_calibration(nbytes=256 << 20):
+    """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE
+    counters (launched only with --copy-calibration)."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] copy calibration: 3 copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
 def main():
@@ -135,14 +198,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--walkers", type=int, default=16384)
     ap.add_argument("--delta", type=float, default=0.6)
+    ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this run has one GPU, "
-              "multi-GPU runs are not part of this build (see DESIGN.md, sharding).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this build runs and reports a single GPU; "
+              "multi-GPU runs (walker sharding) are not part of it.", file=sys.stderr)
         sys.exit(2)
     from champ_amd import rng, system
     s = system.c2_dimer(seed=2)
@@ -151,32 +215,41 @@ def main():
     seeds = rng.walker_seeds("1837465927472523", W)
     res = {}
     for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
-        e, dt, prof = run_engine(s, W, args.steps, args.warmup, prec, args.delta, x0, seeds)
-        res[prec] = dict(engine=e, dt=dt, prof=prof, rate=W * args.steps / dt)
+        e, dt, prof, g = run_engine(s, W, args.steps, args.warmup, prec, args.delta, x0, seeds,
+                                    args.prof_steps)
+        res[prec] = dict(engine=e, dt=dt, prof=prof, rate=W * args.steps / dt, G=g)
         print(f"[bench] {prec}: {args.steps} steps x {W} walkers in {dt:.3f} s "
-              f"({W * args.steps / dt:.4g} walker-moves/s); phases {prof}", file=sys.stderr, flush=True)
+              f"({W * args.steps / dt:.4g} walker-moves/s)", file=sys.stderr, flush=True)
+        for k, (ms, n) in prof.items():
+            if n:
+                print(f"[bench]   {k:12s} {n:5d} launches  avg {ms / n * 1e3:9.1f} us  "
+                      f"total {ms:9.3f} ms", file=sys.stderr, flush=True)
     # exact-configuration energy bias of the mixed engine against FP64
     bias = None
     if "fp64" in res and "mixed" in res:
         xs = res["fp64"]["engine"].get_walkers()
-        ef = res["fp64"]["engine"]
-        em = res["mixed"]["engine"]
+        ef, em = res["fp64"]["engine"], res["mixed"]["engine"]
         ef.set_walkers(xs)
         em.set_walkers(xs)
-        e64 = ef.psi_at()["eloc"]
-        e32 = em.psi_at()["eloc"]
-        d = e32 - e64
+        d = em.psi_at()["eloc"] - ef.psi_at()["eloc"]
         bias = dict(mean_mHa=1e3 * float(d.mean()), sem_mHa=1e3 * float(d.std() / np.sqrt(len(d))),
                     max_abs_mHa=1e3 * float(np.abs(d).max()))
     main_prec = "mixed" if ("mixed" in res and bias is not None and abs(bias["mean_mHa"]) <= 0.1) \
         else ("fp64" if "fp64" in res else "mixed")
     r = res[main_prec]
-    # roofline of the dominant kernel (full evaluation), HIP-event timings
+    # roofline of the dominant kernel (largest total time in the profiled run)
     xs = r["engine"].get_walkers()
-    fl = eval_full_flops(s, xs)
-    ev_avg_ms = r["prof"]["eval_ms"] / max(1, r["prof"]["eval_n"])
-    achieved = fl / (ev_avg_ms * 1e-3) / 1e12
-    peak = PEAK_FP64_TFLOPS if main_prec == "fp64" else PEAK_FP64_TFLOPS
+    dom = max(r["prof"], key=lambda k: r["prof"][k][0])
+    ms_tot, nl = r["prof"][dom]
+    avg_ms = ms_tot / max(1, nl)
+    fl = kernel_flops(s, dom, xs)
+    achieved = fl / (avg_ms * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        tf = json.load(open(TRAFFIC_FILE))
+        ent = tf.get("kernels", {}).get(dom)
+        if ent and ent.get("walkers") == W and ent.get("precision") == main_prec:
+            traffic = ent["hbm_bytes_per_launch"]
     out = {
         "metric": "walker-moves/s + dlnPsi/dp samples/s vs FP64 GPU path at <=0.1 mHa bias",
         "value": r["rate"],
@@ -190,24 +263,28 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "C2-like dimer VMC: 8 valence e, 58 GTO, BFD-like ECP l=0,1 (Q=12), "
-                               "24 CSF / 48 det", "walkers": W, "delta": args.delta,
-                   "precision": main_prec},
+        "config": {"workload": "C2-like dimer VMC: 8 valence e, 58 GTO, ECP l=0,1 nonlocal (Q=12), "
+                               "24 CSF / 48 det, 16384 walkers x 200 steps",
+                   "walkers": W, "delta": args.delta, "precision": main_prec,
+                   "group_width": r["G"]},
         "fp64_walker_moves_per_s": res.get("fp64", {}).get("rate"),
         "mixed_walker_moves_per_s": res.get("mixed", {}).get("rate"),
         "mixed_vs_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]) if len(res) == 2 else None,
         "energy_bias_mixed_vs_fp64": bias,
         "electron_moves_per_s": r["rate"] * s.nelec,
-        "phase_ms": {k: v for k, v in r["prof"].items()},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_eval_full", "avg_ms": ev_avg_ms, "flop_per_launch": fl},
+        "kernel_avg_us": {k: (1e3 * v[0] / v[1] if v[1] else None) for k, v in r["prof"].items()},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "kernel": dom, "avg_us": 1e3 * avg_ms, "flop_per_launch": fl,
+                     "launches_timed": nl},
     }
+    if args.copy_calibration:
+        copy_calibration()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds)
     for v in res.values():
         v["engine"].close()
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -23,6 +23,11 @@ QMCW_FP64 = 0
 QMCW_MIXED = 1
 QMCW_MIXED_BF16 = 2
 
+# kernels timed by qmcw_profile, in the order of the QMCW_PROF_* indices
+PROF_KERNELS = ("k_full_jas", "k_full_det", "k_full_nlpp", "k_full_fin",
+                "k_ratio_1e", "k_accept_1e", "k_rotqua",
+                "k_deriv_jas", "k_deriv_nlpp", "k_derivs")
+
 QMCW_BASIS_SLATER = 1
 QMCW_BASIS_GAUSSIAN = 2
 QMCW_MAX_L = 4
@@ -97,6 +102,8 @@ SIGNATURES = {
     "qmcw_grad_1e": (C.c_int, [_vp, C.c_int32, _pd, _pd]),
     "qmcw_nparm": (C.c_int32, [_vp]),
     "qmcw_derivs": (C.c_int, [_vp, _pd, _pd]),
+    "qmcw_sample_derivs": (C.c_int, [_vp, C.c_int32]),
+    "qmcw_get_derivs": (C.c_int, [_vp, _pd, _pd]),
     "qmcw_set_rng": (C.c_int, [_vp, _pi]),
     "qmcw_get_rng": (C.c_int, [_vp, _pi]),
     "qmcw_vmc_run": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _pd,
@@ -105,6 +112,7 @@ SIGNATURES = {
     "qmcw_get_eloc": (C.c_int, [_vp, _pd]),
     "qmcw_sync": (C.c_int, [_vp]),
     "qmcw_autotune": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "qmcw_set_group_width": (C.c_int, [_vp, C.c_int32]),
     "qmcw_profile": (C.c_int, [_vp, C.c_int32]),
     "qmcw_profile_get": (C.c_int, [_vp, _pd, C.POINTER(C.c_int64)]),
     "qmcw_last_error": (C.c_char_p, [_vp]),

@@ -68,13 +68,16 @@ struct DevSys {
   const int *csf_det_off, *csf_det;
   const double *csf_det_coef, *csf_coef;
   int ncsf;
-  const int *pa_type, *pa_idx, *pb_spin, *pb_idx, *pc_type, *pc_idx;
-  const int *c_dep_n;        // [nterms*nctype] number of params a term feeds
-  const int *c_dep_p;        // [nterms*nctype][maxdep] parameter index
-  const double *c_dep_w;     // [nterms*nctype][maxdep] weight
-  int c_maxdep;
-  const int *porb_i, *porb_j, *porb_rev;
+  const int *porb_i, *porb_j, *porb_rev;  // orbital rotations (labels)
   const int *iocc_lab[2];    // [NO] position in occ or -1
+  // Jastrow derivative slots: one per a-coefficient [nctype][norda+1],
+  // b-coefficient [nspin2b][nordb+1] and e-e-n term [nctype][nterms] (the
+  // terms in the order of DevSys::c); parameter p of the Jastrow block is
+  // sum_q pj_w[q] * slot[pj_slot[q]], q in [pj_off[p], pj_off[p+1]) (the
+  // dependent e-e-n coefficients enter through their cusp weights)
+  int nslot, slot_b0, slot_c0;
+  const int *pj_off, *pj_slot;
+  const double *pj_w;
 };
 
 // Offsets (in doubles) of one walker's persistent state.
@@ -91,12 +94,9 @@ struct WLayout {
 // Offsets (in doubles) of one walker's scratch.
 struct SLayout {
   int64_t phi;               // [max(NB, points)][5] basis values
-  int64_t orb;               // [N][NO][5] all orbitals (full evaluation)
+  int64_t orb;               // [N][NO][5] all orbitals (derivative evaluation)
   int64_t g[2];              // [nocc][ns] G matrices
-  int64_t tfull[2];          // [ns][NO]
   int64_t vd, divv, vj;      // [N][3], [N], [N][3]
-  int64_t pnl;               // [N][NO] psp_nonloc_orb
-  int64_t pnl_c;             // [N*M][NO] per-centre partials
   int64_t red;               // reduction scratch (matrix temporaries)
   int64_t phs;               // [64] scalars passed between the phase kernels
   int64_t cup;               // [ndetu0 + ndetu1] det-coefficient sums
@@ -104,9 +104,27 @@ struct SLayout {
   int64_t vnl;               // [N*M][QMCW_MAX_LPOT+1] nonlocal channel values
   int64_t gproj;             // [N][NB] G(:,i)^T coef of the occupied orbitals
   int64_t mv;                // pending one-electron move buffers
-  int64_t gvalue, gjas, d2g, dpe; // Jastrow parameter derivatives
-  int64_t dpsi, deloc;       // [nparm]
   int64_t stride;
+};
+
+// Derivative scratch (offsets in doubles from the walker's scratch base),
+// present when the system has parameters to optimise.
+struct DLayout {
+  int64_t raw;               // [nslot][N][5] slot pieces: value, grad, lap
+  int64_t gvalue, gjas, d2g; // [npj], [npj][N][3], [npj]
+  int64_t pnl_c;             // [N*M][NO] per (electron, centre) psp_nonloc_orb
+  int64_t dpe_raw;           // [N*M][nslot]
+  int64_t dpe_slot, dpe;     // [nslot], [npj]
+  int64_t pnl;               // [N][NO] psp_nonloc_orb
+  int64_t phq, orbq;         // [64][NB], [64][NO] per-lane quadrature values
+  int64_t B;                 // [N][NO] orbital-resolved local energy
+  int64_t tfull[2], dt[2];   // [ns][NO]
+  int64_t aib;               // [ns][NO]
+  int64_t del[2], c0[2], c1[2];  // [ndetu]
+  int64_t idti[2];           // [ndetu][kmax2]
+  int64_t dy[2], dg[2];      // [nocc][ns]
+  int64_t aux4;              // [nocc][ns]
+  int64_t misc;              // [8]
 };
 
 // Pending one-electron move buffers inside SLayout::mv

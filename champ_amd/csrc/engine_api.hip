@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "engine_kernels.h"
+#include "engine_derivs.h"
 
 using namespace qmcw;
 
@@ -41,6 +41,9 @@ struct qmcw_handle {
   std::vector<int> csf_det_off, csf_det;
   std::vector<double> csf_det_coef;
   int nparm = 0;
+  double *der = nullptr;     // [Wp][2 nparm] dpsi, deloc of the last derivative evaluation
+  int sample_derivs = 0;     // VMC steps evaluate derivatives (CHAMP's ideriv)
+  int der_valid = 0;
   // event timing of the device driver
   int prof = 0;
   std::vector<hipEvent_t> ev;
@@ -210,6 +213,105 @@ std::vector<double> build_cdet(int ndet, int ncsf, const double *csf_coef, const
 
 // e-e-n coefficients from CHAMP's term order (jastrow4.f90:225-244) to the
 // kernels' compile-time order (k, then m, then l)
+// position of each e-e-n term (CHAMP order) in the order of DevSys::c
+static std::vector<int> een_dev_index(int nordc) {
+  std::vector<int> ck, cl, cm;
+  for (int n = 2; n <= nordc; n++)
+    for (int k = n - 1; k >= 0; k--) {
+      int l_hi = (k == 0) ? n - k - 2 : n - k;
+      for (int l = l_hi; l >= 0; l--) {
+        int m = (n - k - l) / 2;
+        if (2 * m == n - k - l) { ck.push_back(k); cl.push_back(l); cm.push_back(m); }
+      }
+    }
+  std::vector<int> pos(ck.size(), 0);
+  int idx = 0;
+  for (int k = 0; k <= nordc; k++)
+    for (int m = 0; 2 * m <= nordc - k; m++)
+      for (int l = 0; l <= nordc - k - 2 * m; l++) {
+        if (!((k >= 1 || m >= 1) && (l + 2 * m >= 1) && (k + l + 2 * m >= 2))) continue;
+        for (size_t t = 0; t < ck.size(); t++)
+          if (ck[t] == k && cl[t] == l && cm[t] == m) pos[t] = idx;
+        idx++;
+      }
+  return pos;
+}
+
+// Jastrow parameter -> derivative slots.  The e-e-n coefficients fixed by
+// the cusp conditions move with the varied ones: the cusp matrix d and the
+// dependent-term list iwc4 of qmc/fit/cuspinit4.f90:1-60, the weights of
+// checkdepend4 (qmc/fit/cuspinit4.f90:62-156).
+struct JasMap {
+  int nslot = 0, slot_b0 = 0, slot_c0 = 0;
+  std::vector<int> off{0}, slot;
+  std::vector<double> w;
+};
+
+static bool build_jas_map(const qmcw_system *s, int nt, JasMap &jm, std::string &why) {
+  const int na = s->norda + 1, nb = s->nordb + 1, nordc = s->nordc;
+  jm.slot_b0 = s->nctype * na;
+  jm.slot_c0 = jm.slot_b0 + s->nspin2b * nb;
+  jm.nslot = jm.slot_c0 + s->nctype * nt;
+  for (int p = 0; p < s->nparma_tot; p++) {
+    int t = s->jas_a_type[p], iw = s->jas_a_idx[p];
+    if (t < 0 || t >= s->nctype || iw < 1 || iw > na) { why = "e-n parameter index out of range"; return false; }
+    jm.slot.push_back(t * na + iw - 1); jm.w.push_back(1.0); jm.off.push_back((int)jm.slot.size());
+  }
+  for (int p = 0; p < s->nparmb_tot; p++) {
+    int sp = s->jas_b_spin[p], iw = s->jas_b_idx[p];
+    if (sp < 0 || sp >= s->nspin2b || iw < 1 || iw > nb) { why = "e-e parameter index out of range"; return false; }
+    jm.slot.push_back(jm.slot_b0 + sp * nb + iw - 1); jm.w.push_back(1.0); jm.off.push_back((int)jm.slot.size());
+  }
+  if (s->nparmc_tot == 0) return true;
+  const int neqs = nordc > 1 ? 2 * (nordc - 1) : 0;
+  std::vector<double> d((size_t)std::max(neqs, 1) * nt, 0.0);
+  std::vector<int> iwc4(std::max(neqs, 1), 0);
+  int i = 0;
+  for (int n = 2; n <= nordc; n++)
+    for (int k = n - 1; k >= 0; k--) {
+      int l_hi = (k == 0) ? n - k - 2 : n - k;
+      for (int l = l_hi; l >= 0; l--) {
+        int m = (n - k - l) / 2;
+        if (2 * m != n - k - l) continue;
+        i++;
+        if (k == 1 && iwc4[n - 2] == 0) iwc4[n - 2] = i;
+        if (k == 0 && iwc4[n + nordc - 3] == 0) iwc4[n + nordc - 3] = i;
+        d[(size_t)(l + 2 * m - 1) * nt + i - 1] -= 2 * k;
+        if (l + m > 0) d[(size_t)(nordc - 1 + k + m - 1) * nt + i - 1] -= (l + m);
+        if (m > 0) d[(size_t)(nordc - 1 + k + l + m - 1) * nt + i - 1] -= m;
+      }
+    }
+  std::vector<int> pos = een_dev_index(nordc);
+  auto D = [&](int e, int j) { return d[(size_t)(e - 1) * nt + j - 1]; };   // 1-based
+  for (int p = 0; p < s->nparmc_tot; p++) {
+    int t = s->jas_c_type[p], idx = s->jas_c_idx[p];
+    if (t < 0 || t >= s->nctype || idx < 1 || idx > nt) { why = "e-e-n parameter index out of range"; return false; }
+    for (int e = 1; e <= neqs; e++)
+      if (iwc4[e - 1] == idx) { why = "a cusp-dependent e-e-n coefficient cannot be varied"; return false; }
+    int c0 = jm.slot_c0 + t * nt;
+    jm.slot.push_back(c0 + pos[idx - 1]); jm.w.push_back(1.0);
+    for (int e = 1; e <= neqs; e++) {
+      if (e == nordc) continue;
+      if (std::fabs(D(e, idx)) > 1e-10) {
+        jm.slot.push_back(c0 + pos[iwc4[e - 1] - 1]);
+        jm.w.push_back(-D(e, idx) / D(e, iwc4[e - 1]));
+      }
+    }
+    if (nordc > 1) {
+      double v = 0.0;
+      for (int e = 1; e <= neqs; e++) {
+        if (e == nordc) continue;
+        double factor = -D(nordc, iwc4[e - 1]) / D(nordc, iwc4[nordc - 1]);
+        v -= factor * (D(e, idx) / D(e, iwc4[e - 1]));
+      }
+      jm.slot.push_back(c0 + pos[iwc4[nordc - 1] - 1]);
+      jm.w.push_back(v);
+    }
+    jm.off.push_back((int)jm.slot.size());
+  }
+  return true;
+}
+
 static std::vector<double> reorder_een(const std::vector<double> &c, int nctype, int nordc) {
   std::vector<int> champ_k, champ_l, champ_m;
   for (int n = 2; n <= nordc; n++)
@@ -321,39 +423,72 @@ __global__ void k_copy_strided(double *dst, int64_t dstride, const double *src, 
 }
 
 // -------------------------------------------------------------------------
+// Full evaluation of every walker; with derivs also the parameter
+// derivatives into h->der (all orbitals, Jastrow slots, pseudopotential
+// pieces, assembly).
 template <int G>
-static void launch_eval(qmcw_handle *h) {
+static void launch_eval(qmcw_handle *h, bool derivs) {
   dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  KArgs A = h->A;
+  A.full_orb = derivs ? 1 : 0;
+  prof_begin(h, QMCW_PROF_FULL_JAS);
   if (h->precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_full_jas<double, G>), grid, block, 0, h->stream, h->A);
+    hipLaunchKernelGGL((k_full_jas<double, G>), grid, block, 0, h->stream, A);
   else
-    hipLaunchKernelGGL((k_full_jas<float, G>), grid, block, 0, h->stream, h->A);
-  hipLaunchKernelGGL((k_full_det<G>), grid, block, 0, h->stream, h->A);
-  if (h->A.S.nloc > 0) {
-    if (h->precision == QMCW_FP64)
-      hipLaunchKernelGGL((k_full_nlpp<double, G>), grid, block, 0, h->stream, h->A);
-    else
-      hipLaunchKernelGGL((k_full_nlpp<float, G>), grid, block, 0, h->stream, h->A);
+    hipLaunchKernelGGL((k_full_jas<float, G>), grid, block, 0, h->stream, A);
+  prof_end(h);
+  if (derivs) {
+    prof_begin(h, QMCW_PROF_DERIV_JAS);
+    hipLaunchKernelGGL((k_deriv_jas<G>), grid, block, 0, h->stream, A);
+    prof_end(h);
   }
-  hipLaunchKernelGGL((k_full_fin<G>), grid, block, 0, h->stream, h->A);
+  prof_begin(h, QMCW_PROF_FULL_DET);
+  hipLaunchKernelGGL((k_full_det<G>), grid, block, 0, h->stream, A);
+  prof_end(h);
+  if (A.S.nloc > 0) {
+    prof_begin(h, QMCW_PROF_FULL_NLPP);
+    if (h->precision == QMCW_FP64)
+      hipLaunchKernelGGL((k_full_nlpp<double, G>), grid, block, 0, h->stream, A);
+    else
+      hipLaunchKernelGGL((k_full_nlpp<float, G>), grid, block, 0, h->stream, A);
+    prof_end(h);
+    if (derivs) {
+      prof_begin(h, QMCW_PROF_DERIV_NLPP);
+      hipLaunchKernelGGL((k_deriv_nlpp<G>), grid, block, 0, h->stream, A);
+      prof_end(h);
+    }
+  }
+  prof_begin(h, QMCW_PROF_FULL_FIN);
+  hipLaunchKernelGGL((k_full_fin<G>), grid, block, 0, h->stream, A);
+  prof_end(h);
+  if (derivs) {
+    prof_begin(h, QMCW_PROF_DERIVS);
+    hipLaunchKernelGGL((k_derivs<G>), grid, block, 0, h->stream, A, h->der);
+    prof_end(h);
+    h->der_valid = 1;
+  }
 }
 template <int G>
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
                          double fbias, double *out) {
   dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  prof_begin(h, QMCW_PROF_RATIO);
   if (h->precision == QMCW_FP64)
     hipLaunchKernelGGL((k_ratio_1e<double, G>), grid, block, 0, h->stream, h->A, ie, mode, xn,
                        delta, fbias, out);
   else
     hipLaunchKernelGGL((k_ratio_1e<float, G>), grid, block, 0, h->stream, h->A, ie, mode, xn,
                        delta, fbias, out);
+  prof_end(h);
 }
 template <int G>
 static void launch_accept(qmcw_handle *h, int ie, int mode, const uint8_t *acc, double delta,
                           double fbias, long long *nacc) {
   dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  prof_begin(h, QMCW_PROF_ACCEPT);
   hipLaunchKernelGGL((k_accept_1e<G>), grid, block, 0, h->stream, h->A, ie, mode, acc, delta,
                      fbias, nacc);
+  prof_end(h);
 }
 template <int G>
 static void launch_grad(qmcw_handle *h, int ie, double *out) {
@@ -506,6 +641,12 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   UP(h->csf_det_coef, S.csf_det_coef); UP(h->csf_coef, S.csf_coef);
   h->d_csf_coef = const_cast<double *>(S.csf_coef);
   // ---- Jastrow
+  if ((s->isc != 2 && s->isc != 4) || s->norda < 1 || s->nordb < 1 || s->norda > QMCW_MAX_ORD ||
+      s->nordb > QMCW_MAX_ORD || s->nordc < 0 || s->nordc > 7 || (s->nspin2b != 1 && s->nspin2b != 2)) {
+    delete h;
+    return fail(nullptr, QMCW_ERR_ARG,
+                "Jastrow: isc must be 2 or 4, 1 <= norda, nordb <= 8, nordc <= 7, nspin2b 1 or 2");
+  }
   S.isc = s->isc; S.nspin2b = s->nspin2b; S.nocuspb = s->nocuspb;
   S.norda = s->norda; S.nordb = s->nordb; S.nordc = s->nordc; S.scalek = s->scalek;
   S.nterms = qmcw_nterms4(s->nordc);
@@ -547,6 +688,27 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   S.nporb = s->nparmorb; S.npj = S.npa + S.npb + S.npc;
   S.nparm = S.npcsf + S.npj + S.nporb;
   h->nparm = S.nparm;
+  {
+    JasMap jm;
+    std::string why;
+    if (!build_jas_map(s, S.nterms, jm, why)) {
+      delete h;
+      return fail(nullptr, QMCW_ERR_ARG, why);
+    }
+    S.nslot = jm.nslot; S.slot_b0 = jm.slot_b0; S.slot_c0 = jm.slot_c0;
+    UP(jm.off, S.pj_off); UP(jm.slot, S.pj_slot); UP(jm.w, S.pj_w);
+    std::vector<int> pcsf(s->csf_parm, s->csf_parm + s->nparmcsf);
+    for (int c : pcsf)
+      if (c < 0 || c >= s->ncsf) { delete h; return fail(nullptr, QMCW_ERR_ARG, "CSF parameter index out of range"); }
+    std::vector<int> oi(s->orb_parm_i, s->orb_parm_i + s->nparmorb),
+        oj(s->orb_parm_j, s->orb_parm_j + s->nparmorb), orv(s->orb_parm_rev, s->orb_parm_rev + s->nparmorb);
+    for (int q = 0; q < s->nparmorb; q++)
+      if (oi[q] < 0 || oi[q] >= s->norb || oj[q] < 0 || oj[q] >= s->norb) {
+        delete h;
+        return fail(nullptr, QMCW_ERR_ARG, "orbital rotation index out of range");
+      }
+    UP(pcsf, S.pcsf); UP(oi, S.porb_i); UP(oj, S.porb_j); UP(orv, S.porb_rev);
+  }
   // ---- lane-group width and layouts
   int N = S.N, M = S.M, NO = S.NO, NB = S.NB;
   int G = N <= 4 ? 8 : (N <= 16 ? 16 : (N <= 40 ? 32 : 64));
@@ -590,12 +752,9 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SLayout &SL = h->A.SL;
   o = 0;
   SL.phi = take((int64_t)h->chunk * NB * 5);
-  SL.orb = take(h->A.full_orb ? (int64_t)N * NO * 5 : 1);
+  SL.orb = take(S.nparm > 0 ? (int64_t)N * NO * 5 : 1);
   SL.g[0] = take(D.nocc[0] * D.ns[0]); SL.g[1] = take(D.nocc[1] * D.ns[1]);
-  SL.tfull[0] = take(1); SL.tfull[1] = take(1);
   SL.vd = take(3 * N); SL.divv = take(N); SL.vj = take(3 * N);
-  SL.pnl = take((int64_t)N * noccm + (h->A.full_orb ? (int64_t)N * NO : 0));
-  SL.pnl_c = take(1);
   SL.red = take(std::max(std::max(64, nsm * nsm), noccm * nsm) + 64);
   SL.cup = take(D.ndetu[0] + D.ndetu[1]);
   SL.phs = take(64);
@@ -603,8 +762,25 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SL.vnl = take((int64_t)N * M * (QMCW_MAX_LPOT + 1));
   SL.gproj = take((int64_t)N * NB);
   SL.mv = take(ML.stride);
-  SL.gvalue = take(1); SL.gjas = take(1); SL.d2g = take(1); SL.dpe = take(1);
-  SL.dpsi = take(1); SL.deloc = take(1);
+  if (S.nparm > 0) {
+    DLayout &DL = h->A.DL;
+    int64_t NM = (int64_t)N * M;
+    DL.raw = take((int64_t)S.nslot * N * 5);
+    DL.gvalue = take(S.npj); DL.gjas = take((int64_t)S.npj * N * 3); DL.d2g = take(S.npj);
+    DL.pnl_c = take(NM * NO); DL.dpe_raw = take(NM * S.nslot);
+    DL.dpe_slot = take(S.nslot); DL.dpe = take(S.npj); DL.pnl = take((int64_t)N * NO);
+    DL.phq = take((int64_t)kWave * NB); DL.orbq = take((int64_t)kWave * NO);
+    DL.B = take((int64_t)N * NO);
+    for (int sp = 0; sp < 2; sp++) {
+      DL.tfull[sp] = take((int64_t)D.ns[sp] * NO); DL.dt[sp] = take((int64_t)D.ns[sp] * NO);
+      DL.del[sp] = take(D.ndetu[sp]); DL.c0[sp] = take(D.ndetu[sp]); DL.c1[sp] = take(D.ndetu[sp]);
+      DL.idti[sp] = take((int64_t)D.ndetu[sp] * S.kmax2);
+      DL.dy[sp] = take(D.nocc[sp] * D.ns[sp]); DL.dg[sp] = take(D.nocc[sp] * D.ns[sp]);
+    }
+    DL.aib = take((int64_t)nsm * NO);
+    DL.aux4 = take(noccm * nsm);
+    DL.misc = take(8);
+  }
   SL.stride = (o + 7) / 8 * 8;
 #undef UP
   // ---- device allocations
@@ -623,6 +799,10 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   h->allocs.push_back(p); h->nacc = (long long *)p;
   if (hipMalloc(&p, (size_t)h->Wp) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "flag alloc"); }
   h->allocs.push_back(p); h->acc_buf = (uint8_t *)p;
+  if (S.nparm > 0) {
+    if (hipMalloc(&p, (size_t)h->Wp * 2 * S.nparm * sizeof(double)) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "derivative alloc"); }
+    h->allocs.push_back(p); h->der = (double *)p;
+  }
   h->io_doubles = (size_t)h->Wp * std::max(8 + 4 * N, 8);
   if (hipMalloc(&p, h->io_doubles * sizeof(double)) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_NOMEM, "io alloc"); }
   h->allocs.push_back(p); h->io = (double *)p;
@@ -766,7 +946,7 @@ static int check_flags(qmcw_handle *h) {
 extern "C" int qmcw_eval_full(qmcw_handle *h, double *logpsid, double *sgn, double *psij,
                               double *grad, double *lap, double *eloc, double *epot) {
   if (!h) return QMCW_ERR_ARG;
-  DISPATCH_G(h, launch_eval);
+  DISPATCH_G(h, launch_eval, false);
   HIPCHK(h, hipGetLastError());
   int N = h->A.S.N, stride = 8 + 4 * N;
   hipLaunchKernelGGL(k_gather_full, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->A, h->io, h->W);
@@ -835,22 +1015,18 @@ extern "C" int qmcw_grad_1e(qmcw_handle *h, int32_t ie, double *grad, double *la
 
 static int vmc_steps(qmcw_handle *h, int nstep, double delta, double fbias, bool trace) {
   int N = h->A.S.N;
-  DISPATCH_G(h, launch_eval);
+  DISPATCH_G(h, launch_eval, false);
   for (int is = 0; is < nstep; is++) {
     if (h->A.S.nloc > 0) {
-      prof_begin(h, 2);
+      prof_begin(h, QMCW_PROF_ROTQUA);
       hipLaunchKernelGGL(k_rotqua, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, h->A);
       prof_end(h);
     }
-    prof_begin(h, 1);
     for (int ie = 0; ie < N; ie++) {
       DISPATCH_G(h, launch_ratio, ie, 1, (const double *)nullptr, delta, fbias, (double *)nullptr);
       DISPATCH_G(h, launch_accept, ie, 1, (const uint8_t *)nullptr, delta, fbias, h->nacc);
     }
-    prof_end(h);
-    prof_begin(h, 0);
-    DISPATCH_G(h, launch_eval);
-    prof_end(h);
+    DISPATCH_G(h, launch_eval, h->sample_derivs != 0);
     if (trace)
       hipLaunchKernelGGL(k_trace_eloc, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->A,
                          h->trace, is, h->W);
@@ -896,10 +1072,10 @@ extern "C" int qmcw_profile(qmcw_handle *h, int32_t enable) {
 extern "C" int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count) {
   if (!h || !ms || !count) return QMCW_ERR_ARG;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  for (int p = 0; p < 3; p++) { ms[p] = 0.0; count[p] = 0; }
+  for (int p = 0; p < QMCW_PROF_NKERNEL; p++) { ms[p] = 0.0; count[p] = 0; }
   for (size_t i = 0; i + 1 < h->ev_used; i += 2) {
     int ph = h->ev_phase[i];
-    if (ph < 0 || ph > 2) continue;
+    if (ph < 0 || ph >= QMCW_PROF_NKERNEL) continue;
     float t = 0.0f;
     HIPCHK(h, hipEventElapsedTime(&t, h->ev[i], h->ev[i + 1]));
     ms[ph] += t;
@@ -930,7 +1106,7 @@ extern "C" int qmcw_autotune(qmcw_handle *h, int32_t *chosen) {
     double tot = 0.0;
     for (int rep = 0; rep < 2; rep++) {
       HIPCHK(h, hipEventRecord(e0, h->stream));
-      DISPATCH_G(h, launch_eval);
+      DISPATCH_G(h, launch_eval, false);
       for (int ie = 0; ie < N; ie++) {
         hipLaunchKernelGGL(k_copy_strided, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, xin,
                            (int64_t)3, (const double *)(h->A.st + h->A.L.x + 3 * ie),
@@ -949,9 +1125,18 @@ extern "C" int qmcw_autotune(qmcw_handle *h, int32_t *chosen) {
   (void)hipEventDestroy(e1);
   h->G = bestG;
   // restore a consistent resident state with the chosen width
-  DISPATCH_G(h, launch_eval);
+  DISPATCH_G(h, launch_eval, false);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (chosen) *chosen = bestG;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_set_group_width(qmcw_handle *h, int32_t G) {
+  if (!h) return QMCW_ERR_ARG;
+  if (G != 8 && G != 16 && G != 32 && G != 64)
+    return fail(h, QMCW_ERR_ARG, "group width must be 8, 16, 32 or 64");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->G = G;
   return QMCW_OK;
 }
 
@@ -965,9 +1150,38 @@ extern "C" int qmcw_get_eloc(const qmcw_handle *hc, double *eloc) {
   return QMCW_OK;
 }
 
+static int copy_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
+  const int P = h->nparm;
+  std::vector<double> o((size_t)h->W * 2 * P);
+  HIPCHK(h, hipMemcpyAsync(o.data(), h->der, o.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int w = 0; w < h->W; w++) {
+    const double *r = o.data() + (size_t)w * 2 * P;
+    if (dpsi) memcpy(dpsi + (size_t)w * P, r, P * sizeof(double));
+    if (deloc) memcpy(deloc + (size_t)w * P, r + P, P * sizeof(double));
+  }
+  return QMCW_OK;
+}
+
 extern "C" int qmcw_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
-  (void)dpsi;
-  (void)deloc;
   if (!h) return QMCW_ERR_ARG;
-  return fail(h, QMCW_ERR_ARG, "parameter derivatives are not built into this library yet");
+  if (h->nparm == 0) return fail(h, QMCW_ERR_ARG, "the system has no parameters to differentiate");
+  DISPATCH_G(h, launch_eval, true);
+  HIPCHK(h, hipGetLastError());
+  int st = copy_derivs(h, dpsi, deloc);
+  if (st) return st;
+  return check_flags(h);
+}
+
+extern "C" int qmcw_sample_derivs(qmcw_handle *h, int32_t enable) {
+  if (!h) return QMCW_ERR_ARG;
+  if (enable && h->nparm == 0) return fail(h, QMCW_ERR_ARG, "the system has no parameters to differentiate");
+  h->sample_derivs = enable != 0;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_get_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
+  if (!h) return QMCW_ERR_ARG;
+  if (!h->der_valid) return fail(h, QMCW_ERR_ARG, "no derivative evaluation has run on this handle");
+  return copy_derivs(h, dpsi, deloc);
 }

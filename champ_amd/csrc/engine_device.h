@@ -432,7 +432,7 @@ __device__ __forceinline__ void solid_harm(int l, T x, T y, T z, T *Sv, T (*Sg)[
 
 __device__ __forceinline__ int m_slot(int m) { return m > 0 ? 2 * m - 1 : (m < 0 ? -2 * m : 0); }
 
-template <typename T, int L>
+template <typename T, int L, int ST>
 __device__ __forceinline__ void shell_write(T x, T y, T z, T f, T fr, T lapfac, int want,
                                             const int *bas, T *phi) {
   T Sv[2 * L + 1], Sg[2 * L + 1][3];
@@ -441,9 +441,9 @@ __device__ __forceinline__ void shell_write(T x, T y, T z, T f, T fr, T lapfac, 
   for (int q = 0; q < 2 * L + 1; q++) {
     int b = bas[q];
     if (b < 0) continue;
-    T *o = phi + 5 * b;
+    T *o = phi + ST * b;
     o[0] = Sv[q] * f;
-    if (want) {
+    if (ST == 5 && want) {
       o[1] = f * Sg[q][0] + Sv[q] * fr * x;
       o[2] = f * Sg[q][1] + Sv[q] * fr * y;
       o[3] = f * Sg[q][2] + Sv[q] * fr * z;
@@ -453,8 +453,9 @@ __device__ __forceinline__ void shell_write(T x, T y, T z, T f, T fr, T lapfac, 
 }
 
 // Evaluate shell s at displacement (x, y, z) from its centre, distance r.
-// want: 0 value only, 1 value + gradient + laplacian.
-template <typename T>
+// want: 0 value only, 1 value + gradient + laplacian.  ST is the stride of
+// phi per basis function (5: value, gradient, laplacian; 1: value only).
+template <typename T, int ST = 5>
 __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
                            T *phi) {
   int l = S.sh_l[s], a = S.sh_a[s];
@@ -494,11 +495,11 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
   T lapfac = f2 + T(2 * (l + 1)) * fr;
   const int *bas = S.sh_bas + s * kMaxShellM;
   switch (l) {
-    case 0: shell_write<T, 0>(x, y, z, f, fr, lapfac, want, bas, phi); break;
-    case 1: shell_write<T, 1>(x, y, z, f, fr, lapfac, want, bas, phi); break;
-    case 2: shell_write<T, 2>(x, y, z, f, fr, lapfac, want, bas, phi); break;
-    case 3: shell_write<T, 3>(x, y, z, f, fr, lapfac, want, bas, phi); break;
-    default: shell_write<T, 4>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 0: shell_write<T, 0, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 1: shell_write<T, 1, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 2: shell_write<T, 2, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 3: shell_write<T, 3, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    default: shell_write<T, 4, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
   }
 }
 

@@ -13,6 +13,7 @@ struct KArgs {
   WLayout L;
   SLayout SL;
   MLayout ML;
+  DLayout DL;
   double *st;        // walker state  [Wp][L.stride]
   double *sc;        // walker scratch [Wp][SL.stride]
   int *rng;          // [Wp][4]

@@ -128,8 +128,19 @@ class Engine:
     grad_1e = calc_1e_derivatives
 
     def derivs(self):
+        """Full evaluation with parameter derivatives: (dpsi, deloc), [W][P]."""
         dp, de = np.zeros((self.W, self.P)), np.zeros((self.W, self.P))
         self._chk(self.lib.qmcw_derivs(self.h, _d(dp), _d(de)))
+        return dp, de
+
+    def sample_derivs(self, enable: bool = True):
+        """Every VMC step's full evaluation also forms the derivatives."""
+        self._chk(self.lib.qmcw_sample_derivs(self.h, int(enable)))
+
+    def get_derivs(self):
+        """(dpsi, deloc) of the last derivative evaluation."""
+        dp, de = np.zeros((self.W, self.P)), np.zeros((self.W, self.P))
+        self._chk(self.lib.qmcw_get_derivs(self.h, _d(dp), _d(de)))
         return dp, de
 
     # ---- drivers
@@ -153,14 +164,19 @@ class Engine:
         self._chk(self.lib.qmcw_autotune(self.h, C.byref(g)))
         return g.value
 
+    def set_group_width(self, g: int):
+        self._chk(self.lib.qmcw_set_group_width(self.h, int(g)))
+
     def profile(self, enable: bool = True):
         self._chk(self.lib.qmcw_profile(self.h, int(enable)))
 
     def profile_get(self):
-        ms, cnt = np.zeros(3), np.zeros(3, np.int64)
+        """{kernel name: (total ms, launches)} of the launches timed since
+        profile(True)."""
+        n = len(abi.PROF_KERNELS)
+        ms, cnt = np.zeros(n), np.zeros(n, np.int64)
         self._chk(self.lib.qmcw_profile_get(self.h, _d(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64))))
-        return dict(eval_ms=ms[0], eval_n=int(cnt[0]), sweep_ms=ms[1], sweep_n=int(cnt[1]),
-                    rot_ms=ms[2], rot_n=int(cnt[2]))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(abi.PROF_KERNELS)}
 
     def sync(self):
         self._chk(self.lib.qmcw_sync(self.h))

@@ -173,12 +173,23 @@ int qmcw_accept_1e(qmcw_handle *h, int32_t ie, const uint8_t *accept);
    Replaces calc_1e_derivatives (qmc/psi_utils_mod.f90:457-483). */
 int qmcw_grad_1e(qmcw_handle *h, int32_t ie, double *grad, double *lap);
 
-/* Number of parameters P and, per walker, d ln Psi/dp and dE_L/dp at the
-   last full evaluation, in CHAMP's order CSF, Jastrow, orbitals
-   (qmc/deriv_mod.f90:219-239).  Replaces dpsi_bld / deloc_bld
-   (qmc/deriv_mod.f90:248-556).  Arrays are [W][P]. */
+/* Number of parameters P (CSF + Jastrow + orbital rotations). */
 int32_t qmcw_nparm(const qmcw_handle *h);
+
+/* Full evaluation of every walker at its current position together with the
+   parameter derivatives d ln Psi/dp and dE_L/dp, in CHAMP's order CSF,
+   Jastrow (a, b, c), orbital rotations (qmc/deriv_mod.f90:219-239).
+   Replaces hpsi with ideriv set (qmc/vmc/hpsi.f90:2-381) followed by
+   dpsi_bld / deloc_bld (qmc/deriv_mod.f90:248-556).  dpsi, deloc: [W][P]
+   host arrays (either may be NULL).  QMCW_ERR_ARG when P = 0. */
 int qmcw_derivs(qmcw_handle *h, double *dpsi, double *deloc);
+
+/* Sampling mode: with enable != 0 the full evaluation that ends every step
+   of qmcw_vmc_run / qmcw_vmc_run_device also forms the derivatives (CHAMP's
+   ideriv during an optimisation run, qmc/vmc/vmc.f90:259-272); they stay on
+   the GPU until read with qmcw_get_derivs ([W][P] each, last step). */
+int qmcw_sample_derivs(qmcw_handle *h, int32_t enable);
+int qmcw_get_derivs(qmcw_handle *h, double *dpsi, double *deloc);
 
 /* Per-walker random streams (CHAMP's rannyu, qmc/vmc/rannyu.f90:53-74):
    seed[W][4] 12-bit limbs as in setrn (qmc/vmc/rannyu.f90:11-21). */
@@ -202,12 +213,24 @@ int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta,
 /* Read back per-walker local energies of the last evaluation. */
 int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
 
-/* Kernel timing of the device VMC driver with HIP events recorded on the
-   handle's stream (no counterpart in the reference; for measurement).
-   enable != 0 clears and starts accumulation.  qmcw_profile_get returns, per
-   phase (0 full evaluation, 1 the one-electron sweep of ratio + accept
-   kernels, 2 quadrature rotation), the summed milliseconds and the number
-   of timed launches of that phase. */
+/* Kernel timing with HIP events recorded on the handle's stream around every
+   launch of the psi kernels (no counterpart in the reference; for
+   measurement).  enable != 0 clears and starts accumulation; the events add
+   a little host work per launch, so time throughput with profiling off.
+   qmcw_profile_get fills ms[QMCW_PROF_NKERNEL] with the summed milliseconds
+   and count[QMCW_PROF_NKERNEL] with the number of timed launches, per kernel
+   in the order of the QMCW_PROF_* indices. */
+#define QMCW_PROF_FULL_JAS 0   /* k_full_jas: distances + full Jastrow */
+#define QMCW_PROF_FULL_DET 1   /* k_full_det: basis, orbitals, AMF dets */
+#define QMCW_PROF_FULL_NLPP 2  /* k_full_nlpp: pseudopotential quadrature */
+#define QMCW_PROF_FULL_FIN 3   /* k_full_fin: local energy assembly */
+#define QMCW_PROF_RATIO 4      /* k_ratio_1e: one-electron move ratio */
+#define QMCW_PROF_ACCEPT 5     /* k_accept_1e: Metropolis test + update */
+#define QMCW_PROF_ROTQUA 6     /* k_rotqua: random rotation of the grid */
+#define QMCW_PROF_DERIV_JAS 7  /* k_deriv_jas: Jastrow parameter derivatives */
+#define QMCW_PROF_DERIV_NLPP 8 /* k_deriv_nlpp: pseudopotential derivative pieces */
+#define QMCW_PROF_DERIVS 9     /* k_derivs: d ln Psi/dp, d E_L/dp assembly */
+#define QMCW_PROF_NKERNEL 10
 int qmcw_profile(qmcw_handle *h, int32_t enable);
 int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count);
 
@@ -215,6 +238,10 @@ int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count);
    candidates on the resident walkers with side-effect-free launches (no
    counterpart in the reference).  chosen (may be NULL) receives the width. */
 int qmcw_autotune(qmcw_handle *h, int32_t *chosen);
+
+/* Set the lane-group width (8, 16, 32 or 64 lanes per walker) explicitly;
+   results are identical for every width (no counterpart in the reference). */
+int qmcw_set_group_width(qmcw_handle *h, int32_t G);
 
 /* Wait for all work queued on the handle's stream. */
 int qmcw_sync(qmcw_handle *h);

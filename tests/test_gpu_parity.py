@@ -109,3 +109,37 @@ def test_vmc_bit_exact_decisions(name):
     assert np.array_equal(e.get_walkers(), o["x"])
     assert np.array_equal(e.get_rng(), o["rng"])
     assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
+
+
+@pytest.mark.parametrize("G", [8, 16, 32, 64])
+def test_every_group_width_gives_identical_results(G):
+    s = SYS["c2"]
+    W = 21
+    x = system.random_walkers(s, W, seed=14)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    e = Engine(s, W)
+    e.set_group_width(G)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    g = e.vmc(3, delta=0.6)
+    o = oracle.Oracle(s).vmc_run(x, seeds, 3, 0.6)
+    assert np.array_equal(g["naccept"], o["naccept"])
+    assert np.array_equal(e.get_walkers(), o["x"])
+    assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
+
+
+def test_autotune_keeps_state_and_parity():
+    s = SYS["c2"]
+    W = 40
+    x = system.random_walkers(s, W, seed=15)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    e = Engine(s, W)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    assert e.autotune() in (8, 16, 32, 64)
+    assert np.array_equal(e.get_walkers(), x)
+    assert np.array_equal(e.get_rng(), np.asarray(seeds, np.int32))
+    g = e.vmc(2, delta=0.6)
+    o = oracle.Oracle(s).vmc_run(x, seeds, 2, 0.6)
+    assert np.array_equal(e.get_walkers(), o["x"])
+    assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
