@@ -70,6 +70,37 @@ class QmcwSystem(C.Structure):
     ]
 
 
+class DmcParams(C.Structure):
+    """struct qmcw_dmc_params; defaults are CHAMP's."""
+    _fields_ = [("tau", C.c_double), ("etrial", C.c_double), ("eest", C.c_double),
+                ("esigma", C.c_double), ("pop_ctrl", C.c_double), ("taueff", C.c_double),
+                ("c_rewt", C.c_double), ("pow_rewt", C.c_double), ("limit_rewt", C.c_double),
+                ("adrift", C.c_double), ("n_drift", C.c_int32), ("itau_eff", C.c_int32),
+                ("itau_integ", C.c_int32), ("iacc_rej", C.c_int32), ("icross", C.c_int32),
+                ("ene_int", C.c_int32), ("idmc", C.c_int32), ("pad_", C.c_int32)]
+
+    def __init__(self, tau, etrial, eest=None, esigma=None, pop_ctrl=1.0, taueff=None, **kw):
+        super().__init__()
+        self.tau, self.etrial = tau, etrial
+        self.eest = etrial if eest is None else eest
+        self.esigma = 1.0 if esigma is None else esigma
+        self.pop_ctrl = pop_ctrl
+        self.taueff = tau if taueff is None else taueff
+        self.c_rewt, self.pow_rewt, self.limit_rewt, self.adrift = 3.5, 0.5, 10.0, 0.5
+        self.n_drift, self.itau_eff, self.itau_integ = 2, 1, 1
+        self.iacc_rej, self.icross, self.ene_int, self.idmc = 1, 1, 0, 2
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class DmcStats(C.Structure):
+    """struct qmcw_dmc_stats"""
+    _fields_ = [("wsum", C.c_double), ("wesum", C.c_double), ("we2sum", C.c_double),
+                ("wtausum", C.c_double), ("pacc", C.c_double), ("nacc", C.c_double),
+                ("ntry", C.c_double), ("ncross", C.c_double), ("nwalk", C.c_int32),
+                ("nbranch", C.c_int32)]
+
+
 def dptr(a):
     """Pointer to a contiguous float64 numpy array (or NULL)."""
     if a is None:
@@ -116,6 +147,11 @@ SIGNATURES = {
     "qmcw_profile": (C.c_int, [_vp, C.c_int32]),
     "qmcw_profile_get": (C.c_int, [_vp, _pd, C.POINTER(C.c_int64)]),
     "qmcw_last_error": (C.c_char_p, [_vp]),
+    "qmcw_dmc_begin": (C.c_int, [_vp, C.c_int32, _pd]),
+    "qmcw_dmc_step": (C.c_int, [_vp, C.c_void_p, C.c_void_p]),
+    "qmcw_dmc_get": (C.c_int, [_vp, C.POINTER(C.c_int32), _pd]),
+    "qmcw_set_branch_rng": (C.c_int, [_vp, _pi]),
+    "qmcw_get_branch_rng": (C.c_int, [_vp, _pi]),
 }
 
 _lib = None

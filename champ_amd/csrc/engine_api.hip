@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "engine_derivs.h"
+#include "engine_dmc.h"
 
 using namespace qmcw;
 
@@ -23,6 +24,7 @@ static thread_local std::string g_lib_err;
 
 struct qmcw_handle {
   int W = 0, Wp = 0, G = 16, precision = QMCW_FP64, chunk = 1;
+  int Wl = 0;                // walkers the kernels run over (multiple of 8, <= Wp)
   hipStream_t stream = nullptr;
   KArgs A{};
   std::vector<void *> allocs;
@@ -44,6 +46,11 @@ struct qmcw_handle {
   double *der = nullptr;     // [Wp][2 nparm] dpsi, deloc of the last derivative evaluation
   int sample_derivs = 0;     // VMC steps evaluate derivatives (CHAMP's ideriv)
   int der_valid = 0;
+  // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
+  int dmc_on = 0, nact = 0;
+  double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
+  int *dorigin = nullptr;
+  int32_t brng[4] = {0, 0, 0, 1};
   // event timing of the device driver
   int prof = 0;
   std::vector<hipEvent_t> ev;
@@ -428,7 +435,7 @@ __global__ void k_copy_strided(double *dst, int64_t dstride, const double *src, 
 // pieces, assembly).
 template <int G>
 static void launch_eval(qmcw_handle *h, bool derivs) {
-  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  dim3 grid(h->Wl / (kWave / G)), block(kWave);
   KArgs A = h->A;
   A.full_orb = derivs ? 1 : 0;
   prof_begin(h, QMCW_PROF_FULL_JAS);
@@ -471,7 +478,7 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
 template <int G>
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
                          double fbias, double *out) {
-  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  dim3 grid(h->Wl / (kWave / G)), block(kWave);
   prof_begin(h, QMCW_PROF_RATIO);
   if (h->precision == QMCW_FP64)
     hipLaunchKernelGGL((k_ratio_1e<double, G>), grid, block, 0, h->stream, h->A, ie, mode, xn,
@@ -484,7 +491,7 @@ static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, dou
 template <int G>
 static void launch_accept(qmcw_handle *h, int ie, int mode, const uint8_t *acc, double delta,
                           double fbias, long long *nacc) {
-  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  dim3 grid(h->Wl / (kWave / G)), block(kWave);
   prof_begin(h, QMCW_PROF_ACCEPT);
   hipLaunchKernelGGL((k_accept_1e<G>), grid, block, 0, h->stream, h->A, ie, mode, acc, delta,
                      fbias, nacc);
@@ -492,7 +499,7 @@ static void launch_accept(qmcw_handle *h, int ie, int mode, const uint8_t *acc, 
 }
 template <int G>
 static void launch_grad(qmcw_handle *h, int ie, double *out) {
-  dim3 grid(h->Wp / (kWave / G)), block(kWave);
+  dim3 grid(h->Wl / (kWave / G)), block(kWave);
   hipLaunchKernelGGL((k_grad_1e<G>), grid, block, 0, h->stream, h->A, ie, out);
 }
 
@@ -811,6 +818,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   (void)hipMemsetAsync(h->A.rng, 0, (size_t)h->Wp * 4 * sizeof(int), h->stream);
   (void)hipMemsetAsync(h->nacc, 0, (size_t)h->Wp * sizeof(long long), h->stream);
   h->A.Wp = h->Wp;
+  h->Wl = h->Wp;
   if (hipStreamSynchronize(h->stream) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_HIP, "init sync"); }
   *out = h;
   return QMCW_OK;
@@ -893,6 +901,9 @@ extern "C" int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b
 extern "C" int qmcw_set_walkers(qmcw_handle *h, const double *x) {
   if (!h || !x) return QMCW_ERR_ARG;
   int N = h->A.S.N;
+  h->dmc_on = 0;            // a new configuration ends a DMC population
+  h->Wl = h->Wp;
+  h->A.Wp = h->Wp;
   HIPCHK(h, hipMemcpyAsync(h->io, x, (size_t)h->W * 3 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_copy_strided, dim3((h->W + 63) / 64), dim3(64), 0, h->stream,
                      h->A.st + h->A.L.x, (int64_t)h->A.L.stride, (const double *)h->io,
@@ -933,8 +944,9 @@ extern "C" int qmcw_get_rng(const qmcw_handle *hc, int32_t *seed) {
 }
 
 static int check_flags(qmcw_handle *h) {
-  std::vector<int> fl(h->W);
-  HIPCHK(h, hipMemcpyAsync(fl.data(), h->A.flags, h->W * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  const int n = h->dmc_on ? h->nact : h->W;
+  std::vector<int> fl(n);
+  HIPCHK(h, hipMemcpyAsync(fl.data(), h->A.flags, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   int any = 0;
   for (int f : fl) any |= f;
@@ -1019,7 +1031,7 @@ static int vmc_steps(qmcw_handle *h, int nstep, double delta, double fbias, bool
   for (int is = 0; is < nstep; is++) {
     if (h->A.S.nloc > 0) {
       prof_begin(h, QMCW_PROF_ROTQUA);
-      hipLaunchKernelGGL(k_rotqua, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, h->A);
+      hipLaunchKernelGGL(k_rotqua, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A);
       prof_end(h);
     }
     for (int ie = 0; ie < N; ie++) {
@@ -1184,4 +1196,219 @@ extern "C" int qmcw_get_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
   if (!h) return QMCW_ERR_ARG;
   if (!h->der_valid) return fail(h, QMCW_ERR_ARG, "no derivative evaluation has run on this handle");
   return copy_derivs(h, dpsi, deloc);
+}
+
+// ---------------------------------------------------------------------------
+// DMC
+
+static int dmc_alloc(qmcw_handle *h) {
+  if (h->dm) return QMCW_OK;
+  void *p;
+  size_t Wp = h->Wp;
+  HIPCHK(h, hipMalloc(&p, Wp * kDmStride * sizeof(double))); h->allocs.push_back(p); h->dm = (double *)p;
+  HIPCHK(h, hipMalloc(&p, Wp * kDmStride * sizeof(double))); h->allocs.push_back(p); h->dm2 = (double *)p;
+  HIPCHK(h, hipMalloc(&p, Wp * h->A.L.stride * sizeof(double))); h->allocs.push_back(p); h->st2 = (double *)p;
+  HIPCHK(h, hipMalloc(&p, 8 * sizeof(double))); h->allocs.push_back(p); h->dstat = (double *)p;
+  HIPCHK(h, hipMalloc(&p, Wp * sizeof(double))); h->allocs.push_back(p); h->dwt = (double *)p;
+  HIPCHK(h, hipMalloc(&p, Wp * sizeof(int))); h->allocs.push_back(p); h->dorigin = (int *)p;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_set_branch_rng(qmcw_handle *h, const int32_t *seed) {
+  if (!h || !seed) return QMCW_ERR_ARG;
+  for (int k = 0; k < 4; k++) h->brng[k] = seed[k] & 4095;
+  h->brng[3] |= 1;     // setrn makes the last limb odd (qmc/vmc/rannyu.f90:11-21)
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_get_branch_rng(const qmcw_handle *h, int32_t *seed) {
+  if (!h || !seed) return QMCW_ERR_ARG;
+  for (int k = 0; k < 4; k++) seed[k] = h->brng[k];
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weight) {
+  if (!h || nwalk <= 0 || nwalk > h->W) return fail(h, QMCW_ERR_ARG, "nwalk must be in 1..W");
+  if (h->A.S.nloc == 0)
+    return fail(h, QMCW_ERR_ARG,
+                "DMC is built for pseudopotential systems (the all-electron cusp mover is not)");
+  int st = dmc_alloc(h);
+  if (st) return st;
+  h->nact = nwalk;
+  h->Wl = (nwalk + 7) / 8 * 8;
+  h->A.Wp = h->Wl;
+  std::vector<double> wt(h->Wp, 1.0);
+  if (weight) std::copy(weight, weight + nwalk, wt.begin());
+  HIPCHK(h, hipMemcpyAsync(h->dwt, wt.data(), h->Wp * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  DISPATCH_G(h, launch_eval, false);
+  hipLaunchKernelGGL(k_dmc_init, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A, h->dm,
+                     (const double *)h->dwt, nwalk);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->dmc_on = 1;
+  st = check_flags(h);
+  return st;
+}
+
+// split_join (qmc/dmc/dmc_elec/split_join_mod.f90:3-219) on the host: the
+// decisions are sequential in the walker index and draw from the branching
+// stream.  Produces the new population as copies of old records: origin[d],
+// the new weights and the streams (a split copy continues on four 12-bit
+// limbs drawn from the branching stream, the last made odd).
+static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<int32_t> &rng,
+                          std::vector<int> &origin, int &n_out, int &nbranch) {
+  const int n = h->nact, MW = h->W;
+  origin.resize(MW);
+  for (int i = 0; i < MW; i++) origin[i] = i;
+  std::vector<int> under;
+  int ipair = 0, iw2 = 0, nbr = 0;
+  for (int iw = 0; iw < n; iw++) {
+    if (!(w[iw] < 0.5)) continue;
+    if (w[iw] == 0.0) {
+      nbr++;
+      under.push_back(iw);
+    } else if (ipair == 0) {
+      ipair = 1;
+      iw2 = iw;
+    } else {
+      nbr++;
+      ipair = 0;
+      const double wt = w[iw] + w[iw2];
+      if (rannyu(h->brng) > w[iw] / wt) {
+        w[iw2] = wt;
+        under.push_back(iw);
+      } else {
+        w[iw] = wt;
+        under.push_back(iw2);
+      }
+    }
+  }
+  const int nafter = n - (int)under.size(), nmax_dupl = MW - nafter;
+  std::vector<double> srt(w.begin(), w.begin() + n);
+  std::sort(srt.begin(), srt.end());
+  auto at1 = [&](int i) { return srt[std::max(1, i) - 1]; };   // 1-based, clamped below
+  double wt_split = std::max(2.0, at1(n - nmax_dupl));
+  int nmore = 0;
+  for (int iw = std::max(1, n - nmax_dupl); iw <= n; iw++) {
+    const double r = srt[iw - 1] / wt_split;
+    if (r >= 8.0) nmore += 3;
+    else if (r >= 4.0) nmore += 2;
+    else if (r >= 2.0) nmore += 1;
+  }
+  if (n - nmax_dupl + nmore < n) wt_split = std::max(2.0, at1(n - nmax_dupl + nmore));
+  else wt_split = 1e99;
+  int n2 = n;
+  for (int iw = 0; iw < n; iw++) {
+    if (!(w[iw] > wt_split)) continue;
+    nbr++;
+    int dst;
+    if (!under.empty()) {
+      dst = under.back();
+      under.pop_back();
+    } else {
+      if (n2 >= MW) return fail(h, QMCW_ERR_NOMEM, "DMC population exceeds the handle's walkers");
+      dst = n2++;
+    }
+    w[iw] *= 0.5;
+    w[dst] = w[iw];
+    origin[dst] = origin[iw];
+    for (int k = 0; k < 4; k++) rng[4 * dst + k] = (int32_t)(rannyu(h->brng) * 4096.0) & 4095;
+    rng[4 * dst + 3] |= 1;
+  }
+  while (!under.empty()) {
+    const int dst = under.back(), src = n2 - 1;
+    under.pop_back();
+    n2--;
+    if (dst != src) {
+      w[dst] = w[src];
+      origin[dst] = origin[src];
+      for (int k = 0; k < 4; k++) rng[4 * dst + k] = rng[4 * src + k];
+    }
+  }
+  n_out = n2;
+  nbranch = nbr;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_stats *stats) {
+  if (!h || !p) return QMCW_ERR_ARG;
+  if (!h->dmc_on) return fail(h, QMCW_ERR_ARG, "qmcw_dmc_begin has not been called");
+  if (!(p->tau > 0.0) || p->n_drift < 1) return fail(h, QMCW_ERR_ARG, "tau > 0 and n_drift >= 1 required");
+  const int N = h->A.S.N;
+  const dim3 g1((h->Wl + 63) / 64), b1(64);
+  double *gbuf = h->io, *xin = h->io + (size_t)h->Wp * 8;
+  if (h->A.S.nloc > 0) hipLaunchKernelGGL(k_rotqua, g1, b1, 0, h->stream, h->A);
+  hipLaunchKernelGGL(k_dmc_gen_begin, g1, b1, 0, h->stream, h->A, h->dm);
+  for (int ie = 0; ie < N; ie++) {
+    DISPATCH_G(h, launch_grad, ie, gbuf);
+    hipLaunchKernelGGL(k_dmc_propose, g1, b1, 0, h->stream, h->A, ie, (const double *)gbuf, xin,
+                       h->dm, *p);
+    DISPATCH_G(h, launch_ratio, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
+    hipLaunchKernelGGL(k_dmc_decide, g1, b1, 0, h->stream, h->A, ie, h->dm, h->acc_buf, *p);
+    DISPATCH_G(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+  }
+  DISPATCH_G(h, launch_eval, false);
+  hipLaunchKernelGGL(k_dmc_finish, g1, b1, 0, h->stream, h->A, h->dm, *p);
+  hipLaunchKernelGGL(k_dmc_sums, dim3(1), dim3(256), 0, h->stream, h->A, (const double *)h->dm,
+                     h->nact, h->dstat);
+  hipLaunchKernelGGL(k_dmc_weights, g1, b1, 0, h->stream, (const double *)h->dm, h->nact, h->dwt);
+  HIPCHK(h, hipGetLastError());
+  double sums[8];
+  std::vector<double> w(h->Wp, 0.0);
+  std::vector<int32_t> rng((size_t)h->Wp * 4, 0);   // the padding records reach Wp
+  HIPCHK(h, hipMemcpyAsync(sums, h->dstat, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(w.data(), h->dwt, h->nact * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(rng.data(), h->A.rng, (size_t)h->nact * 4 * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  int st = check_flags(h);
+  if (st) return st;
+  std::vector<int> origin;
+  int n2 = 0, nbr = 0;
+  st = dmc_split_join(h, w, rng, origin, n2, nbr);
+  if (st) return st;
+  const int Wl2 = (n2 + 7) / 8 * 8;
+  std::vector<int> org(Wl2);
+  std::vector<double> wn(Wl2, 0.0);
+  for (int d = 0; d < Wl2; d++) {
+    org[d] = d < n2 ? origin[d] : origin[0];
+    wn[d] = d < n2 ? w[d] : 0.0;
+    if (d >= n2)
+      for (int k = 0; k < 4; k++) rng[4 * d + k] = rng[k];
+  }
+  HIPCHK(h, hipMemcpyAsync(h->dorigin, org.data(), Wl2 * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->dwt, wn.data(), Wl2 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->A.rng, rng.data(), (size_t)Wl2 * 4 * sizeof(int32_t),
+                           hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_dmc_gather, dim3(Wl2), dim3(256), 0, h->stream, (const double *)h->A.st,
+                     h->st2, (int64_t)h->A.L.stride, (const double *)h->dm, h->dm2,
+                     (const int *)h->dorigin, (const double *)h->dwt, Wl2);
+  HIPCHK(h, hipGetLastError());
+  std::swap(h->A.st, h->st2);
+  std::swap(h->dm, h->dm2);
+  h->nact = n2;
+  h->Wl = Wl2;
+  h->A.Wp = Wl2;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (stats) {
+    stats->wsum = sums[0]; stats->wesum = sums[1]; stats->we2sum = sums[2];
+    stats->wtausum = sums[3]; stats->pacc = sums[4]; stats->nacc = sums[5];
+    stats->ncross = sums[6]; stats->ntry = sums[7];
+    stats->nwalk = n2;
+    stats->nbranch = nbr;
+  }
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_dmc_get(qmcw_handle *h, int32_t *nwalk, double *weight) {
+  if (!h) return QMCW_ERR_ARG;
+  if (!h->dmc_on) return fail(h, QMCW_ERR_ARG, "qmcw_dmc_begin has not been called");
+  if (nwalk) *nwalk = h->nact;
+  if (weight) {
+    hipLaunchKernelGGL(k_dmc_weights, dim3((h->nact + 63) / 64), dim3(64), 0, h->stream,
+                       (const double *)h->dm, h->nact, h->dwt);
+    HIPCHK(h, hipMemcpyAsync(weight, h->dwt, h->nact * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
 }

@@ -610,7 +610,7 @@ __device__ __forceinline__ double ecp_radial(const DevSys &S, double r, int l0, 
 }
 
 // CHAMP's rannyu on a 4 x 12-bit limb state (qmc/vmc/rannyu.f90:53-74)
-__device__ __forceinline__ double rannyu(int *l) {
+__host__ __device__ __forceinline__ double rannyu(int *l) {
   const int m1 = 502, m2 = 1521, m3 = 4071, m4 = 2107;
   const double two12i = 2.44140625e-4;
   int l1 = l[0], l2 = l[1], l3 = l[2], l4 = l[3];

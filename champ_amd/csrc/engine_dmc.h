@@ -1,0 +1,250 @@
+// engine_dmc.h -- kernels of one DMC generation (qmc/rundmc_mod.f90:97-330)
+// for the pseudopotential mover: drift-diffusion proposal with the averaged
+// velocity (qmc/green_functions_mod.f90:70-80, 598-633), Metropolis test with
+// the forward and backward Green's functions (:257-265, rundmc_mod.f90:
+// 142-196), and reweighting with ene_int_rewt (green_functions_mod.f90:
+// 382-472).  The one-electron ratio and the commit reuse k_ratio_1e (given
+// position) and k_accept_1e (given decision).  Every walker draws from its
+// own rannyu stream.
+#pragma once
+#include "engine_kernels.h"
+
+namespace qmcw {
+
+// per-walker DMC record, kDmStride doubles
+enum {
+  kDmEloc = 0,    // E_L at the start of the generation
+  kDmG2 = 1,      // |grad ln Psi|^2 (all electrons) of the current state
+  kDmW = 2,       // weight
+  kDmSdd = 3,     // sum of diffusion displacements squared
+  kDmSpd = 4,     // sum pacc * dd
+  kDmSad = 5,     // sum accepted * dd
+  kDmAcc = 6,     // any move accepted
+  kDmPacc = 7,    // sum of acceptance probabilities
+  kDmNacc = 8,    // accepted moves
+  kDmNcross = 9,  // node-crossing proposals
+  kDmTau = 10,    // taunow of the generation
+  kDmV = 11,      // averaged velocity of the current proposal (3)
+  kDmEpot = 14,   // potential energy at the start of the generation
+  kDmDd = 15,     // dd of the current proposal
+  kDmStride = 16
+};
+
+// averaged_1e_velocity with adrift = adrift_default
+__device__ __forceinline__ void dmc_vav(double tau, double adrift, int nd, const double *g,
+                                        double *v) {
+  const double vn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  double vav;
+  if (adrift * vn * vn * tau <= 0.0) vav = vn;
+  else vav = (-1.0 + pow(1.0 + nd * adrift * vn * vn * tau, 1.0 / nd)) / (adrift * vn * tau);
+  for (int k = 0; k < 3; k++) v[k] = vav * g[k] / vn;
+}
+
+// drift-diffusion Green's function of the move xo -> xn with velocity vav
+__device__ __forceinline__ double dmc_prob(double tau, const double *vav, const double *xo,
+                                           const double *xn) {
+  const double pi = 3.141592653589793;
+  double d2 = 0.0;
+  for (int k = 0; k < 3; k++) {
+    const double d = xn[k] - xo[k] - vav[k] * tau;
+    d2 += d * d;
+  }
+  const double nrm = sqrt(2.0 * pi * tau);
+  return exp(-d2 / (2.0 * tau)) / (nrm * nrm * nrm);
+}
+
+// CHAMP's gauss (qmc/dmc/gauss.f90): cos(2 pi u1) sqrt(-2 ln u2)
+__device__ __forceinline__ double dmc_gauss(int *st4) {
+  const double pi = 3.141592653589793;
+  const double c = cos(2.0 * pi * rannyu(st4));
+  return c * sqrt(-2.0 * log(rannyu(st4)));
+}
+
+__global__ void k_dmc_gen_begin(KArgs A, double *dm) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  double *d = dm + (int64_t)w * kDmStride;
+  d[kDmEloc] = ws[A.L.eloc];
+  d[kDmEpot] = ws[A.L.epot];
+  for (int q = kDmSdd; q <= kDmNcross; q++) d[q] = 0.0;
+}
+
+// proposal for electron ie; g[w][4] holds its current gradient
+__global__ void k_dmc_propose(KArgs A, int ie, const double *g, double *xin, double *dm,
+                              qmcw_dmc_params p) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  double *d = dm + (int64_t)w * kDmStride;
+  double v[3];
+  dmc_vav(p.tau, p.adrift, p.n_drift, g + 4 * w, v);
+  int *rs = A.rng + 4 * w;
+  int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+  const double rt = sqrt(p.tau);
+  double d2 = 0.0;
+  for (int k = 0; k < 3; k++) {
+    const double xd = rt * dmc_gauss(st4);
+    xin[3 * w + k] = ws[A.L.x + 3 * ie + k] + v[k] * p.tau + xd;
+    d2 += xd * xd;
+  }
+  rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+  for (int k = 0; k < 3; k++) d[kDmV + k] = v[k];
+  d[kDmDd] = d2;
+  d[kDmSdd] += d2;
+}
+
+// Metropolis decision for the pending move of electron ie (k_ratio_1e has
+// left the ratio and the new gradient in the walker's move buffer)
+__global__ void k_dmc_decide(KArgs A, int ie, double *dm, uint8_t *acc, qmcw_dmc_params p) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const DevSys &S = A.S;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  const double *mv = A.sc + (int64_t)w * A.SL.stride + A.SL.mv;
+  double *d = dm + (int64_t)w * kDmStride;
+  const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  const double ratio = mv[A.ML.misc + 9 + ns + nocc];
+  const double *gn = mv + A.ML.misc + 10 + ns + nocc;
+  const double *xn = mv + A.ML.xn;
+  const double *xo = ws + A.L.x + 3 * ie;
+  double vn[3];
+  dmc_vav(p.tau, p.adrift, p.n_drift, gn, vn);
+  const double po = dmc_prob(p.tau, d + kDmV, xo, xn);
+  const double pn = dmc_prob(p.tau, vn, xn, xo);
+  double pa = fmin(1.0, ratio * ratio * pn / po);
+  if (p.iacc_rej == 0) pa = 1.0;
+  if (ratio <= 0.0) {
+    if (p.icross <= 0) pa = 0.0;
+    d[kDmNcross] += 1.0;
+  }
+  int *rs = A.rng + 4 * w;
+  int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+  const int a = rannyu(st4) < pa;
+  rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+  acc[w] = (uint8_t)a;
+  const double dd = d[kDmDd];
+  d[kDmSpd] += pa * dd;
+  d[kDmSad] += a * dd;
+  d[kDmPacc] += pa;
+  d[kDmNacc] += a;
+  if (a) d[kDmAcc] = 1.0;
+}
+
+// after the full evaluation: walkers that moved no electron keep the local
+// energy of the start of the generation (CHAMP does not re-evaluate them);
+// effective time step, reweighting (ene_int_v9 or no_ene_int)
+__global__ void k_dmc_finish(KArgs A, double *dm, qmcw_dmc_params p) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const DevSys &S = A.S;
+  double *ws = A.st + (int64_t)w * A.L.stride;
+  const double *ss = A.sc + (int64_t)w * A.SL.stride;
+  double *d = dm + (int64_t)w * kDmStride;
+  if (d[kDmAcc] == 0.0) {
+    ws[A.L.eloc] = d[kDmEloc];
+    ws[A.L.epot] = d[kDmEpot];
+  }
+  const double *vj = ss + A.SL.vj, *vd = ss + A.SL.vd;
+  double g2 = 0.0;
+  for (int q = 0; q < 3 * S.N; q++) {
+    const double v = vj[q] + vd[q];
+    g2 += v * v;
+  }
+  double taunow = p.tau;
+  if (p.itau_eff >= 1) taunow = p.tau * (p.itau_eff == 2 ? d[kDmSad] : d[kDmSpd]) / d[kDmSdd];
+  double tauint = p.tau;
+  if (p.itau_integ >= 1) tauint = (p.itau_eff == 3) ? p.taueff : taunow;
+  double wt = d[kDmW];
+  const double en = ws[A.L.eloc];
+  if (p.idmc > 0) {
+    const double pi = 3.141592653589793;
+    const double eo = d[kDmEloc];
+    double ewto, ewtn;
+    if (p.ene_int == 1) {
+      ewto = eo;
+      ewtn = en;
+    } else {
+      const double nf = pow((double)S.N, p.pow_rewt);
+      const double facto = fmax(sqrt(d[kDmG2]) / nf * tauint * p.c_rewt, 1e-9);
+      const double factn = fmax(sqrt(g2) / nf * tauint * p.c_rewt, 1e-9);
+      ewto = p.eest + (eo - p.eest) * (sqrt(pi) / 2.0) * erf(facto) / facto;
+      ewtn = p.eest + (en - p.eest) * (sqrt(pi) / 2.0) * erf(factn) / factn;
+    }
+    double eint = (ewto + ewtn) / 2.0 - p.etrial;
+    const double emin = p.eest - p.etrial - p.limit_rewt * p.esigma / sqrt(sqrt(p.tau));
+    if (eint < emin) eint = emin;
+    const double enetau = eint * ((p.itau_eff == 3) ? p.taueff : taunow);
+    wt = wt * exp(-enetau) * p.pop_ctrl;
+  }
+  d[kDmW] = wt;
+  d[kDmTau] = taunow;
+  d[kDmG2] = g2;
+}
+
+// |grad ln Psi|^2 of the current state into the records (start of a run)
+__global__ void k_dmc_init(KArgs A, double *dm, const double *weight, int nact) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  const double *ss = A.sc + (int64_t)w * A.SL.stride;
+  double *d = dm + (int64_t)w * kDmStride;
+  double g2 = 0.0;
+  for (int q = 0; q < 3 * A.S.N; q++) {
+    const double v = ss[A.SL.vj + q] + ss[A.SL.vd + q];
+    g2 += v * v;
+  }
+  for (int q = 0; q < kDmStride; q++) d[q] = 0.0;
+  d[kDmG2] = g2;
+  d[kDmEloc] = ws[A.L.eloc];
+  d[kDmW] = w < nact ? weight[w] : 0.0;
+}
+
+// population sums of one generation: one workgroup, fixed-order tree
+// out[8]: w, w E, w E^2, w taunow, pacc, nacc, ncross, ntry
+__global__ __launch_bounds__(256) void k_dmc_sums(KArgs A, const double *dm, int nact,
+                                                  double *out) {
+  __shared__ double sh[8][256];
+  const int t = threadIdx.x;
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int w = t; w < nact; w += 256) {
+    const double *d = dm + (int64_t)w * kDmStride;
+    const double e = A.st[(int64_t)w * A.L.stride + A.L.eloc];
+    const double wt = d[kDmW];
+    a[0] += wt; a[1] += wt * e; a[2] += wt * e * e; a[3] += wt * d[kDmTau];
+    a[4] += d[kDmPacc]; a[5] += d[kDmNacc]; a[6] += d[kDmNcross]; a[7] += A.S.N;
+  }
+  for (int q = 0; q < 8; q++) sh[q][t] = a[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 8; q++) sh[q][t] += sh[q][t + o];
+    __syncthreads();
+  }
+  if (t < 8) out[t] = sh[t][0];
+}
+
+// weights of the population (for branching on the host)
+__global__ void k_dmc_weights(const double *dm, int nact, double *out) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nact) out[w] = dm[(int64_t)w * kDmStride + kDmW];
+}
+
+// population after branching: record d is a copy of record origin[d]
+__global__ void k_dmc_gather(const double *st_src, double *st_dst, int64_t stride,
+                             const double *dm_src, double *dm_dst, const int *origin,
+                             const double *wnew, int nl) {
+  const int d = blockIdx.x;
+  if (d >= nl) return;
+  const int o = origin[d];
+  const double *src = st_src + (int64_t)o * stride;
+  double *dst = st_dst + (int64_t)d * stride;
+  for (int64_t q = threadIdx.x; q < stride; q += blockDim.x) dst[q] = src[q];
+  if (threadIdx.x < kDmStride) {
+    double v = dm_src[(int64_t)o * kDmStride + threadIdx.x];
+    if (threadIdx.x == kDmW) v = wnew[d];
+    dm_dst[(int64_t)d * kDmStride + threadIdx.x] = v;
+  }
+}
+
+}  // namespace qmcw

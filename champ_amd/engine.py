@@ -154,6 +154,35 @@ class Engine:
     def vmc_device(self, nstep: int, delta: float, fbias: float = 1.0):
         self._chk(self.lib.qmcw_vmc_run_device(self.h, int(nstep), float(delta), float(fbias)))
 
+    # ---- DMC (qmc/rundmc_mod.f90)
+    def dmc_begin(self, nwalk: int, weight=None):
+        w = None if weight is None else np.ascontiguousarray(weight, np.float64)
+        self._chk(self.lib.qmcw_dmc_begin(self.h, int(nwalk), _d(w)))
+
+    def dmc_step(self, params):
+        """One generation (moves, reweighting, split_join); returns the
+        qmcw_dmc_stats of the generation."""
+        st = abi.DmcStats()
+        self._chk(self.lib.qmcw_dmc_step(self.h, C.byref(params), C.byref(st)))
+        return st
+
+    def dmc_population(self):
+        """(x[nwalk][N][3], weight[nwalk], rng[nwalk][4]) of the population."""
+        n = C.c_int32(0)
+        self._chk(self.lib.qmcw_dmc_get(self.h, C.byref(n), None))
+        w = np.zeros(max(n.value, 1))
+        self._chk(self.lib.qmcw_dmc_get(self.h, C.byref(n), _d(w)))
+        return self.get_walkers()[:n.value], w[:n.value], self.get_rng()[:n.value]
+
+    def set_branch_rng(self, seed):
+        s = np.ascontiguousarray(seed, np.int32).reshape(4)
+        self._chk(self.lib.qmcw_set_branch_rng(self.h, s.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def get_branch_rng(self):
+        s = np.zeros(4, np.int32)
+        self._chk(self.lib.qmcw_get_branch_rng(self.h, s.ctypes.data_as(C.POINTER(C.c_int32))))
+        return s
+
     def get_eloc(self):
         e = np.zeros(self.W)
         self._chk(self.lib.qmcw_get_eloc(self.h, _d(e)))

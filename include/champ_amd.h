@@ -210,6 +210,72 @@ int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double fbias,
 int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta,
                         double fbias);
 
+/* Parameters of one DMC generation (qmc/rundmc_mod.f90:97-330).  The
+   defaults in brackets are CHAMP's (qmc/control_mod.f90:11-16) and those of
+   its example inputs (idmc 2, itau_eff 1, iacc_rej 1, icross 1). */
+typedef struct qmcw_dmc_params {
+  double tau;          /* time step */
+  double etrial;       /* trial energy */
+  double eest;         /* current energy estimate */
+  double esigma;       /* energy spread used by the reweighting cut-off */
+  double pop_ctrl;     /* population-control factor of this generation */
+  double taueff;       /* running effective time step (itau_eff = 3) */
+  double c_rewt;       /* [3.5] */
+  double pow_rewt;     /* [0.5] */
+  double limit_rewt;   /* limit_rewt_dmc [10] */
+  double adrift;       /* adrift_default [0.5] */
+  int32_t n_drift;     /* n_drift_param [2] */
+  int32_t itau_eff;    /* [1] 0: tau; 1: tau sum(pacc dr2)/sum(dr2); 2: accepted only; 3: taueff */
+  int32_t itau_integ;  /* [1] */
+  int32_t iacc_rej;    /* [1] 0: accept every move */
+  int32_t icross;      /* [1] <= 0: reject node-crossing moves */
+  int32_t ene_int;     /* [0] 0: ene_int_v9, 1: no_ene_int */
+  int32_t idmc;        /* [2] > 0: reweight */
+  int32_t pad_;
+} qmcw_dmc_params;
+
+/* Sums over the population of one generation (before branching) and the
+   population after branching. */
+typedef struct qmcw_dmc_stats {
+  double wsum;         /* sum of weights */
+  double wesum;        /* sum w E_L */
+  double we2sum;       /* sum w E_L^2 */
+  double wtausum;      /* sum w taunow */
+  double pacc;         /* sum of acceptance probabilities */
+  double nacc;         /* accepted moves */
+  double ntry;         /* proposed moves */
+  double ncross;       /* proposed moves that crossed a node */
+  int32_t nwalk;       /* population after branching */
+  int32_t nbranch;     /* kills + joins + splits */
+} qmcw_dmc_stats;
+
+/* DMC population (qmc/rundmc_mod.f90).  qmcw_dmc_begin makes the first
+   nwalk walkers (positions set with qmcw_set_walkers, streams with
+   qmcw_set_rng) the population, with weights (NULL: 1), and evaluates them;
+   the handle's W is the population's capacity (MWALK).  Pseudopotential
+   systems only (the reference's nloc > 0 mover). */
+int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weight);
+
+/* One DMC generation (qmc/rundmc_mod.f90:97-330): for every walker and
+   electron the drift-diffusion proposal (qmc/green_functions_mod.f90:70-80),
+   the one-electron ratio, the Metropolis test with the forward/backward
+   Green's functions (rundmc_mod.f90:142-196) and the commit; the full
+   evaluation of the walkers that moved; the effective time step and the
+   reweighting (ene_int_rewt, green_functions_mod.f90:382-472); then
+   split_join (qmc/dmc/dmc_elec/split_join_mod.f90:3-219) with the branching
+   stream.  Each walker draws from its own rannyu stream; a split copy
+   continues on a stream drawn from the branching stream.  stats (may be
+   NULL) receives the population sums before branching and the population
+   after it.  Positions of the population: qmcw_get_walkers, first nwalk. */
+int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_stats *stats);
+
+/* Current population size and weights ([nwalk], may be NULL). */
+int qmcw_dmc_get(qmcw_handle *h, int32_t *nwalk, double *weight);
+
+/* rannyu stream of split_join (4 limbs as in setrn). */
+int qmcw_set_branch_rng(qmcw_handle *h, const int32_t *seed);
+int qmcw_get_branch_rng(const qmcw_handle *h, int32_t *seed);
+
 /* Read back per-walker local energies of the last evaluation. */
 int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
 

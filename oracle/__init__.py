@@ -74,6 +74,12 @@ def lib():
                                       C.c_double, _pd, C.POINTER(C.c_int64), C.c_int32]),
             "orc_eval_batch": (C.c_int, [_vp, C.c_int32, _pd, _pd, _pd, _pd, _pd, _pd,
                                          C.c_int32]),
+            "orc_dmc_create": (_vp, [_vp, C.c_int32]),
+            "orc_dmc_destroy": (None, [_vp]),
+            "orc_dmc_set": (C.c_int, [_vp, C.c_int32, _pd, _pi, _pd]),
+            "orc_dmc_generation": (C.c_int, [_vp, _vp, _vp, C.c_int32]),
+            "orc_dmc_branch": (C.c_int, [_vp, _pi, _vp]),
+            "orc_dmc_get": (C.c_int32, [_vp, _pd, _pi, _pd, _pd]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -250,3 +256,50 @@ def matinv(a):
     flat = np.ascontiguousarray(a.T.reshape(-1))   # column-major storage
     det = lib().orc_matinv(_d(flat), n)
     return flat.reshape(n, n).T.copy(), det
+
+
+class OracleDMC:
+    """A DMC population in the oracle (orc_dmc): per-walker streams, one
+    generation of moves + reweighting, split_join with a branching stream."""
+
+    def __init__(self, system, cap):
+        self.o = Oracle(system)
+        self.N = system.nelec
+        self.cap = int(cap)
+        self.h = self.o.L.orc_dmc_create(self.o.h, self.cap)
+
+    def set(self, x, rng, weight=None):
+        x = np.ascontiguousarray(x, np.float64)
+        rng = np.ascontiguousarray(rng, np.int32)
+        w = None if weight is None else np.ascontiguousarray(weight, np.float64)
+        return self.o.L.orc_dmc_set(self.h, x.shape[0], _d(x), _i(rng), _d(w))
+
+    def generation(self, params, nthreads=0):
+        from champ_amd.abi import DmcStats
+        st = DmcStats()
+        r = self.o.L.orc_dmc_generation(self.h, C.byref(params), C.byref(st), nthreads)
+        return r, st
+
+    def branch(self, brng):
+        from champ_amd.abi import DmcStats
+        st = DmcStats()
+        r = self.o.L.orc_dmc_branch(self.h, _i(brng), C.byref(st))
+        return r, st
+
+    def get(self):
+        x = np.zeros((self.cap, self.N, 3))
+        rng = np.zeros((self.cap, 4), np.int32)
+        w, e = np.zeros(self.cap), np.zeros(self.cap)
+        n = self.o.L.orc_dmc_get(self.h, _d(x), _i(rng), _d(w), _d(e))
+        return dict(x=x[:n], rng=rng[:n], weight=w[:n], eloc=e[:n])
+
+    def close(self):
+        if self.h:
+            self.o.L.orc_dmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -2414,3 +2414,303 @@ void orc_walker_set_quad(orc_walker *w, const double *xq, const double *yq,
   memcpy(w->yq, yq, sizeof(double) * nq);
   memcpy(w->zq, zq, sizeof(double) * nq);
 }
+
+/* ------------------------------------------------------------------ */
+/* DMC                                                                 */
+/* ------------------------------------------------------------------ */
+/* averaged_1e_velocity, qmc/green_functions_mod.f90:598-633, with
+   adrift = adrift_default (no modified adrift, no nucleus term) */
+static void dmc_vav(double tau, double adrift, int ndrift, const double *g, double *v) {
+  double vn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  double vav;
+  if (adrift * vn * vn * tau <= 0) vav = vn;
+  else vav = (-1 + pow(1 + ndrift * adrift * vn * vn * tau, 1.0 / ndrift)) / (adrift * vn * tau);
+  for (int k = 0; k < 3; k++) v[k] = vav * g[k] / vn;
+}
+
+/* sample_1e_prob_drift_diffusion, qmc/green_functions_mod.f90:257-265 */
+static double dmc_prob(double tau, const double *vav, const double *xo, const double *xn) {
+  const double pi = 3.141592653589793;
+  double d2 = 0;
+  for (int k = 0; k < 3; k++) {
+    double d = xn[k] - xo[k] - vav[k] * tau;
+    d2 += d * d;
+  }
+  double nrm = sqrt(2 * pi * tau);
+  return exp(-d2 / (2 * tau)) / (nrm * nrm * nrm);
+}
+
+/* ene_int_rewt, qmc/green_functions_mod.f90:382-472 (ene_int_v9 or
+   no_ene_int) */
+static double dmc_eint(const qmcw_dmc_params *p, int N, double tauint, double eo,
+                       const double *go, double en, const double *gn) {
+  const double pi = 3.141592653589793;
+  double ewto, ewtn;
+  if (p->ene_int == 1) {
+    ewto = eo;
+    ewtn = en;
+  } else {
+    double so = 0, sn = 0;
+    for (int q = 0; q < 3 * N; q++) { so += go[q] * go[q]; sn += gn[q] * gn[q]; }
+    double facto = fmax(sqrt(so) / pow((double)N, p->pow_rewt) * tauint * p->c_rewt, 1e-9);
+    double factn = fmax(sqrt(sn) / pow((double)N, p->pow_rewt) * tauint * p->c_rewt, 1e-9);
+    ewto = p->eest + (eo - p->eest) * (sqrt(pi) / 2) * erf(facto) / facto;
+    ewtn = p->eest + (en - p->eest) * (sqrt(pi) / 2) * erf(factn) / factn;
+  }
+  double eint = (ewto + ewtn) / 2 - p->etrial;
+  double rtrttau = sqrt(sqrt(p->tau));
+  double emin = p->eest - p->etrial - p->limit_rewt * p->esigma / rtrttau;
+  return eint < emin ? emin : eint;
+}
+
+struct orc_dmc {
+  const orc_sys *o;
+  int cap, nwalk;
+  orc_walker **w;
+  int32_t *rng;       /* [cap][4] */
+  double *weight;     /* [cap] */
+  double *wout;       /* [cap][8] per-walker generation results */
+};
+
+orc_dmc *orc_dmc_create(const orc_sys *o, int32_t cap) {
+  orc_dmc *d = calloc(1, sizeof(orc_dmc));
+  d->o = o;
+  d->cap = cap;
+  d->w = calloc(cap, sizeof(orc_walker *));
+  for (int i = 0; i < cap; i++) d->w[i] = orc_walker_create(o);
+  d->rng = calloc((size_t)cap * 4, sizeof(int32_t));
+  d->weight = calloc(cap, sizeof(double));
+  d->wout = calloc((size_t)cap * 8, sizeof(double));
+  return d;
+}
+
+void orc_dmc_destroy(orc_dmc *d) {
+  if (!d) return;
+  for (int i = 0; i < d->cap; i++) orc_walker_destroy(d->w[i]);
+  free(d->w); free(d->rng); free(d->weight); free(d->wout);
+  free(d);
+}
+
+/* population of nwalk walkers at x with their streams and weights (NULL:
+   1); each gets a full evaluation (psi_at) */
+int orc_dmc_set(orc_dmc *d, int32_t nwalk, const double *x, const int32_t *rng,
+                const double *weight) {
+  int N = d->o->nelec, err = 0;
+  if (nwalk > d->cap) return QMCW_ERR_ARG;
+  d->nwalk = nwalk;
+  for (int i = 0; i < nwalk; i++) {
+    int st = orc_eval_full(d->o, d->w[i], x + (size_t)i * 3 * N, NULL, NULL, NULL);
+    if (st) err = st;
+    memcpy(d->rng + 4 * i, rng + 4 * i, 4 * sizeof(int32_t));
+    d->weight[i] = weight ? weight[i] : 1.0;
+  }
+  return err;
+}
+
+/* one walker's generation: qmc/rundmc_mod.f90:114-303 for the nloc > 0
+   mover (sample_1e_move_drift_diffusion, green_functions_mod.f90:70-80),
+   the walker's own rannyu stream for the grid rotation, the gaussians and
+   the Metropolis tests */
+static int dmc_walker(const orc_sys *o, orc_walker *w, int32_t *rng, double *weight,
+                      const qmcw_dmc_params *p, double *res) {
+  int N = o->nelec;
+  double go[3 * 128 + 3], eo = w->eloc;
+  for (int q = 0; q < 3 * N; q++) go[q] = w->vj[q] + w->vd[q];
+  if (o->nloc > 0) orc_rotqua(rng, o->xq0, o->yq0, o->zq0, o->nquad, w->xq, w->yq, w->zq);
+  double dd[128], pacc[128], sdd = 0, spd = 0, sad = 0, pa_sum = 0, nacc = 0, ncross = 0;
+  int l_accept = 0, st = 0;
+  double rt = sqrt(p->tau);
+  for (int ie = 0; ie < N; ie++) {
+    double gold[3], vo[3], vn[3], gnew[3], xo[3], xn[3], out4[4], lap;
+    orc_grad_1e(o, w, ie, gold, NULL);
+    dmc_vav(p->tau, p->adrift, p->n_drift, gold, vo);
+    double d2 = 0;
+    for (int k = 0; k < 3; k++) {
+      double xd = rt * orc_gauss(rng);
+      xo[k] = w->x[3 * ie + k];
+      xn[k] = xo[k] + vo[k] * p->tau + xd;
+      d2 += xd * xd;
+    }
+    dd[ie] = d2;
+    int s1 = orc_ratio_1e(o, w, ie, xn, out4, gnew, &lap);
+    if (s1) st = s1;
+    dmc_vav(p->tau, p->adrift, p->n_drift, gnew, vn);
+    double po = dmc_prob(p->tau, vo, xo, xn);
+    double pn = dmc_prob(p->tau, vn, xn, xo);
+    double psirat = out4[0] * out4[0];
+    double pa = fmin(1.0, psirat * pn / po);
+    if (p->iacc_rej == 0) pa = 1.0;
+    if (out4[0] <= 0) {
+      if (p->icross <= 0) pa = 0.0;
+      ncross += 1;
+    }
+    pacc[ie] = pa;
+    int acc = orc_rannyu(rng) < pa;
+    orc_accept_1e(o, w, ie, acc);
+    l_accept |= acc;
+    nacc += acc;
+    pa_sum += pa;
+    sdd += d2;
+    spd += pa * d2;
+    sad += acc * d2;
+  }
+  if (l_accept) {
+    int s2 = eval_full_w(o, w, 0);
+    if (s2) st = s2;
+  }
+  double taunow = p->tau;
+  if (p->itau_eff >= 1) taunow = p->tau * (p->itau_eff == 2 ? sad : spd) / sdd;
+  double tauint = p->tau;
+  if (p->itau_integ >= 1) tauint = (p->itau_eff == 3) ? p->taueff : taunow;
+  double wo = *weight;
+  if (p->idmc > 0) {
+    double gn[3 * 128 + 3];
+    for (int q = 0; q < 3 * N; q++) gn[q] = w->vj[q] + w->vd[q];
+    double eint = dmc_eint(p, N, tauint, eo, go, w->eloc, gn);
+    double enetau = eint * ((p->itau_eff == 3) ? p->taueff : taunow);
+    wo = wo * exp(-enetau) * p->pop_ctrl;
+  }
+  *weight = wo;
+  res[0] = taunow; res[1] = pa_sum; res[2] = nacc; res[3] = ncross;
+  (void)pacc; (void)dd;
+  return st;
+}
+
+int orc_dmc_generation(orc_dmc *d, const qmcw_dmc_params *p, qmcw_dmc_stats *stats,
+                       int32_t nthreads) {
+  int err = 0;
+  if (d->o->nelec > 128) return QMCW_ERR_ARG;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < d->nwalk; i++) {
+    int st = dmc_walker(d->o, d->w[i], d->rng + 4 * i, d->weight + i, p, d->wout + 8 * i);
+    if (st) {
+#pragma omp atomic write
+      err = st;
+    }
+  }
+  memset(stats, 0, sizeof(*stats));
+  for (int i = 0; i < d->nwalk; i++) {
+    double w = d->weight[i], e = d->w[i]->eloc;
+    stats->wsum += w;
+    stats->wesum += w * e;
+    stats->we2sum += w * e * e;
+    stats->wtausum += w * d->wout[8 * i];
+    stats->pacc += d->wout[8 * i + 1];
+    stats->nacc += d->wout[8 * i + 2];
+    stats->ncross += d->wout[8 * i + 3];
+    stats->ntry += d->o->nelec;
+  }
+  stats->nwalk = d->nwalk;
+  return err;
+}
+
+/* a split copy continues on a stream drawn from the branching stream:
+   four 12-bit limbs, the last one odd as setrn makes it */
+static void dmc_new_stream(int32_t *brng, int32_t *dst) {
+  for (int k = 0; k < 4; k++) dst[k] = (int32_t)(orc_rannyu(brng) * 4096.0) & 4095;
+  dst[3] |= 1;
+}
+
+static int cmp_dbl(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
+/* split_join, qmc/dmc/dmc_elec/split_join_mod.f90:3-219: join pairs of
+   walkers with weight < 1/2, kill zero weights, split weights above
+   max(2, wt_split) into holes or at the end, then close the holes. */
+int orc_dmc_branch(orc_dmc *d, int32_t *brng, qmcw_dmc_stats *stats) {
+  const orc_sys *o = d->o;
+  int n = d->nwalk, MW = d->cap;
+  int *iwundr = malloc(sizeof(int) * (MW + 1));
+  double *wsav = malloc(sizeof(double) * (MW + 1));
+  double *w = d->weight;
+  int iunder = 0, ipair = 0, iw2 = 0, nbr = 0;
+  for (int iw = 0; iw < n; iw++) {
+    if (w[iw] < 0.5) {
+      if (w[iw] == 0.0) {
+        nbr++;
+        iwundr[iunder++] = iw;
+      } else if (ipair == 0) {
+        ipair = 1;
+        iw2 = iw;
+      } else {
+        nbr++;
+        ipair = 0;
+        double wt = w[iw] + w[iw2];
+        if (orc_rannyu(brng) > w[iw] / wt) {
+          w[iw2] = wt;
+          iwundr[iunder++] = iw;
+        } else {
+          w[iw] = wt;
+          iwundr[iunder++] = iw2;
+        }
+      }
+    }
+  }
+  int nafter = n - iunder, nmax_dupl = MW - nafter;
+  for (int iw = 0; iw < n; iw++) wsav[iw] = w[iw];
+  qsort(wsav, n, sizeof(double), cmp_dbl);
+  /* 1-based wt_sav(max(1, n - nmax_dupl)) */
+  int i1 = n - nmax_dupl; if (i1 < 1) i1 = 1;
+  double wt_split = fmax(2.0, wsav[i1 - 1]);
+  int nmore = 0;
+  for (int iw = i1; iw <= n; iw++) {
+    double r = wsav[iw - 1] / wt_split;
+    if (r >= 8.0) nmore += 3;
+    else if (r >= 4.0) nmore += 2;
+    else if (r >= 2.0) nmore += 1;
+  }
+  if (n - nmax_dupl + nmore < n) {
+    int i2 = n - nmax_dupl + nmore; if (i2 < 1) i2 = 1;
+    wt_split = fmax(2.0, wsav[i2 - 1]);
+  } else {
+    wt_split = 1e99;
+  }
+  int n2 = n, err = 0;
+  for (int iw = 0; iw < n; iw++) {
+    if (w[iw] > wt_split) {
+      nbr++;
+      int dst;
+      if (iunder > 0) dst = iwundr[--iunder];
+      else {
+        if (n2 >= MW) { err = QMCW_ERR_NOMEM; break; }
+        dst = n2++;
+      }
+      w[iw] *= 0.5;
+      w[dst] = w[iw];
+      orc_walker_copy(o, d->w[dst], d->w[iw]);
+      dmc_new_stream(brng, d->rng + 4 * dst);
+    }
+  }
+  for (int j = iunder - 1; j >= 0; j--) {
+    int dst = iwundr[j], src = n2 - 1;
+    n2--;
+    if (dst != src) {
+      w[dst] = w[src];
+      orc_walker_copy(o, d->w[dst], d->w[src]);
+      memcpy(d->rng + 4 * dst, d->rng + 4 * src, 4 * sizeof(int32_t));
+    }
+  }
+  d->nwalk = n2;
+  if (stats) { stats->nwalk = n2; stats->nbranch = nbr; }
+  free(iwundr);
+  free(wsav);
+  return err;
+}
+
+int32_t orc_dmc_get(const orc_dmc *d, double *x, int32_t *rng, double *weight, double *eloc) {
+  int N = d->o->nelec;
+  for (int i = 0; i < d->nwalk; i++) {
+    if (x) memcpy(x + (size_t)i * 3 * N, d->w[i]->x, sizeof(double) * 3 * N);
+    if (rng) memcpy(rng + 4 * i, d->rng + 4 * i, 4 * sizeof(int32_t));
+    if (weight) weight[i] = d->weight[i];
+    if (eloc) eloc[i] = d->w[i]->eloc;
+  }
+  return d->nwalk;
+}

@@ -109,6 +109,20 @@ int orc_eval_batch(const orc_sys *o, int32_t nw, const double *x,
                    double *out9, double *grad, double *divv, double *dpsi,
                    double *deloc, int32_t nthreads);
 
+/* DMC (qmc/rundmc_mod.f90, nloc > 0 mover): a population of up to cap
+   walkers with per-walker rannyu streams; one generation = moves of every
+   electron, full evaluation of walkers that moved, reweighting; branching =
+   split_join with its own stream. */
+typedef struct orc_dmc orc_dmc;
+orc_dmc *orc_dmc_create(const orc_sys *o, int32_t cap);
+void orc_dmc_destroy(orc_dmc *d);
+int orc_dmc_set(orc_dmc *d, int32_t nwalk, const double *x, const int32_t *rng,
+                const double *weight);
+int orc_dmc_generation(orc_dmc *d, const qmcw_dmc_params *p, qmcw_dmc_stats *stats,
+                       int32_t nthreads);
+int orc_dmc_branch(orc_dmc *d, int32_t *brng, qmcw_dmc_stats *stats);
+int32_t orc_dmc_get(const orc_dmc *d, double *x, int32_t *rng, double *weight, double *eloc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""Pins of the oracle's DMC (orc_dmc_*), independent of the GPU.
+
+  reweighting      with the exact ground state every E_L is -Z^2/2, so each
+                   generation multiplies every weight by exp(-(E0 - ET) tau)
+                   pop_ctrl (ene_int_v9 collapses to eest when E_L = eest)
+  drift-diffusion  the walk samples |Psi|^2 Green's-function-wise: for the
+                   exact hydrogenic 1s state <r> = 3/(2Z) within 4 sigma
+  split_join       total weight is conserved, no walker keeps weight zero,
+                   population = n - kills - joins + splits, and every walker
+                   after branching is a copy of one before
+"""
+import numpy as np
+import pytest
+
+import oracle
+from champ_amd import rng, system
+from champ_amd.abi import DmcParams
+
+
+def _hydrogenic_pp(Z):
+    """1 electron, exact 1s orbital, no Jastrow; a pseudopotential table with
+    only the local channel (which is empty), so that V = -Z/r and the
+    pseudopotential mover of the reference applies."""
+    bb = system._BasisBuilder(system.QMCW_BASIS_SLATER, 1)
+    r = bb.add_radial(1, [float(Z)], [1.0])
+    bb.add_function(0, r, 0, 0)
+    barr, anorm = bb.arrays(np.array([0], np.int32))
+    ecp = system._ecp_arrays([dict(local=[], l0=[], l1=[])])
+    npotd = np.array([1], np.int32)
+    lpotp1 = np.array([1], np.int32)
+    return system.System(nelec=1, nup=1, cent=np.zeros((1, 3)), iwctype=np.array([0], np.int32),
+                         znuc=np.array([float(Z)]), coef=np.array([[anorm[0]]]),
+                         det_up=np.array([[0]], np.int32), det_dn=np.zeros((1, 0), np.int32),
+                         csf_coef=np.array([1.0]), csf_det_off=np.array([0, 1], np.int32),
+                         csf_det=np.array([0], np.int32), csf_det_coef=np.array([1.0]),
+                         isc=4, scalek=1.0, norda=2, nordb=2, nordc=0,
+                         a4=np.zeros((1, 3)), b=np.array([[0.5, 1.0, 0.0]]), c=np.zeros((1, 0)),
+                         nloc=6, npotd=npotd, lpotp1=lpotp1, ecp_off=np.zeros_like(ecp[2]),
+                         ecp_coef=ecp[3][:0], ecp_power=ecp[4][:0], ecp_expo=ecp[5][:0],
+                         nquad=12, **barr)
+
+
+@pytest.mark.parametrize("Z", [1, 2])
+def test_exact_wavefunction_weights(Z):
+    s = _hydrogenic_pp(Z)
+    W = 64
+    x = system.random_walkers(s, W, seed=3, sigma=1.0 / Z)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    d = oracle.OracleDMC(s, W)
+    assert d.set(x, seeds) == 0
+    e0, et, tau = -0.5 * Z * Z, -0.5 * Z * Z + 0.3, 0.02
+    p = DmcParams(tau, et, eest=e0, esigma=0.5, pop_ctrl=1.1, itau_eff=0)
+    w = np.ones(W)
+    for _ in range(3):
+        st, stats = d.generation(p)
+        assert st == 0
+        w *= np.exp(-(e0 - et) * tau) * 1.1
+        g = d.get()
+        assert np.allclose(g["eloc"], e0, rtol=0, atol=1e-12)
+        assert np.allclose(g["weight"], w, rtol=1e-12)
+        assert stats.nacc > 0 and stats.ntry == W
+
+
+def test_drift_diffusion_samples_the_ground_state():
+    Z = 1
+    s = _hydrogenic_pp(Z)
+    W = 2000
+    x = system.random_walkers(s, W, seed=5, sigma=1.0)
+    seeds = rng.walker_seeds("1122334455667789", W)
+    d = oracle.OracleDMC(s, W)
+    assert d.set(x, seeds) == 0
+    p = DmcParams(0.01, -0.5, eest=-0.5, esigma=0.5, idmc=0)
+    rs = []
+    for gen in range(120):
+        assert d.generation(p)[0] == 0
+        if gen >= 40 and gen % 4 == 0:
+            rs.append(np.linalg.norm(d.get()["x"][:, 0, :], axis=1))
+    r = np.array(rs)
+    # generations 4 apart are correlated; use block means per walker
+    m = r.mean(axis=0)
+    est, err = m.mean(), m.std() / np.sqrt(W)
+    assert abs(est - 1.5 / Z) < 4 * err + 0.01, (est, err)
+
+
+def test_split_join_invariants():
+    s = system.c2_dimer(2)
+    W, cap = 24, 48
+    x = system.random_walkers(s, W, seed=9)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    d = oracle.OracleDMC(s, cap)
+    rs = np.random.default_rng(2)
+    w = rs.choice([0.0, 0.2, 0.3, 0.45, 1.0, 1.3, 2.6, 5.5, 9.0], size=W)
+    assert d.set(x, seeds, w) == 0
+    before = d.get()
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    st, stats = d.branch(brng)
+    assert st == 0
+    after = d.get()
+    assert abs(after["weight"].sum() - w.sum()) < 1e-12 * w.sum()
+    assert (after["weight"] > 0).all()
+    nkill = int((w == 0).sum())
+    nsmall = int(((w > 0) & (w < 0.5)).sum())
+    njoin = nsmall // 2
+    nsplit = stats.nbranch - nkill - njoin
+    assert nsplit >= int((w > 9.0 - 1e-12).sum())
+    assert stats.nwalk == W - nkill - njoin + nsplit == len(after["weight"])
+    # copies: each walker after branching has the positions of one before
+    flat = before["x"].reshape(W, -1)
+    for xa in after["x"].reshape(len(after["weight"]), -1):
+        assert (np.abs(flat - xa).max(axis=1) == 0).any()
+    # at most one unpaired walker below 1/2 remains
+    assert int((after["weight"] < 0.5).sum()) <= 1
