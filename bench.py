@@ -178,16 +178,87 @@ def run_engine(s, W, steps, warmup, precision, delta, x, seeds, prof_steps):
     return e, t1 - t0, prof, g
 
 
+def run_derivs(W, steps, warmup, precision, delta, prof_steps):
+    """Derivative samples: VMC steps whose full evaluation also forms
+    d ln Psi/dp and dE_L/dp (the optimisation sample), C2-like system with
+    its parameter set (CSF, Jastrow, orbital rotations)."""
+    from champ_amd import rng, system
+    from champ_amd.engine import Engine
+    s = system.c2_dimer(seed=2, derivs=True)
+    e = Engine(s, W, precision)
+    e.set_walkers(system.random_walkers(s, W, seed=8))
+    e.set_rng(rng.walker_seeds("1837465927472523", W, stride=1 << 20))
+    g = e.autotune()
+    e.sample_derivs(True)
+    e.vmc_device(max(1, warmup), delta)
+    e.sync()
+    t0 = time.perf_counter()
+    e.vmc_device(steps, delta)
+    e.sync()
+    dt = time.perf_counter() - t0
+    e.profile(True)
+    e.vmc_device(prof_steps, delta)
+    e.sync()
+    prof = e.profile_get()
+    e.profile(False)
+    return dict(engine=e, rate=W * steps / dt, dt=dt, prof=prof, G=g, P=e.P, system=s)
+
+
+def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
+    """Multi-system batch: nsys synthetic molecules (4-128 electrons, 1-24
+    nuclei, 10-800 GTOs), 256-4096 walkers each, one handle (and HIP stream)
+    per system.  Each system is timed alone, then all together: every
+    handle's steps are queued before any is awaited, so the systems run
+    concurrently on their own streams."""
+    from champ_amd import rng, system
+    from champ_amd.engine import Engine
+    rs = np.random.default_rng(seed0)
+    engines = []
+    for k in range(nsys):
+        s = system.random_molecule(seed0 + k)
+        W = int(rs.integers(256, 4097))
+        e = Engine(s, W, precision)
+        e.set_walkers(system.random_walkers(s, W, seed=seed0 + k))
+        e.set_rng(rng.walker_seeds("1837465927472523", W, stride=(1 << 20) + k))
+        g = e.autotune()
+        engines.append((s, W, e, g))
+    per = []
+    for s, W, e, g in engines:
+        e.vmc_device(max(1, warmup), delta)
+        e.sync()
+        t0 = time.perf_counter()
+        e.vmc_device(steps, delta)
+        e.sync()
+        dt = time.perf_counter() - t0
+        per.append(dict(nelec=int(s.nelec), ncent=int(s.ncent), nbasis=int(s.nbasis), walkers=W,
+                        G=g, walker_moves_per_s=W * steps / dt))
+    t0 = time.perf_counter()
+    for s, W, e, g in engines:
+        e.vmc_device(steps, delta)
+    for s, W, e, g in engines:
+        e.sync()
+    dt = time.perf_counter() - t0
+    agg = sum(W for _, W, _, _ in engines) * steps / dt
+    for _, _, e, _ in engines:
+        e.close()
+    return dict(aggregate=agg, per_system=per, seconds=dt)
+
+
 def copy_calibration(nbytes=256 << 20):
-    """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE
-    counters (launched only with --copy-calibration)."""
-    import torch
-    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
+    """Three device-to-device copies of nbytes (HIP's blit copy kernel), the
+    known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters
+    (launched only with --copy-calibration)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(nbytes)
+    assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    assert hip.hipMemset(a, 0, n) == 0
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
+        assert hip.hipMemcpy(b, a, n, 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
     print(f"[bench] copy calibration: 3 copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
@@ -203,11 +274,27 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--deriv-walkers", type=int, default=8192)
+    ap.add_argument("--deriv-steps", type=int, default=40)
+    ap.add_argument("--derivs", action="store_true", help="also measure derivative samples/s")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="only run the multi-system batch with this many systems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs and reports a single GPU; "
               "multi-GPU runs (walker sharding) are not part of it.", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        prec = "mixed" if args.precision == "both" else args.precision
+        b = run_batch(args.batch, args.steps, args.warmup, prec, args.delta)
+        print(json.dumps({"metric": "walker-moves/s (multi-system batch)", "value": b["aggregate"],
+                          "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "higher_is_better": True, "dtype":
+                          "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
+                          "config": {"workload": f"{args.batch} synthetic molecules, one stream each",
+                                     "precision": prec},
+                          "per_system": b["per_system"]}), flush=True)
+        return
     from champ_amd import rng, system
     s = system.c2_dimer(seed=2)
     W = args.walkers
@@ -224,6 +311,14 @@ def main():
             if n:
                 print(f"[bench]   {k:12s} {n:5d} launches  avg {ms / n * 1e3:9.1f} us  "
                       f"total {ms:9.3f} ms", file=sys.stderr, flush=True)
+    # derivative samples/s (the optimisation sample)
+    dres = {}
+    if args.derivs:
+        for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+            dres[prec] = run_derivs(args.deriv_walkers, args.deriv_steps, args.warmup, prec,
+                                    args.delta, max(2, args.prof_steps // 2))
+            print(f"[bench] {prec} derivatives: {dres[prec]['rate']:.4g} samples/s "
+                  f"(P={dres[prec]['P']}, G={dres[prec]['G']})", file=sys.stderr, flush=True)
     # exact-configuration energy bias of the mixed engine against FP64
     bias = None
     if "fp64" in res and "mixed" in res:
@@ -273,6 +368,14 @@ def main():
         "energy_bias_mixed_vs_fp64": bias,
         "electron_moves_per_s": r["rate"] * s.nelec,
         "kernel_avg_us": {k: (1e3 * v[0] / v[1] if v[1] else None) for k, v in r["prof"].items()},
+        "deriv_samples_per_s": {k: v["rate"] for k, v in dres.items()} or None,
+        "deriv_config": ({"workload": "C2-like dimer with CSF, Jastrow and orbital-rotation "
+                                      "parameters, VMC steps with derivatives",
+                          "walkers": args.deriv_walkers, "nparm": next(iter(dres.values()))["P"]}
+                         if dres else None),
+        "deriv_mixed_vs_fp64": (dres["mixed"]["rate"] / dres["fp64"]["rate"]) if len(dres) == 2 else None,
+        "deriv_kernel_avg_us": ({k: (1e3 * v[0] / v[1] if v[1] else None)
+                                 for k, v in next(iter(dres.values()))["prof"].items()} if dres else None),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": dom, "avg_us": 1e3 * avg_ms, "flop_per_launch": fl,
@@ -282,7 +385,7 @@ def main():
         copy_calibration()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, args.cpu_seconds)
-    for v in res.values():
+    for v in list(res.values()) + list(dres.values()):
         v["engine"].close()
     print(json.dumps(out), flush=True)
 

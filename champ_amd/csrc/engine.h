@@ -46,6 +46,11 @@ struct DevSys {
   const int *exl[2];         // [ndetu][ns] excited orbital labels
   const int *exo[2];         // [ndetu][ns] excited orbital positions in occ
   const int *iwdet[2];       // [ndet] unique det of each spin
+  const int *udet_off[2];    // [ndetu+1] CSR: dets whose spin-s part is unique det u
+  const int *udet_d[2];      // [ndet]
+  const int *yent_off[2];    // [nocc*ns+1] CSR: contributions to Y(jo, ir)
+  const int *yent_u[2];      // unique det of the contribution
+  const int *yent_pos[2];    // its alpha-inverse element (u*kmax2 + jj + k*ii)
   const double *cdet;        // [ndet] cdet_in_wf
   int kmax2;                 // kmax*kmax (>= 1)
   // Jastrow ijas=4

@@ -46,6 +46,14 @@ struct qmcw_handle {
   double *der = nullptr;     // [Wp][2 nparm] dpsi, deloc of the last derivative evaluation
   int sample_derivs = 0;     // VMC steps evaluate derivatives (CHAMP's ideriv)
   int der_valid = 0;
+  // one VMC step captured as a HIP graph (qmcw_vmc_run_device) and its key
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_ok = 0, graph_G = 0, graph_Wl = 0, graph_derivs = 0;
+  double graph_delta = 0.0, graph_fbias = 0.0;
+  const double *graph_st = nullptr;
+  int use_graph = 1;
+  int use_lds = 1;           // LDS staging of the moved electron's basis values
   // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
   int dmc_on = 0, nact = 0;
   double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
@@ -479,12 +487,17 @@ template <int G>
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
                          double fbias, double *out) {
   dim3 grid(h->Wl / (kWave / G)), block(kWave);
+  // the moved electron's basis values in LDS when they fit (<= 48 KB per block)
+  const size_t lds = (size_t)(kWave / G) * h->A.S.NB * 5 * sizeof(double);
+  KArgs A = h->A;
+  A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;
+  const size_t smem = A.lds_phi ? lds : 0;
   prof_begin(h, QMCW_PROF_RATIO);
   if (h->precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_ratio_1e<double, G>), grid, block, 0, h->stream, h->A, ie, mode, xn,
+    hipLaunchKernelGGL((k_ratio_1e<double, G>), grid, block, smem, h->stream, A, ie, mode, xn,
                        delta, fbias, out);
   else
-    hipLaunchKernelGGL((k_ratio_1e<float, G>), grid, block, 0, h->stream, h->A, ie, mode, xn,
+    hipLaunchKernelGGL((k_ratio_1e<float, G>), grid, block, smem, h->stream, A, ie, mode, xn,
                        delta, fbias, out);
   prof_end(h);
 }
@@ -634,6 +647,34 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     UP(D.ivir[sp], S.ivir[sp]); UP(D.ord[sp], S.ord[sp]); UP(D.sgn[sp], S.dsgn[sp]);
     UP(D.rf[sp], S.rf[sp]); UP(D.exl[sp], S.exl[sp]); UP(D.exo[sp], S.exo[sp]);
     UP(D.iwdet[sp], S.iwdet[sp]); UP(D.iocc_lab[sp], S.iocc_lab[sp]);
+    // CSR lists of eval_y: dets of each unique det; contributions to each Y entry
+    {
+      const int nu = D.ndetu[sp], ns = D.ns[sp], nocc = D.nocc[sp];
+      std::vector<int> uoff(nu + 1, 0), ud;
+      for (int u = 0; u < nu; u++) {
+        for (int d = 0; d < s->ndet; d++)
+          if (D.iwdet[sp][d] == u) ud.push_back(d);
+        uoff[u + 1] = (int)ud.size();
+      }
+      std::vector<int> yoff(nocc * ns + 1, 0), yu, yp;
+      for (int e = 0; e < nocc * ns; e++) {
+        const int jo = e / ns, ir = e % ns;
+        for (int u = 1; u < nu; u++) {
+          const int k = D.ord[sp][u];
+          int jj = -1, ii = -1;
+          for (int q = 0; q < k; q++) {
+            if (D.exo[sp][u * ns + q] == jo) jj = q;
+            if (D.rf[sp][u * ns + q] == ir) ii = q;
+          }
+          if (jj < 0 || ii < 0) continue;
+          yu.push_back(u);
+          yp.push_back(u * S.kmax2 + jj + k * ii);
+        }
+        yoff[e + 1] = (int)yu.size();
+      }
+      UP(uoff, S.udet_off[sp]); UP(ud, S.udet_d[sp]);
+      UP(yoff, S.yent_off[sp]); UP(yu, S.yent_u[sp]); UP(yp, S.yent_pos[sp]);
+    }
   }
   h->ncsf = s->ncsf; h->ndet = s->ndet;
   h->csf_det_off.assign(s->csf_det_off, s->csf_det_off + s->ncsf + 1);
@@ -819,13 +860,18 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   (void)hipMemsetAsync(h->nacc, 0, (size_t)h->Wp * sizeof(long long), h->stream);
   h->A.Wp = h->Wp;
   h->Wl = h->Wp;
+  if (const char *e = getenv("QMCW_NO_GRAPH")) h->use_graph = atoi(e) == 0;
+  if (const char *e = getenv("QMCW_NO_LDS")) h->use_lds = atoi(e) == 0;
   if (hipStreamSynchronize(h->stream) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_HIP, "init sync"); }
   *out = h;
   return QMCW_OK;
 }
 
+static void drop_graph(qmcw_handle *h);
+
 extern "C" int qmcw_destroy(qmcw_handle *h) {
   if (!h) return QMCW_OK;
+  drop_graph(h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   for (void *p : h->allocs) (void)hipFree(p);
@@ -847,9 +893,12 @@ extern "C" int qmcw_sync(qmcw_handle *h) {
   return QMCW_OK;
 }
 
+static void drop_graph(qmcw_handle *h);
+
 extern "C" int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b, const double *c,
                                const double *csf_coef, const double *coef) {
   if (!h) return QMCW_ERR_ARG;
+  drop_graph(h);   // the captured kernel arguments hold the Jastrow asymptotes
   DevSys &S = h->A.S;
   if (a4) h->a4.assign(a4, a4 + h->a4.size());
   if (b) h->b.assign(b, b + h->b.size());
@@ -1025,20 +1074,62 @@ extern "C" int qmcw_grad_1e(qmcw_handle *h, int32_t ie, double *grad, double *la
   return QMCW_OK;
 }
 
+// one VMC step: grid rotation, sweep over the electrons, full evaluation
+static void vmc_one_step(qmcw_handle *h, double delta, double fbias) {
+  const int N = h->A.S.N;
+  if (h->A.S.nloc > 0) {
+    prof_begin(h, QMCW_PROF_ROTQUA);
+    hipLaunchKernelGGL(k_rotqua, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A);
+    prof_end(h);
+  }
+  for (int ie = 0; ie < N; ie++) {
+    DISPATCH_G(h, launch_ratio, ie, 1, (const double *)nullptr, delta, fbias, (double *)nullptr);
+    DISPATCH_G(h, launch_accept, ie, 1, (const uint8_t *)nullptr, delta, fbias, h->nacc);
+  }
+  DISPATCH_G(h, launch_eval, h->sample_derivs != 0);
+}
+
+static void drop_graph(qmcw_handle *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->graph_ok = 0;
+}
+
+// The step's launches (kernels only; the arguments, including the walker
+// state pointer, are captured by value) as a graph, rebuilt when a captured
+// value changes.
+static int step_graph(qmcw_handle *h, double delta, double fbias) {
+  const int derivs = h->sample_derivs != 0;
+  if (h->graph_ok && h->graph_G == h->G && h->graph_Wl == h->Wl && h->graph_derivs == derivs &&
+      h->graph_delta == delta && h->graph_fbias == fbias && h->graph_st == h->A.st)
+    return QMCW_OK;
+  drop_graph(h);
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  vmc_one_step(h, delta, fbias);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess) return fail(h, QMCW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  h->graph = g;
+  HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  h->graph_ok = 1;
+  h->graph_G = h->G; h->graph_Wl = h->Wl; h->graph_derivs = derivs;
+  h->graph_delta = delta; h->graph_fbias = fbias; h->graph_st = h->A.st;
+  return QMCW_OK;
+}
+
 static int vmc_steps(qmcw_handle *h, int nstep, double delta, double fbias, bool trace) {
-  int N = h->A.S.N;
   DISPATCH_G(h, launch_eval, false);
+  if (!trace && !h->prof && h->use_graph && nstep > 0) {
+    int st = step_graph(h, delta, fbias);
+    if (st) return st;
+    for (int is = 0; is < nstep; is++) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+    HIPCHK(h, hipGetLastError());
+    return QMCW_OK;
+  }
   for (int is = 0; is < nstep; is++) {
-    if (h->A.S.nloc > 0) {
-      prof_begin(h, QMCW_PROF_ROTQUA);
-      hipLaunchKernelGGL(k_rotqua, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A);
-      prof_end(h);
-    }
-    for (int ie = 0; ie < N; ie++) {
-      DISPATCH_G(h, launch_ratio, ie, 1, (const double *)nullptr, delta, fbias, (double *)nullptr);
-      DISPATCH_G(h, launch_accept, ie, 1, (const uint8_t *)nullptr, delta, fbias, h->nacc);
-    }
-    DISPATCH_G(h, launch_eval, h->sample_derivs != 0);
+    vmc_one_step(h, delta, fbias);
     if (trace)
       hipLaunchKernelGGL(k_trace_eloc, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->A,
                          h->trace, is, h->W);

@@ -506,20 +506,20 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
 // sum over the shell's components of gp[basis] * phi_basis at (x, y, z).
 // Shells are complete: slot q holds m = 0, 1, -1, 2, -2, ...; an absent
 // component has basis index -1.
-template <typename T, int L>
-__device__ __forceinline__ T shell_dot_l(T x, T y, T z, const int *bas, const T *gp) {
+template <typename T, int L, typename GP>
+__device__ __forceinline__ T shell_dot_l(T x, T y, T z, const int *bas, const GP *gp) {
   T Sv[2 * L + 1], Sg[2 * L + 1][3];
   solid_harm<T>(L, x, y, z, Sv, Sg);
   T acc = 0;
 #pragma unroll
   for (int q = 0; q < 2 * L + 1; q++) {
     int b = bas[q];
-    if (b >= 0) acc += gp[b] * Sv[q];
+    if (b >= 0) acc += T(gp[b]) * Sv[q];
   }
   return acc;
 }
-template <typename T>
-__device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const T *gp) {
+template <typename T, typename GP = T>
+__device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const GP *gp) {
   int l = S.sh_l[s], a = S.sh_a[s];
   const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
   T g = 0;
@@ -531,11 +531,11 @@ __device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const T *gp) 
   const int *bas = S.sh_bas + s * kMaxShellM;
   T acc;
   switch (l) {
-    case 0: acc = shell_dot_l<T, 0>(x, y, z, bas, gp); break;
-    case 1: acc = shell_dot_l<T, 1>(x, y, z, bas, gp); break;
-    case 2: acc = shell_dot_l<T, 2>(x, y, z, bas, gp); break;
-    case 3: acc = shell_dot_l<T, 3>(x, y, z, bas, gp); break;
-    default: acc = shell_dot_l<T, 4>(x, y, z, bas, gp); break;
+    case 0: acc = shell_dot_l<T, 0, GP>(x, y, z, bas, gp); break;
+    case 1: acc = shell_dot_l<T, 1, GP>(x, y, z, bas, gp); break;
+    case 2: acc = shell_dot_l<T, 2, GP>(x, y, z, bas, gp); break;
+    case 3: acc = shell_dot_l<T, 3, GP>(x, y, z, bas, gp); break;
+    default: acc = shell_dot_l<T, 4, GP>(x, y, z, bas, gp); break;
   }
   return acc * g;
 }

@@ -21,6 +21,7 @@ struct KArgs {
   int Wp;
   int derivs;        // compute Jastrow/NLPP parameter derivative pieces
   int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
+  int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
   int chunk;         // electrons per basis chunk in the full evaluation
 };
 
@@ -125,9 +126,9 @@ __device__ double group_eval_y(const DevSys &S, const double *const inv[2],
   for (int t = lane; t < S.ndetu[0] + S.ndetu[1]; t += G) {
     int s = t < S.ndetu[0] ? 0 : 1, u = s ? t - S.ndetu[0] : t;
     double cu = 0.0;
-    for (int d = 0; d < S.ndet; d++) {
-      if (S.iwdet[s][d] != u) continue;
-      int iu = S.iwdet[0][d], id = S.iwdet[1][d];
+    for (int q = S.udet_off[s][u]; q < S.udet_off[s][u + 1]; q++) {
+      const int d = S.udet_d[s][q];
+      const int iu = S.iwdet[0][d], id = S.iwdet[1][d];
       cu += S.cdet[d] * S.dsgn[0][iu] * S.dsgn[1][id] * detv[0][iu] * detv[1][id];
     }
     cup[t] = cu;
@@ -137,18 +138,9 @@ __device__ double group_eval_y(const DevSys &S, const double *const inv[2],
     int ns = S.ns[s], nocc = S.nocc[s];
     const double *cus = cup + (s ? S.ndetu[0] : 0);
     for (int e = lane; e < nocc * ns; e += G) {
-      int jo = e / ns, ir = e % ns;
       double acc = 0.0;
-      for (int u = 1; u < S.ndetu[s]; u++) {   // u = 0 is the reference, k = 0
-        int k = S.ord[s][u];
-        int jj = -1, ii = -1;
-        for (int q = 0; q < k; q++) {
-          if (S.exo[s][u * ns + q] == jo) jj = q;
-          if (S.rf[s][u * ns + q] == ir) ii = q;
-        }
-        if (jj < 0 || ii < 0) continue;
-        acc += cus[u] * inv[s][u * S.kmax2 + jj + k * ii];
-      }
+      for (int q = S.yent_off[s][e]; q < S.yent_off[s][e + 1]; q++)
+        acc += cus[S.yent_u[s][q]] * inv[s][S.yent_pos[s][q]];
       y[s][e] = acc * chii;
     }
   }
@@ -499,7 +491,8 @@ __global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
       const double *gpi = gp + (int64_t)i * S.NB;
       for (int sh = 0; sh < S.nshell; sh++) {
         int jc = S.sh_cent[sh];
-        rd += shell_dot<double>(S, sh, rvnew[3 * jc], rvnew[3 * jc + 1], rvnew[3 * jc + 2], rnew[jc], gpi);
+        rd += (double)shell_dot<T, double>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1],
+                                           (T)rvnew[3 * jc + 2], (T)rnew[jc], gpi);
       }
       // Jastrow value change
       T delta = jas_en_val<T>(S, rnew) - (T)fso[i * N + i];
@@ -691,9 +684,8 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
          *lapn = mv + ML.lapn_i, *fjn = mv + ML.fjn;
   const double *ren_o = ws + L.r_en, *rven_o = ws + L.rvec_en;
   double dsum = 0.0, gix = 0, giy = 0, giz = 0, lapi = 0;
-  for (int j = lane; j <= N; j += G) {
-    if (j == ie) continue;
-    if (j == N) {  // e-n diagonal term
+  for (int j = lane; j < N; j += G) {
+    if (j == ie) {  // e-n diagonal term, on the lane the pair list leaves free
       T f, g[3], lp;
       jas_en<T>(S, ren, rven, f, g, lp);
       fsn[ie] = f;
@@ -743,7 +735,9 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   if (lane == 0) { fjn[3 * ie] = fji[0]; fjn[3 * ie + 1] = fji[1]; fjn[3 * ie + 2] = fji[2]; }
   double lapjn = lapi + fji[0] * fji[0] + fji[1] * fji[1] + fji[2] * fji[2];
   // ---- orbitals of the new position (occupied set of the spin)
-  double *phi = ss + SL.phi;
+  // basis values of the new position: LDS when the launch reserved it
+  extern __shared__ double dyn_lds[];
+  double *phi = A.lds_phi ? dyn_lds + (int64_t)grp * S.NB * 5 : ss + SL.phi;
   for (int sh = lane; sh < S.nshell; sh += G) {
     int ic = S.sh_cent[sh];
     eval_shell<double>(S, sh, rven[3 * ic], rven[3 * ic + 1], rven[3 * ic + 2], ren[ic], 1, phi);

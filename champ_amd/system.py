@@ -766,6 +766,59 @@ def hchain(seed: int = 4, ncarbon: int = 16) -> System:
                   name="hchain", **barr)
 
 
+def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = None) -> System:
+    """Synthetic molecule of the multi-system batch: electron count uniform
+    in [4, 128] (split evenly between spins, the extra one up), 1-24 nuclei
+    with Z_eff in {1, 4, 6} placed by random packing at >= 2 bohr spacing,
+    contracted GTOs (Z=1: 2s1p, Z_eff=4: 3s2p1d, Z_eff=6: 3s3p2d1f),
+    pseudopotentials on the Z_eff > 1 centres, random orthonormal MOs, one
+    determinant, e-n / e-e / e-e-n Jastrow.  Charged systems are allowed:
+    the electron count is drawn independently of the nuclei, then capped so
+    that the basis holds the occupied orbitals of each spin plus two."""
+    rng = np.random.default_rng(seed)
+    N = int(nelec if nelec is not None else rng.integers(4, 129))
+    M = int(ncent if ncent is not None else rng.integers(1, 25))
+    zs = rng.choice([1.0, 4.0, 6.0], size=M)
+    if not (zs > 1).any():
+        zs[0] = 4.0
+    cent = []
+    box = 2.0 * (M ** (1.0 / 3.0)) + 1.0
+    while len(cent) < M:
+        c = rng.uniform(-box, box, 3)
+        if all(np.linalg.norm(c - q) >= 2.0 for q in cent):
+            cent.append(c)
+        else:
+            box *= 1.01
+    types = {1.0: 0, 4.0: 1, 6.0: 2}
+    iwctype = np.array([types[z] for z in zs], np.int32)
+    bb = _BasisBuilder(QMCW_BASIS_GAUSSIAN, 3)
+    _gto_shells(bb, rng, 0, [(0, 2), (1, 1)])
+    _gto_shells(bb, rng, 1, [(0, 3), (1, 2), (2, 1)])
+    _gto_shells(bb, rng, 2, [(0, 3), (1, 3), (2, 2), (3, 1)])
+    barr, anorm = bb.arrays(iwctype)
+    nb = len(anorm)
+    N = max(2, min(N, 2 * (nb - 2)))
+    nup = (N + 1) // 2
+    norb = min(nb, nup + 4)
+    q = _random_orthonormal(rng, nb, norb)
+    coef = q.T * anorm[None, :]
+    cc, off, cdet, cdc = _csf_arrays([(1.0, [(0, 1.0)])], False)
+    a4, b, c = _jastrow_random(rng, 3, 4, 4, 4, 0.03)
+    ecp6 = dict(local=[(c_ * 1.5, p_, e_) for c_, p_, e_ in _ECP_C["local"]],
+                l0=[(c_ * 1.5, p_, e_ * 1.2) for c_, p_, e_ in _ECP_C["l0"]],
+                l1=[(c_ * 1.5, p_, e_ * 1.2) for c_, p_, e_ in _ECP_C["l1"]])
+    ecp = _ecp_arrays([None, _ECP_C, ecp6])
+    return System(nelec=N, nup=nup, cent=np.array(cent), iwctype=iwctype,
+                  znuc=np.array([1.0, 4.0, 6.0]), coef=coef,
+                  det_up=np.arange(nup, dtype=np.int32)[None, :],
+                  det_dn=np.arange(N - nup, dtype=np.int32)[None, :],
+                  csf_coef=cc, csf_det_off=off, csf_det=cdet, csf_det_coef=cdc,
+                  isc=4, scalek=0.8, norda=4, nordb=4, nordc=4, a4=a4, b=b, c=c,
+                  nloc=6, npotd=ecp[0], lpotp1=ecp[1], ecp_off=ecp[2],
+                  ecp_coef=ecp[3], ecp_power=ecp[4], ecp_expo=ecp[5], nquad=12,
+                  name=f"mol{seed}", **barr)
+
+
 def random_walkers(sysm: System, nw: int, seed: int = 7, sigma: float = 1.0):
     """Electrons Gaussian (sigma bohr) around the nuclei, assigned round-robin."""
     rng = np.random.default_rng(seed)
