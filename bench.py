@@ -320,7 +320,7 @@ def main():
             print(f"[bench] {prec} derivatives: {dres[prec]['rate']:.4g} samples/s "
                   f"(P={dres[prec]['P']}, G={dres[prec]['G']})", file=sys.stderr, flush=True)
     # exact-configuration energy bias of the mixed engine against FP64
-    bias = None
+    bias = bias_bf16 = None
     if "fp64" in res and "mixed" in res:
         xs = res["fp64"]["engine"].get_walkers()
         ef, em = res["fp64"]["engine"], res["mixed"]["engine"]
@@ -329,6 +329,15 @@ def main():
         d = em.psi_at()["eloc"] - ef.psi_at()["eloc"]
         bias = dict(mean_mHa=1e3 * float(d.mean()), sem_mHa=1e3 * float(d.std() / np.sqrt(len(d))),
                     max_abs_mHa=1e3 * float(np.abs(d).max()))
+        # bfloat16 inputs to the orbital contraction, same configurations
+        from champ_amd.engine import Engine
+        eb = Engine(s, W, "mixed_bf16")
+        eb.set_walkers(xs)
+        db = eb.psi_at()["eloc"] - ef.psi_at()["eloc"]
+        eb.close()
+        bias_bf16 = dict(mean_mHa=1e3 * float(db.mean()),
+                         sem_mHa=1e3 * float(db.std() / np.sqrt(len(db))),
+                         max_abs_mHa=1e3 * float(np.abs(db).max()))
     main_prec = "mixed" if ("mixed" in res and bias is not None and abs(bias["mean_mHa"]) <= 0.1) \
         else ("fp64" if "fp64" in res else "mixed")
     r = res[main_prec]
@@ -366,6 +375,7 @@ def main():
         "mixed_walker_moves_per_s": res.get("mixed", {}).get("rate"),
         "mixed_vs_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]) if len(res) == 2 else None,
         "energy_bias_mixed_vs_fp64": bias,
+        "energy_bias_bf16_inputs_vs_fp64": bias_bf16,
         "electron_moves_per_s": r["rate"] * s.nelec,
         "kernel_avg_us": {k: (1e3 * v[0] / v[1] if v[1] else None) for k, v in r["prof"].items()},
         "deriv_samples_per_s": {k: v["rate"] for k, v in dres.items()} or None,

@@ -17,6 +17,7 @@
 
 #include "engine_derivs.h"
 #include "engine_dmc.h"
+#include "engine_launch.h"
 
 using namespace qmcw;
 
@@ -438,6 +439,17 @@ __global__ void k_copy_strided(double *dst, int64_t dstride, const double *src, 
 }
 
 // -------------------------------------------------------------------------
+static LaunchCtx make_ctx(const qmcw_handle *h, const KArgs &A) {
+  LaunchCtx c;
+  c.A = A;
+  c.stream = h->stream;
+  c.G = h->G;
+  c.Wl = h->Wl;
+  c.precision = h->precision;
+  c.smem_ratio = 0;
+  return c;
+}
+
 // Full evaluation of every walker; with derivs also the parameter
 // derivatives into h->der (all orbitals, Jastrow slots, pseudopotential
 // pieces, assembly).
@@ -446,30 +458,29 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
   dim3 grid(h->Wl / (kWave / G)), block(kWave);
   KArgs A = h->A;
   A.full_orb = derivs ? 1 : 0;
+  const OrdKernels &K = ord_kernels(A.S.nordc);
+  const LaunchCtx c = make_ctx(h, A);
   prof_begin(h, QMCW_PROF_FULL_JAS);
-  if (h->precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_full_jas<double, G>), grid, block, 0, h->stream, A);
-  else
-    hipLaunchKernelGGL((k_full_jas<float, G>), grid, block, 0, h->stream, A);
+  K.full_jas(c);
   prof_end(h);
   if (derivs) {
     prof_begin(h, QMCW_PROF_DERIV_JAS);
-    hipLaunchKernelGGL((k_deriv_jas<G>), grid, block, 0, h->stream, A);
+    K.deriv_jas(c);
     prof_end(h);
   }
   prof_begin(h, QMCW_PROF_FULL_DET);
-  hipLaunchKernelGGL((k_full_det<G>), grid, block, 0, h->stream, A);
+  if (h->precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_full_det<double, G>), grid, block, 0, h->stream, A);
+  else
+    hipLaunchKernelGGL((k_full_det<float, G>), grid, block, 0, h->stream, A);
   prof_end(h);
   if (A.S.nloc > 0) {
     prof_begin(h, QMCW_PROF_FULL_NLPP);
-    if (h->precision == QMCW_FP64)
-      hipLaunchKernelGGL((k_full_nlpp<double, G>), grid, block, 0, h->stream, A);
-    else
-      hipLaunchKernelGGL((k_full_nlpp<float, G>), grid, block, 0, h->stream, A);
+    K.full_nlpp(c);
     prof_end(h);
     if (derivs) {
       prof_begin(h, QMCW_PROF_DERIV_NLPP);
-      hipLaunchKernelGGL((k_deriv_nlpp<G>), grid, block, 0, h->stream, A);
+      K.deriv_nlpp(c);
       prof_end(h);
     }
   }
@@ -485,20 +496,15 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
 }
 template <int G>
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
-                         double fbias, double *out) {
-  dim3 grid(h->Wl / (kWave / G)), block(kWave);
+                         double fbias, double *out, long long *nacc = nullptr) {
   // the moved electron's basis values in LDS when they fit (<= 48 KB per block)
   const size_t lds = (size_t)(kWave / G) * h->A.S.NB * 5 * sizeof(double);
   KArgs A = h->A;
   A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;
-  const size_t smem = A.lds_phi ? lds : 0;
+  LaunchCtx c = make_ctx(h, A);
+  c.smem_ratio = A.lds_phi ? lds : 0;
   prof_begin(h, QMCW_PROF_RATIO);
-  if (h->precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_ratio_1e<double, G>), grid, block, smem, h->stream, A, ie, mode, xn,
-                       delta, fbias, out);
-  else
-    hipLaunchKernelGGL((k_ratio_1e<float, G>), grid, block, smem, h->stream, A, ie, mode, xn,
-                       delta, fbias, out);
+  ord_kernels(A.S.nordc).ratio(c, ie, mode, xn, delta, fbias, out, nacc);
   prof_end(h);
 }
 template <int G>
@@ -772,6 +778,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   h->A.chunk = h->chunk;
   h->A.full_orb = 0;
   h->A.derivs = 0;
+  h->A.bf16 = precision == QMCW_MIXED_BF16 ? 1 : 0;
   int nsm = std::max(nup, ndn), noccm = S.noccmax;
   int ndum = std::max(D.ndetu[0], D.ndetu[1]);
   WLayout &L = h->A.L;
@@ -1082,10 +1089,9 @@ static void vmc_one_step(qmcw_handle *h, double delta, double fbias) {
     hipLaunchKernelGGL(k_rotqua, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A);
     prof_end(h);
   }
-  for (int ie = 0; ie < N; ie++) {
-    DISPATCH_G(h, launch_ratio, ie, 1, (const double *)nullptr, delta, fbias, (double *)nullptr);
-    DISPATCH_G(h, launch_accept, ie, 1, (const uint8_t *)nullptr, delta, fbias, h->nacc);
-  }
+  for (int ie = 0; ie < N; ie++)   // proposal, ratio, Metropolis test and commit in one launch
+    DISPATCH_G(h, launch_ratio, ie, 2, (const double *)nullptr, delta, fbias, (double *)nullptr,
+               h->nacc);
   DISPATCH_G(h, launch_eval, h->sample_derivs != 0);
 }
 

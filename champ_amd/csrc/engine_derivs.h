@@ -113,27 +113,11 @@ __device__ __forceinline__ void een_slots(double U, double A, double B, double U
         }
 }
 
-template <bool FULL>
-__device__ __forceinline__ void een_slots_rt(int ord, double U, double A, double B, double Ud1,
-                                             double Ud2, double Ad1, double Ad2, double rij,
-                                             double ri, const double *rvij, const double *rvi,
-                                             double u2pst, double cv, const SlotAcc &acc,
-                                             int slot0) {
-  switch (ord) {
-    case 2: een_slots<FULL, 2>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij, rvi, u2pst, cv, acc, slot0); break;
-    case 3: een_slots<FULL, 3>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij, rvi, u2pst, cv, acc, slot0); break;
-    case 4: een_slots<FULL, 4>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij, rvi, u2pst, cv, acc, slot0); break;
-    case 5: een_slots<FULL, 5>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij, rvi, u2pst, cv, acc, slot0); break;
-    case 6: een_slots<FULL, 6>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij, rvi, u2pst, cv, acc, slot0); break;
-    default: een_slots<FULL, 7>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij, rvi, u2pst, cv, acc, slot0); break;
-  }
-}
-
 // b-coefficient and e-e-n term pieces of the pair (i, j) seen from electron
 // i: rij, rvij = r_i - r_j, the distances of i and j to the centres and i's
 // vectors to them (qmc/vmc/deriv_jastrow4.f90:95-720).  cv scales the value
 // pieces (0: none); FULL adds the gradient wrt r_i and i's laplacian part.
-template <bool FULL>
+template <bool FULL, int ORD>
 __device__ void pair_slots(const DevSys &S, bool par, double rij, const double *rvij,
                            const double *ri_v, const double *rj_v, const double *rvi, double cv,
                            const SlotAcc &acc) {
@@ -196,9 +180,9 @@ __device__ void pair_slots(const DevSys &S, bool par, double rij, const double *
     B = 1.0 - k * B;
     (void)Bd1; (void)Bd2;
     const double u2pst = rij * rij + ri * ri - rj * rj;
-    een_slots_rt<FULL>(S.nordc, U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij,
-                       FULL ? rvi + 3 * ic : nullptr, u2pst, cv, acc,
-                       S.slot_c0 + it * S.nterms);
+    een_slots<FULL, ORD>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij,
+                         FULL ? rvi + 3 * ic : nullptr, u2pst, cv, acc,
+                         S.slot_c0 + it * S.nterms);
   }
 }
 
@@ -207,7 +191,7 @@ __device__ void pair_slots(const DevSys &S, bool par, double rij, const double *
 // g(p, i) = d grad_i ln J / dp and d2g(p) = d lap ln J / dp
 // (qmc/vmc/deriv_jastrow4.f90).  Each lane takes whole electrons: it visits
 // every pair (i, j) from i's side, so its rows of the slot pieces are its own.
-template <int G>
+template <int G, int ORD>
 __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, nsl = S.nslot;
@@ -227,7 +211,7 @@ __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
       const double *rv = rvee + 3 * (hi * N + lo);
       const double rvij[3] = {sg * rv[0], sg * rv[1], sg * rv[2]};
       const bool par = (hi < S.nup) || (lo >= S.nup);
-      pair_slots<true>(S, par, ree[hi * N + lo], rvij, ren + i * M, ren + j * M,
+      pair_slots<true, ORD>(S, par, ree[hi * N + lo], rvij, ren + i * M, ren + j * M,
                        rven + i * M * 3, j < i ? 1.0 : 0.0, acc);
     }
   }
@@ -265,7 +249,7 @@ __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
 // w exp(dJ) v P_l (Psi_D ratio) d(dJ)/dp.  Each lane takes whole
 // (electron, centre) pairs and owns their rows; k_full_nlpp has filled the
 // channel values (SLayout::vnl) of the current configuration.
-template <int G>
+template <int G, int ORD>
 __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, NO = S.NO, NB = S.NB, nsl = S.nslot;
@@ -325,7 +309,7 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
         double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
         const double rij = sqrt(dx * dx + dy * dy + dz * dz);
         const bool par = (hi < S.nup) || (lo >= S.nup);
-        delta += jas_pair_val<double>(S, par, rij, rnew, ren + jj * M) - fso[hi * N + lo];
+        delta += jas_pair_val<double, ORD>(S, par, rij, rnew, ren + jj * M) - fso[hi * N + lo];
       }
       double vpot = 0.0;
       for (int l = 1; l <= S.npotd[ict]; l++)
@@ -341,7 +325,7 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
         double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
         const double rij = sqrt(dx * dx + dy * dy + dz * dz);
         const bool par = (hi < S.nup) || (lo >= S.nup);
-        pair_slots<false>(S, par, rij, nullptr, rnew, ren + jj * M, nullptr, wr, acc);
+        pair_slots<false, ORD>(S, par, rij, nullptr, rnew, ren + jj * M, nullptr, wr, acc);
       }
     }
     // the same terms at the current position, weighted by the summed weight
@@ -350,7 +334,7 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
       if (jj == i) continue;
       const int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
       const bool par = (hi < S.nup) || (lo >= S.nup);
-      pair_slots<false>(S, par, ree[hi * N + lo], nullptr, ren + i * M, ren + jj * M, nullptr, -wsum, acc);
+      pair_slots<false, ORD>(S, par, ree[hi * N + lo], nullptr, ren + i * M, ren + jj * M, nullptr, -wsum, acc);
     }
     (void)rvee;
   }

@@ -138,29 +138,6 @@ __device__ __forceinline__ void een_derivs(T U, T A, T B, const double *__restri
         }
 }
 
-template <typename T>
-__device__ __forceinline__ T een_value_rt(int ord, T U, T A, T B, const double *c) {
-  switch (ord) {
-    case 2: return een_value<T, 2>(U, A, B, c);
-    case 3: return een_value<T, 3>(U, A, B, c);
-    case 4: return een_value<T, 4>(U, A, B, c);
-    case 5: return een_value<T, 5>(U, A, B, c);
-    case 6: return een_value<T, 6>(U, A, B, c);
-    default: return een_value<T, 7>(U, A, B, c);
-  }
-}
-template <typename T>
-__device__ __forceinline__ void een_derivs_rt(int ord, T U, T A, T B, const double *c, EenD<T> &o) {
-  switch (ord) {
-    case 2: een_derivs<T, 2>(U, A, B, c, o); break;
-    case 3: een_derivs<T, 3>(U, A, B, c, o); break;
-    case 4: een_derivs<T, 4>(U, A, B, c, o); break;
-    case 5: een_derivs<T, 5>(U, A, B, c, o); break;
-    case 6: een_derivs<T, 6>(U, A, B, c, o); break;
-    default: een_derivs<T, 7>(U, A, B, c, o); break;
-  }
-}
-
 // Pair term of electrons (i, j), i > j: e-e Pade + polynomial and e-e-n
 // polynomial.  Returns value; gi/gj gradient wrt r_i and r_j; li/lj the
 // laplacian wrt r_i and r_j.
@@ -169,7 +146,7 @@ struct PairOut {
   T f, gi[3], gj[3], li, lj;
 };
 
-template <typename T>
+template <typename T, int ORD>
 __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
                          const T *rvij, const double *ri_v, const double *rj_v,
                          const double *rvi, const double *rvj, PairOut<T> &o) {
@@ -226,7 +203,7 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
     A = T(1) - k * A; Ad1 = -k * Ad1; Ad2 = -k * Ad2;
     B = T(1) - k * B; Bd1 = -k * Bd1; Bd2 = -k * Bd2;
     EenD<T> e;
-    een_derivs_rt<T>(S.nordc, U, A, B, S.c + it * S.nterms, e);
+    een_derivs<T, ORD>(U, A, B, S.c + it * S.nterms, e);
     T fc = e.f, fU = e.fU, fUU = e.fUU, fA = e.fA, fAA = e.fAA, fB = e.fB, fBB = e.fBB,
       fUA = e.fUA, fUB = e.fUB;
     // chain rule to unscaled distances
@@ -250,7 +227,7 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
 }
 
 // value only of the pair term (pseudopotential quadrature)
-template <typename T>
+template <typename T, int ORD>
 __device__ T jas_pair_val(const DevSys &S, bool parallel_spin, T rij,
                           const double *ri_v, const double *rj_v) {
   const T k = T(S.scalek);
@@ -278,7 +255,7 @@ __device__ T jas_pair_val(const DevSys &S, bool parallel_spin, T rij,
     T A, B, t1, t2;
     scaled<T>(S.isc, k, (T)ri_v[ic], A, t1, t2);
     scaled<T>(S.isc, k, (T)rj_v[ic], B, t1, t2);
-    f += een_value_rt<T>(S.nordc, U, T(1) - k * A, T(1) - k * B, S.c + it * S.nterms);
+    f += een_value<T, ORD>(U, T(1) - k * A, T(1) - k * B, S.c + it * S.nterms);
   }
   return f;
 }
@@ -514,7 +491,7 @@ __device__ __forceinline__ T shell_dot_l(T x, T y, T z, const int *bas, const GP
 #pragma unroll
   for (int q = 0; q < 2 * L + 1; q++) {
     int b = bas[q];
-    if (b >= 0) acc += T(gp[b]) * Sv[q];
+    if (b >= 0) acc += T(gp[b]) * Sv[q];   // gp already rounded in the BF16 engine
   }
   return acc;
 }

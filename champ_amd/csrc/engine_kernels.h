@@ -22,6 +22,7 @@ struct KArgs {
   int derivs;        // compute Jastrow/NLPP parameter derivative pieces
   int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
   int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
+  int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
 };
 
@@ -39,6 +40,21 @@ struct KArgs {
   double *ws = A.st + (int64_t)w * A.L.stride;                   \
   double *ss = A.sc + (int64_t)w * A.SL.stride;                  \
   (void)ss;
+
+// bfloat16 rounding (nearest, ties to even) of an FP32 value, kept in FP32
+__device__ __forceinline__ float bf16r(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xFFFF0000u);
+}
+
+// one product of the orbital contraction: coefficient times basis value, with
+// bfloat16 inputs in the BF16 engine
+template <typename T>
+__device__ __forceinline__ double cmul(int bf16, double cf, T ph) {
+  if (bf16) return (double)(bf16r((float)cf) * bf16r((float)ph));
+  return cf * (double)ph;
+}
 
 // ---------------------------------------------------------------------------
 // In-place inverse of the column-major n x n block `a` by a group (CHAMP's
@@ -187,7 +203,7 @@ __device__ void group_g(const DevSys &S, int s, const double *ai,
 // live registers; scalars pass through the walker's reduction scratch:
 // phs[50] pe_en, [51] pe_ee, [52] ln J, [53] lap ln J, [54] lap ln D,
 // [55] local pseudopotential, [56] nonlocal pseudopotential (SLayout::phs).
-template <typename T, int G>
+template <typename T, int G, int ORD>
 __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, NO = S.NO;
@@ -247,7 +263,7 @@ __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
       T rvij[3];
       for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * p + q];
       bool par = (i < S.nup) || (j >= S.nup);
-      jas_pair<T>(S, par, (T)ree[p], rvij, ren + i * M, ren + j * M, rven + i * M * 3,
+      jas_pair<T, ORD>(S, par, (T)ree[p], rvij, ren + i * M, ren + j * M, rven + i * M * 3,
                   rven + j * M * 3, o);
       fso[p] = o.f;
       for (int q = 0; q < 3; q++) {
@@ -285,7 +301,7 @@ __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
   }
 }
 
-template <int G>
+template <typename T, int G>
 __global__ QMCW_HEAVY void k_full_det(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, NO = S.NO;
@@ -300,7 +316,7 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
   (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
   __shared__ int s_ipiv[kWave / G][128];
   // ---- orbitals (basis_fns + orbitals_loc_ana), chunks of electrons
-  double *phi = ss + SL.phi;
+  T *phi = reinterpret_cast<T *>(ss + SL.phi);   // basis values in the engine's real type
   double *oc = ws + L.oc;
   double *orbf = ss + SL.orb;
   const int noccmax = S.noccmax;
@@ -311,8 +327,8 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
       int ii = t / S.nshell, s = t % S.nshell;
       int i = i0 + ii, ic = S.sh_cent[s];
       const double *rv = rven + (i * M + ic) * 3;
-      eval_shell<double>(S, s, rv[0], rv[1], rv[2], ren[i * M + ic], 1,
-                         phi + (int64_t)ii * S.NB * 5);
+      eval_shell<T>(S, s, (T)rv[0], (T)rv[1], (T)rv[2], (T)ren[i * M + ic], 1,
+                    phi + (int64_t)ii * S.NB * 5);
     }
     __syncthreads();
     // occupied orbitals of each electron's spin into the resident state
@@ -323,11 +339,12 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
       if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
       int lab = S.occ[s][j];
       double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-      const double *ph = phi + (int64_t)ii * S.NB * 5;
+      const T *ph = phi + (int64_t)ii * S.NB * 5;
       for (int b = 0; b < S.NB; b++) {
         double cf = S.coefT[(int64_t)b * NO + lab];
-        a0 += cf * ph[5 * b]; a1 += cf * ph[5 * b + 1]; a2 += cf * ph[5 * b + 2];
-        a3 += cf * ph[5 * b + 3]; a4 += cf * ph[5 * b + 4];
+        a0 += cmul(A.bf16, cf, ph[5 * b]); a1 += cmul(A.bf16, cf, ph[5 * b + 1]);
+        a2 += cmul(A.bf16, cf, ph[5 * b + 2]); a3 += cmul(A.bf16, cf, ph[5 * b + 3]);
+        a4 += cmul(A.bf16, cf, ph[5 * b + 4]);
       }
       o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
     }
@@ -336,11 +353,12 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
         int ii = t / NO, lab = t % NO;
         int i = i0 + ii;
         double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-        const double *ph = phi + (int64_t)ii * S.NB * 5;
+        const T *ph = phi + (int64_t)ii * S.NB * 5;
         for (int b = 0; b < S.NB; b++) {
           double cf = S.coefT[(int64_t)b * NO + lab];
-          a0 += cf * ph[5 * b]; a1 += cf * ph[5 * b + 1]; a2 += cf * ph[5 * b + 2];
-          a3 += cf * ph[5 * b + 3]; a4 += cf * ph[5 * b + 4];
+          a0 += cmul(A.bf16, cf, ph[5 * b]); a1 += cmul(A.bf16, cf, ph[5 * b + 1]);
+          a2 += cmul(A.bf16, cf, ph[5 * b + 2]); a3 += cmul(A.bf16, cf, ph[5 * b + 3]);
+          a4 += cmul(A.bf16, cf, ph[5 * b + 4]);
         }
         double *o = orbf + ((int64_t)i * NO + lab) * 5;
         o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
@@ -426,12 +444,12 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
       const double *g = ss + SL.g[s];
       double acc = 0.0;
       for (int j = 0; j < S.nocc[s]; j++) acc += g[j * ns + ie] * S.coefT[(int64_t)b * NO + S.occ[s][j]];
-      gp[t] = acc;
+      gp[t] = A.bf16 ? (double)bf16r((float)acc) : acc;
     }
   }
 }
 
-template <typename T, int G>
+template <typename T, int G, int ORD>
 __global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, NO = S.NO;
@@ -502,7 +520,7 @@ __global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
         double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
         double rij = sqrt(dx * dx + dy * dy + dz * dz);
         bool par = (hi < S.nup) || (lo >= S.nup);
-        delta += jas_pair_val<T>(S, par, (T)rij, rnew, ren + jj * M) - (T)fso[hi * N + lo];
+        delta += jas_pair_val<T, ORD>(S, par, (T)rij, rnew, ren + jj * M) - (T)fso[hi * N + lo];
       }
       double vpot = 0.0;
       for (int l = 1; l <= S.npotd[ict]; l++)
@@ -598,16 +616,22 @@ __device__ void group_grad_1e(const KArgs &A, const double *ws, int ie,
     *lapout = acc[3] + 2.0 * (fj[0] * acc[0] + fj[1] * acc[1] + fj[2] * acc[2]) + lapj;
 }
 
+__device__ __forceinline__ int metrop_accept(const KArgs &A, const double *ws, const double *mv,
+                                             int w, int ns, int nocc, double delta,
+                                             double fbias);
+template <int G>
+__device__ void group_commit(const KArgs &A, double *ws, const double *mv, int ie, int lane);
+
 // ---------------------------------------------------------------------------
 // One-electron move of electron ie: propose (mode 1, metrop_mov1 with the
 // walker's rannyu stream) or take xnew (mode 0), then distancese +
 // jastrow4e + orbitals + dete_update + eval_grad into the pending buffers.
 // out[w][8]: ratio, ln|psid_new|, sign, psij_new, grad(3), lap
-template <typename T, int G>
+template <typename T, int G, int ORD>
 __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
                                                  const double *xnew_in,
                                                  double delta, double fbias,
-                                                 double *out) {
+                                                 double *out, long long *nacc) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, noccmax = S.noccmax;
   const WLayout &L = A.L;
@@ -619,8 +643,8 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   const double *x = ws + L.x;
   const double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
   double *xn = mv + ML.xn;
-  // ---- proposal
-  if (mode == 1) {
+  // ---- proposal (mode 1: ratio only; mode 2: ratio, test and commit)
+  if (mode >= 1) {
     double lapo_dummy;
     double vold[3];
     double lapjo = 0.0;  // not needed for the proposal
@@ -705,8 +729,8 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     PairOut<T> po;
     bool par = (hi < S.nup) || (lo >= S.nup);
     // moved electron's arrays go to whichever slot (hi or lo) it occupies
-    if (hi == ie) jas_pair<T>(S, par, (T)ree[j], rvij, rI, rJ, rvI, rvJ, po);
-    else jas_pair<T>(S, par, (T)ree[j], rvij, rJ, rI, rvJ, rvI, po);
+    if (hi == ie) jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rI, rJ, rvI, rvJ, po);
+    else jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rJ, rI, rvJ, rvI, po);
     const T *g_ie = (hi == ie) ? po.gi : po.gj;
     const T *g_j = (hi == ie) ? po.gj : po.gi;
     T l_ie = (hi == ie) ? po.li : po.lj;
@@ -737,10 +761,10 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   // ---- orbitals of the new position (occupied set of the spin)
   // basis values of the new position: LDS when the launch reserved it
   extern __shared__ double dyn_lds[];
-  double *phi = A.lds_phi ? dyn_lds + (int64_t)grp * S.NB * 5 : ss + SL.phi;
+  T *phi = reinterpret_cast<T *>(A.lds_phi ? dyn_lds + (int64_t)grp * S.NB * 5 : ss + SL.phi);
   for (int sh = lane; sh < S.nshell; sh += G) {
     int ic = S.sh_cent[sh];
-    eval_shell<double>(S, sh, rven[3 * ic], rven[3 * ic + 1], rven[3 * ic + 2], ren[ic], 1, phi);
+    eval_shell<T>(S, sh, (T)rven[3 * ic], (T)rven[3 * ic + 1], (T)rven[3 * ic + 2], (T)ren[ic], 1, phi);
   }
   __syncthreads();
   double *orbe = mv + ML.orbe;
@@ -749,7 +773,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     double a[5] = {0, 0, 0, 0, 0};
     for (int b = 0; b < S.NB; b++) {
       double cf = S.coefT[(int64_t)b * S.NO + lab];
-      for (int c = 0; c < 5; c++) a[c] += cf * phi[5 * b + c];
+      for (int c = 0; c < 5; c++) a[c] += cmul(A.bf16, cf, phi[5 * b + c]);
     }
     for (int c = 0; c < 5; c++) orbe[5 * j + c] = a[c];
   }
@@ -826,49 +850,54 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
       o[4] = gnew[0]; o[5] = gnew[1]; o[6] = gnew[2]; o[7] = lapn_tot;
     }
   }
+  // VMC sweep (mode 2): Metropolis test and commit in the same launch
+  if (mode == 2) {
+    __shared__ int s_acc[kWave / G];
+    __syncthreads();
+    if (lane == 0) {
+      const int acc = metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias);
+      s_acc[grp] = acc;
+      if (acc && nacc) nacc[w] += 1;
+    }
+    __syncthreads();
+    if (s_acc[grp]) group_commit<G>(A, ws, mv, ie, lane);
+  }
 }
 
-// Commit or drop the pending move of electron ie.  mode 1: Metropolis test
-// of metrop_mov1 (qmc/vmc/metrop_mov1.f90:90-99, 226-250) with the walker's
-// rannyu; mode 0: accept[w] given.  naccept[w] incremented on acceptance.
+// Metropolis test of metrop_mov1 (qmc/vmc/metrop_mov1.f90:90-99, 226-250)
+// for the pending move of electron ie, with the walker's rannyu (lane 0).
+__device__ __forceinline__ int metrop_accept(const KArgs &A, const double *ws, const double *mv,
+                                             int w, int ns, int nocc, double delta,
+                                             double fbias) {
+  const WLayout &L = A.L;
+  const MLayout &ML = A.ML;
+  int *rs = A.rng + 4 * w;
+  int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+  const double deltfi = 2.0 * fbias / delta;
+  const double logn = mv[ML.misc + 8 + ns + nocc];
+  const double psi2n = 2.0 * (logn + mv[ML.misc + 7]);
+  double fxnp = 1.0;
+  for (int k = 0; k < 3; k++) {
+    const double vn = mv[ML.misc + 10 + ns + nocc + k];
+    fxnp *= (1.0 - mv[ML.misc + 2 + k] * copysign(fmin(fabs(vn), deltfi), vn));
+  }
+  double p = exp(psi2n - ws[L.psi2o]) * fxnp / mv[ML.misc + 1];
+  p = fmin(1.0, p);
+  const int acc = rannyu(st4) < p;
+  rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+  return acc;
+}
+
+// Commit the pending move of electron ie into the walker's state: jassav +
+// detsav + dete_save (qmc/vmc/metrop_slat_mov1.f90:544-546).  All lanes of
+// the group; no barriers.
 template <int G>
-__global__ __launch_bounds__(64) void k_accept_1e(KArgs A, int ie, int mode,
-                                                  const uint8_t *accept_in,
-                                                  double delta, double fbias,
-                                                  long long *naccept) {
-  QMCW_GROUP_SETUP
+__device__ void group_commit(const KArgs &A, double *ws, const double *mv, int ie, int lane) {
+  const DevSys &S = A.S;
   const int N = S.N, M = S.M, noccmax = S.noccmax;
   const WLayout &L = A.L;
-  const SLayout &SL = A.SL;
   const MLayout &ML = A.ML;
-  double *mv = ss + SL.mv;
   const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
-  __shared__ int s_acc[kWave / G];
-  if (lane == 0) {
-    int acc;
-    if (mode == 1) {
-      int *rs = A.rng + 4 * w;
-      int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
-      double deltfi = 2.0 * fbias / delta;
-      double logn = mv[ML.misc + 8 + ns + nocc];
-      double psi2n = 2.0 * (logn + mv[ML.misc + 7]);
-      double fxnp = 1.0;
-      for (int k = 0; k < 3; k++) {
-        double vn = mv[ML.misc + 10 + ns + nocc + k];
-        fxnp *= (1.0 - mv[ML.misc + 2 + k] * copysign(fmin(fabs(vn), deltfi), vn));
-      }
-      double p = exp(psi2n - ws[L.psi2o]) * fxnp / mv[ML.misc + 1];
-      p = fmin(1.0, p);
-      acc = rannyu(st4) < p;
-      rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
-    } else {
-      acc = accept_in[w] != 0;
-    }
-    s_acc[grp] = acc;
-    if (acc && naccept) naccept[w] += 1;
-  }
-  __syncthreads();
-  if (!s_acc[grp]) return;   // uniform within the group; no barriers follow
   double *x = ws + L.x;
   const double *xn = mv + ML.xn;
   if (lane < 3) x[3 * ie + lane] = xn[lane];
@@ -914,10 +943,34 @@ __global__ __launch_bounds__(64) void k_accept_1e(KArgs A, int ie, int mode,
     ws[L.deta[s]] = mv[ML.misc + 6];
     ws[L.fsumo] = mv[ML.misc + 7];
     ws[L.psij] = mv[ML.misc + 7];
-    double logn = mv[ML.misc + 8 + ns + nocc];
+    const double logn = mv[ML.misc + 8 + ns + nocc];
     ws[L.logpsid] = logn;
     ws[L.psi2o] = 2.0 * (logn + mv[ML.misc + 7]);
   }
+}
+
+// Commit or drop the pending move of electron ie.  mode 1: Metropolis test
+// with the walker's rannyu; mode 0: accept[w] given.  naccept[w] incremented
+// on acceptance.
+template <int G>
+__global__ __launch_bounds__(64) void k_accept_1e(KArgs A, int ie, int mode,
+                                                  const uint8_t *accept_in,
+                                                  double delta, double fbias,
+                                                  long long *naccept) {
+  QMCW_GROUP_SETUP
+  const SLayout &SL = A.SL;
+  const double *mv = ss + SL.mv;
+  const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  __shared__ int s_acc[kWave / G];
+  if (lane == 0) {
+    const int acc = (mode == 1) ? metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias)
+                                : (accept_in[w] != 0);
+    s_acc[grp] = acc;
+    if (acc && naccept) naccept[w] += 1;
+  }
+  __syncthreads();
+  if (!s_acc[grp]) return;   // uniform within the group; no barriers follow
+  group_commit<G>(A, ws, mv, ie, lane);
 }
 
 // grad/lap of electron ie in the current state: out[w][4]
@@ -940,6 +993,7 @@ __global__ __launch_bounds__(64) void k_grad_1e(KArgs A, int ie, double *out) {
 }
 
 // random rotation of each walker's quadrature grid (qmc/vmc/rotqua.f90:621-682)
+#ifndef QMCW_ORD   // launched from the host translation unit only
 __global__ void k_rotqua(KArgs A) {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= A.Wp) return;
@@ -986,5 +1040,6 @@ __global__ void k_rotqua(KArgs A) {
   }
   rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
 }
+#endif
 
 }  // namespace qmcw

@@ -1,0 +1,82 @@
+// engine_ord.hip -- the launcher table of one e-e-n order; compiled with
+// -DQMCW_ORD=<2..7>.  See engine_launch.h.
+#include <hip/hip_runtime.h>
+
+#include "engine_derivs.h"
+#include "engine_launch.h"
+
+#ifndef QMCW_ORD
+#error "compile with -DQMCW_ORD=<order of the e-e-n polynomial>"
+#endif
+
+#define QMCW_CAT2(a, b) a##b
+#define QMCW_CAT(a, b) QMCW_CAT2(a, b)
+
+namespace qmcw {
+namespace {
+
+constexpr int ORD = QMCW_ORD;
+
+template <int G>
+void full_jas_g(const LaunchCtx &c) {
+  const dim3 grid(c.Wl / (kWave / G)), block(kWave);
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_full_jas<double, G, ORD>), grid, block, 0, c.stream, c.A);
+  else
+    hipLaunchKernelGGL((k_full_jas<float, G, ORD>), grid, block, 0, c.stream, c.A);
+}
+template <int G>
+void full_nlpp_g(const LaunchCtx &c) {
+  const dim3 grid(c.Wl / (kWave / G)), block(kWave);
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_full_nlpp<double, G, ORD>), grid, block, 0, c.stream, c.A);
+  else
+    hipLaunchKernelGGL((k_full_nlpp<float, G, ORD>), grid, block, 0, c.stream, c.A);
+}
+template <int G>
+void ratio_g(const LaunchCtx &c, int ie, int mode, const double *xn, double delta, double fbias,
+             double *out, long long *nacc) {
+  const dim3 grid(c.Wl / (kWave / G)), block(kWave);
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_ratio_1e<double, G, ORD>), grid, block, c.smem_ratio, c.stream, c.A, ie,
+                       mode, xn, delta, fbias, out, nacc);
+  else
+    hipLaunchKernelGGL((k_ratio_1e<float, G, ORD>), grid, block, c.smem_ratio, c.stream, c.A, ie,
+                       mode, xn, delta, fbias, out, nacc);
+}
+template <int G>
+void deriv_jas_g(const LaunchCtx &c) {
+  const dim3 grid(c.Wl / (kWave / G)), block(kWave);
+  hipLaunchKernelGGL((k_deriv_jas<G, ORD>), grid, block, 0, c.stream, c.A);
+}
+template <int G>
+void deriv_nlpp_g(const LaunchCtx &c) {
+  const dim3 grid(c.Wl / (kWave / G)), block(kWave);
+  hipLaunchKernelGGL((k_deriv_nlpp<G, ORD>), grid, block, 0, c.stream, c.A);
+}
+
+#define QMCW_BY_G(fn, c, ...)                  \
+  switch ((c).G) {                             \
+    case 8: fn<8>(c, ##__VA_ARGS__); break;    \
+    case 16: fn<16>(c, ##__VA_ARGS__); break;  \
+    case 32: fn<32>(c, ##__VA_ARGS__); break;  \
+    default: fn<64>(c, ##__VA_ARGS__); break;  \
+  }
+
+void full_jas(const LaunchCtx &c) { QMCW_BY_G(full_jas_g, c); }
+void full_nlpp(const LaunchCtx &c) { QMCW_BY_G(full_nlpp_g, c); }
+void ratio(const LaunchCtx &c, int ie, int mode, const double *xn, double delta, double fbias,
+           double *out, long long *nacc) {
+  QMCW_BY_G(ratio_g, c, ie, mode, xn, delta, fbias, out, nacc);
+}
+void deriv_jas(const LaunchCtx &c) { QMCW_BY_G(deriv_jas_g, c); }
+void deriv_nlpp(const LaunchCtx &c) { QMCW_BY_G(deriv_nlpp_g, c); }
+
+}  // namespace
+
+#ifndef __HIP_DEVICE_COMPILE__   // a host table of host launchers
+extern const OrdKernels QMCW_CAT(kOrdKernels, QMCW_ORD) = {full_jas, full_nlpp, ratio, deriv_jas,
+                                                            deriv_nlpp};
+#endif
+
+}  // namespace qmcw
