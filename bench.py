@@ -244,6 +244,35 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
     return dict(aggregate=agg, per_system=per, seconds=dt)
 
 
+def run_dmc_chain(W0, steps, warmup, precision, tau=0.01):
+    """DMC of the 98-electron chain (16 Z_eff=4 centres with a
+    pseudopotential, 34 H, 424 GTOs, one determinant): W0 walkers, capacity
+    2 W0, time step tau, under the rundmc driver (population control)."""
+    from champ_amd import rng, system
+    from champ_amd.dmc import DmcDriver
+    from champ_amd.engine import Engine
+    s = system.hchain(4)
+    cap = 2 * W0
+    e = Engine(s, cap, precision)
+    e.set_walkers(system.random_walkers(s, cap, seed=9))
+    e.set_rng(rng.walker_seeds("1837465927472523", cap, stride=1 << 20))
+    e.set_branch_rng(np.array([1, 2, 3, 5], np.int32))
+    g = e.autotune()
+    e.dmc_begin(W0)
+    e0 = float(e.get_eloc()[:W0].mean())
+    drv = DmcDriver(e, s.nelec, W0, tau=tau, etrial=e0)
+    drv.run(max(1, warmup))
+    e.sync()
+    t0 = time.perf_counter()
+    h = drv.run(steps)
+    e.sync()
+    dt = time.perf_counter() - t0
+    moves = sum(x["nwalk"] for x in h)
+    e.close()
+    return dict(rate=moves / dt, G=g, nwalk_final=h[-1]["nwalk"], eave=h[-1]["eave"],
+                acc=float(np.mean([x["acc"] for x in h])), seconds=dt)
+
+
 def copy_calibration(nbytes=256 << 20):
     """Three device-to-device copies of nbytes (HIP's blit copy kernel), the
     known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters
@@ -277,6 +306,8 @@ def main():
     ap.add_argument("--deriv-walkers", type=int, default=8192)
     ap.add_argument("--deriv-steps", type=int, default=40)
     ap.add_argument("--derivs", action="store_true", help="also measure derivative samples/s")
+    ap.add_argument("--dmc", action="store_true",
+                    help="only run DMC of the 98-electron chain (4096 walkers, tau 0.01)")
     ap.add_argument("--batch", type=int, default=0,
                     help="only run the multi-system batch with this many systems")
     args = ap.parse_args()
@@ -284,6 +315,18 @@ def main():
         print("bench.py measures one GPU: this build runs and reports a single GPU; "
               "multi-GPU runs (walker sharding) are not part of it.", file=sys.stderr)
         sys.exit(2)
+    if args.dmc:
+        prec = "mixed" if args.precision == "both" else args.precision
+        d = run_dmc_chain(4096, args.steps, args.warmup, prec)
+        print(json.dumps({"metric": "DMC walker-moves/s (98-electron chain)", "value": d["rate"],
+                          "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "higher_is_better": True,
+                          "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
+                          "config": {"workload": "DMC, 98-electron chain, 4096 walkers, tau 0.01",
+                                     "precision": prec, "group_width": d["G"]},
+                          "nwalk_final": d["nwalk_final"], "acceptance": d["acc"],
+                          "seconds": d["seconds"]}), flush=True)
+        return
     if args.batch:
         prec = "mixed" if args.precision == "both" else args.precision
         b = run_batch(args.batch, args.steps, args.warmup, prec, args.delta)

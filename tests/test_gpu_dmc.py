@@ -66,3 +66,43 @@ def test_dmc_requires_pseudopotential():
     e.set_walkers(system.random_walkers(s, 8, seed=1))
     with pytest.raises(Exception):
         e.dmc_begin(8)
+
+
+class _OracleStepper:
+    def __init__(self, d, brng):
+        self.d, self.brng = d, brng
+
+    def dmc_step(self, params):
+        r, st = self.d.generation(params)
+        assert r == 0
+        r, sb = self.d.branch(self.brng)
+        assert r == 0
+        st.nwalk, st.nbranch = sb.nwalk, sb.nbranch
+        return st
+
+
+def test_dmc_driver_run_matches_oracle():
+    """Six generations under the rundmc driver (population control, running
+    eest/taueff) on the GPU and on the oracle: same populations, energies to
+    1e-9."""
+    from champ_amd.dmc import DmcDriver
+    s = system.c2_dimer(2)
+    n0, cap = 24, 64
+    x, seeds = _setup(s, n0, cap, seed=33)
+    e0 = float(oracle.Oracle(s).eval_batch(x[:n0])["out9"][:, 3].mean())
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    e = Engine(s, cap)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    e.set_branch_rng(brng)
+    e.dmc_begin(n0)
+    d = oracle.OracleDMC(s, cap)
+    assert d.set(x[:n0], seeds[:n0]) == 0
+    dg = DmcDriver(e, s.nelec, n0, tau=0.05, etrial=e0)
+    do = DmcDriver(_OracleStepper(d, brng.copy()), s.nelec, n0, tau=0.05, etrial=e0)
+    for _ in range(6):
+        sg, so = dg.generation(), do.generation()
+        assert sg.nwalk == so.nwalk and sg.nacc == so.nacc
+        assert abs(sg.wesum - so.wesum) <= 1e-9 * abs(so.wesum)
+    assert abs(dg.eest - do.eest) <= 1e-9 * abs(do.eest)

@@ -110,3 +110,54 @@ def test_split_join_invariants():
         assert (np.abs(flat - xa).max(axis=1) == 0).any()
     # at most one unpaired walker below 1/2 remains
     assert int((after["weight"] < 0.5).sum()) <= 1
+
+
+class OracleStepper:
+    """dmc_step over the oracle's population: generation + split_join."""
+
+    def __init__(self, d, brng):
+        self.d, self.brng = d, brng
+
+    def dmc_step(self, params):
+        r, st = self.d.generation(params)
+        assert r == 0
+        r, sb = self.d.branch(self.brng)
+        assert r == 0
+        st.nwalk, st.nbranch = sb.nwalk, sb.nbranch
+        return st
+
+
+def _hydrogen_sto(zeta):
+    s = _hydrogenic_pp(1)
+    bb = system._BasisBuilder(system.QMCW_BASIS_SLATER, 1)
+    r = bb.add_radial(1, [float(zeta)], [1.0])
+    bb.add_function(0, r, 0, 0)
+    barr, anorm = bb.arrays(np.array([0], np.int32))
+    for k, v in barr.items():
+        setattr(s, k, v)
+    s.coef = np.array([[anorm[0]]])
+    return s
+
+
+def test_dmc_recovers_the_exact_energy_of_a_nodeless_state():
+    """H atom with the wrong orbital exp(-1.3 r): VMC gives zeta^2/2 - zeta =
+    -0.455 Ha; one electron has no nodes, so DMC must give -0.5 Ha (time step
+    0.02, error ~ a few mHa)."""
+    from champ_amd.dmc import DmcDriver
+    s = _hydrogen_sto(1.3)
+    W0, cap = 1000, 2500
+    x = system.random_walkers(s, W0, seed=7, sigma=1.0)
+    seeds = rng.walker_seeds("1837465927472523", W0)
+    d = oracle.OracleDMC(s, cap)
+    assert d.set(x, seeds) == 0
+    evmc = float(d.get()["eloc"].mean())
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    drv = DmcDriver(OracleStepper(d, brng), 1, W0, tau=0.02, etrial=-0.45)
+    drv.run(150)                         # equilibration
+    h = drv.run(300)
+    e = np.array([g["eave"] for g in h])
+    blocks = e.reshape(15, 20).mean(axis=1)
+    est, err = blocks.mean(), blocks.std() / np.sqrt(len(blocks))
+    assert abs(est + 0.5) < max(4 * err, 0.004), (est, err, evmc)
+    assert 0.3 * W0 < h[-1]["nwalk"] < 2 * W0      # population control holds
