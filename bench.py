@@ -161,8 +161,10 @@ def run_engine(s, W, steps, warmup, precision, delta, x, seeds, prof_steps):
     e = Engine(s, W, precision)
     e.set_walkers(x)
     e.set_rng(seeds)
-    g = e.autotune()
-    print(f"[bench] {precision}: autotuned lane-group width G={g}", file=sys.stderr, flush=True)
+    e.autotune()
+    g = e.group_widths()
+    print(f"[bench] {precision}: autotuned lane-group widths (jastrow, det, nlpp, moves) = {g}",
+          file=sys.stderr, flush=True)
     e.vmc_device(max(1, warmup), delta)
     e.sync()
     t0 = time.perf_counter()
@@ -413,7 +415,7 @@ def main():
         "config": {"workload": "C2-like dimer VMC: 8 valence e, 58 GTO, ECP l=0,1 nonlocal (Q=12), "
                                "24 CSF / 48 det, 16384 walkers x 200 steps",
                    "walkers": W, "delta": args.delta, "precision": main_prec,
-                   "group_width": r["G"]},
+                   "group_widths": r["G"]},
         "fp64_walker_moves_per_s": res.get("fp64", {}).get("rate"),
         "mixed_walker_moves_per_s": res.get("mixed", {}).get("rate"),
         "mixed_vs_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]) if len(res) == 2 else None,

@@ -144,6 +144,7 @@ SIGNATURES = {
     "qmcw_sync": (C.c_int, [_vp]),
     "qmcw_autotune": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "qmcw_set_group_width": (C.c_int, [_vp, C.c_int32]),
+    "qmcw_get_group_widths": (C.c_int, [_vp, _pi]),
     "qmcw_profile": (C.c_int, [_vp, C.c_int32]),
     "qmcw_profile_get": (C.c_int, [_vp, _pd, C.POINTER(C.c_int64)]),
     "qmcw_last_error": (C.c_char_p, [_vp]),

@@ -20,9 +20,12 @@ FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
          "-Wno-unused-result"]
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
+def build(verbose: bool = False, jobs: int | None = None, out: str | None = None,
+          build_dir: str | None = None) -> str:
     """Compile engine_api.hip and one engine_ord.hip object per e-e-n order
-    in parallel, then link them into libchamp_amd.so."""
+    in parallel, then link them into libchamp_amd.so (or `out`)."""
+    OUT = out or globals()["OUT"]
+    BUILD = build_dir or globals()["BUILD"]
     os.makedirs(BUILD, exist_ok=True)
     units = [(os.path.join(CSRC, "engine_api.hip"), os.path.join(BUILD, "engine_api.o"), [])]
     for o in ORDERS:

@@ -25,6 +25,8 @@ static thread_local std::string g_lib_err;
 
 struct qmcw_handle {
   int W = 0, Wp = 0, G = 16, precision = QMCW_FP64, chunk = 1;
+  // lane-group width of each kernel class (jastrow, determinants, pseudopotential, moves)
+  int Gk[4] = {16, 16, 16, 16};
   int Wl = 0;                // walkers the kernels run over (multiple of 8, <= Wp)
   hipStream_t stream = nullptr;
   KArgs A{};
@@ -55,6 +57,7 @@ struct qmcw_handle {
   const double *graph_st = nullptr;
   int use_graph = 1;
   int use_lds = 1;           // LDS staging of the moved electron's basis values
+  int chunk_override = 0;    // electrons per basis chunk (0: from the L2 budget)
   // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
   int dmc_on = 0, nact = 0;
   double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
@@ -439,69 +442,100 @@ __global__ void k_copy_strided(double *dst, int64_t dstride, const double *src, 
 }
 
 // -------------------------------------------------------------------------
-static LaunchCtx make_ctx(const qmcw_handle *h, const KArgs &A) {
+enum { KC_JAS = 0, KC_DET = 1, KC_NLPP = 2, KC_MOVE = 3 };
+
+static LaunchCtx make_ctx(const qmcw_handle *h, const KArgs &A, int G) {
   LaunchCtx c;
   c.A = A;
   c.stream = h->stream;
-  c.G = h->G;
+  c.G = G;
   c.Wl = h->Wl;
   c.precision = h->precision;
   c.smem_ratio = 0;
   return c;
 }
 
-// Full evaluation of every walker; with derivs also the parameter
-// derivatives into h->der (all orbitals, Jastrow slots, pseudopotential
-// pieces, assembly).
 template <int G>
-static void launch_eval(qmcw_handle *h, bool derivs) {
-  dim3 grid(h->Wl / (kWave / G)), block(kWave);
-  KArgs A = h->A;
-  A.full_orb = derivs ? 1 : 0;
-  const OrdKernels &K = ord_kernels(A.S.nordc);
-  const LaunchCtx c = make_ctx(h, A);
-  prof_begin(h, QMCW_PROF_FULL_JAS);
-  K.full_jas(c);
-  prof_end(h);
-  if (derivs) {
-    prof_begin(h, QMCW_PROF_DERIV_JAS);
-    K.deriv_jas(c);
-    prof_end(h);
-  }
-  prof_begin(h, QMCW_PROF_FULL_DET);
+static void launch_det(qmcw_handle *h, const KArgs &A) {
+  const dim3 grid(h->Wl / (kWave / G)), block(kWave);
   if (h->precision == QMCW_FP64)
     hipLaunchKernelGGL((k_full_det<double, G>), grid, block, 0, h->stream, A);
   else
     hipLaunchKernelGGL((k_full_det<float, G>), grid, block, 0, h->stream, A);
+}
+template <int G>
+static void launch_fin(qmcw_handle *h, const KArgs &A) {
+  const dim3 grid(h->Wl / (kWave / G)), block(kWave);
+  hipLaunchKernelGGL((k_full_fin<G>), grid, block, 0, h->stream, A);
+}
+template <int G>
+static void launch_derivs_asm(qmcw_handle *h, const KArgs &A) {
+  const dim3 grid(h->Wl / (kWave / G)), block(kWave);
+  hipLaunchKernelGGL((k_derivs<G>), grid, block, 0, h->stream, A, h->der);
+}
+
+#define DISPATCH_GW(gw, fn, ...)                  \
+  switch (gw) {                                   \
+    case 8: fn<8>(__VA_ARGS__); break;            \
+    case 16: fn<16>(__VA_ARGS__); break;          \
+    case 32: fn<32>(__VA_ARGS__); break;          \
+    default: fn<64>(__VA_ARGS__); break;          \
+  }
+
+// Full evaluation of every walker; with derivs also the parameter
+// derivatives into h->der (all orbitals, Jastrow slots, pseudopotential
+// pieces, assembly).  Each kernel class runs at its own lane-group width.
+static void launch_eval(qmcw_handle *h, bool derivs) {
+  KArgs A = h->A;
+  A.full_orb = derivs ? 1 : 0;
+  // electrons per basis chunk of k_full_det: a wavefront's basis values
+  // (64/G walkers) stay near 16 KB so that they are read back from L2
+  const int gd = h->Gk[KC_DET];
+  const size_t per_e = (size_t)(kWave / gd) * A.S.NB * 5 *
+                       (h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float));
+  A.chunk = std::max(1, std::min(h->chunk, (int)((16u << 10) / std::max<size_t>(per_e, 1))));
+  if (h->chunk_override > 0) A.chunk = std::min(h->chunk, h->chunk_override);
+  const OrdKernels &K = ord_kernels(A.S.nordc);
+  const LaunchCtx cj = make_ctx(h, A, h->Gk[KC_JAS]), cn = make_ctx(h, A, h->Gk[KC_NLPP]);
+  prof_begin(h, QMCW_PROF_FULL_JAS);
+  K.full_jas(cj);
+  prof_end(h);
+  if (derivs) {
+    prof_begin(h, QMCW_PROF_DERIV_JAS);
+    K.deriv_jas(cj);
+    prof_end(h);
+  }
+  prof_begin(h, QMCW_PROF_FULL_DET);
+  DISPATCH_GW(gd, launch_det, h, A);
   prof_end(h);
   if (A.S.nloc > 0) {
     prof_begin(h, QMCW_PROF_FULL_NLPP);
-    K.full_nlpp(c);
+    K.full_nlpp(cn);
     prof_end(h);
     if (derivs) {
       prof_begin(h, QMCW_PROF_DERIV_NLPP);
-      K.deriv_nlpp(c);
+      K.deriv_nlpp(cn);
       prof_end(h);
     }
   }
   prof_begin(h, QMCW_PROF_FULL_FIN);
-  hipLaunchKernelGGL((k_full_fin<G>), grid, block, 0, h->stream, A);
+  DISPATCH_GW(gd, launch_fin, h, A);
   prof_end(h);
   if (derivs) {
     prof_begin(h, QMCW_PROF_DERIVS);
-    hipLaunchKernelGGL((k_derivs<G>), grid, block, 0, h->stream, A, h->der);
+    DISPATCH_GW(gd, launch_derivs_asm, h, A);
     prof_end(h);
     h->der_valid = 1;
   }
 }
-template <int G>
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
                          double fbias, double *out, long long *nacc = nullptr) {
+  const int G = h->Gk[KC_MOVE];
   // the moved electron's basis values in LDS when they fit (<= 48 KB per block)
   const size_t lds = (size_t)(kWave / G) * h->A.S.NB * 5 * sizeof(double);
   KArgs A = h->A;
   A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;
-  LaunchCtx c = make_ctx(h, A);
+  LaunchCtx c = make_ctx(h, A, G);
   c.smem_ratio = A.lds_phi ? lds : 0;
   prof_begin(h, QMCW_PROF_RATIO);
   ord_kernels(A.S.nordc).ratio(c, ie, mode, xn, delta, fbias, out, nacc);
@@ -522,13 +556,8 @@ static void launch_grad(qmcw_handle *h, int ie, double *out) {
   hipLaunchKernelGGL((k_grad_1e<G>), grid, block, 0, h->stream, h->A, ie, out);
 }
 
-#define DISPATCH_G(h, fn, ...)                    \
-  switch ((h)->G) {                               \
-    case 8: fn<8>(h, ##__VA_ARGS__); break;       \
-    case 16: fn<16>(h, ##__VA_ARGS__); break;     \
-    case 32: fn<32>(h, ##__VA_ARGS__); break;     \
-    default: fn<64>(h, ##__VA_ARGS__); break;     \
-  }
+// one-electron kernels run at the width of the move class
+#define DISPATCH_MOVE(h, fn, ...) DISPATCH_GW((h)->Gk[KC_MOVE], fn, h, ##__VA_ARGS__)
 
 // -------------------------------------------------------------------------
 extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precision,
@@ -771,6 +800,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     if (g == 8 || g == 16 || g == 32 || g == 64) G = g;
   }
   h->G = G;
+  for (int k = 0; k < 4; k++) h->Gk[k] = G;
   // padded walker count: a multiple of the walkers per wavefront of every
   // group width (8 for G = 8), so that the autotuner can switch G freely
   h->Wp = ((nwalk + 7) / 8) * 8;
@@ -869,6 +899,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   h->Wl = h->Wp;
   if (const char *e = getenv("QMCW_NO_GRAPH")) h->use_graph = atoi(e) == 0;
   if (const char *e = getenv("QMCW_NO_LDS")) h->use_lds = atoi(e) == 0;
+  if (const char *e = getenv("QMCW_CHUNK")) h->chunk_override = atoi(e);
   if (hipStreamSynchronize(h->stream) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_HIP, "init sync"); }
   *out = h;
   return QMCW_OK;
@@ -1014,7 +1045,7 @@ static int check_flags(qmcw_handle *h) {
 extern "C" int qmcw_eval_full(qmcw_handle *h, double *logpsid, double *sgn, double *psij,
                               double *grad, double *lap, double *eloc, double *epot) {
   if (!h) return QMCW_ERR_ARG;
-  DISPATCH_G(h, launch_eval, false);
+  launch_eval(h, false);
   HIPCHK(h, hipGetLastError());
   int N = h->A.S.N, stride = 8 + 4 * N;
   hipLaunchKernelGGL(k_gather_full, dim3((h->W + 63) / 64), dim3(64), 0, h->stream, h->A, h->io, h->W);
@@ -1043,7 +1074,7 @@ extern "C" int qmcw_ratio_1e(qmcw_handle *h, int32_t ie, const double *xnew, dou
   for (int w = 0; w < h->Wp; w++)
     for (int q = 0; q < 3; q++) xp[3 * w + q] = xnew[3 * std::min(w, h->W - 1) + q];
   HIPCHK(h, hipMemcpyAsync(xin, xp.data(), xp.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  DISPATCH_G(h, launch_ratio, ie, 0, (const double *)xin, 1.0, 1.0, h->io);
+  launch_ratio(h, ie, 0, (const double *)xin, 1.0, 1.0, h->io);
   HIPCHK(h, hipGetLastError());
   std::vector<double> o((size_t)h->W * 8);
   HIPCHK(h, hipMemcpyAsync(o.data(), h->io, o.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1061,7 +1092,7 @@ extern "C" int qmcw_accept_1e(qmcw_handle *h, int32_t ie, const uint8_t *accept)
   std::vector<uint8_t> a(h->Wp, 0);
   for (int w = 0; w < h->W; w++) a[w] = accept[w];
   HIPCHK(h, hipMemcpyAsync(h->acc_buf, a.data(), h->Wp, hipMemcpyHostToDevice, h->stream));
-  DISPATCH_G(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+  DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return QMCW_OK;
@@ -1069,7 +1100,7 @@ extern "C" int qmcw_accept_1e(qmcw_handle *h, int32_t ie, const uint8_t *accept)
 
 extern "C" int qmcw_grad_1e(qmcw_handle *h, int32_t ie, double *grad, double *lap) {
   if (!h || ie < 0 || ie >= h->A.S.N) return QMCW_ERR_ARG;
-  DISPATCH_G(h, launch_grad, ie, h->io);
+  DISPATCH_MOVE(h, launch_grad, ie, h->io);
   HIPCHK(h, hipGetLastError());
   std::vector<double> o((size_t)h->W * 4);
   HIPCHK(h, hipMemcpyAsync(o.data(), h->io, o.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1090,9 +1121,9 @@ static void vmc_one_step(qmcw_handle *h, double delta, double fbias) {
     prof_end(h);
   }
   for (int ie = 0; ie < N; ie++)   // proposal, ratio, Metropolis test and commit in one launch
-    DISPATCH_G(h, launch_ratio, ie, 2, (const double *)nullptr, delta, fbias, (double *)nullptr,
+    launch_ratio(h, ie, 2, (const double *)nullptr, delta, fbias, (double *)nullptr,
                h->nacc);
-  DISPATCH_G(h, launch_eval, h->sample_derivs != 0);
+  launch_eval(h, h->sample_derivs != 0);
 }
 
 static void drop_graph(qmcw_handle *h) {
@@ -1108,7 +1139,8 @@ static void drop_graph(qmcw_handle *h) {
 // value changes.
 static int step_graph(qmcw_handle *h, double delta, double fbias) {
   const int derivs = h->sample_derivs != 0;
-  if (h->graph_ok && h->graph_G == h->G && h->graph_Wl == h->Wl && h->graph_derivs == derivs &&
+  const int gkey = h->Gk[0] | (h->Gk[1] << 8) | (h->Gk[2] << 16) | (h->Gk[3] << 24);
+  if (h->graph_ok && h->graph_G == gkey && h->graph_Wl == h->Wl && h->graph_derivs == derivs &&
       h->graph_delta == delta && h->graph_fbias == fbias && h->graph_st == h->A.st)
     return QMCW_OK;
   drop_graph(h);
@@ -1120,13 +1152,13 @@ static int step_graph(qmcw_handle *h, double delta, double fbias) {
   h->graph = g;
   HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
   h->graph_ok = 1;
-  h->graph_G = h->G; h->graph_Wl = h->Wl; h->graph_derivs = derivs;
+  h->graph_G = gkey; h->graph_Wl = h->Wl; h->graph_derivs = derivs;
   h->graph_delta = delta; h->graph_fbias = fbias; h->graph_st = h->A.st;
   return QMCW_OK;
 }
 
 static int vmc_steps(qmcw_handle *h, int nstep, double delta, double fbias, bool trace) {
-  DISPATCH_G(h, launch_eval, false);
+  launch_eval(h, false);
   if (!trace && !h->prof && h->use_graph && nstep > 0) {
     int st = step_graph(h, delta, fbias);
     if (st) return st;
@@ -1193,50 +1225,65 @@ extern "C" int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count) {
   return QMCW_OK;
 }
 
-// Group-width autotuner: times the full evaluation plus one ratio launch per
-// electron (mode 0 at the current positions; neither changes the walkers or
-// their random streams) for each admissible G and keeps the fastest.
+// Group-width autotuner: for each admissible width, times every kernel of the
+// full evaluation and one ratio launch per electron (mode 0 at the current
+// positions: neither changes the walkers or their random streams) with HIP
+// events, and keeps the fastest width of each kernel class.
 extern "C" int qmcw_autotune(qmcw_handle *h, int32_t *chosen) {
   if (!h) return QMCW_ERR_ARG;
   const int N = h->A.S.N;
-  int cand[4] = {8, 16, 32, 64};
-  double best = 1e30;
-  int bestG = h->G;
+  const int cand[4] = {8, 16, 32, 64};
+  // kernel classes of the profiled kernels
+  const int cls[QMCW_PROF_NKERNEL] = {KC_JAS, KC_DET, KC_NLPP, KC_DET, KC_MOVE, KC_MOVE, -1,
+                                      KC_JAS, KC_NLPP, KC_DET};
+  double best[4] = {1e30, 1e30, 1e30, 1e30};
+  int bestG[4] = {h->Gk[0], h->Gk[1], h->Gk[2], h->Gk[3]};
   double *xin = h->io + (size_t)h->Wp * 8;
-  // current positions of electron ie as the proposal
-  hipEvent_t e0, e1;
-  HIPCHK(h, hipEventCreate(&e0));
-  HIPCHK(h, hipEventCreate(&e1));
+  const int prof_save = h->prof;
+  drop_graph(h);
   for (int c = 0; c < 4; c++) {
-    int G = cand[c];
+    const int G = cand[c];
     if (G > 16 && N <= 8) continue;      // lanes would mostly idle
     if (G < 16 && N > 48) continue;
-    h->G = G;
-    double tot = 0.0;
+    for (int k = 0; k < 4; k++) h->Gk[k] = G;
+    double t[4] = {0, 0, 0, 0};
     for (int rep = 0; rep < 2; rep++) {
-      HIPCHK(h, hipEventRecord(e0, h->stream));
-      DISPATCH_G(h, launch_eval, false);
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      h->prof = rep == 1;          // the first repetition warms up
+      h->ev_used = 0;
+      launch_eval(h, false);
       for (int ie = 0; ie < N; ie++) {
-        hipLaunchKernelGGL(k_copy_strided, dim3((h->Wp + 63) / 64), dim3(64), 0, h->stream, xin,
+        hipLaunchKernelGGL(k_copy_strided, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, xin,
                            (int64_t)3, (const double *)(h->A.st + h->A.L.x + 3 * ie),
-                           (int64_t)h->A.L.stride, 3, h->Wp);
-        DISPATCH_G(h, launch_ratio, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
+                           (int64_t)h->A.L.stride, 3, h->Wl);
+        launch_ratio(h, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
       }
-      HIPCHK(h, hipEventRecord(e1, h->stream));
-      HIPCHK(h, hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
-      if (rep == 1) tot = ms;   // first repetition warms up
+      HIPCHK(h, hipStreamSynchronize(h->stream));
     }
-    if (tot < best) { best = tot; bestG = G; }
+    for (size_t i = 0; i + 1 < h->ev_used; i += 2) {
+      const int ph = h->ev_phase[i];
+      if (ph < 0 || ph >= QMCW_PROF_NKERNEL || cls[ph] < 0) continue;
+      float ms = 0.f;
+      HIPCHK(h, hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+      t[cls[ph]] += ms;
+    }
+    for (int k = 0; k < 4; k++)
+      if (t[k] > 0 && t[k] < best[k]) { best[k] = t[k]; bestG[k] = G; }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  h->G = bestG;
-  // restore a consistent resident state with the chosen width
-  DISPATCH_G(h, launch_eval, false);
+  h->prof = prof_save;
+  h->ev_used = 0;
+  for (int k = 0; k < 4; k++) h->Gk[k] = bestG[k];
+  h->G = bestG[KC_MOVE];
+  // restore a consistent resident state with the chosen widths
+  launch_eval(h, false);
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (chosen) *chosen = bestG;
+  if (chosen) *chosen = h->G;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_get_group_widths(const qmcw_handle *h, int32_t *widths) {
+  if (!h || !widths) return QMCW_ERR_ARG;
+  for (int k = 0; k < 4; k++) widths[k] = h->Gk[k];
   return QMCW_OK;
 }
 
@@ -1246,6 +1293,7 @@ extern "C" int qmcw_set_group_width(qmcw_handle *h, int32_t G) {
     return fail(h, QMCW_ERR_ARG, "group width must be 8, 16, 32 or 64");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->G = G;
+  for (int k = 0; k < 4; k++) h->Gk[k] = G;
   return QMCW_OK;
 }
 
@@ -1275,7 +1323,7 @@ static int copy_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
 extern "C" int qmcw_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
   if (!h) return QMCW_ERR_ARG;
   if (h->nparm == 0) return fail(h, QMCW_ERR_ARG, "the system has no parameters to differentiate");
-  DISPATCH_G(h, launch_eval, true);
+  launch_eval(h, true);
   HIPCHK(h, hipGetLastError());
   int st = copy_derivs(h, dpsi, deloc);
   if (st) return st;
@@ -1337,7 +1385,7 @@ extern "C" int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weigh
   std::vector<double> wt(h->Wp, 1.0);
   if (weight) std::copy(weight, weight + nwalk, wt.begin());
   HIPCHK(h, hipMemcpyAsync(h->dwt, wt.data(), h->Wp * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  DISPATCH_G(h, launch_eval, false);
+  launch_eval(h, false);
   hipLaunchKernelGGL(k_dmc_init, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A, h->dm,
                      (const double *)h->dwt, nwalk);
   HIPCHK(h, hipGetLastError());
@@ -1437,14 +1485,14 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
   if (h->A.S.nloc > 0) hipLaunchKernelGGL(k_rotqua, g1, b1, 0, h->stream, h->A);
   hipLaunchKernelGGL(k_dmc_gen_begin, g1, b1, 0, h->stream, h->A, h->dm);
   for (int ie = 0; ie < N; ie++) {
-    DISPATCH_G(h, launch_grad, ie, gbuf);
+    DISPATCH_MOVE(h, launch_grad, ie, gbuf);
     hipLaunchKernelGGL(k_dmc_propose, g1, b1, 0, h->stream, h->A, ie, (const double *)gbuf, xin,
                        h->dm, *p);
-    DISPATCH_G(h, launch_ratio, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
+    launch_ratio(h, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
     hipLaunchKernelGGL(k_dmc_decide, g1, b1, 0, h->stream, h->A, ie, h->dm, h->acc_buf, *p);
-    DISPATCH_G(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+    DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
   }
-  DISPATCH_G(h, launch_eval, false);
+  launch_eval(h, false);
   hipLaunchKernelGGL(k_dmc_finish, g1, b1, 0, h->stream, h->A, h->dm, *p);
   hipLaunchKernelGGL(k_dmc_sums, dim3(1), dim3(256), 0, h->stream, h->A, (const double *)h->dm,
                      h->nact, h->dstat);

@@ -249,7 +249,7 @@ __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
 // w exp(dJ) v P_l (Psi_D ratio) d(dJ)/dp.  Each lane takes whole
 // (electron, centre) pairs and owns their rows; k_full_nlpp has filled the
 // channel values (SLayout::vnl) of the current configuration.
-template <int G, int ORD>
+template <typename T, int G, int ORD>
 __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, NO = S.NO, NB = S.NB, nsl = S.nslot;
@@ -275,7 +275,7 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
     const double *g = ss + SL.g[s];
     const double rei = ren[p];
     double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
-    double *phq = ss + DL.phq + (int64_t)lane * NB;
+    T *phq = reinterpret_cast<T *>(ss + DL.phq + (int64_t)lane * NB);
     double *orbq = ss + DL.orbq + (int64_t)lane * NO;
     const SlotAcc acc{drow, 1};
     double wsum = 0.0;
@@ -290,26 +290,27 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
         rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
         rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
       }
-      for (int b = 0; b < NB; b++) phq[b] = 0.0;
+      for (int b = 0; b < NB; b++) phq[b] = T(0);
       for (int sh = 0; sh < S.nshell; sh++) {
         const int jc = S.sh_cent[sh];
-        eval_shell<double, 1>(S, sh, rvnew[3 * jc], rvnew[3 * jc + 1], rvnew[3 * jc + 2], rnew[jc], 0, phq);
+        eval_shell<T, 1>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1], (T)rvnew[3 * jc + 2],
+                         (T)rnew[jc], 0, phq);
       }
       for (int q = 0; q < NO; q++) {
         double a = 0.0;
-        for (int b = 0; b < NB; b++) a += S.coefT[(int64_t)b * NO + q] * phq[b];
+        for (int b = 0; b < NB; b++) a += S.coefT[(int64_t)b * NO + q] * (double)phq[b];
         orbq[q] = a;
       }
       double rat = 0.0;
       for (int j = 0; j < nocc; j++) rat += g[j * ns + ie] * orbq[S.occ[s][j]];
-      double delta = jas_en_val<double>(S, rnew) - fso[i * N + i];
+      double delta = (double)jas_en_val<T>(S, rnew) - fso[i * N + i];
       for (int jj = 0; jj < N; jj++) {
         if (jj == i) continue;
         const int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
         double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
         const double rij = sqrt(dx * dx + dy * dy + dz * dz);
         const bool par = (hi < S.nup) || (lo >= S.nup);
-        delta += jas_pair_val<double, ORD>(S, par, rij, rnew, ren + jj * M) - fso[hi * N + lo];
+        delta += (double)jas_pair_val<T, ORD>(S, par, (T)rij, rnew, ren + jj * M) - fso[hi * N + lo];
       }
       double vpot = 0.0;
       for (int l = 1; l <= S.npotd[ict]; l++)

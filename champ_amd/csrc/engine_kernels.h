@@ -48,12 +48,32 @@ __device__ __forceinline__ float bf16r(float x) {
   return __uint_as_float(u & 0xFFFF0000u);
 }
 
-// one product of the orbital contraction: coefficient times basis value, with
-// bfloat16 inputs in the BF16 engine
+// orbital value, gradient and laplacian o[0..4] = sum_b coefT[b][lab] phi_b[0..4]
+// (phi with stride 5); BF16: inputs rounded to bfloat16
+template <bool BF16, typename T>
+__device__ __forceinline__ void contract5(const double *__restrict__ coefT, int NO, int NB, int lab,
+                                          const T *__restrict__ ph, double *o) {
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+  for (int b = 0; b < NB; b++) {
+    const double cf = coefT[(int64_t)b * NO + lab];
+    const T *p = ph + 5 * b;
+    if (BF16) {
+      const float c = bf16r((float)cf);
+      a0 += (double)(c * bf16r((float)p[0])); a1 += (double)(c * bf16r((float)p[1]));
+      a2 += (double)(c * bf16r((float)p[2])); a3 += (double)(c * bf16r((float)p[3]));
+      a4 += (double)(c * bf16r((float)p[4]));
+    } else {
+      a0 += cf * (double)p[0]; a1 += cf * (double)p[1]; a2 += cf * (double)p[2];
+      a3 += cf * (double)p[3]; a4 += cf * (double)p[4];
+    }
+  }
+  o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
+}
 template <typename T>
-__device__ __forceinline__ double cmul(int bf16, double cf, T ph) {
-  if (bf16) return (double)(bf16r((float)cf) * bf16r((float)ph));
-  return cf * (double)ph;
+__device__ __forceinline__ void contract5(int bf16, const double *coefT, int NO, int NB, int lab,
+                                          const T *ph, double *o) {
+  if (bf16) contract5<true, T>(coefT, NO, NB, lab, ph, o);
+  else contract5<false, T>(coefT, NO, NB, lab, ph, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,31 +357,14 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
       int i = i0 + ii, s = (i < S.nup) ? 0 : 1;
       double *o = oc + ((int64_t)i * noccmax + j) * 5;
       if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
-      int lab = S.occ[s][j];
-      double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-      const T *ph = phi + (int64_t)ii * S.NB * 5;
-      for (int b = 0; b < S.NB; b++) {
-        double cf = S.coefT[(int64_t)b * NO + lab];
-        a0 += cmul(A.bf16, cf, ph[5 * b]); a1 += cmul(A.bf16, cf, ph[5 * b + 1]);
-        a2 += cmul(A.bf16, cf, ph[5 * b + 2]); a3 += cmul(A.bf16, cf, ph[5 * b + 3]);
-        a4 += cmul(A.bf16, cf, ph[5 * b + 4]);
-      }
-      o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
+      contract5<T>(A.bf16, S.coefT, NO, S.NB, S.occ[s][j], phi + (int64_t)ii * S.NB * 5, o);
     }
     if (A.full_orb) {
       for (int t = lane; t < nch * NO; t += G) {
         int ii = t / NO, lab = t % NO;
         int i = i0 + ii;
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-        const T *ph = phi + (int64_t)ii * S.NB * 5;
-        for (int b = 0; b < S.NB; b++) {
-          double cf = S.coefT[(int64_t)b * NO + lab];
-          a0 += cmul(A.bf16, cf, ph[5 * b]); a1 += cmul(A.bf16, cf, ph[5 * b + 1]);
-          a2 += cmul(A.bf16, cf, ph[5 * b + 2]); a3 += cmul(A.bf16, cf, ph[5 * b + 3]);
-          a4 += cmul(A.bf16, cf, ph[5 * b + 4]);
-        }
-        double *o = orbf + ((int64_t)i * NO + lab) * 5;
-        o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
+        contract5<T>(A.bf16, S.coefT, NO, S.NB, lab, phi + (int64_t)ii * S.NB * 5,
+                     orbf + ((int64_t)i * NO + lab) * 5);
       }
     }
   }
@@ -769,13 +772,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   __syncthreads();
   double *orbe = mv + ML.orbe;
   for (int j = lane; j < nocc; j += G) {
-    int lab = S.occ[s][j];
-    double a[5] = {0, 0, 0, 0, 0};
-    for (int b = 0; b < S.NB; b++) {
-      double cf = S.coefT[(int64_t)b * S.NO + lab];
-      for (int c = 0; c < 5; c++) a[c] += cmul(A.bf16, cf, phi[5 * b + c]);
-    }
-    for (int c = 0; c < 5; c++) orbe[5 * j + c] = a[c];
+    contract5<T>(A.bf16, S.coefT, S.NO, S.NB, S.occ[s][j], phi, orbe + 5 * j);
   }
   __syncthreads();
   // ---- dete_update (qmc/dete_mod.f90:114-214)

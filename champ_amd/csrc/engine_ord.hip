@@ -52,7 +52,10 @@ void deriv_jas_g(const LaunchCtx &c) {
 template <int G>
 void deriv_nlpp_g(const LaunchCtx &c) {
   const dim3 grid(c.Wl / (kWave / G)), block(kWave);
-  hipLaunchKernelGGL((k_deriv_nlpp<G, ORD>), grid, block, 0, c.stream, c.A);
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_deriv_nlpp<double, G, ORD>), grid, block, 0, c.stream, c.A);
+  else
+    hipLaunchKernelGGL((k_deriv_nlpp<float, G, ORD>), grid, block, 0, c.stream, c.A);
 }
 
 #define QMCW_BY_G(fn, c, ...)                  \

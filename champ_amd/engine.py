@@ -193,6 +193,13 @@ class Engine:
         self._chk(self.lib.qmcw_autotune(self.h, C.byref(g)))
         return g.value
 
+    def group_widths(self):
+        """Lane-group widths per kernel class (Jastrow, determinants,
+        pseudopotential, moves)."""
+        g = np.zeros(4, np.int32)
+        self._chk(self.lib.qmcw_get_group_widths(self.h, g.ctypes.data_as(C.POINTER(C.c_int32))))
+        return [int(v) for v in g]
+
     def set_group_width(self, g: int):
         self._chk(self.lib.qmcw_set_group_width(self.h, int(g)))
 

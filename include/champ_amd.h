@@ -300,14 +300,22 @@ int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
 int qmcw_profile(qmcw_handle *h, int32_t enable);
 int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count);
 
-/* Choose the lane-group width (walkers per wavefront) by timing the
-   candidates on the resident walkers with side-effect-free launches (no
-   counterpart in the reference).  chosen (may be NULL) receives the width. */
+/* Choose the lane-group width (walkers per wavefront) of each kernel class
+   by timing the candidates on the resident walkers with side-effect-free
+   launches (no counterpart in the reference).  chosen (may be NULL)
+   receives the width of the one-electron moves. */
 int qmcw_autotune(qmcw_handle *h, int32_t *chosen);
 
-/* Set the lane-group width (8, 16, 32 or 64 lanes per walker) explicitly;
-   results are identical for every width (no counterpart in the reference). */
+/* Set the lane-group width (8, 16, 32 or 64 lanes per walker) of every
+   kernel explicitly; results are identical for every width (no counterpart
+   in the reference). */
 int qmcw_set_group_width(qmcw_handle *h, int32_t G);
+
+/* Lane-group widths in use, per kernel class: widths[0] Jastrow,
+   [1] orbitals/determinants/local energy, [2] pseudopotential, [3]
+   one-electron moves (qmcw_autotune chooses each; no counterpart in the
+   reference). */
+int qmcw_get_group_widths(const qmcw_handle *h, int32_t *widths);
 
 /* Wait for all work queued on the handle's stream. */
 int qmcw_sync(qmcw_handle *h);
