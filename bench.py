@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--deriv-walkers", type=int, default=8192)
     ap.add_argument("--deriv-steps", type=int, default=40)
-    ap.add_argument("--derivs", action="store_true", help="also measure derivative samples/s")
+    ap.add_argument("--no-derivs", action="store_true", help="skip the derivative samples/s measurement")
     ap.add_argument("--dmc", action="store_true",
                     help="only run DMC of the 98-electron chain (4096 walkers, tau 0.01)")
     ap.add_argument("--batch", type=int, default=0,
@@ -358,7 +358,7 @@ def main():
                       f"total {ms:9.3f} ms", file=sys.stderr, flush=True)
     # derivative samples/s (the optimisation sample)
     dres = {}
-    if args.derivs:
+    if not args.no_derivs:
         for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
             dres[prec] = run_derivs(args.deriv_walkers, args.deriv_steps, args.warmup, prec,
                                     args.delta, max(2, args.prof_steps // 2))

@@ -108,6 +108,8 @@ struct SLayout {
   int64_t qd;                // [G][4M] quadrature-point distances
   int64_t vnl;               // [N*M][QMCW_MAX_LPOT+1] nonlocal channel values
   int64_t gproj;             // [N][NB] G(:,i)^T coef of the occupied orbitals
+  int64_t bt;                // [N][M] e-e-n variable 1 - k scaled(r_j,ic) (as the kernel's T)
+  int64_t uq;                // [64][N] per-lane e-e-n variable of the pairs of a point
   int64_t mv;                // pending one-electron move buffers
   int64_t stride;
 };
@@ -117,11 +119,12 @@ struct SLayout {
 struct DLayout {
   int64_t raw;               // [nslot][N][5] slot pieces: value, grad, lap
   int64_t gvalue, gjas, d2g; // [npj], [npj][N][3], [npj]
-  int64_t pnl_c;             // [N*M][NO] per (electron, centre) psp_nonloc_orb
+  int64_t pbc;               // [N*M][NB] per (electron, centre) weighted basis sums
   int64_t dpe_raw;           // [N*M][nslot]
   int64_t dpe_slot, dpe;     // [nslot], [npj]
   int64_t pnl;               // [N][NO] psp_nonloc_orb
-  int64_t phq, orbq;         // [64][NB], [64][NO] per-lane quadrature values
+  int64_t phq;               // [64][NB] per-lane quadrature-point basis values
+  int64_t btd, uqd;          // [N][M] e-e-n variable B(r_j,ic), [64][N] per-lane U of a point
   int64_t B;                 // [N][NO] orbital-resolved local energy
   int64_t tfull[2], dt[2];   // [ns][NO]
   int64_t aib;               // [ns][NO]

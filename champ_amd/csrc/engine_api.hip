@@ -846,15 +846,18 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SL.qd = take((int64_t)kWave * 4 * M);
   SL.vnl = take((int64_t)N * M * (QMCW_MAX_LPOT + 1));
   SL.gproj = take((int64_t)N * NB);
+  SL.bt = take((int64_t)N * M);
+  SL.uq = take((int64_t)kWave * N);
   SL.mv = take(ML.stride);
   if (S.nparm > 0) {
     DLayout &DL = h->A.DL;
     int64_t NM = (int64_t)N * M;
     DL.raw = take((int64_t)S.nslot * N * 5);
     DL.gvalue = take(S.npj); DL.gjas = take((int64_t)S.npj * N * 3); DL.d2g = take(S.npj);
-    DL.pnl_c = take(NM * NO); DL.dpe_raw = take(NM * S.nslot);
+    DL.pbc = take(NM * NB); DL.dpe_raw = take(NM * S.nslot);
     DL.dpe_slot = take(S.nslot); DL.dpe = take(S.npj); DL.pnl = take((int64_t)N * NO);
-    DL.phq = take((int64_t)kWave * NB); DL.orbq = take((int64_t)kWave * NO);
+    DL.phq = take((int64_t)kWave * NB);
+    DL.btd = take(NM); DL.uqd = take((int64_t)kWave * N);
     DL.B = take((int64_t)N * NO);
     for (int sp = 0; sp < 2; sp++) {
       DL.tfull[sp] = take((int64_t)D.ns[sp] * NO); DL.dt[sp] = take((int64_t)D.ns[sp] * NO);

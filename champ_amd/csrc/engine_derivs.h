@@ -117,10 +117,9 @@ __device__ __forceinline__ void een_slots(double U, double A, double B, double U
 // i: rij, rvij = r_i - r_j, the distances of i and j to the centres and i's
 // vectors to them (qmc/vmc/deriv_jastrow4.f90:95-720).  cv scales the value
 // pieces (0: none); FULL adds the gradient wrt r_i and i's laplacian part.
-template <bool FULL, int ORD>
-__device__ void pair_slots(const DevSys &S, bool par, double rij, const double *rvij,
-                           const double *ri_v, const double *rj_v, const double *rvi, double cv,
-                           const SlotAcc &acc) {
+template <bool FULL>
+__device__ void b_slots(const DevSys &S, bool par, double rij, const double *rvij, double cv,
+                        const SlotAcc &acc) {
   const double k = S.scalek, ar = S.asymp_r;
   double sspinn = 1.0;
   int isb = 0;
@@ -166,6 +165,14 @@ __device__ void pair_slots(const DevSys &S, bool par, double rij, const double *
       acc.add(slot, 4, geeuu + 2.0 * geeu);
     }
   }
+}
+
+template <bool FULL, int ORD>
+__device__ void pair_slots(const DevSys &S, bool par, double rij, const double *rvij,
+                           const double *ri_v, const double *rj_v, const double *rvi, double cv,
+                           const SlotAcc &acc) {
+  const double k = S.scalek;
+  b_slots<FULL>(S, par, rij, rvij, cv, acc);
   if (S.nordc <= 1) return;
   double U, Ud1, Ud2;
   scaled<double>(S.isc, k, rij, U, Ud1, Ud2);
@@ -183,6 +190,71 @@ __device__ void pair_slots(const DevSys &S, bool par, double rij, const double *
     een_slots<FULL, ORD>(U, A, B, Ud1, Ud2, Ad1, Ad2, rij, ri, rvij,
                          FULL ? rvi + 3 * ic : nullptr, u2pst, cv, acc,
                          S.slot_c0 + it * S.nterms);
+  }
+}
+
+// Value pieces, weighted by cv, of the b-coefficient and e-e-n terms of all
+// pairs (i, j), electron i at xn (distances ri_v to the centres) and every
+// other electron where it is (btd[j * M + ic] = 1 - k scaled(r_j,ic);
+// uqd: per-lane scratch of N values).  Summed over j, the term values of a
+// centre are A^(l+m) S(k, m) + A^m S(k, l+m) with S(k, n) = sum_j U_j^k B_j^n,
+// so each slot takes one addition per call.
+template <int ORD>
+__device__ void point_pair_slots(const DevSys &S, int i, const double *xn, const double *x,
+                                 const double *ri_v, const double *btd, double *uqd, double cv,
+                                 const SlotAcc &acc) {
+  const int N = S.N, M = S.M;
+  const double k = S.scalek;
+  for (int jj = 0; jj < N; jj++) {
+    if (jj == i) continue;
+    const double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
+    const double rij = sqrt(dx * dx + dy * dy + dz * dz);
+    b_slots<false>(S, (i < S.nup) == (jj < S.nup), rij, nullptr, cv, acc);
+    double u, d1, d2;
+    scaled<double>(S.isc, k, rij, u, d1, d2);
+    uqd[jj] = 1.0 - k * u;
+  }
+  if (S.nordc <= 1) return;
+  constexpr int NG = een_gi<ORD>(ORD, 0) + 1;
+  for (int ic = 0; ic < M; ic++) {
+    double sg[NG];
+#pragma unroll
+    for (int q = 0; q < NG; q++) sg[q] = 0.0;
+    for (int jj = 0; jj < N; jj++) {
+      if (jj == i) continue;
+      const double U = uqd[jj], B = btd[jj * M + ic];
+      double uk = 1.0;
+#pragma unroll
+      for (int kk = 0; kk <= ORD; kk++) {
+        double t = uk;
+#pragma unroll
+        for (int n = 0; n <= ORD - kk; n++) {
+          sg[een_gi<ORD>(kk, n)] += t;
+          t *= B;
+        }
+        uk *= U;
+      }
+    }
+    double A, t1, t2;
+    scaled<double>(S.isc, k, ri_v[ic], A, t1, t2);
+    A = 1.0 - k * A;
+    double ap[ORD + 1];
+    ap[0] = 1.0;
+#pragma unroll
+    for (int q = 1; q <= ORD; q++) ap[q] = ap[q - 1] * A;
+    const int slot0 = S.slot_c0 + S.iwctype[ic] * S.nterms;
+    int idx = 0;
+#pragma unroll
+    for (int kk = 0; kk <= ORD; kk++)
+#pragma unroll
+      for (int m = 0; 2 * m <= ORD - kk; m++)
+#pragma unroll
+        for (int l = 0; l <= ORD - kk - 2 * m; l++)
+          if (QMCW_EEN_TERM(kk, l, m)) {
+            acc.add(slot0 + idx, 0, cv * (ap[l + m] * sg[een_gi<ORD>(kk, m)] +
+                                          ap[m] * sg[een_gi<ORD>(kk, l + m)]));
+            idx++;
+          }
   }
 }
 
@@ -249,6 +321,11 @@ __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
 // w exp(dJ) v P_l (Psi_D ratio) d(dJ)/dp.  Each lane takes whole
 // (electron, centre) pairs and owns their rows; k_full_nlpp has filled the
 // channel values (SLayout::vnl) of the current configuration.
+// psp_nonloc_orb is linear in the orbitals, psp_nonloc_orb(i, q) =
+// sum_b coef(b, q) sum_points fac phi_b(point), so per point only the
+// weighted basis values are accumulated (NB terms) and the contraction with
+// the coefficients runs once per electron; the determinant ratio at a point
+// is sum_b gproj_i[b] phi_b(point).
 template <typename T, int G, int ORD>
 __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   QMCW_GROUP_SETUP
@@ -257,26 +334,34 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   const SLayout &SL = A.SL;
   const DLayout &DL = A.DL;
   const double *x = ws + L.x;
-  const double *ren = ws + L.r_en, *rven = ws + L.rvec_en, *ree = ws + L.r_ee, *rvee = ws + L.rvec_ee;
+  const double *ren = ws + L.r_en, *rven = ws + L.rvec_en;
   const double *fso = ws + L.fso;
   const double *vnl = ss + SL.vnl;
   const double *quad = ws + L.quad;
-  double *pnlc = ss + DL.pnl_c, *draw = ss + DL.dpe_raw;
+  const double *gp = ss + SL.gproj;
+  const T *bt = reinterpret_cast<const T *>(ss + SL.bt);    // filled by k_full_nlpp
+  T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
+  double *pbc = ss + DL.pbc, *draw = ss + DL.dpe_raw;
+  double *btd = ss + DL.btd, *uqd = ss + DL.uqd + (int64_t)lane * N;
   const int nq = S.nquad;
   for (int p = lane; p < N * M; p += G) {
-    double *prow = pnlc + (int64_t)p * NO;
+    double bu, t1, t2;
+    scaled<double>(S.isc, S.scalek, ren[p], bu, t1, t2);
+    btd[p] = 1.0 - S.scalek * bu;
+  }
+  __syncthreads();
+  for (int p = lane; p < N * M; p += G) {
+    double *pbrow = pbc + (int64_t)p * NB;
     double *drow = draw + (int64_t)p * nsl;
-    for (int q = 0; q < NO; q++) prow[q] = 0.0;
     for (int q = 0; q < nsl; q++) drow[q] = 0.0;
     const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
     if (v[QMCW_MAX_LPOT] == 0.0) continue;
+    for (int b = 0; b < NB; b++) pbrow[b] = 0.0;
     const int i = p / M, ic = p % M, ict = S.iwctype[ic];
-    const int s = i < S.nup ? 0 : 1, ie = s ? i - S.nup : i, ns = S.ns[s], nocc = S.nocc[s];
-    const double *g = ss + SL.g[s];
+    const double *gpi = gp + (int64_t)i * NB;
     const double rei = ren[p];
     double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
     T *phq = reinterpret_cast<T *>(ss + DL.phq + (int64_t)lane * NB);
-    double *orbq = ss + DL.orbq + (int64_t)lane * NO;
     const SlotAcc acc{drow, 1};
     double wsum = 0.0;
     for (int iq = 0; iq < nq; iq++) {
@@ -296,56 +381,36 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
         eval_shell<T, 1>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1], (T)rvnew[3 * jc + 2],
                          (T)rnew[jc], 0, phq);
       }
-      for (int q = 0; q < NO; q++) {
-        double a = 0.0;
-        for (int b = 0; b < NB; b++) a += S.coefT[(int64_t)b * NO + q] * (double)phq[b];
-        orbq[q] = a;
-      }
       double rat = 0.0;
-      for (int j = 0; j < nocc; j++) rat += g[j * ns + ie] * orbq[S.occ[s][j]];
-      double delta = (double)jas_en_val<T>(S, rnew) - fso[i * N + i];
-      for (int jj = 0; jj < N; jj++) {
-        if (jj == i) continue;
-        const int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
-        double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
-        const double rij = sqrt(dx * dx + dy * dy + dz * dz);
-        const bool par = (hi < S.nup) || (lo >= S.nup);
-        delta += (double)jas_pair_val<T, ORD>(S, par, (T)rij, rnew, ren + jj * M) - fso[hi * N + lo];
-      }
+      for (int b = 0; b < NB; b++) rat += gpi[b] * (double)phq[b];
+      double fold = fso[i * N + i];
+      for (int jj = 0; jj < N; jj++)
+        if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
+      const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
       double vpot = 0.0;
       for (int l = 1; l <= S.npotd[ict]; l++)
         if (l != S.lpotp1[ict]) vpot += v[l - 1] * legendre_w(l, costh);
       const double fac = S.q0[3 * nq + iq] * exp(delta) * vpot;
-      for (int q = 0; q < NO; q++) prow[q] += fac * orbq[q];
+      for (int b = 0; b < NB; b++) pbrow[b] += fac * (double)phq[b];
       const double wr = fac * rat;
       wsum += wr;
       en_slots<false>(S, rnew, rvnew, wr, acc);
-      for (int jj = 0; jj < N; jj++) {
-        if (jj == i) continue;
-        const int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
-        double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
-        const double rij = sqrt(dx * dx + dy * dy + dz * dz);
-        const bool par = (hi < S.nup) || (lo >= S.nup);
-        pair_slots<false, ORD>(S, par, rij, nullptr, rnew, ren + jj * M, nullptr, wr, acc);
-      }
+      point_pair_slots<ORD>(S, i, xn, x, rnew, btd, uqd, wr, acc);
     }
     // the same terms at the current position, weighted by the summed weight
     en_slots<false>(S, ren + i * M, rven + i * M * 3, -wsum, acc);
-    for (int jj = 0; jj < N; jj++) {
-      if (jj == i) continue;
-      const int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
-      const bool par = (hi < S.nup) || (lo >= S.nup);
-      pair_slots<false, ORD>(S, par, ree[hi * N + lo], nullptr, ren + i * M, ren + jj * M, nullptr, -wsum, acc);
-    }
-    (void)rvee;
+    point_pair_slots<ORD>(S, i, x + 3 * i, x, ren + i * M, btd, uqd, -wsum, acc);
   }
   __syncthreads();
-  double *pnl = ss + DL.pnl;
-  for (int t = lane; t < N * NO; t += G) {
-    const int i = t / NO, q = t % NO;
+  // per-electron basis sums over the centres in range, kept in the row of
+  // centre 0 (each lane reads and writes only its own (i, b) column)
+  for (int t = lane; t < N * NB; t += G) {
+    const int i = t / NB, b = t % NB;
     double a = 0.0;
-    for (int ic = 0; ic < M; ic++) a += pnlc[(int64_t)(i * M + ic) * NO + q];
-    pnl[t] = a;
+    for (int ic = 0; ic < M; ic++)
+      if (vnl[(i * M + ic) * (QMCW_MAX_LPOT + 1) + QMCW_MAX_LPOT] != 0.0)
+        a += pbc[(int64_t)(i * M + ic) * NB + b];
+    pbc[(int64_t)i * M * NB + b] = a;
   }
   double *dsl = ss + DL.dpe_slot;
   for (int sl = lane; sl < nsl; sl += G) {
@@ -354,6 +419,14 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
     dsl[sl] = a;
   }
   __syncthreads();
+  double *pnl = ss + DL.pnl;
+  for (int t = lane; t < N * NO; t += G) {
+    const int i = t / NO, q = t % NO;
+    const double *row = pbc + (int64_t)i * M * NB;
+    double a = 0.0;
+    for (int b = 0; b < NB; b++) a += S.coefT[(int64_t)b * NO + q] * row[b];
+    pnl[t] = a;
+  }
   double *dpe = ss + DL.dpe;
   for (int p = lane; p < S.npj; p += G) {
     double a = 0.0;

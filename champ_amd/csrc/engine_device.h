@@ -226,40 +226,6 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
   }
 }
 
-// value only of the pair term (pseudopotential quadrature)
-template <typename T, int ORD>
-__device__ T jas_pair_val(const DevSys &S, bool parallel_spin, T rij,
-                          const double *ri_v, const double *rj_v) {
-  const T k = T(S.scalek);
-  T sspinn = T(1);
-  int isb = 0;
-  double asymp_b = S.asymp_jasb0;
-  if (parallel_spin) {
-    if (S.nspin2b == 2) isb = 1;
-    else if (S.nocuspb == 0) sspinn = T(0.5);
-    asymp_b = S.asymp_jasb1;
-  }
-  const double *b = S.b + isb * (S.nordb + 1);
-  T u, d1, d2;
-  scaled<T>(S.isc, k, rij, u, d1, d2);
-  T f = sspinn * T(b[0]) * u / (T(1) + T(b[1]) * u) - T(asymp_b);
-  T up = u;
-  for (int iord = 2; iord <= S.nordb; iord++) {
-    up *= u;
-    f += T(b[iord]) * up;
-  }
-  if (S.nordc <= 1) return f;
-  T U = T(1) - k * u;   // e-e-n variable (same scaling as the e-e term)
-  for (int ic = 0; ic < S.M; ic++) {
-    int it = S.iwctype[ic];
-    T A, B, t1, t2;
-    scaled<T>(S.isc, k, (T)ri_v[ic], A, t1, t2);
-    scaled<T>(S.isc, k, (T)rj_v[ic], B, t1, t2);
-    f += een_value<T, ORD>(U, T(1) - k * A, T(1) - k * B, S.c + it * S.nterms);
-  }
-  return f;
-}
-
 template <typename T>
 __device__ T jas_en_val(const DevSys &S, const double *ri_v) {
   const T k = T(S.scalek);
@@ -276,6 +242,108 @@ __device__ T jas_en_val(const DevSys &S, const double *ri_v) {
       fe += T(a[iord]) * up;
     }
     f += fe;
+  }
+  return f;
+}
+
+// e-e-n polynomial of a fixed A as a bivariate polynomial in (U, B):
+// sum_t c_t U^k (A^(l+m) B^m + A^m B^(l+m)) = sum_k U^k sum_n gam(k, n) B^n,
+// n <= ORD - k, rows of gam stored one after the other.  With A fixed for
+// a point and a centre, each other electron then costs one Horner
+// evaluation instead of the term-by-term sum.
+template <int ORD>
+__host__ __device__ constexpr int een_gi(int k, int n) { return k * (ORD + 1) - k * (k - 1) / 2 + n; }
+
+template <typename T, int ORD>
+__device__ __forceinline__ void een_gamma(T A, const double *__restrict__ c, T *gam) {
+  T ap[ORD + 1];
+  ap[0] = T(1);
+#pragma unroll
+  for (int q = 1; q <= ORD; q++) ap[q] = ap[q - 1] * A;
+#pragma unroll
+  for (int q = 0; q < een_gi<ORD>(ORD, 0) + 1; q++) gam[q] = T(0);
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k <= ORD; k++)
+#pragma unroll
+    for (int m = 0; 2 * m <= ORD - k; m++)
+#pragma unroll
+      for (int l = 0; l <= ORD - k - 2 * m; l++)
+        if (QMCW_EEN_TERM(k, l, m)) {
+          const T ct = T(c[idx]);
+          idx++;
+          gam[een_gi<ORD>(k, m)] += ct * ap[l + m];
+          gam[een_gi<ORD>(k, l + m)] += ct * ap[m];
+        }
+}
+
+template <typename T, int ORD>
+__device__ __forceinline__ T een_horner(const T *gam, T U, T B) {
+  T f = T(0);
+#pragma unroll
+  for (int k = ORD; k >= 0; k--) {
+    T row = T(0);
+#pragma unroll
+    for (int n = ORD - k; n >= 0; n--) row = row * B + gam[een_gi<ORD>(k, n)];
+    f = f * U + row;
+  }
+  return f;
+}
+
+// e-e Pade + polynomial of a pair at scaled distance u
+template <typename T>
+__device__ __forceinline__ T jas_b_val(const DevSys &S, bool parallel_spin, T u) {
+  T sspinn = T(1);
+  int isb = 0;
+  double asymp_b = S.asymp_jasb0;
+  if (parallel_spin) {
+    if (S.nspin2b == 2) isb = 1;
+    else if (S.nocuspb == 0) sspinn = T(0.5);
+    asymp_b = S.asymp_jasb1;
+  }
+  const double *b = S.b + isb * (S.nordb + 1);
+  T f = sspinn * T(b[0]) * u / (T(1) + T(b[1]) * u) - T(asymp_b);
+  T up = u;
+  for (int iord = 2; iord <= S.nordb; iord++) {
+    up *= u;
+    f += T(b[iord]) * up;
+  }
+  return f;
+}
+
+// Jastrow value of electron i placed at xn (distances rnew to the centres),
+// with every other electron where it is: its e-n terms plus the e-e and
+// e-e-n terms of all pairs (i, j).  bt[j * M + ic] = 1 - k scaled(r_j,ic)
+// of the current configuration; uq is per-lane scratch of N values.
+template <typename T, int ORD>
+__device__ T jas_point_val(const DevSys &S, int i, const double *xn, const double *x,
+                           const double *rnew, const T *bt, T *uq) {
+  const int N = S.N, M = S.M;
+  const T k = T(S.scalek);
+  T f = jas_en_val<T>(S, rnew);
+  for (int jj = 0; jj < N; jj++) {
+    if (jj == i) continue;
+    const double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
+    const T rij = (T)sqrt(dx * dx + dy * dy + dz * dz);
+    const bool par = (i < S.nup) == (jj < S.nup);
+    T u, d1, d2;
+    scaled<T>(S.isc, k, rij, u, d1, d2);
+    f += jas_b_val<T>(S, par, u);
+    uq[jj] = T(1) - k * u;
+  }
+  if (S.nordc <= 1) return f;
+  constexpr int NG = een_gi<ORD>(ORD, 0) + 1;
+  for (int ic = 0; ic < M; ic++) {
+    T A, t1, t2;
+    scaled<T>(S.isc, k, (T)rnew[ic], A, t1, t2);
+    T gam[NG];
+    een_gamma<T, ORD>(T(1) - k * A, S.c + S.iwctype[ic] * S.nterms, gam);
+    T fc = T(0);
+    for (int jj = 0; jj < N; jj++) {
+      if (jj == i) continue;
+      fc += een_horner<T, ORD>(gam, uq[jj], bt[jj * M + ic]);
+    }
+    f += fc;
   }
   return f;
 }

@@ -472,6 +472,7 @@ __global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
   double vpsp = 0.0, ploc = 0.0;
   if (S.nloc > 0) {
     double *vnl = ss + SL.vnl;        // [N*M][QMCW_MAX_LPOT + 1] (last: in range)
+    T *bt = reinterpret_cast<T *>(ss + SL.bt);
     for (int p = lane; p < N * M; p += G) {
       int ic = p % M, ict = S.iwctype[ic];
       double r = fmax(1e-10, ren[p]);
@@ -484,10 +485,15 @@ __global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
         v[l - 1] = vl;
       }
       v[QMCW_MAX_LPOT] = inr ? 1.0 : 0.0;
+      // e-e-n variable of the other electrons at their current positions
+      T bu, t1, t2;
+      scaled<T>(S.isc, (T)S.scalek, (T)ren[p], bu, t1, t2);
+      bt[p] = T(1) - (T)S.scalek * bu;
     }
     ploc = gsum<G>(ploc);
     __syncthreads();
     const double *gp = ss + SL.gproj;
+    T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
     const double *quad = ws + L.quad;
     const int nq = S.nquad;
     for (int t = lane; t < N * M * nq; t += G) {
@@ -516,19 +522,14 @@ __global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
                                            (T)rvnew[3 * jc + 2], (T)rnew[jc], gpi);
       }
       // Jastrow value change
-      T delta = jas_en_val<T>(S, rnew) - (T)fso[i * N + i];
-      for (int jj = 0; jj < N; jj++) {
-        if (jj == i) continue;
-        int hi = jj < i ? i : jj, lo = jj < i ? jj : i;
-        double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
-        double rij = sqrt(dx * dx + dy * dy + dz * dz);
-        bool par = (hi < S.nup) || (lo >= S.nup);
-        delta += jas_pair_val<T, ORD>(S, par, (T)rij, rnew, ren + jj * M) - (T)fso[hi * N + lo];
-      }
+      double fold = fso[i * N + i];
+      for (int jj = 0; jj < N; jj++)
+        if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
+      const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
       double vpot = 0.0;
       for (int l = 1; l <= S.npotd[ict]; l++)
         if (l != S.lpotp1[ict]) vpot += v[l - 1] * legendre_w(l, costh);
-      vpsp += S.q0[3 * nq + iq] * exp((double)delta) * vpot * rd;
+      vpsp += S.q0[3 * nq + iq] * exp(delta) * vpot * rd;
     }
     vpsp = gsum<G>(vpsp);
   }

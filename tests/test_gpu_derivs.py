@@ -49,6 +49,25 @@ def test_derivs_match_oracle(name):
     assert _rel(g["eloc"], r["out9"][:, 3], 1e-9) < 1e-9
 
 
+def test_mixed_precision_derivs_close_to_oracle():
+    """Mixed precision (FP32 basis values and Jastrow at the quadrature
+    points, FP64 accumulation): d ln Psi/dp and dE_L/dp within 2e-3
+    relative (absolute floor 1e-2).  Measured on the C2-like system: 2.6e-4
+    and 4.7e-4, the FP32 rounding of the basis values carried through the
+    orbital-rotation blocks; a wrong term gives errors of order one."""
+    s = system.c2_dimer(2, derivs=True)
+    W = 13
+    x = system.random_walkers(s, W, seed=23)
+    e = Engine(s, W, "mixed")
+    e.set_walkers(x)
+    dp, de = e.derivs()
+    r = oracle.Oracle(s).eval_batch(x, derivs=True)
+    edp, ede = _rel(dp, r["dpsi"], 1e-2), _rel(de, r["deloc"], 1e-2)
+    print(f"mixed derivs: dpsi rel {edp:.3g}, deloc rel {ede:.3g}")
+    assert edp < 2e-3
+    assert ede < 2e-3
+
+
 def test_sampling_mode_gives_last_step_derivatives():
     s = system.c2_dimer(2, derivs=True)
     W = 11
