@@ -397,7 +397,8 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         tf = json.load(open(TRAFFIC_FILE))
         ent = tf.get("kernels", {}).get(dom)
-        if ent and ent.get("walkers") == W and ent.get("precision") == main_prec:
+        if ent and ent.get("walkers") == W and ent.get("precision") == main_prec \
+                and ent.get("group_widths") == list(r["G"]):
             traffic = ent["hbm_bytes_per_launch"]
     out = {
         "metric": "walker-moves/s + dlnPsi/dp samples/s vs FP64 GPU path at <=0.1 mHa bias",
