@@ -52,6 +52,9 @@ struct DevSys {
   const int *yent_u[2];      // unique det of the contribution
   const int *yent_pos[2];    // its alpha-inverse element (u*kmax2 + jj + k*ii)
   const double *cdet;        // [ndet] cdet_in_wf
+  const double *cdd;         // [ndet] cdet * sign(up part) * sign(down part)
+  const double *ucf[2];      // per udet_d entry: cdd of the det
+  const int *uiu[2], *uid[2];  // per udet_d entry: its up and down unique dets
   int kmax2;                 // kmax*kmax (>= 1)
   // Jastrow ijas=4
   int isc, nspin2b, nocuspb, norda, nordb, nordc, nterms;

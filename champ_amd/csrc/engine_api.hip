@@ -39,6 +39,8 @@ struct qmcw_handle {
   int trace_steps = 0;
   // host copies of the parameter arrays that set_params can replace
   std::vector<double> a4, b, c, csf_coef, coef, cdet, asymp_jasa;
+  double *d_cdd = nullptr, *d_ucf[2] = {nullptr, nullptr};
+  std::vector<int> iwdet_h[2], dsgn_h[2], udet_d[2];   // det bookkeeping for re-folding signs
   double *d_a4 = nullptr, *d_b = nullptr, *d_c = nullptr, *d_cdet = nullptr,
          *d_coefT = nullptr, *d_asymp = nullptr, *d_csf_coef = nullptr;
   // determinant bookkeeping (host) used to rebuild cdet
@@ -229,6 +231,20 @@ std::vector<double> build_cdet(int ndet, int ncsf, const double *csf_coef, const
 }
 
 }  // namespace
+
+// cdet with the permutation signs of both spin parts folded in (exact: the
+// signs are +-1), per determinant and per entry of the dets-of-a-unique-det
+// lists, so that the kernels skip the sign lookups
+static void fold_det_signs(const qmcw_handle *h, std::vector<double> &cdd,
+                           std::vector<double> (&ucf)[2]) {
+  cdd.resize(h->cdet.size());
+  for (size_t d = 0; d < cdd.size(); d++)
+    cdd[d] = h->cdet[d] * h->dsgn_h[0][h->iwdet_h[0][d]] * h->dsgn_h[1][h->iwdet_h[1][d]];
+  for (int sp = 0; sp < 2; sp++) {
+    ucf[sp].clear();
+    for (int d : h->udet_d[sp]) ucf[sp].push_back(cdd[d]);
+  }
+}
 
 // e-e-n coefficients from CHAMP's term order (jastrow4.f90:225-244) to the
 // kernels' compile-time order (k, then m, then l)
@@ -532,7 +548,8 @@ static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, dou
                          double fbias, double *out, long long *nacc = nullptr) {
   const int G = h->Gk[KC_MOVE];
   // the moved electron's basis values in LDS when they fit (<= 48 KB per block)
-  const size_t lds = (size_t)(kWave / G) * h->A.S.NB * 5 * sizeof(double);
+  const size_t lds = (size_t)(kWave / G) * h->A.S.NB * 5 *
+                     (h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float));
   KArgs A = h->A;
   A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;
   LaunchCtx c = make_ctx(h, A, G);
@@ -682,6 +699,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     UP(D.ivir[sp], S.ivir[sp]); UP(D.ord[sp], S.ord[sp]); UP(D.sgn[sp], S.dsgn[sp]);
     UP(D.rf[sp], S.rf[sp]); UP(D.exl[sp], S.exl[sp]); UP(D.exo[sp], S.exo[sp]);
     UP(D.iwdet[sp], S.iwdet[sp]); UP(D.iocc_lab[sp], S.iocc_lab[sp]);
+    h->iwdet_h[sp] = D.iwdet[sp]; h->dsgn_h[sp] = D.sgn[sp];
     // CSR lists of eval_y: dets of each unique det; contributions to each Y entry
     {
       const int nu = D.ndetu[sp], ns = D.ns[sp], nocc = D.nocc[sp];
@@ -707,6 +725,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
         }
         yoff[e + 1] = (int)yu.size();
       }
+      h->udet_d[sp] = ud;
       UP(uoff, S.udet_off[sp]); UP(ud, S.udet_d[sp]);
       UP(yoff, S.yent_off[sp]); UP(yu, S.yent_u[sp]); UP(yp, S.yent_pos[sp]);
     }
@@ -719,6 +738,18 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   h->cdet = build_cdet(s->ndet, s->ncsf, s->csf_coef, s->csf_det_off, s->csf_det, s->csf_det_coef);
   UP(h->cdet, S.cdet);
   h->d_cdet = const_cast<double *>(S.cdet);
+  {
+    std::vector<double> cdd, ucf[2];
+    fold_det_signs(h, cdd, ucf);
+    UP(cdd, S.cdd); h->d_cdd = const_cast<double *>(S.cdd);
+    for (int sp = 0; sp < 2; sp++) {
+      std::vector<int> iu, id;
+      for (int d : h->udet_d[sp]) { iu.push_back(D.iwdet[0][d]); id.push_back(D.iwdet[1][d]); }
+      if (iu.empty()) { iu.push_back(0); id.push_back(0); ucf[sp].push_back(0.0); }
+      UP(ucf[sp], S.ucf[sp]); h->d_ucf[sp] = const_cast<double *>(S.ucf[sp]);
+      UP(iu, S.uiu[sp]); UP(id, S.uid[sp]);
+    }
+  }
   S.ncsf = s->ncsf;
   UP(h->csf_det_off, S.csf_det_off); UP(h->csf_det, S.csf_det);
   UP(h->csf_det_coef, S.csf_det_coef); UP(h->csf_coef, S.csf_coef);
@@ -975,6 +1006,12 @@ extern "C" int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b
     h->cdet = build_cdet(h->ndet, h->ncsf, h->csf_coef.data(), h->csf_det_off.data(),
                          h->csf_det.data(), h->csf_det_coef.data());
     HIPCHK(h, hipMemcpy(h->d_cdet, h->cdet.data(), h->cdet.size() * 8, hipMemcpyHostToDevice));
+    std::vector<double> cdd, ucf[2];
+    fold_det_signs(h, cdd, ucf);
+    HIPCHK(h, hipMemcpy(h->d_cdd, cdd.data(), cdd.size() * 8, hipMemcpyHostToDevice));
+    for (int sp = 0; sp < 2; sp++)
+      if (!ucf[sp].empty())
+        HIPCHK(h, hipMemcpy(h->d_ucf[sp], ucf[sp].data(), ucf[sp].size() * 8, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_csf_coef, h->csf_coef.data(), h->csf_coef.size() * 8, hipMemcpyHostToDevice));
   }
   if (coef) {

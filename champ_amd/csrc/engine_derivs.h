@@ -530,7 +530,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
     for (int d = 0; d < S.ndet; d++) {
       if (S.iwdet[s][d] != u) continue;
       const int iu = S.iwdet[0][d], id = S.iwdet[1][d];
-      const double cf = S.cdet[d] * S.dsgn[0][iu] * S.dsgn[1][id] * dv0[iu] * dv1[id];
+      const double cf = S.cdd[d] * dv0[iu] * dv1[id];
       c0 += cf;
       c1 += cf * ((S.ns[0] ? del0[iu] : 0.0) + (S.ns[1] ? del1[id] : 0.0));
     }

@@ -590,6 +590,11 @@ __device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const GP *gp)
 // block held by one lane.  Singular -> det = 0, inverse zeroed.
 __device__ inline double small_matinv(double *a, int n) {
   if (n == 0) return 1.0;
+  if (n == 1) {   // the general path gives 1 * (1 / a) and det 1 * a (0 and 0 when singular)
+    const double a0 = a[0];
+    a[0] = a0 == 0.0 ? 0.0 : 1.0 / a0;
+    return a0 == 0.0 ? 0.0 : a0;
+  }
   int ipiv[kMaxKmax];
   double atemp[kMaxKmax];
   for (int i = 0; i < n; i++) ipiv[i] = i;

@@ -54,6 +54,7 @@ template <bool BF16, typename T>
 __device__ __forceinline__ void contract5(const double *__restrict__ coefT, int NO, int NB, int lab,
                                           const T *__restrict__ ph, double *o) {
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+#pragma unroll 4
   for (int b = 0; b < NB; b++) {
     const double cf = coefT[(int64_t)b * NO + lab];
     const T *p = ph + 5 * b;
@@ -145,41 +146,53 @@ __device__ double group_matinv(double *a, double *tmp, int n, int lane, int *ipi
   return singular ? 0.0 : det;
 }
 
+// Y entries of one spin: y[e] = chii sum_q cus[yent_u[q]] inv[yent_pos[q]]
+// over e = lane, lane + G, ...; the inputs never alias the output, so the
+// loads of an entry's list are issued together.
+template <int G>
+__device__ __forceinline__ void group_y_spin(const int *__restrict__ off, const int *__restrict__ yu,
+                                             const int *__restrict__ yp, const double *__restrict__ cus,
+                                             const double *__restrict__ inv, double *__restrict__ y,
+                                             int n, double chii, int lane) {
+  for (int e = lane; e < n; e += G) {
+    double acc = 0.0;
+    const int q1 = off[e + 1];
+#pragma unroll 4
+    for (int q = off[e]; q < q1; q++) acc += cus[yu[q]] * inv[yp[q]];
+    y[e] = acc * chii;
+  }
+}
+
 // chi and Y of one walker from the alpha inverses/dets of both spins
 // (qmc/deriv_fast_mod.f90:545-598).  inv/detv per spin, y out per spin.
+// The determinant coefficients carry the permutation signs (DevSys::cdd).
 template <int G>
 __device__ double group_eval_y(const DevSys &S, const double *const inv[2],
                                const double *const detv[2], double *const y[2],
                                double *cup, int lane) {
   // chi = sum_d cdet sgn detup detdn ; cup_s(u) = sum over dets with spin-s part u
+  const double *__restrict__ dv0 = detv[0];
+  const double *__restrict__ dv1 = detv[1];
   double chi = 0.0;
-  for (int d = lane; d < S.ndet; d += G) {
-    int iu = S.iwdet[0][d], id = S.iwdet[1][d];
-    chi += S.cdet[d] * S.dsgn[0][iu] * S.dsgn[1][id] * detv[0][iu] * detv[1][id];
-  }
+#pragma unroll 4
+  for (int d = lane; d < S.ndet; d += G) chi += S.cdd[d] * dv0[S.iwdet[0][d]] * dv1[S.iwdet[1][d]];
   chi = gsum<G>(chi);
   const double chii = 1.0 / chi;
   for (int t = lane; t < S.ndetu[0] + S.ndetu[1]; t += G) {
-    int s = t < S.ndetu[0] ? 0 : 1, u = s ? t - S.ndetu[0] : t;
+    const int s = t < S.ndetu[0] ? 0 : 1, u = s ? t - S.ndetu[0] : t;
+    const int *__restrict__ iu = S.uiu[s];
+    const int *__restrict__ id = S.uid[s];
+    const double *__restrict__ cf = S.ucf[s];
     double cu = 0.0;
-    for (int q = S.udet_off[s][u]; q < S.udet_off[s][u + 1]; q++) {
-      const int d = S.udet_d[s][q];
-      const int iu = S.iwdet[0][d], id = S.iwdet[1][d];
-      cu += S.cdet[d] * S.dsgn[0][iu] * S.dsgn[1][id] * detv[0][iu] * detv[1][id];
-    }
+    const int q1 = S.udet_off[s][u + 1];
+#pragma unroll 4
+    for (int q = S.udet_off[s][u]; q < q1; q++) cu += cf[q] * dv0[iu[q]] * dv1[id[q]];
     cup[t] = cu;
   }
   __syncthreads();
-  for (int s = 0; s < 2; s++) {
-    int ns = S.ns[s], nocc = S.nocc[s];
-    const double *cus = cup + (s ? S.ndetu[0] : 0);
-    for (int e = lane; e < nocc * ns; e += G) {
-      double acc = 0.0;
-      for (int q = S.yent_off[s][e]; q < S.yent_off[s][e + 1]; q++)
-        acc += cus[S.yent_u[s][q]] * inv[s][S.yent_pos[s][q]];
-      y[s][e] = acc * chii;
-    }
-  }
+  for (int s = 0; s < 2; s++)
+    group_y_spin<G>(S.yent_off[s], S.yent_u[s], S.yent_pos[s], cup + (s ? S.ndetu[0] : 0), inv[s],
+                    y[s], S.nocc[s] * S.ns[s], chii, lane);
   __syncthreads();
   return chi;
 }
@@ -598,21 +611,28 @@ __device__ void group_grad_1e(const KArgs &A, const double *ws, int ie,
   // grad(c) = sum_k ai(k,e) d_c(ref_k) + sum_i sum_j y(j,i) ai(i,e) v_c(j)
   //   v_c(j) = d_c(occ_j) - sum_k d_c(ref_k) T(k, occ_j)
   double acc[4] = {0, 0, 0, 0};
+  const int *iref = S.iref[s];
   for (int k = lane; k < ns; k += G) {
     double a = ai[k + ns * e];
-    const double *o = orbv + S.iref[s][k] * 5;
+    const double *o = orbv + iref[k] * 5;
     for (int c = 0; c < 4; c++) acc[c] += a * o[1 + c];
   }
-  // w_i = ai(i, e) ; z_j = sum_i y(j,i) w_i
+  // w_i = ai(i, e) ; z_j = sum_i y(j,i) w_i ; the four components of v(j)
+  // in one pass over the reference orbitals (each keeps its own order)
+  const double *aie = ai + ns * e;
   for (int j = lane; j < nocc; j += G) {
     double z = 0.0;
-    for (int i = 0; i < ns; i++) z += y[j * ns + i] * ai[i + ns * e];
+#pragma unroll 4
+    for (int i = 0; i < ns; i++) z += y[j * ns + i] * aie[i];
     if (z == 0.0) continue;
-    for (int c = 0; c < 4; c++) {
-      double v = orbv[j * 5 + 1 + c];
-      for (int k = 0; k < ns; k++) v -= orbv[S.iref[s][k] * 5 + 1 + c] * tocc[k * nocc + j];
-      acc[c] += z * v;
+    double v0 = orbv[j * 5 + 1], v1 = orbv[j * 5 + 2], v2 = orbv[j * 5 + 3], v3 = orbv[j * 5 + 4];
+#pragma unroll 4
+    for (int k = 0; k < ns; k++) {
+      const double *o = orbv + iref[k] * 5;
+      const double t = tocc[k * nocc + j];
+      v0 -= o[1] * t; v1 -= o[2] * t; v2 -= o[3] * t; v3 -= o[4] * t;
     }
+    acc[0] += z * v0; acc[1] += z * v1; acc[2] += z * v2; acc[3] += z * v3;
   }
   for (int c = 0; c < 4; c++) acc[c] = gsum<G>(acc[c]);
   for (int c = 0; c < 3; c++) grad[c] = acc[c] + fj[c];
@@ -625,6 +645,62 @@ __device__ __forceinline__ int metrop_accept(const KArgs &A, const double *ws, c
                                              double fbias);
 template <int G>
 __device__ void group_commit(const KArgs &A, double *ws, const double *mv, int ie, int lane);
+
+// Sherman-Morrison vectors of dete_update (qmc/dete_mod.f90:114-214):
+// vi1(i) = sum_k orb(ref_k) A^-1(k, i) - delta(i, e) and
+// vi2(j) = sum_k orb(ref_k) T(k, j) - orb(occ_j).  Outputs never alias inputs.
+template <int G>
+__device__ __forceinline__ void sm_vectors(double *__restrict__ vi1, double *__restrict__ vi2,
+                                           const double *__restrict__ orbe, const double *__restrict__ ai,
+                                           const double *__restrict__ tocc, const int *__restrict__ iref,
+                                           int ns, int nocc, int e, int lane) {
+  for (int i = lane; i < ns; i += G) {
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < ns; k++) acc += orbe[5 * iref[k]] * ai[k + ns * i];
+    vi1[i] = acc - (i == e ? 1.0 : 0.0);
+  }
+  for (int j = lane; j < nocc; j += G) {
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < ns; k++) acc += orbe[5 * iref[k]] * tocc[k * nocc + j];
+    vi2[j] = acc - orbe[5 * j];
+  }
+}
+
+// rank-1 updates A^-1_new = A^-1 + u vi1^T, T_new = T + u vi2^T with
+// u(k) = -A^-1(k, e) / ratio
+template <int G>
+__device__ __forceinline__ void sm_rank1(double *__restrict__ ain, double *__restrict__ tn,
+                                         const double *__restrict__ ai, const double *__restrict__ tocc,
+                                         const double *__restrict__ vi1, const double *__restrict__ vi2,
+                                         int ns, int nocc, int e, double rat, int lane) {
+#pragma unroll 4
+  for (int q = lane; q < ns * ns; q += G) {
+    const int k = q % ns, i = q / ns;
+    ain[q] = ai[q] + (-ai[k + ns * e] / rat) * vi1[i];
+  }
+#pragma unroll 4
+  for (int q = lane; q < ns * nocc; q += G) {
+    const int k = q / nocc, j = q % nocc;
+    tn[q] = tocc[q] + (-ai[k + ns * e] / rat) * vi2[j];
+  }
+}
+
+// basis values (value, gradient, laplacian) of one position into phi, then
+// the occupied orbitals of spin s into orbe[5 * nocc]
+template <typename T, int G>
+__device__ __forceinline__ void orbitals_1e(const KArgs &A, int s, int nocc, const double *ren,
+                                            const double *rven, T *phi, double *orbe, int lane) {
+  const DevSys &S = A.S;
+  for (int sh = lane; sh < S.nshell; sh += G) {
+    const int ic = S.sh_cent[sh];
+    eval_shell<T>(S, sh, (T)rven[3 * ic], (T)rven[3 * ic + 1], (T)rven[3 * ic + 2], (T)ren[ic], 1, phi);
+  }
+  __syncthreads();
+  for (int j = lane; j < nocc; j += G)
+    contract5<T>(A.bf16, S.coefT, S.NO, S.NB, S.occ[s][j], phi, orbe + 5 * j);
+}
 
 // ---------------------------------------------------------------------------
 // One-electron move of electron ie: propose (mode 1, metrop_mov1 with the
@@ -735,6 +811,11 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     // moved electron's arrays go to whichever slot (hi or lo) it occupies
     if (hi == ie) jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rI, rJ, rvI, rvJ, po);
     else jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rJ, rI, rvJ, rvI, po);
+    // old pair values before the stores to the pending buffers below
+    const double f_old = fso[hi * N + lo];
+    const double gi_old[3] = {fijo[3 * (ie * N + j)], fijo[3 * (ie * N + j) + 1], fijo[3 * (ie * N + j) + 2]};
+    const double gj_old[3] = {fijo[3 * (j * N + ie)], fijo[3 * (j * N + ie) + 1], fijo[3 * (j * N + ie) + 2]};
+    const double fj_old[3] = {fjo[3 * j], fjo[3 * j + 1], fjo[3 * j + 2]};
     const T *g_ie = (hi == ie) ? po.gi : po.gj;
     const T *g_j = (hi == ie) ? po.gj : po.gi;
     T l_ie = (hi == ie) ? po.li : po.lj;
@@ -746,14 +827,14 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     }
     lapn[j] = l_ie;
     mv[ML.lapn_i + N + j] = l_j;
-    dsum += (double)po.f - fso[hi * N + lo];
-    gix += (double)g_ie[0] - fijo[3 * (ie * N + j)];
-    giy += (double)g_ie[1] - fijo[3 * (ie * N + j) + 1];
-    giz += (double)g_ie[2] - fijo[3 * (ie * N + j) + 2];
+    dsum += (double)po.f - f_old;
+    gix += (double)g_ie[0] - gi_old[0];
+    giy += (double)g_ie[1] - gi_old[1];
+    giz += (double)g_ie[2] - gi_old[2];
     lapi += l_ie;
     // other electron's new gradient
     for (int q = 0; q < 3; q++)
-      fjn[3 * j + q] = fjo[3 * j + q] + (double)g_j[q] - fijo[3 * (j * N + ie) + q];
+      fjn[3 * j + q] = fj_old[q] + (double)g_j[q] - gj_old[q];
   }
   dsum = gsum<G>(dsum);
   gix = gsum<G>(gix); giy = gsum<G>(giy); giz = gsum<G>(giz);
@@ -763,18 +844,15 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   if (lane == 0) { fjn[3 * ie] = fji[0]; fjn[3 * ie + 1] = fji[1]; fjn[3 * ie + 2] = fji[2]; }
   double lapjn = lapi + fji[0] * fji[0] + fji[1] * fji[1] + fji[2] * fji[2];
   // ---- orbitals of the new position (occupied set of the spin)
-  // basis values of the new position: LDS when the launch reserved it
+  // basis values of the new position: LDS when the launch reserved it (the
+  // two calls keep the LDS pointer visible as such to the compiler)
   extern __shared__ double dyn_lds[];
-  T *phi = reinterpret_cast<T *>(A.lds_phi ? dyn_lds + (int64_t)grp * S.NB * 5 : ss + SL.phi);
-  for (int sh = lane; sh < S.nshell; sh += G) {
-    int ic = S.sh_cent[sh];
-    eval_shell<T>(S, sh, (T)rven[3 * ic], (T)rven[3 * ic + 1], (T)rven[3 * ic + 2], (T)ren[ic], 1, phi);
-  }
-  __syncthreads();
   double *orbe = mv + ML.orbe;
-  for (int j = lane; j < nocc; j += G) {
-    contract5<T>(A.bf16, S.coefT, S.NO, S.NB, S.occ[s][j], phi, orbe + 5 * j);
-  }
+  if (A.lds_phi)
+    orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * S.NB * 5,
+                      orbe, lane);
+  else
+    orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(ss + SL.phi), orbe, lane);
   __syncthreads();
   // ---- dete_update (qmc/dete_mod.f90:114-214)
   const double *ai = ws + L.ai[s];
@@ -786,25 +864,9 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   // vi1(i) = sum_k orbe(ref_k) ai(k, i) - delta(i, e); ui(k) = -ai(k,e)/rat
   double *vi1 = mv + ML.misc + 8;            // [ns]
   double *vi2 = vi1 + ns;                    // [nocc]
-  for (int i = lane; i < ns; i += G) {
-    double acc = 0.0;
-    for (int k = 0; k < ns; k++) acc += orbe[5 * S.iref[s][k]] * ai[k + ns * i];
-    vi1[i] = acc - (i == e ? 1.0 : 0.0);
-  }
-  for (int j = lane; j < nocc; j += G) {
-    double acc = 0.0;
-    for (int k = 0; k < ns; k++) acc += orbe[5 * S.iref[s][k]] * tocc[k * nocc + j];
-    vi2[j] = acc - orbe[5 * j];
-  }
+  sm_vectors<G>(vi1, vi2, orbe, ai, tocc, S.iref[s], ns, nocc, e, lane);
   __syncthreads();
-  for (int q = lane; q < ns * ns; q += G) {
-    int k = q % ns, i = q / ns;
-    ain[q] = ai[q] + (-ai[k + ns * e] / rat) * vi1[i];
-  }
-  for (int q = lane; q < ns * nocc; q += G) {
-    int k = q / nocc, j = q % nocc;
-    tn[q] = tocc[q] + (-ai[k + ns * e] / rat) * vi2[j];
-  }
+  sm_rank1<G>(ain, tn, ai, tocc, vi1, vi2, ns, nocc, e, rat, lane);
   __syncthreads();
   double *invn = mv + ML.invn, *detn = mv + ML.detn;
   for (int u = lane; u < S.ndetu[s]; u += G) {
@@ -888,7 +950,82 @@ __device__ __forceinline__ int metrop_accept(const KArgs &A, const double *ws, c
 
 // Commit the pending move of electron ie into the walker's state: jassav +
 // detsav + dete_save (qmc/vmc/metrop_slat_mov1.f90:544-546).  All lanes of
-// the group; no barriers.
+// the group; no barriers.  The contiguous blocks (x, distance and orbital
+// rows, gradients, A^-1, T, alpha inverses and determinants, Y) are copied
+// as one flat list of segments, up to kCommitBatch loads in flight per lane
+// before the stores; the pair-array rows follow, one electron per lane.
+constexpr int kCommitSeg = 11;
+constexpr int kCommitBatch = 16;
+
+template <int G>
+__device__ __forceinline__ void commit_pairs(const DevSys &S, const WLayout &L, const MLayout &ML,
+                                             double *__restrict__ ws, const double *__restrict__ mv,
+                                             int ie, int lane) {
+  const int N = S.N;
+  for (int j = lane; j < N; j += G) {
+    if (j == ie) {
+      const double f = mv[ML.fsn + ie], lp = mv[ML.lapn_i + ie];
+      const double g0 = mv[ML.fijn_i + 3 * ie], g1 = mv[ML.fijn_i + 3 * ie + 1], g2 = mv[ML.fijn_i + 3 * ie + 2];
+      ws[L.fso + ie * N + ie] = f;
+      ws[L.fijo + 3 * (ie * N + ie)] = g0;
+      ws[L.fijo + 3 * (ie * N + ie) + 1] = g1;
+      ws[L.fijo + 3 * (ie * N + ie) + 2] = g2;
+      ws[L.lapj + ie * N + ie] = lp;
+      continue;
+    }
+    const int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
+    double v[13];
+    v[0] = mv[ML.ree + j];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      v[1 + q] = mv[ML.rvee + 3 * j + q];
+      v[4 + q] = mv[ML.fijn_i + 3 * j + q];
+      v[7 + q] = mv[ML.fijn_j + 3 * j + q];
+    }
+    v[10] = mv[ML.fsn + j];
+    v[11] = mv[ML.lapn_i + j];
+    v[12] = mv[ML.lapn_i + N + j];
+    ws[L.r_ee + hi * N + lo] = v[0];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      ws[L.rvec_ee + 3 * (hi * N + lo) + q] = v[1 + q];
+      ws[L.fijo + 3 * (ie * N + j) + q] = v[4 + q];
+      ws[L.fijo + 3 * (j * N + ie) + q] = v[7 + q];
+    }
+    ws[L.fso + hi * N + lo] = v[10];
+    ws[L.lapj + ie * N + j] = v[11];
+    ws[L.lapj + j * N + ie] = v[12];
+  }
+}
+
+template <int G>
+__device__ __forceinline__ void commit_blocks(const int (&st)[kCommitSeg + 1], const int (&dso)[kCommitSeg],
+                                              const int (&sso)[kCommitSeg], double *__restrict__ ws,
+                                              const double *__restrict__ mv, int lane) {
+  const int n = st[kCommitSeg];
+  int srel[kCommitSeg], drel[kCommitSeg];
+#pragma unroll
+  for (int k = 0; k < kCommitSeg; k++) { srel[k] = sso[k] - st[k]; drel[k] = dso[k] - st[k]; }
+  for (int base = 0; base < n; base += G * kCommitBatch) {
+    double v[kCommitBatch];
+    int dst[kCommitBatch];
+#pragma unroll
+    for (int u = 0; u < kCommitBatch; u++) {
+      const int e = base + lane + u * G;
+      // segments are consecutive in e: the last one starting at or before e holds it
+      int so = srel[0], d = drel[0];
+#pragma unroll
+      for (int k = 1; k < kCommitSeg; k++)
+        if (e >= st[k]) { so = srel[k]; d = drel[k]; }
+      dst[u] = e < n ? d + e : -1;
+      v[u] = e < n ? mv[so + e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kCommitBatch; u++)
+      if (dst[u] >= 0) ws[dst[u]] = v[u];
+  }
+}
+
 template <int G>
 __device__ void group_commit(const KArgs &A, double *ws, const double *mv, int ie, int lane) {
   const DevSys &S = A.S;
@@ -896,54 +1033,31 @@ __device__ void group_commit(const KArgs &A, double *ws, const double *mv, int i
   const WLayout &L = A.L;
   const MLayout &ML = A.ML;
   const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
-  double *x = ws + L.x;
-  const double *xn = mv + ML.xn;
-  if (lane < 3) x[3 * ie + lane] = xn[lane];
-  for (int ic = lane; ic < M; ic += G) {
-    ws[L.r_en + ie * M + ic] = mv[ML.ren + ic];
-    for (int q = 0; q < 3; q++) ws[L.rvec_en + (ie * M + ic) * 3 + q] = mv[ML.rven + 3 * ic + q];
-  }
-  double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
-  for (int j = lane; j < N; j += G) {
-    if (j == ie) {
-      fso[ie * N + ie] = mv[ML.fsn + ie];
-      for (int q = 0; q < 3; q++) fijo[3 * (ie * N + ie) + q] = mv[ML.fijn_i + 3 * ie + q];
-      lapj[ie * N + ie] = mv[ML.lapn_i + ie];
-      continue;
-    }
-    int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
-    ws[L.r_ee + hi * N + lo] = mv[ML.ree + j];
-    for (int q = 0; q < 3; q++) ws[L.rvec_ee + 3 * (hi * N + lo) + q] = mv[ML.rvee + 3 * j + q];
-    fso[hi * N + lo] = mv[ML.fsn + j];
-    for (int q = 0; q < 3; q++) {
-      fijo[3 * (ie * N + j) + q] = mv[ML.fijn_i + 3 * j + q];
-      fijo[3 * (j * N + ie) + q] = mv[ML.fijn_j + 3 * j + q];
-    }
-    lapj[ie * N + j] = mv[ML.lapn_i + j];
-    lapj[j * N + ie] = mv[ML.lapn_i + N + j];
-  }
-  for (int q = lane; q < 3 * N; q += G) fjo[q] = mv[ML.fjn + q];
-  double *oc = ws + L.oc + (int64_t)ie * noccmax * 5;
-  for (int q = lane; q < nocc * 5; q += G) oc[q] = mv[ML.orbe + q];
-  double *ai = ws + L.ai[s], *tocc = ws + L.tocc[s];
-  for (int q = lane; q < ns * ns; q += G) ai[q] = mv[ML.ain + q];
-  for (int q = lane; q < ns * nocc; q += G) tocc[q] = mv[ML.tn + q];
-  double *inv = ws + L.inv[s], *dv = ws + L.detv[s];
-  for (int q = lane; q < S.ndetu[s] * S.kmax2; q += G) inv[q] = mv[ML.invn + q];
-  for (int q = lane; q < S.ndetu[s]; q += G) dv[q] = mv[ML.detn + q];
-  for (int t = 0; t < 2; t++) {
-    double *y = ws + L.y[t];
-    const double *yn = mv + ML.yn + (t ? S.nocc[0] * S.ns[0] : 0);
-    for (int q = lane; q < S.nocc[t] * S.ns[t]; q += G) y[q] = yn[q];
-  }
+  const int len[kCommitSeg] = {3, M, 3 * M, 3 * N, nocc * 5, ns * ns, ns * nocc,
+                               S.ndetu[s] * S.kmax2, S.ndetu[s], S.nocc[0] * S.ns[0],
+                               S.nocc[1] * S.ns[1]};
+  const int dso[kCommitSeg] = {(int)L.x + 3 * ie, (int)L.r_en + ie * M, (int)L.rvec_en + ie * M * 3,
+                               (int)L.fjo, (int)L.oc + ie * noccmax * 5, (int)L.ai[s],
+                               (int)L.tocc[s], (int)L.inv[s], (int)L.detv[s], (int)L.y[0],
+                               (int)L.y[1]};
+  const int sso[kCommitSeg] = {(int)ML.xn, (int)ML.ren, (int)ML.rven, (int)ML.fjn, (int)ML.orbe,
+                               (int)ML.ain, (int)ML.tn, (int)ML.invn, (int)ML.detn, (int)ML.yn,
+                               (int)ML.yn + S.nocc[0] * S.ns[0]};
+  int st[kCommitSeg + 1];
+  st[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kCommitSeg; k++) st[k + 1] = st[k] + len[k];
+  commit_blocks<G>(st, dso, sso, ws, mv, lane);
+  commit_pairs<G>(S, L, ML, ws, mv, ie, lane);
   if (lane == 0) {
-    ws[L.chi] = mv[ML.misc + 5];
-    ws[L.deta[s]] = mv[ML.misc + 6];
-    ws[L.fsumo] = mv[ML.misc + 7];
-    ws[L.psij] = mv[ML.misc + 7];
+    const double chi = mv[ML.misc + 5], deta = mv[ML.misc + 6], fs = mv[ML.misc + 7];
     const double logn = mv[ML.misc + 8 + ns + nocc];
+    ws[L.chi] = chi;
+    ws[L.deta[s]] = deta;
+    ws[L.fsumo] = fs;
+    ws[L.psij] = fs;
     ws[L.logpsid] = logn;
-    ws[L.psi2o] = 2.0 * (logn + mv[ML.misc + 7]);
+    ws[L.psi2o] = 2.0 * (logn + fs);
   }
 }
 
