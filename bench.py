@@ -270,9 +270,15 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01):
     e.sync()
     dt = time.perf_counter() - t0
     moves = sum(x["nwalk"] for x in h)
+    # kernel times of one more generation (outside the timed region)
+    e.profile(True)
+    drv.run(1)
+    e.sync()
+    prof = e.profile_get()
+    e.profile(False)
     e.close()
     return dict(rate=moves / dt, G=g, nwalk_final=h[-1]["nwalk"], eave=h[-1]["eave"],
-                acc=float(np.mean([x["acc"] for x in h])), seconds=dt)
+                acc=float(np.mean([x["acc"] for x in h])), seconds=dt, prof=prof)
 
 
 def copy_calibration(nbytes=256 << 20):
@@ -320,6 +326,10 @@ def main():
     if args.dmc:
         prec = "mixed" if args.precision == "both" else args.precision
         d = run_dmc_chain(4096, args.steps, args.warmup, prec)
+        for k, (ms, n) in d["prof"].items():
+            if n:
+                print(f"[bench]   {k:12s} {n:5d} launches  avg {ms / n * 1e3:9.1f} us  "
+                      f"total {ms:9.3f} ms", file=sys.stderr, flush=True)
         print(json.dumps({"metric": "DMC walker-moves/s (98-electron chain)", "value": d["rate"],
                           "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
                           "warmup": args.warmup, "higher_is_better": True,
@@ -327,7 +337,9 @@ def main():
                           "config": {"workload": "DMC, 98-electron chain, 4096 walkers, tau 0.01",
                                      "precision": prec, "group_width": d["G"]},
                           "nwalk_final": d["nwalk_final"], "acceptance": d["acc"],
-                          "seconds": d["seconds"]}), flush=True)
+                          "seconds": d["seconds"],
+                          "kernel_avg_us": {k: (ms / n * 1e3 if n else None)
+                                            for k, (ms, n) in d["prof"].items()}}), flush=True)
         return
     if args.batch:
         prec = "mixed" if args.precision == "both" else args.precision

@@ -879,6 +879,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SL.gproj = take((int64_t)N * NB);
   SL.bt = take((int64_t)N * M);
   SL.uq = take((int64_t)kWave * N);
+  SL.plist = take(((int64_t)N * M + 1) / 2);
   SL.mv = take(ML.stride);
   if (S.nparm > 0) {
     DLayout &DL = h->A.DL;

@@ -348,14 +348,18 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
     double bu, t1, t2;
     scaled<double>(S.isc, S.scalek, ren[p], bu, t1, t2);
     btd[p] = 1.0 - S.scalek * bu;
-  }
-  __syncthreads();
-  for (int p = lane; p < N * M; p += G) {
-    double *pbrow = pbc + (int64_t)p * NB;
     double *drow = draw + (int64_t)p * nsl;
     for (int q = 0; q < nsl; q++) drow[q] = 0.0;
+  }
+  // whole pairs in range per lane
+  int *plist = reinterpret_cast<int *>(ss + SL.plist);
+  const int npair = group_compact<G>(vnl + QMCW_MAX_LPOT, QMCW_MAX_LPOT + 1, N * M, plist, lane);
+  __syncthreads();
+  for (int kp = lane; kp < npair; kp += G) {
+    const int p = plist[kp];
+    double *pbrow = pbc + (int64_t)p * NB;
+    double *drow = draw + (int64_t)p * nsl;
     const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
-    if (v[QMCW_MAX_LPOT] == 0.0) continue;
     for (int b = 0; b < NB; b++) pbrow[b] = 0.0;
     const int i = p / M, ic = p % M, ict = S.iwctype[ic];
     const double *gpi = gp + (int64_t)i * NB;

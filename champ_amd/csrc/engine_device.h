@@ -23,6 +23,32 @@ __device__ __forceinline__ float gsumf(float v) {
   for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
   return v;
 }
+// inclusive prefix sum over the G lanes of a group (lane = position in it)
+template <int G>
+__device__ __forceinline__ int gscan_incl(int v, int lane) {
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int t = __shfl_up(v, d, G);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Ordered list of the entries of flags[0..n) (stride fs) that are nonzero,
+// built by the G lanes of a group: each lane takes a contiguous chunk, a
+// prefix sum places the chunks.  Returns the count (all lanes).
+template <int G>
+__device__ __forceinline__ int group_compact(const double *flags, int fs, int n, int *list, int lane) {
+  const int chunk = (n + G - 1) / G, b0 = lane * chunk, b1 = min(n, b0 + chunk);
+  int c = 0;
+  for (int q = b0; q < b1; q++) c += flags[(int64_t)q * fs] != 0.0;
+  const int incl = gscan_incl<G>(c, lane);
+  int o = incl - c;
+  for (int q = b0; q < b1; q++)
+    if (flags[(int64_t)q * fs] != 0.0) list[o++] = q;
+  return __shfl(incl, G - 1, G);
+}
+
 template <int G>
 __device__ __forceinline__ int gsumi(int v) {
 #pragma unroll

@@ -465,8 +465,10 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
   }
 }
 
+// FP32 instances are held to 128 VGPRs (four waves per SIMD)
 template <typename T, int G, int ORD>
-__global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : QMCW_WPE, 8)))
+void k_full_nlpp(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, NO = S.NO;
   const WLayout &L = A.L;
@@ -505,14 +507,17 @@ __global__ QMCW_HEAVY void k_full_nlpp(KArgs A) {
     }
     ploc = gsum<G>(ploc);
     __syncthreads();
+    // quadrature points only of the pairs in range, spread evenly over the lanes
+    int *plist = reinterpret_cast<int *>(ss + SL.plist);
+    const int npair = group_compact<G>(vnl + QMCW_MAX_LPOT, QMCW_MAX_LPOT + 1, N * M, plist, lane);
+    __syncthreads();
     const double *gp = ss + SL.gproj;
     T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
     const double *quad = ws + L.quad;
     const int nq = S.nquad;
-    for (int t = lane; t < N * M * nq; t += G) {
-      int iq = t % nq, p = t / nq, ic = p % M, i = p / M;
+    for (int t = lane; t < npair * nq; t += G) {
+      const int iq = t % nq, p = plist[t / nq], ic = p % M, i = p / M;
       const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
-      if (v[QMCW_MAX_LPOT] == 0.0) continue;
       int ict = S.iwctype[ic];
       double rei = ren[p];
       double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
