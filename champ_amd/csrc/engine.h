@@ -34,6 +34,7 @@ struct DevSys {
   const double *prim_zeta, *prim_coef;
   int maxshell_per_cent;
   const double *coefT;       // [NB][NO] (normalisation folded in)
+  const double *coef_occ[2]; // [NB][nocc] coefT of the occupied orbitals of each spin
   // determinants (spin s = 0 up, 1 down)
   int ns[2], nocc[2], ndetu[2], ndet, kmax, noccmax;
   const int *ref[2];         // [ns] reference orbital labels

@@ -41,6 +41,8 @@ struct qmcw_handle {
   std::vector<double> a4, b, c, csf_coef, coef, cdet, asymp_jasa;
   double *d_cdd = nullptr, *d_ucf[2] = {nullptr, nullptr};
   std::vector<int> iwdet_h[2], dsgn_h[2], udet_d[2];   // det bookkeeping for re-folding signs
+  std::vector<int> occ_h[2];                           // occupied orbital labels per spin
+  double *d_coef_occ[2] = {nullptr, nullptr};
   double *d_a4 = nullptr, *d_b = nullptr, *d_c = nullptr, *d_cdet = nullptr,
          *d_coefT = nullptr, *d_asymp = nullptr, *d_csf_coef = nullptr;
   // determinant bookkeeping (host) used to rebuild cdet
@@ -60,6 +62,7 @@ struct qmcw_handle {
   int use_graph = 1;
   int use_lds = 1;           // LDS staging of the moved electron's basis values
   int chunk_override = 0;    // electrons per basis chunk (0: from the L2 budget)
+  size_t det_lds_budget = 10240;  // LDS bytes per block for k_full_det's basis values
   // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
   int dmc_on = 0, nact = 0;
   double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
@@ -244,6 +247,15 @@ static void fold_det_signs(const qmcw_handle *h, std::vector<double> &cdd,
     ucf[sp].clear();
     for (int d : h->udet_d[sp]) ucf[sp].push_back(cdd[d]);
   }
+}
+
+// coefT restricted to the occupied orbitals of each spin, [NB][nocc]
+static std::vector<double> coef_occ_table(const std::vector<double> &coefT, int NB, int NO,
+                                          const std::vector<int> &occ) {
+  std::vector<double> t((size_t)NB * std::max<size_t>(1, occ.size()), 0.0);
+  for (int b = 0; b < NB; b++)
+    for (size_t j = 0; j < occ.size(); j++) t[(size_t)b * occ.size() + j] = coefT[(size_t)b * NO + occ[j]];
+  return t;
 }
 
 // e-e-n coefficients from CHAMP's term order (jastrow4.f90:225-244) to the
@@ -472,12 +484,12 @@ static LaunchCtx make_ctx(const qmcw_handle *h, const KArgs &A, int G) {
 }
 
 template <int G>
-static void launch_det(qmcw_handle *h, const KArgs &A) {
+static void launch_det(qmcw_handle *h, const KArgs &A, size_t smem) {
   const dim3 grid(h->Wl / (kWave / G)), block(kWave);
   if (h->precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_full_det<double, G>), grid, block, 0, h->stream, A);
+    hipLaunchKernelGGL((k_full_det<double, G>), grid, block, smem, h->stream, A);
   else
-    hipLaunchKernelGGL((k_full_det<float, G>), grid, block, 0, h->stream, A);
+    hipLaunchKernelGGL((k_full_det<float, G>), grid, block, smem, h->stream, A);
 }
 template <int G>
 static void launch_fin(qmcw_handle *h, const KArgs &A) {
@@ -511,6 +523,16 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
                        (h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float));
   A.chunk = std::max(1, std::min(h->chunk, (int)((16u << 10) / std::max<size_t>(per_e, 1))));
   if (h->chunk_override > 0) A.chunk = std::min(h->chunk, h->chunk_override);
+  // in LDS instead, one electron at a time, when that fits the per-block
+  // budget (measured on the C2-like system: one electron in LDS beats both
+  // two, which costs occupancy, and L2-sized chunks in global scratch)
+  size_t det_smem = 0;
+  A.lds_det = 0;
+  if (h->use_lds && per_e <= h->det_lds_budget) {
+    A.chunk = 1;
+    A.lds_det = 1;
+    det_smem = per_e;
+  }
   const OrdKernels &K = ord_kernels(A.S.nordc);
   const LaunchCtx cj = make_ctx(h, A, h->Gk[KC_JAS]), cn = make_ctx(h, A, h->Gk[KC_NLPP]);
   prof_begin(h, QMCW_PROF_FULL_JAS);
@@ -522,7 +544,7 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     prof_end(h);
   }
   prof_begin(h, QMCW_PROF_FULL_DET);
-  DISPATCH_GW(gd, launch_det, h, A);
+  DISPATCH_GW(gd, launch_det, h, A, det_smem);
   prof_end(h);
   if (A.S.nloc > 0) {
     prof_begin(h, QMCW_PROF_FULL_NLPP);
@@ -700,6 +722,12 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     UP(D.rf[sp], S.rf[sp]); UP(D.exl[sp], S.exl[sp]); UP(D.exo[sp], S.exo[sp]);
     UP(D.iwdet[sp], S.iwdet[sp]); UP(D.iocc_lab[sp], S.iocc_lab[sp]);
     h->iwdet_h[sp] = D.iwdet[sp]; h->dsgn_h[sp] = D.sgn[sp];
+    h->occ_h[sp] = D.occ[sp];
+    {
+      std::vector<double> co = coef_occ_table(coefT, S.NB, S.NO, D.occ[sp]);
+      UP(co, S.coef_occ[sp]);
+      h->d_coef_occ[sp] = const_cast<double *>(S.coef_occ[sp]);
+    }
     // CSR lists of eval_y: dets of each unique det; contributions to each Y entry
     {
       const int nu = D.ndetu[sp], ns = D.ns[sp], nocc = D.nocc[sp];
@@ -935,6 +963,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_NO_GRAPH")) h->use_graph = atoi(e) == 0;
   if (const char *e = getenv("QMCW_NO_LDS")) h->use_lds = atoi(e) == 0;
   if (const char *e = getenv("QMCW_CHUNK")) h->chunk_override = atoi(e);
+  if (const char *e = getenv("QMCW_DET_LDS")) h->det_lds_budget = (size_t)atol(e);
   if (hipStreamSynchronize(h->stream) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_HIP, "init sync"); }
   *out = h;
   return QMCW_OK;
@@ -1022,6 +1051,10 @@ extern "C" int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b
     for (int o = 0; o < NO; o++)
       for (int bb = 0; bb < NB; bb++) coefT[(size_t)bb * NO + o] = h->coef[(size_t)o * NB + bb];
     HIPCHK(h, hipMemcpy(h->d_coefT, coefT.data(), coefT.size() * 8, hipMemcpyHostToDevice));
+    for (int sp = 0; sp < 2; sp++) {
+      std::vector<double> co = coef_occ_table(coefT, NB, NO, h->occ_h[sp]);
+      HIPCHK(h, hipMemcpy(h->d_coef_occ[sp], co.data(), co.size() * 8, hipMemcpyHostToDevice));
+    }
   }
   return QMCW_OK;
 }

@@ -22,6 +22,7 @@ struct KArgs {
   int derivs;        // compute Jastrow/NLPP parameter derivative pieces
   int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
   int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
+  int lds_det;       // k_full_det keeps a chunk's basis values in LDS
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
 };
@@ -197,6 +198,66 @@ __device__ double group_eval_y(const DevSys &S, const double *const inv[2],
   return chi;
 }
 
+// Occupied orbitals (value, gradient, laplacian) of every electron into the
+// walker state (and all orbitals with full_orb), A.chunk electrons at a time
+// through phi (basis values in the engine's real type).
+template <typename T, int G>
+__device__ __forceinline__ void full_orbitals(const KArgs &A, double *ws, double *ss, T *phi, int lane) {
+  const DevSys &S = A.S;
+  const WLayout &L = A.L;
+  const int N = S.N, M = S.M, NO = S.NO, noccmax = S.noccmax;
+  const double *rven = ws + L.rvec_en, *ren = ws + L.r_en;
+  double *oc = ws + L.oc;
+  double *orbf = ss + A.SL.orb;
+  for (int i0 = 0; i0 < N; i0 += A.chunk) {
+    const int nch = min(A.chunk, N - i0);
+    __syncthreads();
+    for (int t = lane; t < nch * S.nshell; t += G) {
+      const int ii = t / S.nshell, s = t % S.nshell;
+      const int i = i0 + ii, ic = S.sh_cent[s];
+      const double *rv = rven + (i * M + ic) * 3;
+      eval_shell<T>(S, s, (T)rv[0], (T)rv[1], (T)rv[2], (T)ren[i * M + ic], 1,
+                    phi + (int64_t)ii * S.NB * 5);
+    }
+    __syncthreads();
+    // occupied orbitals of each electron's spin into the resident state
+    for (int t = lane; t < nch * noccmax; t += G) {
+      const int ii = t / noccmax, j = t % noccmax;
+      const int i = i0 + ii, s = (i < S.nup) ? 0 : 1;
+      double *o = oc + ((int64_t)i * noccmax + j) * 5;
+      if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
+      contract5<T>(A.bf16, S.coefT, NO, S.NB, S.occ[s][j], phi + (int64_t)ii * S.NB * 5, o);
+    }
+    if (A.full_orb) {
+      for (int t = lane; t < nch * NO; t += G) {
+        const int ii = t / NO, lab = t % NO;
+        const int i = i0 + ii;
+        contract5<T>(A.bf16, S.coefT, NO, S.NB, lab, phi + (int64_t)ii * S.NB * 5,
+                     orbf + ((int64_t)i * NO + lab) * 5);
+      }
+    }
+  }
+}
+
+// gp[i][b] = sum_j G(j, i) coef(b, occ_j) for every electron (FP64; BF16
+// mode rounds the result), coef_occ holding the occupied columns contiguously
+template <int G>
+__device__ __forceinline__ void group_gproj(const DevSys &S, double *__restrict__ gp,
+                                            const double *__restrict__ g0, const double *__restrict__ g1,
+                                            int bf16, int lane) {
+  const int NB = S.NB;
+  for (int t = lane; t < S.N * NB; t += G) {
+    const int i = t / NB, b = t % NB;
+    const int s = i < S.nup ? 0 : 1, ie = s ? i - S.nup : i, ns = S.ns[s], nocc = S.nocc[s];
+    const double *__restrict__ gc = (s ? g1 : g0) + ie;
+    const double *__restrict__ co = S.coef_occ[s] + (int64_t)b * nocc;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < nocc; j++) acc += gc[j * ns] * co[j];
+    gp[t] = bf16 ? (double)bf16r((float)acc) : acc;
+  }
+}
+
 // G = A^-1 - T Y A^-1 on reference rows, Y A^-1 on the others
 // (qmc/deriv_fast_mod.f90:600-625).  ai col-major ns x ns, tocc ns x nocc,
 // y nocc x ns, g out nocc x ns; tmp >= nocc*ns.
@@ -335,7 +396,8 @@ __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
 }
 
 template <typename T, int G>
-__global__ QMCW_HEAVY void k_full_det(KArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : QMCW_WPE, 8)))
+void k_full_det(KArgs A) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, NO = S.NO;
   const WLayout &L = A.L;
@@ -348,39 +410,15 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
   (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
   (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
   __shared__ int s_ipiv[kWave / G][128];
-  // ---- orbitals (basis_fns + orbitals_loc_ana), chunks of electrons
-  T *phi = reinterpret_cast<T *>(ss + SL.phi);   // basis values in the engine's real type
+  // ---- orbitals (basis_fns + orbitals_loc_ana), chunks of electrons; the
+  // chunk's basis values in LDS when the launch reserved it
+  extern __shared__ double dyn_lds[];
   double *oc = ws + L.oc;
-  double *orbf = ss + SL.orb;
   const int noccmax = S.noccmax;
-  for (int i0 = 0; i0 < N; i0 += A.chunk) {
-    int nch = min(A.chunk, N - i0);
-    __syncthreads();
-    for (int t = lane; t < nch * S.nshell; t += G) {
-      int ii = t / S.nshell, s = t % S.nshell;
-      int i = i0 + ii, ic = S.sh_cent[s];
-      const double *rv = rven + (i * M + ic) * 3;
-      eval_shell<T>(S, s, (T)rv[0], (T)rv[1], (T)rv[2], (T)ren[i * M + ic], 1,
-                    phi + (int64_t)ii * S.NB * 5);
-    }
-    __syncthreads();
-    // occupied orbitals of each electron's spin into the resident state
-    for (int t = lane; t < nch * noccmax; t += G) {
-      int ii = t / noccmax, j = t % noccmax;
-      int i = i0 + ii, s = (i < S.nup) ? 0 : 1;
-      double *o = oc + ((int64_t)i * noccmax + j) * 5;
-      if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
-      contract5<T>(A.bf16, S.coefT, NO, S.NB, S.occ[s][j], phi + (int64_t)ii * S.NB * 5, o);
-    }
-    if (A.full_orb) {
-      for (int t = lane; t < nch * NO; t += G) {
-        int ii = t / NO, lab = t % NO;
-        int i = i0 + ii;
-        contract5<T>(A.bf16, S.coefT, NO, S.NB, lab, phi + (int64_t)ii * S.NB * 5,
-                     orbf + ((int64_t)i * NO + lab) * 5);
-      }
-    }
-  }
+  if (A.lds_det)
+    full_orbitals<T, G>(A, ws, ss, reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * A.chunk * S.NB * 5, lane);
+  else
+    full_orbitals<T, G>(A, ws, ss, reinterpret_cast<T *>(ss + SL.phi), lane);
   __syncthreads();
   // ---- determinants (t_bld, alpha_bld, y_bld, g_bld)
   double *tmp = ss + SL.red;  // >= max(ns^2, nocc*ns) doubles
@@ -452,17 +490,7 @@ __global__ QMCW_HEAVY void k_full_det(KArgs A) {
   d2d = gsum<G>(d2d);
   if (lane == 0) red[54] = d2d;
   // projected coefficients gp_i = G(:,i)^T coef_occ of the pseudopotential
-  if (S.nloc > 0) {
-    double *gp = ss + SL.gproj;       // [N][NB]
-    for (int t = lane; t < N * S.NB; t += G) {
-      int i = t / S.NB, b = t % S.NB;
-      int s = i < S.nup ? 0 : 1, ie = s ? i - S.nup : i, ns = S.ns[s];
-      const double *g = ss + SL.g[s];
-      double acc = 0.0;
-      for (int j = 0; j < S.nocc[s]; j++) acc += g[j * ns + ie] * S.coefT[(int64_t)b * NO + S.occ[s][j]];
-      gp[t] = A.bf16 ? (double)bf16r((float)acc) : acc;
-    }
-  }
+  if (S.nloc > 0) group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
 }
 
 // FP32 instances are held to 128 VGPRs (four waves per SIMD)
