@@ -63,6 +63,8 @@ struct qmcw_handle {
   int use_lds = 1;           // LDS staging of the moved electron's basis values
   int chunk_override = 0;    // electrons per basis chunk (0: from the L2 budget)
   size_t det_lds_budget = 10240;  // LDS bytes per block for k_full_det's basis values
+  int split_move = 0;        // one-electron move as three launches (k_move_a/b/c)
+  int split_forced = 0;      // QMCW_SPLIT_MOVE set: the autotuner keeps split_move
   // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
   int dmc_on = 0, nact = 0;
   double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
@@ -566,6 +568,16 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     h->der_valid = 1;
   }
 }
+template <int G>
+static void launch_move_bc(qmcw_handle *h, const KArgs &A, size_t smem, int ie, int mode,
+                           double delta, double fbias, double *out, long long *nacc) {
+  const dim3 grid(h->Wl / (kWave / G)), block(kWave);
+  if (h->precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_move_b<double, G>), grid, block, smem, h->stream, A, ie);
+  else
+    hipLaunchKernelGGL((k_move_b<float, G>), grid, block, smem, h->stream, A, ie);
+  hipLaunchKernelGGL((k_move_c<G>), grid, block, 0, h->stream, A, ie, mode, delta, fbias, out, nacc);
+}
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
                          double fbias, double *out, long long *nacc = nullptr) {
   const int G = h->Gk[KC_MOVE];
@@ -577,7 +589,12 @@ static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, dou
   LaunchCtx c = make_ctx(h, A, G);
   c.smem_ratio = A.lds_phi ? lds : 0;
   prof_begin(h, QMCW_PROF_RATIO);
-  ord_kernels(A.S.nordc).ratio(c, ie, mode, xn, delta, fbias, out, nacc);
+  if (h->split_move) {
+    ord_kernels(A.S.nordc).move_a(c, ie, mode, xn, delta, fbias);
+    DISPATCH_GW(G, launch_move_bc, h, A, c.smem_ratio, ie, mode, delta, fbias, out, nacc);
+  } else {
+    ord_kernels(A.S.nordc).ratio(c, ie, mode, xn, delta, fbias, out, nacc);
+  }
   prof_end(h);
 }
 template <int G>
@@ -891,7 +908,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   ML.fsn = take(N); ML.fijn_i = take(3 * N); ML.fijn_j = take(3 * N); ML.lapn_i = take(2 * N);
   ML.fjn = take(3 * N); ML.orbe = take(noccm * 5); ML.ain = take(nsm * nsm);
   ML.tn = take(nsm * noccm); ML.invn = take(ndum * S.kmax2); ML.detn = take(ndum);
-  ML.yn = take(D.nocc[0] * D.ns[0] + D.nocc[1] * D.ns[1]); ML.misc = take(16 + nsm + noccm);
+  ML.yn = take(D.nocc[0] * D.ns[0] + D.nocc[1] * D.ns[1]); ML.misc = take(24 + nsm + noccm);
   ML.stride = o;
   SLayout &SL = h->A.SL;
   o = 0;
@@ -964,6 +981,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_NO_LDS")) h->use_lds = atoi(e) == 0;
   if (const char *e = getenv("QMCW_CHUNK")) h->chunk_override = atoi(e);
   if (const char *e = getenv("QMCW_DET_LDS")) h->det_lds_budget = (size_t)atol(e);
+  if (const char *e = getenv("QMCW_SPLIT_MOVE")) { h->split_move = atoi(e) != 0; h->split_forced = 1; }
   if (hipStreamSynchronize(h->stream) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_HIP, "init sync"); }
   *out = h;
   return QMCW_OK;
@@ -1312,11 +1330,17 @@ extern "C" int qmcw_autotune(qmcw_handle *h, int32_t *chosen) {
                                       KC_JAS, KC_NLPP, KC_DET};
   double best[4] = {1e30, 1e30, 1e30, 1e30};
   int bestG[4] = {h->Gk[0], h->Gk[1], h->Gk[2], h->Gk[3]};
+  int best_split = h->split_move;
   double *xin = h->io + (size_t)h->Wp * 8;
   const int prof_save = h->prof;
   drop_graph(h);
+  // the one-electron move is also timed as three launches (k_move_a/b/c)
+  // unless QMCW_SPLIT_MOVE fixed the choice; the other classes once per width
+  const int sm0 = h->split_forced ? h->split_move : 0, sm1 = h->split_forced ? h->split_move : 1;
+  for (int sm = sm0; sm <= sm1; sm++)
   for (int c = 0; c < 4; c++) {
     const int G = cand[c];
+    h->split_move = sm;
     if (G > 16 && N <= 8) continue;      // lanes would mostly idle
     if (G < 16 && N > 48) continue;
     for (int k = 0; k < 4; k++) h->Gk[k] = G;
@@ -1341,11 +1365,18 @@ extern "C" int qmcw_autotune(qmcw_handle *h, int32_t *chosen) {
       HIPCHK(h, hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
       t[cls[ph]] += ms;
     }
-    for (int k = 0; k < 4; k++)
-      if (t[k] > 0 && t[k] < best[k]) { best[k] = t[k]; bestG[k] = G; }
+    for (int k = 0; k < 4; k++) {
+      if (k != KC_MOVE && sm != sm0) continue;
+      if (t[k] > 0 && t[k] < best[k]) {
+        best[k] = t[k];
+        bestG[k] = G;
+        if (k == KC_MOVE) best_split = sm;
+      }
+    }
   }
   h->prof = prof_save;
   h->ev_used = 0;
+  h->split_move = best_split;
   for (int k = 0; k < 4; k++) h->Gk[k] = bestG[k];
   h->G = bestG[KC_MOVE];
   // restore a consistent resident state with the chosen widths

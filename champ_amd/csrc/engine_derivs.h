@@ -25,7 +25,7 @@ __device__ void en_slots(const DevSys &S, const double *r, const double *rv, dou
   const double k = S.scalek, ar = S.asymp_r;
   const int na = S.norda + 1;
   for (int ic = 0; ic < S.M; ic++) {
-    const int it = S.iwctype[ic];
+    const int it = cst(S.iwctype)[ic];
     const double ri = r[ic];
     double rr, d7, d9;
     scaled<double>(S.isc, k, ri, rr, d7, d9);
@@ -178,7 +178,7 @@ __device__ void pair_slots(const DevSys &S, bool par, double rij, const double *
   scaled<double>(S.isc, k, rij, U, Ud1, Ud2);
   U = 1.0 - k * U; Ud1 = -k * Ud1; Ud2 = -k * Ud2;
   for (int ic = 0; ic < S.M; ic++) {
-    const int it = S.iwctype[ic];
+    const int it = cst(S.iwctype)[ic];
     const double ri = ri_v[ic], rj = rj_v[ic];
     double A, Ad1, Ad2, B, Bd1, Bd2;
     scaled<double>(S.isc, k, ri, A, Ad1, Ad2);
@@ -242,7 +242,7 @@ __device__ void point_pair_slots(const DevSys &S, int i, const double *xn, const
     ap[0] = 1.0;
 #pragma unroll
     for (int q = 1; q <= ORD; q++) ap[q] = ap[q - 1] * A;
-    const int slot0 = S.slot_c0 + S.iwctype[ic] * S.nterms;
+    const int slot0 = S.slot_c0 + cst(S.iwctype)[ic] * S.nterms;
     int idx = 0;
 #pragma unroll
     for (int kk = 0; kk <= ORD; kk++)
@@ -292,18 +292,18 @@ __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
   for (int t = lane; t < S.npj * N; t += G) {
     const int p = t / N, i = t % N;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    for (int q = S.pj_off[p]; q < S.pj_off[p + 1]; q++) {
-      const double w = S.pj_w[q];
-      const double *r = raw + ((int64_t)S.pj_slot[q] * N + i) * 5;
+    for (int q = cst(S.pj_off)[p]; q < cst(S.pj_off)[p + 1]; q++) {
+      const double w = cst(S.pj_w)[q];
+      const double *r = raw + ((int64_t)cst(S.pj_slot)[q] * N + i) * 5;
       g0 += w * r[1]; g1 += w * r[2]; g2 += w * r[3];
     }
     gj[3 * t] = g0; gj[3 * t + 1] = g1; gj[3 * t + 2] = g2;
   }
   for (int p = lane; p < S.npj; p += G) {
     double v = 0.0, d = 0.0;
-    for (int q = S.pj_off[p]; q < S.pj_off[p + 1]; q++) {
-      const double w = S.pj_w[q];
-      const double *r = raw + (int64_t)S.pj_slot[q] * N * 5;
+    for (int q = cst(S.pj_off)[p]; q < cst(S.pj_off)[p + 1]; q++) {
+      const double w = cst(S.pj_w)[q];
+      const double *r = raw + (int64_t)cst(S.pj_slot)[q] * N * 5;
       double sv = 0.0, sd = 0.0;
       for (int i = 0; i < N; i++) { sv += r[5 * i]; sd += r[5 * i + 4]; }
       v += w * sv;
@@ -361,7 +361,7 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
     double *drow = draw + (int64_t)p * nsl;
     const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
     for (int b = 0; b < NB; b++) pbrow[b] = 0.0;
-    const int i = p / M, ic = p % M, ict = S.iwctype[ic];
+    const int i = p / M, ic = p % M, ict = cst(S.iwctype)[ic];
     const double *gpi = gp + (int64_t)i * NB;
     const double rei = ren[p];
     double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
@@ -372,16 +372,16 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
       const double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
       const double *rv0 = rven + 3 * p;
       const double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
-      const double xn[3] = {rei * qx + S.cent[3 * ic], rei * qy + S.cent[3 * ic + 1],
-                            rei * qz + S.cent[3 * ic + 2]};
+      const double xn[3] = {rei * qx + cst(S.cent)[3 * ic], rei * qy + cst(S.cent)[3 * ic + 1],
+                            rei * qz + cst(S.cent)[3 * ic + 2]};
       for (int jc = 0; jc < M; jc++) {
-        double dx = xn[0] - S.cent[3 * jc], dy = xn[1] - S.cent[3 * jc + 1], dz = xn[2] - S.cent[3 * jc + 2];
+        double dx = xn[0] - cst(S.cent)[3 * jc], dy = xn[1] - cst(S.cent)[3 * jc + 1], dz = xn[2] - cst(S.cent)[3 * jc + 2];
         rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
         rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
       }
       for (int b = 0; b < NB; b++) phq[b] = T(0);
       for (int sh = 0; sh < S.nshell; sh++) {
-        const int jc = S.sh_cent[sh];
+        const int jc = cst(S.sh_cent)[sh];
         eval_shell<T, 1>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1], (T)rvnew[3 * jc + 2],
                          (T)rnew[jc], 0, phq);
       }
@@ -392,9 +392,9 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
         if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
       const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
       double vpot = 0.0;
-      for (int l = 1; l <= S.npotd[ict]; l++)
-        if (l != S.lpotp1[ict]) vpot += v[l - 1] * legendre_w(l, costh);
-      const double fac = S.q0[3 * nq + iq] * exp(delta) * vpot;
+      for (int l = 1; l <= cst(S.npotd)[ict]; l++)
+        if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
+      const double fac = cst(S.q0)[3 * nq + iq] * exp(delta) * vpot;
       for (int b = 0; b < NB; b++) pbrow[b] += fac * (double)phq[b];
       const double wr = fac * rat;
       wsum += wr;
@@ -428,13 +428,13 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
     const int i = t / NO, q = t % NO;
     const double *row = pbc + (int64_t)i * M * NB;
     double a = 0.0;
-    for (int b = 0; b < NB; b++) a += S.coefT[(int64_t)b * NO + q] * row[b];
+    for (int b = 0; b < NB; b++) a += cst(S.coefT)[(int64_t)b * NO + q] * row[b];
     pnl[t] = a;
   }
   double *dpe = ss + DL.dpe;
   for (int p = lane; p < S.npj; p += G) {
     double a = 0.0;
-    for (int q = S.pj_off[p]; q < S.pj_off[p + 1]; q++) a += S.pj_w[q] * dsl[S.pj_slot[q]];
+    for (int q = cst(S.pj_off)[p]; q < cst(S.pj_off)[p + 1]; q++) a += cst(S.pj_w)[q] * dsl[cst(S.pj_slot)[q]];
     dpe[p] = a;
   }
 }
@@ -495,14 +495,14 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
     for (int e = lane; e < ns * NO; e += G) {
       const int k = e / NO, q = e % NO;
       double a = 0.0;
-      for (int r = 0; r < ns; r++) a += aib[k * NO + S.ref[s][r]] * tf[r * NO + q];
+      for (int r = 0; r < ns; r++) a += aib[k * NO + cst(S.ref[s])[r]] * tf[r * NO + q];
       dt[e] = aib[e] - a;
     }
     __syncthreads();
     const double *inv = ws + L.inv[s];
     double *del = ss + DL.del[s], *idti = ss + DL.idti[s];
     for (int u = lane; u < S.ndetu[s]; u += G) {
-      const int k = S.ord[s][u];
+      const int k = cst(S.ord[s])[u];
       const double *iv = inv + u * S.kmax2;
       const int *rf = S.rf[s] + u * ns, *le = S.exl[s] + u * ns;
       double d = 0.0;
@@ -532,9 +532,9 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
     const int s = t < S.ndetu[0] ? 0 : 1, u = s ? t - S.ndetu[0] : t;
     double c0 = 0.0, c1 = 0.0;
     for (int d = 0; d < S.ndet; d++) {
-      if (S.iwdet[s][d] != u) continue;
-      const int iu = S.iwdet[0][d], id = S.iwdet[1][d];
-      const double cf = S.cdd[d] * dv0[iu] * dv1[id];
+      if (cst(S.iwdet[s])[d] != u) continue;
+      const int iu = cst(S.iwdet[0])[d], id = cst(S.iwdet[1])[d];
+      const double cf = cst(S.cdd)[d] * dv0[iu] * dv1[id];
       c0 += cf;
       c1 += cf * ((S.ns[0] ? del0[iu] : 0.0) + (S.ns[1] ? del1[id] : 0.0));
     }
@@ -547,7 +547,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
     const double *y = ws + L.y[s], *dt = ss + DL.dt[s];
     for (int e = lane; e < nocc * ns; e += G) {
       const int j = e / ns, i = e % ns;
-      es += y[e] * dt[i * NO + S.occ[s][j]];
+      es += y[e] * dt[i * NO + cst(S.occ[s])[j]];
     }
   }
   const double esum = gsum<G>(es);
@@ -562,11 +562,11 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
       const int jo = e / ns, ir = e % ns;
       double a = 0.0;
       for (int u = 1; u < S.ndetu[s]; u++) {
-        const int k = S.ord[s][u];
+        const int k = cst(S.ord[s])[u];
         int jj = -1, ii = -1;
         for (int q = 0; q < k; q++) {
-          if (S.exo[s][u * ns + q] == jo) jj = q;
-          if (S.rf[s][u * ns + q] == ir) ii = q;
+          if (cst(S.exo[s])[u * ns + q] == jo) jj = q;
+          if (cst(S.rf[s])[u * ns + q] == ir) ii = q;
         }
         if (jj < 0 || ii < 0) continue;
         const int o = u * S.kmax2 + jj + k * ii;
@@ -589,16 +589,16 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
       const int k = e / ns, i = e % ns;
       double a = 0.0;
       for (int j = 0; j < nocc; j++) {
-        const int oj = S.occ[s][j];
+        const int oj = cst(S.occ[s])[j];
         a += tf[k * NO + oj] * dy[j * ns + i] + dt[k * NO + oj] * y[j * ns + i];
       }
-      aux[S.iref[s][k] * ns + i] = a;
+      aux[cst(S.iref[s])[k] * ns + i] = a;
     }
     __syncthreads();
     for (int e = lane; e < nocc * ns; e += G) {
       const int j = e / ns, i = e % ns;
       double a3 = 0.0;
-      for (int q = 0; q < ns; q++) a3 += g[j * ns + q] * B[(off + q) * NO + S.ref[s][i]];
+      for (int q = 0; q < ns; q++) a3 += g[j * ns + q] * B[(off + q) * NO + cst(S.ref[s])[i]];
       aux[e] = dy[e] - a3 - aux[e];
     }
     __syncthreads();
@@ -613,12 +613,12 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
   // ---- outputs
   double *o1 = out + (int64_t)w * 2 * P, *o2 = o1 + P;
   for (int ip = lane; ip < S.npcsf; ip += G) {
-    const int icsf = S.pcsf[ip];
+    const int icsf = cst(S.pcsf)[ip];
     double dd = 0.0, de = 0.0;
-    for (int q = S.csf_det_off[icsf]; q < S.csf_det_off[icsf + 1]; q++) {
-      const int d = S.csf_det[q];
-      const int iu = S.iwdet[0][d], id = S.iwdet[1][d];
-      const double cf = S.dsgn[0][iu] * S.dsgn[1][id] * S.csf_det_coef[q] * dv0[iu] * dv1[id] * chii;
+    for (int q = cst(S.csf_det_off)[icsf]; q < cst(S.csf_det_off)[icsf + 1]; q++) {
+      const int d = cst(S.csf_det)[q];
+      const int iu = cst(S.iwdet[0])[d], id = cst(S.iwdet[1])[d];
+      const double cf = cst(S.dsgn[0])[iu] * cst(S.dsgn[1])[id] * cst(S.csf_det_coef)[q] * dv0[iu] * dv1[id] * chii;
       const double dl = (S.ns[0] ? del0[iu] : 0.0) + (S.ns[1] ? del1[id] : 0.0);
       dd += cf;
       de += cf * (dl - esum);
@@ -636,14 +636,14 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
     o2[S.npcsf + ip] = de;
   }
   for (int ip = lane; ip < S.nporb; ip += G) {
-    const int oi = S.porb_i[ip], oj = S.porb_j[ip];
+    const int oi = cst(S.porb_i)[ip], oj = cst(S.porb_j)[ip];
     double dp = 0.0, de = 0.0;
-    for (int pass = 0; pass < (S.porb_rev[ip] ? 2 : 1); pass++) {
+    for (int pass = 0; pass < (cst(S.porb_rev)[ip] ? 2 : 1); pass++) {
       const int a = pass ? oj : oi, b = pass ? oi : oj;
       const double sg = pass ? -1.0 : 1.0;
       for (int s = 0; s < 2; s++) {
         const int ns = S.ns[s], off = s ? S.nup : 0;
-        const int pa = S.iocc_lab[s][a];
+        const int pa = cst(S.iocc_lab[s])[a];
         if (ns == 0 || pa < 0) continue;
         const double *g = ss + SL.g[s], *dg = ss + DL.dg[s];
         for (int k = 0; k < ns; k++) {

@@ -11,6 +11,17 @@
 
 namespace qmcw {
 
+// The system tables (DevSys) are written only by the host between launches.
+// Reading them through the constant address space lets the compiler turn a
+// load at a uniform index into a scalar load through the scalar cache
+// instead of a vector load with its own wait.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> cst(const T *p) {
+  return (cptr<T>)p;
+}
+
 template <int G>
 __device__ __forceinline__ double gsum(double v) {
 #pragma unroll
@@ -97,7 +108,7 @@ __device__ __forceinline__ void scaled(int isc, T k, T r, T &rr, T &d1, T &d2) {
 #define QMCW_EEN_TERM(k, l, m) (((k) >= 1 || (m) >= 1) && ((l) + 2 * (m) >= 1) && ((k) + (l) + 2 * (m) >= 2))
 
 template <typename T, int ORD>
-__device__ __forceinline__ T een_value(T U, T A, T B, const double *__restrict__ c) {
+__device__ __forceinline__ T een_value(T U, T A, T B, cptr<double> c) {
   T up[ORD + 1], ap[ORD + 1], bp[ORD + 1];
   up[0] = ap[0] = bp[0] = T(1);
 #pragma unroll
@@ -123,7 +134,7 @@ template <typename T>
 struct EenD { T f, fU, fUU, fA, fAA, fB, fBB, fUA, fUB; };
 
 template <typename T, int ORD>
-__device__ __forceinline__ void een_derivs(T U, T A, T B, const double *__restrict__ c, EenD<T> &o) {
+__device__ __forceinline__ void een_derivs(T U, T A, T B, cptr<double> c, EenD<T> &o) {
   T up[ORD + 1], ap[ORD + 1], bp[ORD + 1];
   up[0] = ap[0] = bp[0] = T(1);
 #pragma unroll
@@ -221,7 +232,7 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
   Ud1 = -k * Ud1;
   Ud2 = -k * Ud2;
   for (int ic = 0; ic < S.M; ic++) {
-    int it = S.iwctype[ic];
+    int it = cst(S.iwctype)[ic];
     T ri = (T)ri_v[ic], rj = (T)rj_v[ic];
     T A, Ad1, Ad2, B, Bd1, Bd2;
     scaled<T>(S.isc, k, ri, A, Ad1, Ad2);
@@ -229,7 +240,7 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
     A = T(1) - k * A; Ad1 = -k * Ad1; Ad2 = -k * Ad2;
     B = T(1) - k * B; Bd1 = -k * Bd1; Bd2 = -k * Bd2;
     EenD<T> e;
-    een_derivs<T, ORD>(U, A, B, S.c + it * S.nterms, e);
+    een_derivs<T, ORD>(U, A, B, cst(S.c) + it * S.nterms, e);
     T fc = e.f, fU = e.fU, fUU = e.fUU, fA = e.fA, fAA = e.fAA, fB = e.fB, fBB = e.fBB,
       fUA = e.fUA, fUB = e.fUB;
     // chain rule to unscaled distances
@@ -257,11 +268,11 @@ __device__ T jas_en_val(const DevSys &S, const double *ri_v) {
   const T k = T(S.scalek);
   T f = 0;
   for (int ic = 0; ic < S.M; ic++) {
-    int it = S.iwctype[ic];
-    const double *a = S.a4 + it * (S.norda + 1);
+    int it = cst(S.iwctype)[ic];
+    const cptr<double> a = cst(S.a4) + it * (S.norda + 1);
     T u, d1, d2;
     scaled<T>(S.isc, k, (T)ri_v[ic], u, d1, d2);
-    T fe = T(a[0]) * u / (T(1) + T(a[1]) * u) - T(S.asymp_jasa[it]);
+    T fe = T(a[0]) * u / (T(1) + T(a[1]) * u) - T(cst(S.asymp_jasa)[it]);
     T up = u;
     for (int iord = 2; iord <= S.norda; iord++) {
       up *= u;
@@ -281,7 +292,7 @@ template <int ORD>
 __host__ __device__ constexpr int een_gi(int k, int n) { return k * (ORD + 1) - k * (k - 1) / 2 + n; }
 
 template <typename T, int ORD>
-__device__ __forceinline__ void een_gamma(T A, const double *__restrict__ c, T *gam) {
+__device__ __forceinline__ void een_gamma(T A, cptr<double> c, T *gam) {
   T ap[ORD + 1];
   ap[0] = T(1);
 #pragma unroll
@@ -363,7 +374,7 @@ __device__ T jas_point_val(const DevSys &S, int i, const double *xn, const doubl
     T A, t1, t2;
     scaled<T>(S.isc, k, (T)rnew[ic], A, t1, t2);
     T gam[NG];
-    een_gamma<T, ORD>(T(1) - k * A, S.c + S.iwctype[ic] * S.nterms, gam);
+    een_gamma<T, ORD>(T(1) - k * A, cst(S.c) + cst(S.iwctype)[ic] * S.nterms, gam);
     T fc = T(0);
     for (int jj = 0; jj < N; jj++) {
       if (jj == i) continue;
@@ -383,12 +394,12 @@ __device__ void jas_en(const DevSys &S, const double *ri_v, const double *rvi, T
   g[0] = g[1] = g[2] = 0;
   lap = 0;
   for (int ic = 0; ic < S.M; ic++) {
-    int it = S.iwctype[ic];
-    const double *a = S.a4 + it * (S.norda + 1);
+    int it = cst(S.iwctype)[ic];
+    const cptr<double> a = cst(S.a4) + it * (S.norda + 1);
     T r = (T)ri_v[ic], u, d1, d2;
     scaled<T>(S.isc, k, r, u, d1, d2);
     T bot = T(1) + T(a[1]) * u;
-    T fe = T(a[0]) * u / bot - T(S.asymp_jasa[it]);
+    T fe = T(a[0]) * u / bot - T(cst(S.asymp_jasa)[it]);
     T fu = T(a[0]) / (bot * bot);
     T fuu = T(-2) * fu * T(a[1]) / bot;
     T up = u, upm = T(1);
@@ -505,7 +516,7 @@ __device__ __forceinline__ int m_slot(int m) { return m > 0 ? 2 * m - 1 : (m < 0
 
 template <typename T, int L, int ST>
 __device__ __forceinline__ void shell_write(T x, T y, T z, T f, T fr, T lapfac, int want,
-                                            const int *bas, T *phi) {
+                                            cptr<int> bas, T *phi) {
   T Sv[2 * L + 1], Sg[2 * L + 1][3];
   solid_harm<T>(L, x, y, z, Sv, Sg);
 #pragma unroll
@@ -529,13 +540,14 @@ __device__ __forceinline__ void shell_write(T x, T y, T z, T f, T fr, T lapfac, 
 template <typename T, int ST = 5>
 __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
                            T *phi) {
-  int l = S.sh_l[s], a = S.sh_a[s];
+  int l = cst(S.sh_l)[s], a = cst(S.sh_a)[s];
   if (r == T(0)) { r = T(1e-30); x = T(1e-30); }
   // radial f = r^a g(r), g = sum d exp(-z r^q); need f, f'/r, f''
   T g = 0, g1 = 0, g2 = 0;  // g, dg/dr, d2g/dr2
   const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
-  for (int p = S.sh_prim_off[s]; p < S.sh_prim_off[s + 1]; p++) {
-    T zeta = T(S.prim_zeta[p]), d = T(S.prim_coef[p]);
+  const cptr<double> pz = cst(S.prim_zeta), pc = cst(S.prim_coef);
+  for (int p = cst(S.sh_prim_off)[s]; p < cst(S.sh_prim_off)[s + 1]; p++) {
+    T zeta = T(pz[p]), d = T(pc[p]);
     if (gauss) {
       T e = d * exp(-zeta * r * r);
       g += e;
@@ -564,7 +576,7 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
   }
   // laplacian of S f(r): S (f'' + 2 (l+1) f'/r)
   T lapfac = f2 + T(2 * (l + 1)) * fr;
-  const int *bas = S.sh_bas + s * kMaxShellM;
+  const cptr<int> bas = cst(S.sh_bas) + s * kMaxShellM;
   switch (l) {
     case 0: shell_write<T, 0, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
     case 1: shell_write<T, 1, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
@@ -578,7 +590,7 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
 // Shells are complete: slot q holds m = 0, 1, -1, 2, -2, ...; an absent
 // component has basis index -1.
 template <typename T, int L, typename GP>
-__device__ __forceinline__ T shell_dot_l(T x, T y, T z, const int *bas, const GP *gp) {
+__device__ __forceinline__ T shell_dot_l(T x, T y, T z, cptr<int> bas, const GP *gp) {
   T Sv[2 * L + 1], Sg[2 * L + 1][3];
   solid_harm<T>(L, x, y, z, Sv, Sg);
   T acc = 0;
@@ -591,15 +603,16 @@ __device__ __forceinline__ T shell_dot_l(T x, T y, T z, const int *bas, const GP
 }
 template <typename T, typename GP = T>
 __device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const GP *gp) {
-  int l = S.sh_l[s], a = S.sh_a[s];
+  int l = cst(S.sh_l)[s], a = cst(S.sh_a)[s];
   const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
   T g = 0;
-  for (int p = S.sh_prim_off[s]; p < S.sh_prim_off[s + 1]; p++) {
-    T zeta = T(S.prim_zeta[p]), d = T(S.prim_coef[p]);
+  const cptr<double> pz = cst(S.prim_zeta), pc = cst(S.prim_coef);
+  for (int p = cst(S.sh_prim_off)[s]; p < cst(S.sh_prim_off)[s + 1]; p++) {
+    T zeta = T(pz[p]), d = T(pc[p]);
     g += d * exp(gauss ? -zeta * r * r : -zeta * r);
   }
   for (int q = 0; q < a; q++) g *= r;
-  const int *bas = S.sh_bas + s * kMaxShellM;
+  const cptr<int> bas = cst(S.sh_bas) + s * kMaxShellM;
   T acc;
   switch (l) {
     case 0: acc = shell_dot_l<T, 0, GP>(x, y, z, bas, gp); break;
@@ -675,12 +688,12 @@ __device__ __forceinline__ double legendre_w(int l1, double c) {
 
 __device__ __forceinline__ double ecp_radial(const DevSys &S, double r, int l0, int ict) {
   double v = 0, rsq = r * r;
-  int b = S.ecp_off[ict * QMCW_MAX_LPOT + l0], e = S.ecp_off[ict * QMCW_MAX_LPOT + l0 + 1];
+  int b = cst(S.ecp_off)[ict * QMCW_MAX_LPOT + l0], e = cst(S.ecp_off)[ict * QMCW_MAX_LPOT + l0 + 1];
   for (int q = b; q < e; q++) {
-    int pw = S.ecp_power[q];
+    int pw = cst(S.ecp_power)[q];
     double p = 1.0;
     if (pw != 2) p = pow(r, (double)(pw - 2));
-    v += p * S.ecp_coef[q] * exp(-S.ecp_expo[q] * rsq);
+    v += p * cst(S.ecp_coef)[q] * exp(-cst(S.ecp_expo)[q] * rsq);
   }
   return v;
 }

@@ -32,6 +32,12 @@ struct KArgs {
 #define QMCW_WPE 2
 #endif
 #define QMCW_HEAVY __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(QMCW_WPE, 8)))
+#ifndef QMCW_MOVE_WPE
+#define QMCW_MOVE_WPE 2
+#endif
+#define QMCW_MOVE __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(QMCW_MOVE_WPE, 8)))
+// offset (past ML.misc + ns + nocc) of the Jastrow sums handed between the move kernels
+constexpr int kMvJas = 16;
 
 #define QMCW_GROUP_SETUP                                         \
   const int lane = threadIdx.x % G;                              \
@@ -176,7 +182,7 @@ __device__ double group_eval_y(const DevSys &S, const double *const inv[2],
   const double *__restrict__ dv1 = detv[1];
   double chi = 0.0;
 #pragma unroll 4
-  for (int d = lane; d < S.ndet; d += G) chi += S.cdd[d] * dv0[S.iwdet[0][d]] * dv1[S.iwdet[1][d]];
+  for (int d = lane; d < S.ndet; d += G) chi += cst(S.cdd)[d] * dv0[cst(S.iwdet[0])[d]] * dv1[cst(S.iwdet[1])[d]];
   chi = gsum<G>(chi);
   const double chii = 1.0 / chi;
   for (int t = lane; t < S.ndetu[0] + S.ndetu[1]; t += G) {
@@ -185,9 +191,9 @@ __device__ double group_eval_y(const DevSys &S, const double *const inv[2],
     const int *__restrict__ id = S.uid[s];
     const double *__restrict__ cf = S.ucf[s];
     double cu = 0.0;
-    const int q1 = S.udet_off[s][u + 1];
+    const int q1 = cst(S.udet_off[s])[u + 1];
 #pragma unroll 4
-    for (int q = S.udet_off[s][u]; q < q1; q++) cu += cf[q] * dv0[iu[q]] * dv1[id[q]];
+    for (int q = cst(S.udet_off[s])[u]; q < q1; q++) cu += cf[q] * dv0[iu[q]] * dv1[id[q]];
     cup[t] = cu;
   }
   __syncthreads();
@@ -214,7 +220,7 @@ __device__ __forceinline__ void full_orbitals(const KArgs &A, double *ws, double
     __syncthreads();
     for (int t = lane; t < nch * S.nshell; t += G) {
       const int ii = t / S.nshell, s = t % S.nshell;
-      const int i = i0 + ii, ic = S.sh_cent[s];
+      const int i = i0 + ii, ic = cst(S.sh_cent)[s];
       const double *rv = rven + (i * M + ic) * 3;
       eval_shell<T>(S, s, (T)rv[0], (T)rv[1], (T)rv[2], (T)ren[i * M + ic], 1,
                     phi + (int64_t)ii * S.NB * 5);
@@ -226,7 +232,7 @@ __device__ __forceinline__ void full_orbitals(const KArgs &A, double *ws, double
       const int i = i0 + ii, s = (i < S.nup) ? 0 : 1;
       double *o = oc + ((int64_t)i * noccmax + j) * 5;
       if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
-      contract5<T>(A.bf16, S.coefT, NO, S.NB, S.occ[s][j], phi + (int64_t)ii * S.NB * 5, o);
+      contract5<T>(A.bf16, S.coefT, NO, S.NB, cst(S.occ[s])[j], phi + (int64_t)ii * S.NB * 5, o);
     }
     if (A.full_orb) {
       for (int t = lane; t < nch * NO; t += G) {
@@ -284,7 +290,7 @@ __device__ void group_g(const DevSys &S, int s, const double *ai,
     int k = e / ns, i = e % ns;
     double acc = 0.0;
     for (int j = 0; j < nocc; j++) acc += tocc[k * nocc + j] * tmp[j * ns + i];
-    g[S.iref[s][k] * ns + i] = ai[k + ns * i] - acc;
+    g[cst(S.iref[s])[k] * ns + i] = ai[k + ns * i] - acc;
   }
   __syncthreads();
 }
@@ -316,13 +322,13 @@ __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
     int i = p / M, ic = p % M;
     double s2 = 0.0;
     for (int q = 0; q < 3; q++) {
-      double d = x[3 * i + q] - S.cent[3 * ic + q];
+      double d = x[3 * i + q] - cst(S.cent)[3 * ic + q];
       rven[3 * p + q] = d;
       s2 += d * d;
     }
     double r = sqrt(s2);
     ren[p] = r;
-    if (S.nloc == 0) pe_en -= S.znuc[S.iwctype[ic]] / r;
+    if (S.nloc == 0) pe_en -= cst(S.znuc)[cst(S.iwctype)[ic]] / r;
   }
   for (int p = lane; p < N * N; p += G) {
     int i = p / N, j = p % N;
@@ -433,7 +439,7 @@ void k_full_det(KArgs A) {
     // A(i, k) = orb(i, ref_k): position of ref_k in occ is iref[k]
     for (int e = lane; e < ns * ns; e += G) {
       int i = e % ns, k = e / ns;
-      ai[i + ns * k] = oc[((int64_t)(off + i) * noccmax + S.iref[s][k]) * 5];
+      ai[i + ns * k] = oc[((int64_t)(off + i) * noccmax + cst(S.iref[s])[k]) * 5];
     }
     __syncthreads();
     double det = group_matinv<G>(ai, tmp, ns, lane, s_ipiv[grp]);
@@ -451,11 +457,11 @@ void k_full_det(KArgs A) {
     // alpha blocks of the unique dets
     double *inv = ws + L.inv[s], *dv = ws + L.detv[s];
     for (int u = lane; u < S.ndetu[s]; u += G) {
-      int k = S.ord[s][u];
+      int k = cst(S.ord[s])[u];
       double *blk = inv + u * S.kmax2;
       for (int ii = 0; ii < k; ii++)
         for (int jj = 0; jj < k; jj++)
-          blk[jj + k * ii] = tocc[S.rf[s][u * ns + jj] * nocc + S.exo[s][u * ns + ii]];
+          blk[jj + k * ii] = tocc[cst(S.rf[s])[u * ns + jj] * nocc + cst(S.exo[s])[u * ns + ii]];
       dv[u] = small_matinv(blk, k);
     }
     __syncthreads();
@@ -517,13 +523,13 @@ void k_full_nlpp(KArgs A) {
     double *vnl = ss + SL.vnl;        // [N*M][QMCW_MAX_LPOT + 1] (last: in range)
     T *bt = reinterpret_cast<T *>(ss + SL.bt);
     for (int p = lane; p < N * M; p += G) {
-      int ic = p % M, ict = S.iwctype[ic];
+      int ic = p % M, ict = cst(S.iwctype)[ic];
       double r = fmax(1e-10, ren[p]);
       double *v = vnl + p * (QMCW_MAX_LPOT + 1);
       bool inr = false;
-      for (int l = 1; l <= S.npotd[ict]; l++) {
+      for (int l = 1; l <= cst(S.npotd)[ict]; l++) {
         double vl = ecp_radial(S, r, l - 1, ict);
-        if (l == S.lpotp1[ict]) { ploc += vl - S.znuc[ict] / r; vl = 0.0; }
+        if (l == cst(S.lpotp1)[ict]) { ploc += vl - cst(S.znuc)[ict] / r; vl = 0.0; }
         else if (fabs(vl) > 1e-4) inr = true;
         v[l - 1] = vl;
       }
@@ -546,16 +552,16 @@ void k_full_nlpp(KArgs A) {
     for (int t = lane; t < npair * nq; t += G) {
       const int iq = t % nq, p = plist[t / nq], ic = p % M, i = p / M;
       const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
-      int ict = S.iwctype[ic];
+      int ict = cst(S.iwctype)[ic];
       double rei = ren[p];
       double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
       const double *rv0 = rven + 3 * p;
       double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
-      double xn[3] = {rei * qx + S.cent[3 * ic], rei * qy + S.cent[3 * ic + 1],
-                      rei * qz + S.cent[3 * ic + 2]};
+      double xn[3] = {rei * qx + cst(S.cent)[3 * ic], rei * qy + cst(S.cent)[3 * ic + 1],
+                      rei * qz + cst(S.cent)[3 * ic + 2]};
       double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
       for (int jc = 0; jc < M; jc++) {
-        double dx = xn[0] - S.cent[3 * jc], dy = xn[1] - S.cent[3 * jc + 1], dz = xn[2] - S.cent[3 * jc + 2];
+        double dx = xn[0] - cst(S.cent)[3 * jc], dy = xn[1] - cst(S.cent)[3 * jc + 1], dz = xn[2] - cst(S.cent)[3 * jc + 2];
         rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
         rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
       }
@@ -563,7 +569,7 @@ void k_full_nlpp(KArgs A) {
       double rd = 0.0;
       const double *gpi = gp + (int64_t)i * S.NB;
       for (int sh = 0; sh < S.nshell; sh++) {
-        int jc = S.sh_cent[sh];
+        int jc = cst(S.sh_cent)[sh];
         rd += (double)shell_dot<T, double>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1],
                                            (T)rvnew[3 * jc + 2], (T)rnew[jc], gpi);
       }
@@ -573,9 +579,9 @@ void k_full_nlpp(KArgs A) {
         if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
       const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
       double vpot = 0.0;
-      for (int l = 1; l <= S.npotd[ict]; l++)
-        if (l != S.lpotp1[ict]) vpot += v[l - 1] * legendre_w(l, costh);
-      vpsp += S.q0[3 * nq + iq] * exp(delta) * vpot * rd;
+      for (int l = 1; l <= cst(S.npotd)[ict]; l++)
+        if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
+      vpsp += cst(S.q0)[3 * nq + iq] * exp(delta) * vpot * rd;
     }
     vpsp = gsum<G>(vpsp);
   }
@@ -727,12 +733,12 @@ __device__ __forceinline__ void orbitals_1e(const KArgs &A, int s, int nocc, con
                                             const double *rven, T *phi, double *orbe, int lane) {
   const DevSys &S = A.S;
   for (int sh = lane; sh < S.nshell; sh += G) {
-    const int ic = S.sh_cent[sh];
+    const int ic = cst(S.sh_cent)[sh];
     eval_shell<T>(S, sh, (T)rven[3 * ic], (T)rven[3 * ic + 1], (T)rven[3 * ic + 2], (T)ren[ic], 1, phi);
   }
   __syncthreads();
   for (int j = lane; j < nocc; j += G)
-    contract5<T>(A.bf16, S.coefT, S.NO, S.NB, S.occ[s][j], phi, orbe + 5 * j);
+    contract5<T>(A.bf16, S.coefT, S.NO, S.NB, cst(S.occ[s])[j], phi, orbe + 5 * j);
 }
 
 // ---------------------------------------------------------------------------
@@ -797,7 +803,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   for (int ic = lane; ic < M; ic += G) {
     double s2 = 0.0;
     for (int q = 0; q < 3; q++) {
-      double d = xn[q] - S.cent[3 * ic + q];
+      double d = xn[q] - cst(S.cent)[3 * ic + q];
       rven[3 * ic + q] = d;
       s2 += d * d;
     }
@@ -849,10 +855,12 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     const double gi_old[3] = {fijo[3 * (ie * N + j)], fijo[3 * (ie * N + j) + 1], fijo[3 * (ie * N + j) + 2]};
     const double gj_old[3] = {fijo[3 * (j * N + ie)], fijo[3 * (j * N + ie) + 1], fijo[3 * (j * N + ie) + 2]};
     const double fj_old[3] = {fjo[3 * j], fjo[3 * j + 1], fjo[3 * j + 2]};
-    const T *g_ie = (hi == ie) ? po.gi : po.gj;
-    const T *g_j = (hi == ie) ? po.gj : po.gi;
-    T l_ie = (hi == ie) ? po.li : po.lj;
-    T l_j = (hi == ie) ? po.lj : po.li;
+    // per-component selects keep po in registers
+    const bool mi = hi == ie;
+    const T g_ie[3] = {mi ? po.gi[0] : po.gj[0], mi ? po.gi[1] : po.gj[1], mi ? po.gi[2] : po.gj[2]};
+    const T g_j[3] = {mi ? po.gj[0] : po.gi[0], mi ? po.gj[1] : po.gi[1], mi ? po.gj[2] : po.gi[2]};
+    T l_ie = mi ? po.li : po.lj;
+    T l_j = mi ? po.lj : po.li;
     fsn[j] = po.f;
     for (int q = 0; q < 3; q++) {
       fijn_i[3 * j + q] = g_ie[q];
@@ -892,7 +900,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   const double *tocc = ws + L.tocc[s];
   double *ain = mv + ML.ain, *tn = mv + ML.tn;
   double rat = 0.0;
-  for (int k = lane; k < ns; k += G) rat += orbe[5 * S.iref[s][k]] * ai[k + ns * e];
+  for (int k = lane; k < ns; k += G) rat += orbe[5 * cst(S.iref[s])[k]] * ai[k + ns * e];
   rat = gsum<G>(rat);
   // vi1(i) = sum_k orbe(ref_k) ai(k, i) - delta(i, e); ui(k) = -ai(k,e)/rat
   double *vi1 = mv + ML.misc + 8;            // [ns]
@@ -903,14 +911,273 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   __syncthreads();
   double *invn = mv + ML.invn, *detn = mv + ML.detn;
   for (int u = lane; u < S.ndetu[s]; u += G) {
-    int k = S.ord[s][u];
+    int k = cst(S.ord[s])[u];
     double *blk = invn + u * S.kmax2;
     for (int ii = 0; ii < k; ii++)
       for (int jj = 0; jj < k; jj++)
-        blk[jj + k * ii] = tn[S.rf[s][u * ns + jj] * nocc + S.exo[s][u * ns + ii]];
+        blk[jj + k * ii] = tn[cst(S.rf[s])[u * ns + jj] * nocc + cst(S.exo[s])[u * ns + ii]];
     detn[u] = small_matinv(blk, k);
   }
   __syncthreads();
+  double chin;
+  {
+    const double *invs[2], *dvs[2];
+    double *ys[2];
+    invs[s] = invn; dvs[s] = detn; ys[s] = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
+    invs[1 - s] = ws + L.inv[1 - s]; dvs[1 - s] = ws + L.detv[1 - s];
+    ys[1 - s] = mv + ML.yn + (s ? 0 : S.nocc[0] * S.ns[0]);
+    chin = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
+  }
+  __syncthreads();
+  const double *yn = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
+  double deta_n = ws[L.deta[s]] * rat;
+  double gnew[3], lapn_tot;
+  group_grad_1e<G>(A, ws, ie, orbe, ain, tn, yn, fji, lapjn, gnew, &lapn_tot, lane);
+  if (lane == 0) {
+    double psio = ws[L.chi] * ws[L.deta[0]] * ws[L.deta[1]];
+    double psin = chin * deta_n * ws[L.deta[1 - s]];
+    double logn = log(fabs(chin)) + log(fabs(deta_n)) + log(fabs(ws[L.deta[1 - s]]));
+    mv[ML.misc + 0] = rat;
+    mv[ML.misc + 5] = chin;
+    mv[ML.misc + 6] = deta_n;
+    mv[ML.misc + 7] = fsumn;
+    mv[ML.misc + 8 + ns + nocc] = logn;
+    double *o = out ? out + 8 * w : nullptr;
+    double ratio = (psin / psio) * exp(fsumn - ws[L.fsumo]);
+    mv[ML.misc + 9 + ns + nocc] = ratio;
+    for (int q = 0; q < 3; q++) mv[ML.misc + 10 + ns + nocc + q] = gnew[q];
+    if (o) {
+      o[0] = ratio; o[1] = logn; o[2] = psin < 0 ? -1.0 : 1.0; o[3] = fsumn;
+      o[4] = gnew[0]; o[5] = gnew[1]; o[6] = gnew[2]; o[7] = lapn_tot;
+    }
+  }
+  // VMC sweep (mode 2): Metropolis test and commit in the same launch
+  if (mode == 2) {
+    __shared__ int s_acc[kWave / G];
+    __syncthreads();
+    if (lane == 0) {
+      const int acc = metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias);
+      s_acc[grp] = acc;
+      if (acc && nacc) nacc[w] += 1;
+    }
+    __syncthreads();
+    if (s_acc[grp]) group_commit<G>(A, ws, mv, ie, lane);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same one-electron move as k_ratio_1e in three launches, each with its
+// own register budget: (a) proposal, distances and Jastrow row; (b) orbitals
+// of the new position and dete_update; (c) eval_y, the new gradient, the
+// ratio and (mode 2) the Metropolis test and commit.  The pending buffers
+// carry everything between them; the Jastrow sums that (c) needs go to
+// ML.misc + kMvJas + {0..3} (fji, lap ln J).
+template <typename T, int G, int ORD>
+__global__ QMCW_MOVE void k_move_a(KArgs A, int ie, int mode, const double *xnew_in, double delta,
+                                   double fbias) {
+  QMCW_GROUP_SETUP
+  const int N = S.N, M = S.M, noccmax = S.noccmax;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const MLayout &ML = A.ML;
+  double *mv = ss + SL.mv;
+  const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  const int e = s ? ie - S.nup : ie;
+  const double *x = ws + L.x;
+  const double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
+  double *xn = mv + ML.xn;
+  // ---- proposal (mode 1: ratio only; mode 2: ratio, test and commit)
+  if (mode >= 1) {
+    double lapo_dummy;
+    double vold[3];
+    double lapjo = 0.0;  // not needed for the proposal
+    group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, ws + L.ai[s],
+                     ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lapjo, vold, nullptr, lane);
+    (void)lapo_dummy;
+    if (lane == 0) {
+      int *rs = A.rng + 4 * w;
+      int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+      double deltfi = 2.0 * fbias / delta;
+      double fxop = 1.0;
+      for (int k = 0; k < 3; k++) {
+        double fx = fmin(fabs(vold[k]), deltfi);
+        double ftest = fx * delta * 0.5;
+        fx = copysign(fx, vold[k]);
+        double dx;
+        if (ftest <= rannyu(st4)) {
+          dx = (rannyu(st4) - 0.5) * delta;
+        } else {
+          double r1 = rannyu(st4), r2 = rannyu(st4);
+          dx = (fmax(r1, r2) - 0.5) * delta;
+          if (vold[k] < 0) dx = -dx;
+        }
+        fxop *= (1.0 + fx * dx);
+        xn[k] = x[3 * ie + k] + dx;
+        mv[ML.misc + 2 + k] = dx;
+      }
+      mv[ML.misc + 1] = fxop;
+      rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+    }
+  } else {
+    if (lane < 3) xn[lane] = xnew_in[3 * w + lane];
+  }
+  __syncthreads();
+  // ---- new distances of electron ie
+  double *ren = mv + ML.ren, *rven = mv + ML.rven, *ree = mv + ML.ree, *rvee = mv + ML.rvee;
+  for (int ic = lane; ic < M; ic += G) {
+    double s2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      double d = xn[q] - cst(S.cent)[3 * ic + q];
+      rven[3 * ic + q] = d;
+      s2 += d * d;
+    }
+    ren[ic] = sqrt(s2);
+  }
+  for (int j = lane; j < N; j += G) {
+    if (j == ie) continue;
+    int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
+    double s2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      double xi = (hi == ie) ? xn[q] : x[3 * hi + q];
+      double xj = (lo == ie) ? xn[q] : x[3 * lo + q];
+      rvee[3 * j + q] = xi - xj;
+      s2 += (xi - xj) * (xi - xj);
+    }
+    ree[j] = sqrt(s2);
+  }
+  __syncthreads();
+  // ---- jastrow4e (+ per-electron laplacian)
+  double *fsn = mv + ML.fsn, *fijn_i = mv + ML.fijn_i, *fijn_j = mv + ML.fijn_j,
+         *lapn = mv + ML.lapn_i, *fjn = mv + ML.fjn;
+  const double *ren_o = ws + L.r_en, *rven_o = ws + L.rvec_en;
+  double dsum = 0.0, gix = 0, giy = 0, giz = 0, lapi = 0;
+  for (int j = lane; j < N; j += G) {
+    if (j == ie) {  // e-n diagonal term, on the lane the pair list leaves free
+      T f, g[3], lp;
+      jas_en<T>(S, ren, rven, f, g, lp);
+      fsn[ie] = f;
+      for (int q = 0; q < 3; q++) fijn_i[3 * ie + q] = g[q];
+      lapn[ie] = lp;
+      dsum += (double)f - fso[ie * N + ie];
+      gix += (double)g[0] - fijo[3 * (ie * N + ie)];
+      giy += (double)g[1] - fijo[3 * (ie * N + ie) + 1];
+      giz += (double)g[2] - fijo[3 * (ie * N + ie) + 2];
+      lapi += lp;
+      continue;
+    }
+    int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
+    T rvij[3];
+    for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * j + q];
+    const double *rI = ren, *rvI = rven, *rJ = ren_o + j * M, *rvJ = rven_o + j * M * 3;
+    PairOut<T> po;
+    bool par = (hi < S.nup) || (lo >= S.nup);
+    // moved electron's arrays go to whichever slot (hi or lo) it occupies
+    if (hi == ie) jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rI, rJ, rvI, rvJ, po);
+    else jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rJ, rI, rvJ, rvI, po);
+    // old pair values before the stores to the pending buffers below
+    const double f_old = fso[hi * N + lo];
+    const double gi_old[3] = {fijo[3 * (ie * N + j)], fijo[3 * (ie * N + j) + 1], fijo[3 * (ie * N + j) + 2]};
+    const double gj_old[3] = {fijo[3 * (j * N + ie)], fijo[3 * (j * N + ie) + 1], fijo[3 * (j * N + ie) + 2]};
+    const double fj_old[3] = {fjo[3 * j], fjo[3 * j + 1], fjo[3 * j + 2]};
+    // per-component selects keep po in registers
+    const bool mi = hi == ie;
+    const T g_ie[3] = {mi ? po.gi[0] : po.gj[0], mi ? po.gi[1] : po.gj[1], mi ? po.gi[2] : po.gj[2]};
+    const T g_j[3] = {mi ? po.gj[0] : po.gi[0], mi ? po.gj[1] : po.gi[1], mi ? po.gj[2] : po.gi[2]};
+    T l_ie = mi ? po.li : po.lj;
+    T l_j = mi ? po.lj : po.li;
+    fsn[j] = po.f;
+    for (int q = 0; q < 3; q++) {
+      fijn_i[3 * j + q] = g_ie[q];
+      fijn_j[3 * j + q] = g_j[q];
+    }
+    lapn[j] = l_ie;
+    mv[ML.lapn_i + N + j] = l_j;
+    dsum += (double)po.f - f_old;
+    gix += (double)g_ie[0] - gi_old[0];
+    giy += (double)g_ie[1] - gi_old[1];
+    giz += (double)g_ie[2] - gi_old[2];
+    lapi += l_ie;
+    // other electron's new gradient
+    for (int q = 0; q < 3; q++)
+      fjn[3 * j + q] = fj_old[q] + (double)g_j[q] - gj_old[q];
+  }
+  dsum = gsum<G>(dsum);
+  gix = gsum<G>(gix); giy = gsum<G>(giy); giz = gsum<G>(giz);
+  lapi = gsum<G>(lapi);
+  const double fsumn = ws[L.fsumo] + dsum;
+  double fji[3] = {fjo[3 * ie] + gix, fjo[3 * ie + 1] + giy, fjo[3 * ie + 2] + giz};
+  if (lane == 0) { fjn[3 * ie] = fji[0]; fjn[3 * ie + 1] = fji[1]; fjn[3 * ie + 2] = fji[2]; }
+  double lapjn = lapi + fji[0] * fji[0] + fji[1] * fji[1] + fji[2] * fji[2];
+  if (lane == 0) {
+    const int o = ML.misc + kMvJas + ns + nocc;
+    mv[o] = fji[0]; mv[o + 1] = fji[1]; mv[o + 2] = fji[2]; mv[o + 3] = lapjn;
+    mv[ML.misc + 7] = fsumn;
+  }
+}
+
+template <typename T, int G>
+__global__ QMCW_MOVE void k_move_b(KArgs A, int ie) {
+  QMCW_GROUP_SETUP
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const MLayout &ML = A.ML;
+  double *mv = ss + SL.mv;
+  const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  const int e = s ? ie - S.nup : ie;
+  const double *ren = mv + ML.ren, *rven = mv + ML.rven;
+  // ---- orbitals of the new position (occupied set of the spin)
+  // basis values of the new position: LDS when the launch reserved it (the
+  // two calls keep the LDS pointer visible as such to the compiler)
+  extern __shared__ double dyn_lds[];
+  double *orbe = mv + ML.orbe;
+  if (A.lds_phi)
+    orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * S.NB * 5,
+                      orbe, lane);
+  else
+    orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(ss + SL.phi), orbe, lane);
+  __syncthreads();
+  // ---- dete_update (qmc/dete_mod.f90:114-214)
+  const double *ai = ws + L.ai[s];
+  const double *tocc = ws + L.tocc[s];
+  double *ain = mv + ML.ain, *tn = mv + ML.tn;
+  double rat = 0.0;
+  for (int k = lane; k < ns; k += G) rat += orbe[5 * cst(S.iref[s])[k]] * ai[k + ns * e];
+  rat = gsum<G>(rat);
+  // vi1(i) = sum_k orbe(ref_k) ai(k, i) - delta(i, e); ui(k) = -ai(k,e)/rat
+  double *vi1 = mv + ML.misc + 8;            // [ns]
+  double *vi2 = vi1 + ns;                    // [nocc]
+  sm_vectors<G>(vi1, vi2, orbe, ai, tocc, S.iref[s], ns, nocc, e, lane);
+  __syncthreads();
+  sm_rank1<G>(ain, tn, ai, tocc, vi1, vi2, ns, nocc, e, rat, lane);
+  __syncthreads();
+  double *invn = mv + ML.invn, *detn = mv + ML.detn;
+  for (int u = lane; u < S.ndetu[s]; u += G) {
+    int k = cst(S.ord[s])[u];
+    double *blk = invn + u * S.kmax2;
+    for (int ii = 0; ii < k; ii++)
+      for (int jj = 0; jj < k; jj++)
+        blk[jj + k * ii] = tn[cst(S.rf[s])[u * ns + jj] * nocc + cst(S.exo[s])[u * ns + ii]];
+    detn[u] = small_matinv(blk, k);
+  }
+  __syncthreads();
+  if (lane == 0) mv[ML.misc + 0] = rat;
+}
+
+template <int G>
+__global__ QMCW_MOVE void k_move_c(KArgs A, int ie, int mode, double delta, double fbias,
+                                   double *out, long long *nacc) {
+  QMCW_GROUP_SETUP
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const MLayout &ML = A.ML;
+  double *mv = ss + SL.mv;
+  const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
+  double *orbe = mv + ML.orbe, *ain = mv + ML.ain, *tn = mv + ML.tn;
+  double *invn = mv + ML.invn, *detn = mv + ML.detn;
+  const double rat = mv[ML.misc + 0], fsumn = mv[ML.misc + 7];
+  const int oj = ML.misc + kMvJas + ns + nocc;
+  const double fji[3] = {mv[oj], mv[oj + 1], mv[oj + 2]};
+  const double lapjn = mv[oj + 3];
   double chin;
   {
     const double *invs[2], *dvs[2];
@@ -1178,7 +1445,7 @@ __global__ void k_rotqua(KArgs A) {
   int nq = S.nquad;
   double *quad = A.st + (int64_t)w * A.L.stride + A.L.quad;
   for (int iq = 0; iq < nq; iq++) {
-    double a = S.q0[iq], b = S.q0[nq + iq], c = S.q0[2 * nq + iq];
+    double a = cst(S.q0)[iq], b = cst(S.q0)[nq + iq], c = cst(S.q0)[2 * nq + iq];
     quad[iq] = a * x1 + b * y1 + c * z1;
     quad[nq + iq] = a * x2 + b * y2 + c * z2;
     quad[2 * nq + iq] = a * x3 + b * y3 + c * z3;

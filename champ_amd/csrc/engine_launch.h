@@ -25,6 +25,7 @@ struct OrdKernels {
                 double fbias, double *out, long long *nacc);
   void (*deriv_jas)(const LaunchCtx &);
   void (*deriv_nlpp)(const LaunchCtx &);
+  void (*move_a)(const LaunchCtx &, int ie, int mode, const double *xn, double delta, double fbias);
 };
 
 extern const OrdKernels kOrdKernels2, kOrdKernels3, kOrdKernels4, kOrdKernels5, kOrdKernels6,

@@ -72,6 +72,17 @@ void ratio(const LaunchCtx &c, int ie, int mode, const double *xn, double delta,
            double *out, long long *nacc) {
   QMCW_BY_G(ratio_g, c, ie, mode, xn, delta, fbias, out, nacc);
 }
+template <int G>
+void move_a_g(const LaunchCtx &c, int ie, int mode, const double *xn, double delta, double fbias) {
+  const dim3 grid(c.Wl / (kWave / G)), block(kWave);
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_move_a<double, G, ORD>), grid, block, 0, c.stream, c.A, ie, mode, xn, delta, fbias);
+  else
+    hipLaunchKernelGGL((k_move_a<float, G, ORD>), grid, block, 0, c.stream, c.A, ie, mode, xn, delta, fbias);
+}
+void move_a(const LaunchCtx &c, int ie, int mode, const double *xn, double delta, double fbias) {
+  QMCW_BY_G(move_a_g, c, ie, mode, xn, delta, fbias);
+}
 void deriv_jas(const LaunchCtx &c) { QMCW_BY_G(deriv_jas_g, c); }
 void deriv_nlpp(const LaunchCtx &c) { QMCW_BY_G(deriv_nlpp_g, c); }
 
@@ -79,7 +90,7 @@ void deriv_nlpp(const LaunchCtx &c) { QMCW_BY_G(deriv_nlpp_g, c); }
 
 #ifndef __HIP_DEVICE_COMPILE__   // a host table of host launchers
 extern const OrdKernels QMCW_CAT(kOrdKernels, QMCW_ORD) = {full_jas, full_nlpp, ratio, deriv_jas,
-                                                            deriv_nlpp};
+                                                            deriv_nlpp, move_a};
 #endif
 
 }  // namespace qmcw
