@@ -65,6 +65,8 @@ struct qmcw_handle {
   size_t det_lds_budget = 10240;  // LDS bytes per block for k_full_det's basis values
   int split_move = 0;        // one-electron move as three launches (k_move_a/b/c)
   int split_forced = 0;      // QMCW_SPLIT_MOVE set: the autotuner keeps split_move
+  int lds_y = 1;             // LDS staging of eval_y in the move kernels
+  int ncu = 256;             // compute units of the device (set at create)
   // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
   int dmc_on = 0, nact = 0;
   double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
@@ -569,14 +571,15 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
   }
 }
 template <int G>
-static void launch_move_bc(qmcw_handle *h, const KArgs &A, size_t smem, int ie, int mode,
-                           double delta, double fbias, double *out, long long *nacc) {
+static void launch_move_bc(qmcw_handle *h, const KArgs &A, size_t smem_b, size_t smem_c, int ie,
+                           int mode, double delta, double fbias, double *out, long long *nacc) {
   const dim3 grid(h->Wl / (kWave / G)), block(kWave);
   if (h->precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_move_b<double, G>), grid, block, smem, h->stream, A, ie);
+    hipLaunchKernelGGL((k_move_b<double, G>), grid, block, smem_b, h->stream, A, ie);
   else
-    hipLaunchKernelGGL((k_move_b<float, G>), grid, block, smem, h->stream, A, ie);
-  hipLaunchKernelGGL((k_move_c<G>), grid, block, 0, h->stream, A, ie, mode, delta, fbias, out, nacc);
+    hipLaunchKernelGGL((k_move_b<float, G>), grid, block, smem_b, h->stream, A, ie);
+  hipLaunchKernelGGL((k_move_c<G>), grid, block, smem_c, h->stream, A, ie, mode, delta, fbias, out,
+                     nacc);
 }
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
                          double fbias, double *out, long long *nacc = nullptr) {
@@ -586,12 +589,24 @@ static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, dou
                      (h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float));
   KArgs A = h->A;
   A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;
+  const size_t phi_b = A.lds_phi ? (lds + 7) / 8 * 8 : 0;
+  // eval_y's determinant values and alpha inverses staged in LDS after the
+  // basis values, when the two together stay within 48 KB per block and
+  // still let every block of the launch be resident at once (160 KB of LDS
+  // per compute unit)
+  const size_t y_b = (size_t)(kWave / G) *
+                     y_lds_doubles(A.S.ndetu[0], A.S.ndetu[1], A.S.kmax2) * sizeof(double);
+  const size_t blocks_per_cu = ((size_t)h->Wl / (kWave / G) + h->ncu - 1) / h->ncu;
+  const size_t lds_cap = std::min<size_t>(48 * 1024, (160 * 1024) / std::max<size_t>(1, blocks_per_cu));
+  const bool ly = h->use_lds && h->lds_y && phi_b + y_b <= lds_cap;
+  A.lds_y = ly ? y_lds_doubles(A.S.ndetu[0], A.S.ndetu[1], A.S.kmax2) : 0;
+  A.lds_y_off = (int)(phi_b / 8);
   LaunchCtx c = make_ctx(h, A, G);
-  c.smem_ratio = A.lds_phi ? lds : 0;
+  c.smem_ratio = phi_b + (ly ? y_b : 0);
   prof_begin(h, QMCW_PROF_RATIO);
   if (h->split_move) {
     ord_kernels(A.S.nordc).move_a(c, ie, mode, xn, delta, fbias);
-    DISPATCH_GW(G, launch_move_bc, h, A, c.smem_ratio, ie, mode, delta, fbias, out, nacc);
+    DISPATCH_GW(G, launch_move_bc, h, A, phi_b, c.smem_ratio, ie, mode, delta, fbias, out, nacc);
   } else {
     ord_kernels(A.S.nordc).ratio(c, ie, mode, xn, delta, fbias, out, nacc);
   }
@@ -982,6 +997,13 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_CHUNK")) h->chunk_override = atoi(e);
   if (const char *e = getenv("QMCW_DET_LDS")) h->det_lds_budget = (size_t)atol(e);
   if (const char *e = getenv("QMCW_SPLIT_MOVE")) { h->split_move = atoi(e) != 0; h->split_forced = 1; }
+  if (const char *e = getenv("QMCW_NO_LDS_Y")) h->lds_y = atoi(e) == 0;
+  {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      h->ncu = n;
+  }
   if (hipStreamSynchronize(h->stream) != hipSuccess) { qmcw_destroy(h); return fail(nullptr, QMCW_ERR_HIP, "init sync"); }
   *out = h;
   return QMCW_OK;

@@ -23,6 +23,8 @@ struct KArgs {
   int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
   int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
   int lds_det;       // k_full_det keeps a chunk's basis values in LDS
+  int lds_y;         // doubles per group of the LDS staging of eval_y in the move kernels (0: none)
+  int lds_y_off;     // its offset (doubles) in the dynamic LDS, after the basis values
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
 };
@@ -55,10 +57,11 @@ __device__ __forceinline__ float bf16r(float x) {
   return __uint_as_float(u & 0xFFFF0000u);
 }
 
-// orbital value, gradient and laplacian o[0..4] = sum_b coefT[b][lab] phi_b[0..4]
-// (phi with stride 5); BF16: inputs rounded to bfloat16
+// orbital value, gradient and laplacian o[0..4] = sum_b coef[b * NO + lab] phi_b[0..4]
+// (coef: coefT with NO = all orbitals, or an occupied-orbital table with NO =
+// nocc; phi with stride 5); BF16: inputs rounded to bfloat16
 template <bool BF16, typename T>
-__device__ __forceinline__ void contract5(const double *__restrict__ coefT, int NO, int NB, int lab,
+__device__ __forceinline__ void contract5(cptr<double> coefT, int NO, int NB, int lab,
                                           const T *__restrict__ ph, double *o) {
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
 #pragma unroll 4
@@ -78,7 +81,7 @@ __device__ __forceinline__ void contract5(const double *__restrict__ coefT, int 
   o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
 }
 template <typename T>
-__device__ __forceinline__ void contract5(int bf16, const double *coefT, int NO, int NB, int lab,
+__device__ __forceinline__ void contract5(int bf16, cptr<double> coefT, int NO, int NB, int lab,
                                           const T *ph, double *o) {
   if (bf16) contract5<true, T>(coefT, NO, NB, lab, ph, o);
   else contract5<false, T>(coefT, NO, NB, lab, ph, o);
@@ -232,13 +235,13 @@ __device__ __forceinline__ void full_orbitals(const KArgs &A, double *ws, double
       const int i = i0 + ii, s = (i < S.nup) ? 0 : 1;
       double *o = oc + ((int64_t)i * noccmax + j) * 5;
       if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
-      contract5<T>(A.bf16, S.coefT, NO, S.NB, cst(S.occ[s])[j], phi + (int64_t)ii * S.NB * 5, o);
+      contract5<T>(A.bf16, cst(S.coef_occ[s]), S.nocc[s], S.NB, j, phi + (int64_t)ii * S.NB * 5, o);
     }
     if (A.full_orb) {
       for (int t = lane; t < nch * NO; t += G) {
         const int ii = t / NO, lab = t % NO;
         const int i = i0 + ii;
-        contract5<T>(A.bf16, S.coefT, NO, S.NB, lab, phi + (int64_t)ii * S.NB * 5,
+        contract5<T>(A.bf16, cst(S.coefT), NO, S.NB, lab, phi + (int64_t)ii * S.NB * 5,
                      orbf + ((int64_t)i * NO + lab) * 5);
       }
     }
@@ -262,6 +265,73 @@ __device__ __forceinline__ void group_gproj(const DevSys &S, double *__restrict_
     for (int j = 0; j < nocc; j++) acc += gc[j * ns] * co[j];
     gp[t] = bf16 ? (double)bf16r((float)acc) : acc;
   }
+}
+
+// group_eval_y with the walker's determinant values and alpha inverses of
+// both spins staged in LDS first (one batched copy), so that the index
+// chains of the chi, cup and Y sums end in LDS reads.  lds: this group's
+// region of y_lds_doubles(S) doubles (derived from the kernel's dynamic LDS
+// so that the compiler addresses it as LDS).
+__host__ __device__ inline int y_lds_doubles(int ndetu0, int ndetu1, int kmax2) {
+  return (ndetu0 + ndetu1) * (2 + kmax2);
+}
+template <int G>
+__device__ __forceinline__ double group_eval_y_lds(const DevSys &S, const double *const inv[2],
+                                                   const double *const detv[2], double *const y[2],
+                                                   double *lds, int lane) {
+  const int nu0 = S.ndetu[0], nu1 = S.ndetu[1], k2 = S.kmax2;
+  double *dv0 = lds, *dv1 = dv0 + nu0, *iv0 = dv1 + nu1, *iv1 = iv0 + nu0 * k2, *cup = iv1 + nu1 * k2;
+  {
+    // segments: detv[0], detv[1], inv[0], inv[1]; four loads in flight per lane
+    const int st1 = nu0, st2 = st1 + nu1, st3 = st2 + nu0 * k2, n = st3 + nu1 * k2;
+    for (int base = 0; base < n; base += 4 * G) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = base + lane + u * G;
+        const double *src = e < st1 ? detv[0] + e : e < st2 ? detv[1] + (e - st1)
+                          : e < st3 ? inv[0] + (e - st2) : inv[1] + (e - st3);
+        v[u] = e < n ? *src : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = base + lane + u * G;
+        if (e < n) lds[e] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  double chi = 0.0;
+#pragma unroll 4
+  for (int d = lane; d < S.ndet; d += G)
+    chi += cst(S.cdd)[d] * dv0[cst(S.iwdet[0])[d]] * dv1[cst(S.iwdet[1])[d]];
+  chi = gsum<G>(chi);
+  const double chii = 1.0 / chi;
+  for (int t = lane; t < nu0 + nu1; t += G) {
+    const int s = t < nu0 ? 0 : 1, u = s ? t - nu0 : t;
+    const cptr<int> iu = cst(S.uiu[s]), id = cst(S.uid[s]);
+    const cptr<double> cf = cst(S.ucf[s]);
+    double cu = 0.0;
+    const int q1 = cst(S.udet_off[s])[u + 1];
+#pragma unroll 4
+    for (int q = cst(S.udet_off[s])[u]; q < q1; q++) cu += cf[q] * dv0[iu[q]] * dv1[id[q]];
+    cup[t] = cu;
+  }
+  __syncthreads();
+  for (int s = 0; s < 2; s++) {
+    const cptr<int> off = cst(S.yent_off[s]), yu = cst(S.yent_u[s]), yp = cst(S.yent_pos[s]);
+    const double *cus = cup + (s ? nu0 : 0), *iv = s ? iv1 : iv0;
+    double *ys = y[s];
+    for (int e = lane; e < S.nocc[s] * S.ns[s]; e += G) {
+      double acc = 0.0;
+      const int q1 = off[e + 1];
+#pragma unroll 4
+      for (int q = off[e]; q < q1; q++) acc += cus[yu[q]] * iv[yp[q]];
+      ys[e] = acc * chii;
+    }
+  }
+  __syncthreads();
+  return chi;
 }
 
 // G = A^-1 - T Y A^-1 on reference rows, Y A^-1 on the others
@@ -738,7 +808,7 @@ __device__ __forceinline__ void orbitals_1e(const KArgs &A, int s, int nocc, con
   }
   __syncthreads();
   for (int j = lane; j < nocc; j += G)
-    contract5<T>(A.bf16, S.coefT, S.NO, S.NB, cst(S.occ[s])[j], phi, orbe + 5 * j);
+    contract5<T>(A.bf16, cst(S.coef_occ[s]), nocc, S.NB, j, phi, orbe + 5 * j);
 }
 
 // ---------------------------------------------------------------------------
@@ -926,7 +996,12 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     invs[s] = invn; dvs[s] = detn; ys[s] = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
     invs[1 - s] = ws + L.inv[1 - s]; dvs[1 - s] = ws + L.detv[1 - s];
     ys[1 - s] = mv + ML.yn + (s ? 0 : S.nocc[0] * S.ns[0]);
-    chin = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
+    if (A.lds_y) {
+      extern __shared__ double dyn_lds[];
+      chin = group_eval_y_lds<G>(S, invs, dvs, ys, dyn_lds + A.lds_y_off + (int64_t)grp * A.lds_y, lane);
+    } else {
+      chin = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
+    }
   }
   __syncthreads();
   const double *yn = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
@@ -1185,7 +1260,12 @@ __global__ QMCW_MOVE void k_move_c(KArgs A, int ie, int mode, double delta, doub
     invs[s] = invn; dvs[s] = detn; ys[s] = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
     invs[1 - s] = ws + L.inv[1 - s]; dvs[1 - s] = ws + L.detv[1 - s];
     ys[1 - s] = mv + ML.yn + (s ? 0 : S.nocc[0] * S.ns[0]);
-    chin = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
+    if (A.lds_y) {
+      extern __shared__ double dyn_lds[];
+      chin = group_eval_y_lds<G>(S, invs, dvs, ys, dyn_lds + A.lds_y_off + (int64_t)grp * A.lds_y, lane);
+    } else {
+      chin = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
+    }
   }
   __syncthreads();
   const double *yn = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
