@@ -484,6 +484,7 @@ static LaunchCtx make_ctx(const qmcw_handle *h, const KArgs &A, int G) {
   c.Wl = h->Wl;
   c.precision = h->precision;
   c.smem_ratio = 0;
+  c.smem_nlpp = 0;
   return c;
 }
 
@@ -538,7 +539,22 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     det_smem = per_e;
   }
   const OrdKernels &K = ord_kernels(A.S.nordc);
-  const LaunchCtx cj = make_ctx(h, A, h->Gk[KC_JAS]), cn = make_ctx(h, A, h->Gk[KC_NLPP]);
+  const LaunchCtx cj = make_ctx(h, A, h->Gk[KC_JAS]);
+  // k_full_nlpp's per-lane point scratch (4M distances, N values of U) in
+  // LDS when all of the launch's blocks stay resident with it
+  LaunchCtx cn = make_ctx(h, A, h->Gk[KC_NLPP]);
+  {
+    const int M = A.S.M, N = A.S.N;
+    const size_t tsz = h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float);
+    const int per_lane = 4 * M + (int)((N * tsz + 7) / 8);
+    const size_t blk = (size_t)kWave * per_lane * sizeof(double);
+    const size_t bpc = ((size_t)h->Wl / (kWave / h->Gk[KC_NLPP]) + h->ncu - 1) / h->ncu;
+    const size_t cap = std::min<size_t>(48 * 1024, (160 * 1024) / std::max<size_t>(1, bpc));
+    if (h->use_lds && blk <= cap) {
+      cn.A.lds_nlpp = per_lane;
+      cn.smem_nlpp = blk;
+    }
+  }
   prof_begin(h, QMCW_PROF_FULL_JAS);
   K.full_jas(cj);
   prof_end(h);

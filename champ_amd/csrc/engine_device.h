@@ -353,7 +353,7 @@ __device__ __forceinline__ T jas_b_val(const DevSys &S, bool parallel_spin, T u)
 // e-e-n terms of all pairs (i, j).  bt[j * M + ic] = 1 - k scaled(r_j,ic)
 // of the current configuration; uq is per-lane scratch of N values.
 template <typename T, int ORD>
-__device__ T jas_point_val(const DevSys &S, int i, const double *xn, const double *x,
+__device__ __forceinline__ T jas_point_val(const DevSys &S, int i, const double *xn, const double *x,
                            const double *rnew, const T *bt, T *uq) {
   const int N = S.N, M = S.M;
   const T k = T(S.scalek);

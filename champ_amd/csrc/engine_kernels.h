@@ -25,6 +25,7 @@ struct KArgs {
   int lds_det;       // k_full_det keeps a chunk's basis values in LDS
   int lds_y;         // doubles per group of the LDS staging of eval_y in the move kernels (0: none)
   int lds_y_off;     // its offset (doubles) in the dynamic LDS, after the basis values
+  int lds_nlpp;      // doubles per lane of k_full_nlpp's point scratch in LDS (0: global)
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
 };
@@ -569,6 +570,62 @@ void k_full_det(KArgs A) {
   if (S.nloc > 0) group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
 }
 
+// Pseudopotential quadrature points of the in-range pairs plist[0..npair)
+// taken by this lane: sum of w exp(dJ) v P_l (Psi_D ratio).  rnew: 4M
+// doubles and uq: N values of per-lane scratch (LDS or global).
+template <typename T, int G, int ORD>
+__device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, const double *ss,
+                                              int npair, const int *plist, double *rnew, T *uq,
+                                              int lane) {
+  const DevSys &S = A.S;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const int N = S.N, M = S.M;
+  const double *x = ws + L.x, *rven = ws + L.rvec_en, *ren = ws + L.r_en, *fso = ws + L.fso;
+  const double *vnl = ss + SL.vnl;
+  const T *bt = reinterpret_cast<const T *>(ss + SL.bt);
+  const double *gp = ss + SL.gproj;
+  const double *quad = ws + L.quad;
+  const int nq = S.nquad;
+  double *rvnew = rnew + M;
+  double vpsp = 0.0;
+  for (int t = lane; t < npair * nq; t += G) {
+    const int iq = t % nq, p = plist[t / nq], ic = p % M, i = p / M;
+    const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
+    const int ict = cst(S.iwctype)[ic];
+    const double rei = ren[p];
+    const double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
+    const double *rv0 = rven + 3 * p;
+    const double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
+    const double xn[3] = {rei * qx + cst(S.cent)[3 * ic], rei * qy + cst(S.cent)[3 * ic + 1],
+                          rei * qz + cst(S.cent)[3 * ic + 2]};
+    for (int jc = 0; jc < M; jc++) {
+      const double dx = xn[0] - cst(S.cent)[3 * jc], dy = xn[1] - cst(S.cent)[3 * jc + 1],
+                   dz = xn[2] - cst(S.cent)[3 * jc + 2];
+      rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
+      rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    // determinant ratio of moving electron i to the point
+    double rd = 0.0;
+    const double *gpi = gp + (int64_t)i * S.NB;
+    for (int sh = 0; sh < S.nshell; sh++) {
+      const int jc = cst(S.sh_cent)[sh];
+      rd += (double)shell_dot<T, double>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1],
+                                         (T)rvnew[3 * jc + 2], (T)rnew[jc], gpi);
+    }
+    // Jastrow value change
+    double fold = fso[i * N + i];
+    for (int jj = 0; jj < N; jj++)
+      if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
+    const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
+    double vpot = 0.0;
+    for (int l = 1; l <= cst(S.npotd)[ict]; l++)
+      if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
+    vpsp += cst(S.q0)[3 * nq + iq] * exp(delta) * vpot * rd;
+  }
+  return vpsp;
+}
+
 // FP32 instances are held to 128 VGPRs (four waves per SIMD)
 template <typename T, int G, int ORD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : QMCW_WPE, 8)))
@@ -615,43 +672,15 @@ void k_full_nlpp(KArgs A) {
     int *plist = reinterpret_cast<int *>(ss + SL.plist);
     const int npair = group_compact<G>(vnl + QMCW_MAX_LPOT, QMCW_MAX_LPOT + 1, N * M, plist, lane);
     __syncthreads();
-    const double *gp = ss + SL.gproj;
-    T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
-    const double *quad = ws + L.quad;
-    const int nq = S.nquad;
-    for (int t = lane; t < npair * nq; t += G) {
-      const int iq = t % nq, p = plist[t / nq], ic = p % M, i = p / M;
-      const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
-      int ict = cst(S.iwctype)[ic];
-      double rei = ren[p];
-      double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
-      const double *rv0 = rven + 3 * p;
-      double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
-      double xn[3] = {rei * qx + cst(S.cent)[3 * ic], rei * qy + cst(S.cent)[3 * ic + 1],
-                      rei * qz + cst(S.cent)[3 * ic + 2]};
-      double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
-      for (int jc = 0; jc < M; jc++) {
-        double dx = xn[0] - cst(S.cent)[3 * jc], dy = xn[1] - cst(S.cent)[3 * jc + 1], dz = xn[2] - cst(S.cent)[3 * jc + 2];
-        rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
-        rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
-      }
-      // determinant ratio of moving electron i to the point
-      double rd = 0.0;
-      const double *gpi = gp + (int64_t)i * S.NB;
-      for (int sh = 0; sh < S.nshell; sh++) {
-        int jc = cst(S.sh_cent)[sh];
-        rd += (double)shell_dot<T, double>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1],
-                                           (T)rvnew[3 * jc + 2], (T)rnew[jc], gpi);
-      }
-      // Jastrow value change
-      double fold = fso[i * N + i];
-      for (int jj = 0; jj < N; jj++)
-        if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
-      const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
-      double vpot = 0.0;
-      for (int l = 1; l <= cst(S.npotd)[ict]; l++)
-        if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
-      vpsp += cst(S.q0)[3 * nq + iq] * exp(delta) * vpot * rd;
+    // per-lane distances of the point and U of the other electrons: LDS
+    // when the launch reserved it
+    if (A.lds_nlpp) {
+      extern __shared__ double dyn_lds[];
+      double *lb = dyn_lds + (int64_t)threadIdx.x * A.lds_nlpp;
+      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, lb, reinterpret_cast<T *>(lb + 4 * M), lane);
+    } else {
+      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, ss + SL.qd + (int64_t)lane * 4 * M,
+                                    reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N), lane);
     }
     vpsp = gsum<G>(vpsp);
   }

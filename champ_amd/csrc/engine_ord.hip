@@ -29,9 +29,9 @@ template <int G>
 void full_nlpp_g(const LaunchCtx &c) {
   const dim3 grid(c.Wl / (kWave / G)), block(kWave);
   if (c.precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_full_nlpp<double, G, ORD>), grid, block, 0, c.stream, c.A);
+    hipLaunchKernelGGL((k_full_nlpp<double, G, ORD>), grid, block, c.smem_nlpp, c.stream, c.A);
   else
-    hipLaunchKernelGGL((k_full_nlpp<float, G, ORD>), grid, block, 0, c.stream, c.A);
+    hipLaunchKernelGGL((k_full_nlpp<float, G, ORD>), grid, block, c.smem_nlpp, c.stream, c.A);
 }
 template <int G>
 void ratio_g(const LaunchCtx &c, int ie, int mode, const double *xn, double delta, double fbias,
