@@ -128,7 +128,6 @@ struct DLayout {
   int64_t dpe_raw;           // [N*M][nslot]
   int64_t dpe_slot, dpe;     // [nslot], [npj]
   int64_t pnl;               // [N][NO] psp_nonloc_orb
-  int64_t phq;               // [64][NB] per-lane quadrature-point basis values
   int64_t btd, uqd;          // [N][M] e-e-n variable B(r_j,ic), [64][N] per-lane U of a point
   int64_t B;                 // [N][NO] orbital-resolved local energy
   int64_t tfull[2], dt[2];   // [ns][NO]

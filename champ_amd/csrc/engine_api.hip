@@ -485,6 +485,7 @@ static LaunchCtx make_ctx(const qmcw_handle *h, const KArgs &A, int G) {
   c.precision = h->precision;
   c.smem_ratio = 0;
   c.smem_nlpp = 0;
+  c.smem_dnlpp = 0;
   return c;
 }
 
@@ -553,6 +554,13 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     if (h->use_lds && blk <= cap) {
       cn.A.lds_nlpp = per_lane;
       cn.smem_nlpp = blk;
+    }
+    // k_deriv_nlpp: the same plus N doubles per lane
+    const int per_lane_d = per_lane + N;
+    const size_t blk_d = (size_t)kWave * per_lane_d * sizeof(double);
+    if (derivs && h->use_lds && blk_d <= cap) {
+      cn.A.lds_dnlpp = per_lane_d;
+      cn.smem_dnlpp = blk_d;
     }
   }
   prof_begin(h, QMCW_PROF_FULL_JAS);
@@ -964,7 +972,6 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     DL.gvalue = take(S.npj); DL.gjas = take((int64_t)S.npj * N * 3); DL.d2g = take(S.npj);
     DL.pbc = take(NM * NB); DL.dpe_raw = take(NM * S.nslot);
     DL.dpe_slot = take(S.nslot); DL.dpe = take(S.npj); DL.pnl = take((int64_t)N * NO);
-    DL.phq = take((int64_t)kWave * NB);
     DL.btd = take(NM); DL.uqd = take((int64_t)kWave * N);
     DL.B = take((int64_t)N * NO);
     for (int sp = 0; sp < 2; sp++) {

@@ -314,6 +314,74 @@ __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
   }
 }
 
+// The (electron, centre) pairs plist[kp] taken by this lane, whole: the
+// weighted basis sums (pbc rows) and the slot pieces of dpe (dpe_raw rows)
+// over the pair's quadrature points.  rnew: 4M doubles, uq: N values of the
+// kernel's T, uqd: N doubles of per-lane scratch (LDS or global).
+template <typename T, int G, int ORD>
+__device__ __forceinline__ void deriv_nlpp_pairs(const KArgs &A, const double *ws, double *ss, int npair,
+                                                 const int *plist, double *rnew, T *uq, double *uqd,
+                                                 int lane) {
+  const DevSys &S = A.S;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const DLayout &DL = A.DL;
+  const int N = S.N, M = S.M, NB = S.NB, nsl = S.nslot, nq = S.nquad;
+  const double *x = ws + L.x, *ren = ws + L.r_en, *rven = ws + L.rvec_en, *fso = ws + L.fso;
+  const double *vnl = ss + SL.vnl, *quad = ws + L.quad, *gp = ss + SL.gproj, *btd = ss + DL.btd;
+  const T *bt = reinterpret_cast<const T *>(ss + SL.bt);    // filled by k_full_nlpp
+  double *pbc = ss + DL.pbc, *draw = ss + DL.dpe_raw;
+  double *rvnew = rnew + M;
+  for (int kp = lane; kp < npair; kp += G) {
+    const int p = plist[kp];
+    double *pbrow = pbc + (int64_t)p * NB;
+    double *drow = draw + (int64_t)p * nsl;
+    const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
+    for (int b = 0; b < NB; b++) pbrow[b] = 0.0;
+    const int i = p / M, ic = p % M, ict = cst(S.iwctype)[ic];
+    const double *gpi = gp + (int64_t)i * NB;
+    const double rei = ren[p];
+    const SlotAcc acc{drow, 1};
+    double fold = fso[i * N + i];
+    for (int jj = 0; jj < N; jj++)
+      if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
+    double wsum = 0.0;
+    for (int iq = 0; iq < nq; iq++) {
+      const double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
+      const double *rv0 = rven + 3 * p;
+      const double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
+      const double xn[3] = {rei * qx + cst(S.cent)[3 * ic], rei * qy + cst(S.cent)[3 * ic + 1],
+                            rei * qz + cst(S.cent)[3 * ic + 2]};
+      for (int jc = 0; jc < M; jc++) {
+        const double dx = xn[0] - cst(S.cent)[3 * jc], dy = xn[1] - cst(S.cent)[3 * jc + 1],
+                     dz = xn[2] - cst(S.cent)[3 * jc + 2];
+        rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
+        rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
+      }
+      // Jastrow factor and channel sum first, then one pass over the shells
+      // forms both the determinant ratio and the weighted basis sums
+      const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
+      double vpot = 0.0;
+      for (int l = 1; l <= cst(S.npotd)[ict]; l++)
+        if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
+      const double fac = cst(S.q0)[3 * nq + iq] * exp(delta) * vpot;
+      double rat = 0.0;
+      for (int sh = 0; sh < S.nshell; sh++) {
+        const int jc = cst(S.sh_cent)[sh];
+        shell_val_axpy<T>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1], (T)rvnew[3 * jc + 2],
+                          (T)rnew[jc], gpi, fac, pbrow, rat);
+      }
+      const double wr = fac * rat;
+      wsum += wr;
+      en_slots<false>(S, rnew, rvnew, wr, acc);
+      point_pair_slots<ORD>(S, i, xn, x, rnew, btd, uqd, wr, acc);
+    }
+    // the same terms at the current position, weighted by the summed weight
+    en_slots<false>(S, ren + i * M, rven + i * M * 3, -wsum, acc);
+    point_pair_slots<ORD>(S, i, x + 3 * i, x, ren + i * M, btd, uqd, -wsum, acc);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Pseudopotential pieces of the derivatives (qmc/vmc/nonloc.f90:184-197,
 // qmc/fit/deriv_nonloc_pot.f90:59-64): psp_nonloc_orb(i, q) over all
@@ -333,17 +401,11 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
   const DLayout &DL = A.DL;
-  const double *x = ws + L.x;
-  const double *ren = ws + L.r_en, *rven = ws + L.rvec_en;
-  const double *fso = ws + L.fso;
+  const double *ren = ws + L.r_en;
   const double *vnl = ss + SL.vnl;
-  const double *quad = ws + L.quad;
-  const double *gp = ss + SL.gproj;
-  const T *bt = reinterpret_cast<const T *>(ss + SL.bt);    // filled by k_full_nlpp
   T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
   double *pbc = ss + DL.pbc, *draw = ss + DL.dpe_raw;
   double *btd = ss + DL.btd, *uqd = ss + DL.uqd + (int64_t)lane * N;
-  const int nq = S.nquad;
   for (int p = lane; p < N * M; p += G) {
     double bu, t1, t2;
     scaled<double>(S.isc, S.scalek, ren[p], bu, t1, t2);
@@ -355,55 +417,15 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   int *plist = reinterpret_cast<int *>(ss + SL.plist);
   const int npair = group_compact<G>(vnl + QMCW_MAX_LPOT, QMCW_MAX_LPOT + 1, N * M, plist, lane);
   __syncthreads();
-  for (int kp = lane; kp < npair; kp += G) {
-    const int p = plist[kp];
-    double *pbrow = pbc + (int64_t)p * NB;
-    double *drow = draw + (int64_t)p * nsl;
-    const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
-    for (int b = 0; b < NB; b++) pbrow[b] = 0.0;
-    const int i = p / M, ic = p % M, ict = cst(S.iwctype)[ic];
-    const double *gpi = gp + (int64_t)i * NB;
-    const double rei = ren[p];
-    double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
-    T *phq = reinterpret_cast<T *>(ss + DL.phq + (int64_t)lane * NB);
-    const SlotAcc acc{drow, 1};
-    double wsum = 0.0;
-    for (int iq = 0; iq < nq; iq++) {
-      const double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
-      const double *rv0 = rven + 3 * p;
-      const double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
-      const double xn[3] = {rei * qx + cst(S.cent)[3 * ic], rei * qy + cst(S.cent)[3 * ic + 1],
-                            rei * qz + cst(S.cent)[3 * ic + 2]};
-      for (int jc = 0; jc < M; jc++) {
-        double dx = xn[0] - cst(S.cent)[3 * jc], dy = xn[1] - cst(S.cent)[3 * jc + 1], dz = xn[2] - cst(S.cent)[3 * jc + 2];
-        rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
-        rnew[jc] = (jc == ic) ? rei : sqrt(dx * dx + dy * dy + dz * dz);
-      }
-      for (int b = 0; b < NB; b++) phq[b] = T(0);
-      for (int sh = 0; sh < S.nshell; sh++) {
-        const int jc = cst(S.sh_cent)[sh];
-        eval_shell<T, 1>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1], (T)rvnew[3 * jc + 2],
-                         (T)rnew[jc], 0, phq);
-      }
-      double rat = 0.0;
-      for (int b = 0; b < NB; b++) rat += gpi[b] * (double)phq[b];
-      double fold = fso[i * N + i];
-      for (int jj = 0; jj < N; jj++)
-        if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
-      const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
-      double vpot = 0.0;
-      for (int l = 1; l <= cst(S.npotd)[ict]; l++)
-        if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
-      const double fac = cst(S.q0)[3 * nq + iq] * exp(delta) * vpot;
-      for (int b = 0; b < NB; b++) pbrow[b] += fac * (double)phq[b];
-      const double wr = fac * rat;
-      wsum += wr;
-      en_slots<false>(S, rnew, rvnew, wr, acc);
-      point_pair_slots<ORD>(S, i, xn, x, rnew, btd, uqd, wr, acc);
-    }
-    // the same terms at the current position, weighted by the summed weight
-    en_slots<false>(S, ren + i * M, rven + i * M * 3, -wsum, acc);
-    point_pair_slots<ORD>(S, i, x + 3 * i, x, ren + i * M, btd, uqd, -wsum, acc);
+  // per-lane distances of the points and e-e-n variables of the other
+  // electrons: LDS when the launch reserved it
+  if (A.lds_dnlpp) {
+    extern __shared__ double dyn_lds[];
+    double *lb = dyn_lds + (int64_t)threadIdx.x * A.lds_dnlpp;
+    T *uql = reinterpret_cast<T *>(lb + 4 * M);
+    deriv_nlpp_pairs<T, G, ORD>(A, ws, ss, npair, plist, lb, uql, lb + A.lds_dnlpp - N, lane);
+  } else {
+    deriv_nlpp_pairs<T, G, ORD>(A, ws, ss, npair, plist, ss + SL.qd + (int64_t)lane * 4 * M, uq, uqd, lane);
   }
   __syncthreads();
   // per-electron basis sums over the centres in range, kept in the row of

@@ -624,6 +624,46 @@ __device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const GP *gp)
   return acc * g;
 }
 
+// Values of shell s's basis functions at (x, y, z), r, accumulated in FP64:
+// rat += sum gp[b] phi_b and pb[b] += fac phi_b (the pseudopotential's
+// determinant ratio and weighted basis sums in one pass, no stored values).
+template <typename T, int L>
+__device__ __forceinline__ void shell_axpy_l(T x, T y, T z, T g, cptr<int> bas, const double *gp,
+                                             double fac, double *pb, double &rat) {
+  T Sv[2 * L + 1], Sg[2 * L + 1][3];
+  solid_harm<T>(L, x, y, z, Sv, Sg);
+#pragma unroll
+  for (int q = 0; q < 2 * L + 1; q++) {
+    const int b = bas[q];
+    if (b >= 0) {
+      const double v = (double)(Sv[q] * g);
+      rat += gp[b] * v;
+      pb[b] += fac * v;
+    }
+  }
+}
+template <typename T>
+__device__ __forceinline__ void shell_val_axpy(const DevSys &S, int s, T x, T y, T z, T r,
+                                               const double *gp, double fac, double *pb, double &rat) {
+  const int l = cst(S.sh_l)[s], a = cst(S.sh_a)[s];
+  const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
+  T g = 0;
+  const cptr<double> pz = cst(S.prim_zeta), pc = cst(S.prim_coef);
+  for (int p = cst(S.sh_prim_off)[s]; p < cst(S.sh_prim_off)[s + 1]; p++) {
+    const T zeta = T(pz[p]), d = T(pc[p]);
+    g += d * exp(gauss ? -zeta * r * r : -zeta * r);
+  }
+  for (int q = 0; q < a; q++) g *= r;
+  const cptr<int> bas = cst(S.sh_bas) + s * kMaxShellM;
+  switch (l) {
+    case 0: shell_axpy_l<T, 0>(x, y, z, g, bas, gp, fac, pb, rat); break;
+    case 1: shell_axpy_l<T, 1>(x, y, z, g, bas, gp, fac, pb, rat); break;
+    case 2: shell_axpy_l<T, 2>(x, y, z, g, bas, gp, fac, pb, rat); break;
+    case 3: shell_axpy_l<T, 3>(x, y, z, g, bas, gp, fac, pb, rat); break;
+    default: shell_axpy_l<T, 4>(x, y, z, g, bas, gp, fac, pb, rat); break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // CHAMP's pivoted Gauss-Jordan (qmc/vmc/matinv.f90) on a column-major k x k
 // block held by one lane.  Singular -> det = 0, inverse zeroed.

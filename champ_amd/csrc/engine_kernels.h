@@ -26,6 +26,7 @@ struct KArgs {
   int lds_y;         // doubles per group of the LDS staging of eval_y in the move kernels (0: none)
   int lds_y_off;     // its offset (doubles) in the dynamic LDS, after the basis values
   int lds_nlpp;      // doubles per lane of k_full_nlpp's point scratch in LDS (0: global)
+  int lds_dnlpp;     // doubles per lane of k_deriv_nlpp's point scratch in LDS (0: global)
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
 };
@@ -1077,8 +1078,8 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
 // carry everything between them; the Jastrow sums that (c) needs go to
 // ML.misc + kMvJas + {0..3} (fji, lap ln J).
 template <typename T, int G, int ORD>
-__global__ QMCW_MOVE void k_move_a(KArgs A, int ie, int mode, const double *xnew_in, double delta,
-                                   double fbias) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : QMCW_MOVE_WPE, 8)))
+void k_move_a(KArgs A, int ie, int mode, const double *xnew_in, double delta, double fbias) {
   QMCW_GROUP_SETUP
   const int N = S.N, M = S.M, noccmax = S.noccmax;
   const WLayout &L = A.L;

@@ -17,6 +17,7 @@ struct LaunchCtx {
   int precision;        // QMCW_FP64 / QMCW_MIXED / QMCW_MIXED_BF16
   size_t smem_ratio;    // dynamic LDS of k_ratio_1e (0 when A.lds_phi == 0)
   size_t smem_nlpp;     // dynamic LDS of k_full_nlpp (0 when A.lds_nlpp == 0)
+  size_t smem_dnlpp;    // dynamic LDS of k_deriv_nlpp (0 when A.lds_dnlpp == 0)
 };
 
 struct OrdKernels {

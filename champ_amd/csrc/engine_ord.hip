@@ -53,9 +53,9 @@ template <int G>
 void deriv_nlpp_g(const LaunchCtx &c) {
   const dim3 grid(c.Wl / (kWave / G)), block(kWave);
   if (c.precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_deriv_nlpp<double, G, ORD>), grid, block, 0, c.stream, c.A);
+    hipLaunchKernelGGL((k_deriv_nlpp<double, G, ORD>), grid, block, c.smem_dnlpp, c.stream, c.A);
   else
-    hipLaunchKernelGGL((k_deriv_nlpp<float, G, ORD>), grid, block, 0, c.stream, c.A);
+    hipLaunchKernelGGL((k_deriv_nlpp<float, G, ORD>), grid, block, c.smem_dnlpp, c.stream, c.A);
 }
 
 #define QMCW_BY_G(fn, c, ...)                  \
