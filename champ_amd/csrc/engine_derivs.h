@@ -204,15 +204,62 @@ __device__ void point_pair_slots(const DevSys &S, int i, const double *xn, const
                                  const double *ri_v, const double *btd, double *uqd, double cv,
                                  const SlotAcc &acc) {
   const int N = S.N, M = S.M;
-  const double k = S.scalek;
+  const double k = S.scalek, ar = S.asymp_r;
+  // b-coefficient pieces summed over the other electrons in registers
+  // (b_slots' value terms), one update per slot afterwards
+  const int nb = S.nordb + 1;
+  double ab0[QMCW_MAX_ORD + 1], ab1[QMCW_MAX_ORD + 1];
+#pragma unroll
+  for (int q = 0; q <= QMCW_MAX_ORD; q++) ab0[q] = ab1[q] = 0.0;
   for (int jj = 0; jj < N; jj++) {
     if (jj == i) continue;
     const double dx = xn[0] - x[3 * jj], dy = xn[1] - x[3 * jj + 1], dz = xn[2] - x[3 * jj + 2];
     const double rij = sqrt(dx * dx + dy * dy + dz * dz);
-    b_slots<false>(S, (i < S.nup) == (jj < S.nup), rij, nullptr, cv, acc);
     double u, d1, d2;
     scaled<double>(S.isc, k, rij, u, d1, d2);
     uqd[jj] = 1.0 - k * u;
+    double sspinn = 1.0;
+    int isb = 0;
+    if ((i < S.nup) == (jj < S.nup)) {
+      if (S.nspin2b == 2) isb = 1;
+      else if (S.nocuspb == 0) sspinn = 0.5;
+    }
+    const cptr<double> b = cst(S.b) + isb * nb;
+    double up_ = u, arp = ar;
+#pragma unroll
+    for (int iw = 1; iw <= QMCW_MAX_ORD + 1; iw++) {
+      if (iw > nb) continue;
+      double gee;
+      if (iw == 1) {
+        const double bt = 1.0 + b[1] * u, bta = 1.0 + b[1] * ar;
+        gee = sspinn * u / bt - sspinn * ar / bta;
+      } else if (iw == 2) {
+        const double t = -sspinn * b[0] * u * u;
+        const double bot0 = 1.0 + b[1] * u, bt = bot0 * bot0, bta = 1.0 + b[1] * ar;
+        gee = t / bt + sspinn * b[0] * ar * ar / (bta * bta);
+      } else {
+        up_ *= u;
+        arp *= ar;
+        gee = up_ - arp;
+      }
+      if (isb) ab1[iw - 1] += cv * gee;
+      else ab0[iw - 1] += cv * gee;
+    }
+  }
+  {
+    double *bs = acc.base + (int64_t)S.slot_b0 * acc.rs;
+    const int nsb = S.nspin2b == 2 ? 2 : 1;
+    double o0[QMCW_MAX_ORD + 1], o1[QMCW_MAX_ORD + 1];
+#pragma unroll
+    for (int q = 0; q <= QMCW_MAX_ORD; q++) {
+      o0[q] = q < nb ? bs[(int64_t)q * acc.rs] : 0.0;
+      o1[q] = (nsb == 2 && q < nb) ? bs[(int64_t)(nb + q) * acc.rs] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q <= QMCW_MAX_ORD; q++) {
+      if (q < nb) bs[(int64_t)q * acc.rs] = o0[q] + ab0[q];
+      if (nsb == 2 && q < nb) bs[(int64_t)(nb + q) * acc.rs] = o1[q] + ab1[q];
+    }
   }
   if (S.nordc <= 1) return;
   constexpr int NG = een_gi<ORD>(ORD, 0) + 1;

@@ -632,13 +632,17 @@ __device__ __forceinline__ void shell_axpy_l(T x, T y, T z, T g, cptr<int> bas, 
                                              double fac, double *pb, double &rat) {
   T Sv[2 * L + 1], Sg[2 * L + 1][3];
   solid_harm<T>(L, x, y, z, Sv, Sg);
+  // the shell's sums first (its components are distinct), then the stores
+  double old[2 * L + 1];
+#pragma unroll
+  for (int q = 0; q < 2 * L + 1; q++) old[q] = bas[q] >= 0 ? pb[bas[q]] : 0.0;
 #pragma unroll
   for (int q = 0; q < 2 * L + 1; q++) {
     const int b = bas[q];
     if (b >= 0) {
       const double v = (double)(Sv[q] * g);
       rat += gp[b] * v;
-      pb[b] += fac * v;
+      pb[b] = old[q] + fac * v;
     }
   }
 }
