@@ -14,6 +14,14 @@ struct SlotAcc {
   double *base;
   int64_t rs;
   __device__ __forceinline__ void add(int slot, int c, double v) const { base[slot * rs + c] += v; }
+  // all five pieces of a slot: the old values are read together before the
+  // stores, one memory round trip instead of five
+  __device__ __forceinline__ void add5(int slot, double v, double g0, double g1, double g2,
+                                       double l) const {
+    double *p = base + slot * rs;
+    const double o0 = p[0], o1 = p[1], o2 = p[2], o3 = p[3], o4 = p[4];
+    p[0] = o0 + v; p[1] = o1 + g0; p[2] = o2 + g1; p[3] = o3 + g2; p[4] = o4 + l;
+  }
 };
 
 // a-coefficient pieces of the e-n terms of one electron (distances r[M],
@@ -57,12 +65,13 @@ __device__ void en_slots(const DevSys &S, const double *r, const double *rv, dou
         genii = iord * (iord - 1) * rpm2;
       }
       const int slot = it * na + iw - 1;
-      if (cv != 0.0) acc.add(slot, 0, cv * gen);
       if (FULL) {
         genii = genii * d7 * d7 + geni * d9;
         geni = geni * d7 / ri;
-        for (int q = 0; q < 3; q++) acc.add(slot, 1 + q, geni * rv[3 * ic + q]);
-        acc.add(slot, 4, genii + 2.0 * geni);
+        acc.add5(slot, cv != 0.0 ? cv * gen : 0.0, geni * rv[3 * ic], geni * rv[3 * ic + 1],
+                 geni * rv[3 * ic + 2], genii + 2.0 * geni);
+      } else if (cv != 0.0) {
+        acc.add(slot, 0, cv * gen);
       }
     }
   }
@@ -92,7 +101,7 @@ __device__ __forceinline__ void een_slots(double U, double A, double B, double U
           const int slot = slot0 + idx;
           idx++;
           const double P = ap[lm] * bp[m] + ap[m] * bp[lm];
-          if (cv != 0.0) acc.add(slot, 0, cv * up[k] * P);
+          if (!FULL && cv != 0.0) acc.add(slot, 0, cv * up[k] * P);
           if (FULL) {
             double PA = (lm ? lm * ap[lm > 0 ? lm - 1 : 0] * bp[m] : 0.0) +
                         (m ? m * ap[m > 0 ? m - 1 : 0] * bp[lm] : 0.0);
@@ -107,8 +116,9 @@ __device__ __forceinline__ void een_slots(double U, double A, double B, double U
             gui = gui * Ud1 * Ad1;
             gii = gii * Ad1 * Ad1 + gi * Ad2;
             gi = gi * Ad1 / ri;
-            for (int q = 0; q < 3; q++) acc.add(slot, 1 + q, gi * rvi[q] + gu * rvij[q]);
-            acc.add(slot, 4, 2.0 * (gu + gi) + guu + gii + gui * u2pst / (ri * rij));
+            acc.add5(slot, cv != 0.0 ? cv * up[k] * P : 0.0, gi * rvi[0] + gu * rvij[0],
+                     gi * rvi[1] + gu * rvij[1], gi * rvi[2] + gu * rvij[2],
+                     2.0 * (gu + gi) + guu + gii + gui * u2pst / (ri * rij));
           }
         }
 }
@@ -157,12 +167,13 @@ __device__ void b_slots(const DevSys &S, bool par, double rij, const double *rvi
       geeuu = iord * (iord - 1) * upm2;
     }
     const int slot = S.slot_b0 + isb * nb + iw - 1;
-    if (cv != 0.0) acc.add(slot, 0, cv * gee);
     if (FULL) {
       geeuu = geeuu * dd1 * dd1 + geeu * dd2;
       geeu = geeu * dd1 / rij;
-      for (int q = 0; q < 3; q++) acc.add(slot, 1 + q, geeu * rvij[q]);
-      acc.add(slot, 4, geeuu + 2.0 * geeu);
+      acc.add5(slot, cv != 0.0 ? cv * gee : 0.0, geeu * rvij[0], geeu * rvij[1], geeu * rvij[2],
+               geeuu + 2.0 * geeu);
+    } else if (cv != 0.0) {
+      acc.add(slot, 0, cv * gee);
     }
   }
 }
