@@ -608,13 +608,18 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
   const double chii = 1.0 / ws[L.chi];
   const double *dv0 = ws + L.detv[0], *dv1 = ws + L.detv[1];
   const double *del0 = ss + DL.del[0], *del1 = ss + DL.del[1];
+  // over the dets that contain unique det u (the CSR lists of eval_y, in
+  // ascending det order)
   for (int t = lane; t < S.ndetu[0] + S.ndetu[1]; t += G) {
     const int s = t < S.ndetu[0] ? 0 : 1, u = s ? t - S.ndetu[0] : t;
+    const cptr<int> uiu = cst(S.uiu[s]), uid = cst(S.uid[s]);
+    const cptr<double> ucf = cst(S.ucf[s]);
     double c0 = 0.0, c1 = 0.0;
-    for (int d = 0; d < S.ndet; d++) {
-      if (cst(S.iwdet[s])[d] != u) continue;
-      const int iu = cst(S.iwdet[0])[d], id = cst(S.iwdet[1])[d];
-      const double cf = cst(S.cdd)[d] * dv0[iu] * dv1[id];
+    const int q1 = cst(S.udet_off[s])[u + 1];
+#pragma unroll 4
+    for (int q = cst(S.udet_off[s])[u]; q < q1; q++) {
+      const int iu = uiu[q], id = uid[q];
+      const double cf = ucf[q] * dv0[iu] * dv1[id];
       c0 += cf;
       c1 += cf * ((S.ns[0] ? del0[iu] : 0.0) + (S.ns[1] ? del1[id] : 0.0));
     }
@@ -638,18 +643,14 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
     const double *inv = ws + L.inv[s], *idti = ss + DL.idti[s], *y = ws + L.y[s];
     const double *c0 = ss + DL.c0[s], *c1 = ss + DL.c1[s];
     double *dy = ss + DL.dy[s];
+    // contributions to entry e (the Y-entry CSR lists of eval_y, ascending u)
+    const cptr<int> yoff = cst(S.yent_off[s]), yu = cst(S.yent_u[s]), yp = cst(S.yent_pos[s]);
     for (int e = lane; e < nocc * ns; e += G) {
-      const int jo = e / ns, ir = e % ns;
       double a = 0.0;
-      for (int u = 1; u < S.ndetu[s]; u++) {
-        const int k = cst(S.ord[s])[u];
-        int jj = -1, ii = -1;
-        for (int q = 0; q < k; q++) {
-          if (cst(S.exo[s])[u * ns + q] == jo) jj = q;
-          if (cst(S.rf[s])[u * ns + q] == ir) ii = q;
-        }
-        if (jj < 0 || ii < 0) continue;
-        const int o = u * S.kmax2 + jj + k * ii;
+      const int q1 = yoff[e + 1];
+#pragma unroll 4
+      for (int q = yoff[e]; q < q1; q++) {
+        const int u = yu[q], o = yp[q];
         a += c1[u] * inv[o] - c0[u] * idti[o];
       }
       dy[e] = a * chii - esum * y[e];
