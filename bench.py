@@ -182,8 +182,9 @@ def run_engine(s, W, steps, warmup, precision, delta, x, seeds, prof_steps):
 
 def run_derivs(W, steps, warmup, precision, delta, prof_steps):
     """Derivative samples: VMC steps whose full evaluation also forms
-    d ln Psi/dp and dE_L/dp (the optimisation sample), C2-like system with
-    its parameter set (CSF, Jastrow, orbital rotations)."""
+    d ln Psi/dp and dE_L/dp (the optimisation sample) and adds them to the
+    linear method's product sums, C2-like system with its parameter set
+    (CSF, Jastrow, orbital rotations)."""
     from champ_amd import rng, system
     from champ_amd.engine import Engine
     s = system.c2_dimer(seed=2, derivs=True)
@@ -191,7 +192,8 @@ def run_derivs(W, steps, warmup, precision, delta, prof_steps):
     e.set_walkers(system.random_walkers(s, W, seed=8))
     e.set_rng(rng.walker_seeds("1837465927472523", W, stride=1 << 20))
     g = e.autotune()
-    e.sample_derivs(True)
+    e.accum_reset()
+    e.sample_derivs(2)          # derivatives and the linear method's product sums every step
     e.vmc_device(max(1, warmup), delta)
     e.sync()
     t0 = time.perf_counter()
@@ -204,6 +206,24 @@ def run_derivs(W, steps, warmup, precision, delta, prof_steps):
     prof = e.profile_get()
     e.profile(False)
     return dict(engine=e, rate=W * steps / dt, dt=dt, prof=prof, G=g, P=e.P, system=s)
+
+
+def accum_roofline(dres, W):
+    """FP64 MFMA rate of the product sums (k_accum_*: three [W x P]^T [W x P]
+    products, 2 W P^2 flops each, plus O(W P) vector sums) in the derivative
+    run's profile."""
+    if not dres:
+        return None
+    d = next(iter(dres.values()))
+    ms, n = d["prof"].get("k_accum", (0.0, 0))
+    if not n:
+        return None
+    P = d["P"]
+    fl = 3 * 2.0 * W * P * P
+    ach = fl / (ms / n * 1e-3) / 1e12
+    return {"kernel": "k_accum (gemm, vector sums, chunk reduction)", "avg_us": 1e3 * ms / n,
+            "flop_per_launch": fl, "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_FP64_TFLOPS}
 
 
 def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
@@ -438,12 +458,14 @@ def main():
         "kernel_avg_us": {k: (1e3 * v[0] / v[1] if v[1] else None) for k, v in r["prof"].items()},
         "deriv_samples_per_s": {k: v["rate"] for k, v in dres.items()} or None,
         "deriv_config": ({"workload": "C2-like dimer with CSF, Jastrow and orbital-rotation "
-                                      "parameters, VMC steps with derivatives",
+                                      "parameters, VMC steps with derivatives and the linear "
+                                      "method's product sums",
                           "walkers": args.deriv_walkers, "nparm": next(iter(dres.values()))["P"]}
                          if dres else None),
         "deriv_mixed_vs_fp64": (dres["mixed"]["rate"] / dres["fp64"]["rate"]) if len(dres) == 2 else None,
         "deriv_kernel_avg_us": ({k: (1e3 * v[0] / v[1] if v[1] else None)
                                  for k, v in next(iter(dres.values()))["prof"].items()} if dres else None),
+        "accum_mfma": accum_roofline(dres, args.deriv_walkers),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": dom, "avg_us": 1e3 * avg_ms, "flop_per_launch": fl,

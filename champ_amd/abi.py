@@ -26,7 +26,7 @@ QMCW_MIXED_BF16 = 2
 # kernels timed by qmcw_profile, in the order of the QMCW_PROF_* indices
 PROF_KERNELS = ("k_full_jas", "k_full_det", "k_full_nlpp", "k_full_fin",
                 "k_ratio_1e", "k_accept_1e", "k_rotqua",
-                "k_deriv_jas", "k_deriv_nlpp", "k_derivs")
+                "k_deriv_jas", "k_deriv_nlpp", "k_derivs", "k_accum")
 
 QMCW_BASIS_SLATER = 1
 QMCW_BASIS_GAUSSIAN = 2
@@ -135,6 +135,9 @@ SIGNATURES = {
     "qmcw_derivs": (C.c_int, [_vp, _pd, _pd]),
     "qmcw_sample_derivs": (C.c_int, [_vp, C.c_int32]),
     "qmcw_get_derivs": (C.c_int, [_vp, _pd, _pd]),
+    "qmcw_accum_reset": (C.c_int, [_vp]),
+    "qmcw_accum_add": (C.c_int, [_vp]),
+    "qmcw_accum_get": (C.c_int, [_vp, _pd]),
     "qmcw_set_rng": (C.c_int, [_vp, _pi]),
     "qmcw_get_rng": (C.c_int, [_vp, _pi]),
     "qmcw_vmc_run": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _pd,

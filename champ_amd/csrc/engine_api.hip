@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "engine_derivs.h"
+#include "engine_accum.h"
 #include "engine_dmc.h"
 #include "engine_launch.h"
 
@@ -67,6 +68,8 @@ struct qmcw_handle {
   int split_forced = 0;      // QMCW_SPLIT_MOVE set: the autotuner keeps split_move
   int lds_y = 1;             // LDS staging of eval_y in the move kernels
   int ncu = 256;             // compute units of the device (set at create)
+  double *acc = nullptr, *acc_part = nullptr, *acc_partv = nullptr;  // product sums and chunk partials
+  int acc_nch = 0;
   // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
   int dmc_on = 0, nact = 0;
   double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
@@ -1250,6 +1253,40 @@ extern "C" int qmcw_grad_1e(qmcw_handle *h, int32_t ie, double *grad, double *la
 }
 
 // one VMC step: grid rotation, sweep over the electrons, full evaluation
+// Product sums of the derivative samples (engine_accum.h); buffers sized by
+// W and P at first use, outside any graph capture.
+static int accum_alloc(qmcw_handle *h) {
+  if (h->acc) return QMCW_OK;
+  const int P = h->nparm;
+  if (P <= 0) return fail(h, QMCW_ERR_ARG, "the system has no parameters to differentiate");
+  h->acc_nch = (h->W + kAccChunk - 1) / kAccChunk;
+  void *p;
+  if (hipMalloc(&p, accum_doubles(P) * sizeof(double)) != hipSuccess) return fail(h, QMCW_ERR_NOMEM, "accumulator alloc");
+  h->allocs.push_back(p); h->acc = (double *)p;
+  if (hipMalloc(&p, (size_t)h->acc_nch * 3 * P * P * sizeof(double)) != hipSuccess) return fail(h, QMCW_ERR_NOMEM, "accumulator alloc");
+  h->allocs.push_back(p); h->acc_part = (double *)p;
+  if (hipMalloc(&p, (size_t)h->acc_nch * (3 + 3 * P) * sizeof(double)) != hipSuccess) return fail(h, QMCW_ERR_NOMEM, "accumulator alloc");
+  h->allocs.push_back(p); h->acc_partv = (double *)p;
+  const int64_t n = accum_doubles(P);
+  hipLaunchKernelGGL(k_accum_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->acc, n);
+  return QMCW_OK;
+}
+
+static void launch_accum(qmcw_handle *h) {
+  const int P = h->nparm, nb = (P + kAccBlk - 1) / kAccBlk, nch = h->acc_nch;
+  const int64_t n = accum_doubles(P);
+  prof_begin(h, QMCW_PROF_ACCUM);
+  hipLaunchKernelGGL(k_accum_gemm, dim3(nb * nb, nch), dim3(64), 0, h->stream, (const double *)h->der,
+                     2 * P, P, (const double *)h->A.st, (int64_t)h->A.L.stride, (int64_t)h->A.L.eloc,
+                     h->W, h->acc_part);
+  hipLaunchKernelGGL(k_accum_vec, dim3((P + 1 + 63) / 64, nch), dim3(64), 0, h->stream,
+                     (const double *)h->der, 2 * P, P, (const double *)h->A.st,
+                     (int64_t)h->A.L.stride, (int64_t)h->A.L.eloc, h->W, h->acc_partv);
+  hipLaunchKernelGGL(k_accum_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                     (const double *)h->acc_part, (const double *)h->acc_partv, nch, P, h->acc);
+  prof_end(h);
+}
+
 static void vmc_one_step(qmcw_handle *h, double delta, double fbias) {
   const int N = h->A.S.N;
   if (h->A.S.nloc > 0) {
@@ -1261,6 +1298,7 @@ static void vmc_one_step(qmcw_handle *h, double delta, double fbias) {
     launch_ratio(h, ie, 2, (const double *)nullptr, delta, fbias, (double *)nullptr,
                h->nacc);
   launch_eval(h, h->sample_derivs != 0);
+  if (h->sample_derivs == 2) launch_accum(h);
 }
 
 static void drop_graph(qmcw_handle *h) {
@@ -1275,7 +1313,7 @@ static void drop_graph(qmcw_handle *h) {
 // state pointer, are captured by value) as a graph, rebuilt when a captured
 // value changes.
 static int step_graph(qmcw_handle *h, double delta, double fbias) {
-  const int derivs = h->sample_derivs != 0;
+  const int derivs = h->sample_derivs;
   const int gkey = h->Gk[0] | (h->Gk[1] << 8) | (h->Gk[2] << 16) | (h->Gk[3] << 24);
   if (h->graph_ok && h->graph_G == gkey && h->graph_Wl == h->Wl && h->graph_derivs == derivs &&
       h->graph_delta == delta && h->graph_fbias == fbias && h->graph_st == h->A.st)
@@ -1372,7 +1410,7 @@ extern "C" int qmcw_autotune(qmcw_handle *h, int32_t *chosen) {
   const int cand[4] = {8, 16, 32, 64};
   // kernel classes of the profiled kernels
   const int cls[QMCW_PROF_NKERNEL] = {KC_JAS, KC_DET, KC_NLPP, KC_DET, KC_MOVE, KC_MOVE, -1,
-                                      KC_JAS, KC_NLPP, KC_DET};
+                                      KC_JAS, KC_NLPP, KC_DET, -1};
   double best[4] = {1e30, 1e30, 1e30, 1e30};
   int bestG[4] = {h->Gk[0], h->Gk[1], h->Gk[2], h->Gk[3]};
   int best_split = h->split_move;
@@ -1483,7 +1521,12 @@ extern "C" int qmcw_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
 extern "C" int qmcw_sample_derivs(qmcw_handle *h, int32_t enable) {
   if (!h) return QMCW_ERR_ARG;
   if (enable && h->nparm == 0) return fail(h, QMCW_ERR_ARG, "the system has no parameters to differentiate");
-  h->sample_derivs = enable != 0;
+  if (enable < 0 || enable > 2) return fail(h, QMCW_ERR_ARG, "sample_derivs takes 0, 1 or 2");
+  if (enable == 2) {
+    int st = accum_alloc(h);
+    if (st) return st;
+  }
+  h->sample_derivs = enable;
   return QMCW_OK;
 }
 
@@ -1491,6 +1534,33 @@ extern "C" int qmcw_get_derivs(qmcw_handle *h, double *dpsi, double *deloc) {
   if (!h) return QMCW_ERR_ARG;
   if (!h->der_valid) return fail(h, QMCW_ERR_ARG, "no derivative evaluation has run on this handle");
   return copy_derivs(h, dpsi, deloc);
+}
+
+extern "C" int qmcw_accum_reset(qmcw_handle *h) {
+  if (!h) return QMCW_ERR_ARG;
+  if (!h->acc) return accum_alloc(h);
+  const int64_t n = accum_doubles(h->nparm);
+  hipLaunchKernelGGL(k_accum_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->acc, n);
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_accum_add(qmcw_handle *h) {
+  if (!h) return QMCW_ERR_ARG;
+  if (!h->der_valid) return fail(h, QMCW_ERR_ARG, "no derivative evaluation has run on this handle");
+  int st = accum_alloc(h);
+  if (st) return st;
+  launch_accum(h);
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_accum_get(qmcw_handle *h, double *out) {
+  if (!h || !out) return QMCW_ERR_ARG;
+  int st = accum_alloc(h);
+  if (st) return st;
+  HIPCHK(h, hipMemcpyAsync(out, h->acc, accum_doubles(h->nparm) * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
 }
 
 // ---------------------------------------------------------------------------

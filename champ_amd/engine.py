@@ -133,8 +133,9 @@ class Engine:
         self._chk(self.lib.qmcw_derivs(self.h, _d(dp), _d(de)))
         return dp, de
 
-    def sample_derivs(self, enable: bool = True):
-        """Every VMC step's full evaluation also forms the derivatives."""
+    def sample_derivs(self, enable=True):
+        """Every VMC step's full evaluation also forms the derivatives;
+        enable=2 also adds them to the product sums (accum_get)."""
         self._chk(self.lib.qmcw_sample_derivs(self.h, int(enable)))
 
     def get_derivs(self):
@@ -142,6 +143,25 @@ class Engine:
         dp, de = np.zeros((self.W, self.P)), np.zeros((self.W, self.P))
         self._chk(self.lib.qmcw_get_derivs(self.h, _d(dp), _d(de)))
         return dp, de
+
+    # ---- sums of the linear method's derivative products (qmcw_accum_*)
+    def accum_reset(self):
+        self._chk(self.lib.qmcw_accum_reset(self.h))
+
+    def accum_add(self):
+        """Add the last derivative sample of every walker."""
+        self._chk(self.lib.qmcw_accum_add(self.h))
+
+    def accum_get(self):
+        """dict: n, se, se2 (sums of 1, E_L, E_L^2); sd, sde, sl [P] (sums of
+        dpsi, E_L dpsi, deloc); dd, dde, dl [P][P] (sums of dpsi_i dpsi_j,
+        E_L dpsi_i dpsi_j, dpsi_i deloc_j)."""
+        P = self.P
+        o = np.zeros(3 + 3 * P + 3 * P * P)
+        self._chk(self.lib.qmcw_accum_get(self.h, _d(o)))
+        v = o[3:3 + 3 * P].reshape(3, P)
+        m = o[3 + 3 * P:].reshape(3, P, P)
+        return dict(n=o[0], se=o[1], se2=o[2], sd=v[0], sde=v[1], sl=v[2], dd=m[0], dde=m[1], dl=m[2])
 
     # ---- drivers
     def vmc(self, nstep: int, delta: float, fbias: float = 1.0, trace: bool = True):

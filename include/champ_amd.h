@@ -187,9 +187,26 @@ int qmcw_derivs(qmcw_handle *h, double *dpsi, double *deloc);
 /* Sampling mode: with enable != 0 the full evaluation that ends every step
    of qmcw_vmc_run / qmcw_vmc_run_device also forms the derivatives (CHAMP's
    ideriv during an optimisation run, qmc/vmc/vmc.f90:259-272); they stay on
-   the GPU until read with qmcw_get_derivs ([W][P] each, last step). */
+   the GPU until read with qmcw_get_derivs ([W][P] each, last step).  With
+   enable == 2 every step also adds them to the product sums below. */
 int qmcw_sample_derivs(qmcw_handle *h, int32_t enable);
 int qmcw_get_derivs(qmcw_handle *h, double *dpsi, double *deloc);
+
+/* Sums over walkers (and over calls, until reset) of the products the
+   linear method averages: dpsi_dpsi_bld, dpsi_dpsi_eloc_bld, dpsi_eloc_bld
+   and dpsi_deloc_bld (qmc/deriv_mod.f90:683-723, 898-926, 960-1000,
+   1375-1412), with the sums of dpsi, deloc and E_L behind dpsi_av, deloc_av
+   and eloc_av; consumed by ovlp_lin_bld / ham_lin_energy_bld
+   (qmc/opt_lin_mod.f90:249-348, 404-620).  qmcw_accum_add adds the last
+   derivative sample of every walker; qmcw_sample_derivs(h, 2) makes every
+   VMC step do so.  qmcw_accum_get copies QMCW_ACCUM_DOUBLES(P) doubles:
+   [0] samples, [1] sum E_L, [2] sum E_L^2, sum dpsi[P], sum E_L dpsi[P],
+   sum deloc[P], then row-major P x P: sum dpsi_i dpsi_j,
+   sum E_L dpsi_i dpsi_j, sum dpsi_i deloc_j. */
+#define QMCW_ACCUM_DOUBLES(P) (3 + 3 * (int64_t)(P) + 3 * (int64_t)(P) * (P))
+int qmcw_accum_reset(qmcw_handle *h);
+int qmcw_accum_add(qmcw_handle *h);
+int qmcw_accum_get(qmcw_handle *h, double *out);
 
 /* Per-walker random streams (CHAMP's rannyu, qmc/vmc/rannyu.f90:53-74):
    seed[W][4] 12-bit limbs as in setrn (qmc/vmc/rannyu.f90:11-21). */
@@ -296,7 +313,8 @@ int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
 #define QMCW_PROF_DERIV_JAS 7  /* k_deriv_jas: Jastrow parameter derivatives */
 #define QMCW_PROF_DERIV_NLPP 8 /* k_deriv_nlpp: pseudopotential derivative pieces */
 #define QMCW_PROF_DERIVS 9     /* k_derivs: d ln Psi/dp, d E_L/dp assembly */
-#define QMCW_PROF_NKERNEL 10
+#define QMCW_PROF_ACCUM 10     /* k_accum_*: sums of the linear method's derivative products */
+#define QMCW_PROF_NKERNEL 11
 int qmcw_profile(qmcw_handle *h, int32_t enable);
 int qmcw_profile_get(qmcw_handle *h, double *ms, int64_t *count);
 

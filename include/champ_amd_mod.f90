@@ -136,6 +136,24 @@ module champ_amd_mod
       real(c_double), intent(out) :: dpsi(*), deloc(*)
       integer(c_int) :: st
     end function
+    ! sums of the linear method's derivative products (qmcw_accum_get layout:
+    ! 3 + 3 nparm + 3 nparm**2 doubles)
+    function qmcw_accum_reset(h) bind(C, name="qmcw_accum_reset") result(st)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int) :: st
+    end function
+    function qmcw_accum_add(h) bind(C, name="qmcw_accum_add") result(st)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int) :: st
+    end function
+    function qmcw_accum_get(h, acc) bind(C, name="qmcw_accum_get") result(st)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: acc(*)
+      integer(c_int) :: st
+    end function
     function qmcw_set_rng(h, seed) bind(C, name="qmcw_set_rng") result(st)
       import :: c_ptr, c_int, c_int32_t
       type(c_ptr), value :: h

@@ -69,6 +69,7 @@ def lib():
             "orc_accept_1e": (None, [_vp, _vp, C.c_int32, C.c_int]),
             "orc_grad_1e": (None, [_vp, _vp, C.c_int32, _pd, _pd]),
             "orc_derivs": (C.c_int, [_vp, _vp, _pd, _pd]),
+            "orc_accum": (None, [C.c_int32, C.c_int32, _pd, _pd, _pd]),
             "orc_walker_get": (None, [_vp, _vp, _pd, _pd]),
             "orc_vmc_run": (C.c_int, [_vp, C.c_int32, _pd, _pi, C.c_int32, C.c_double,
                                       C.c_double, _pd, C.POINTER(C.c_int64), C.c_int32]),
@@ -248,6 +249,27 @@ def rng_stream(seed4, n):
     s = np.ascontiguousarray(seed4, np.int32)
     L.orc_setrn(_i(st), _i(s))
     return np.array([L.orc_rannyu(_i(st)) for _ in range(n)]), st
+
+
+def accum(dpsi, deloc, eloc):
+    """Sums over samples of the linear method's derivative products (orc_accum):
+    dict with n, se, se2, sd, sde, sl [P] and dd, dde, dl [P][P]."""
+    dpsi, deloc = np.asarray(dpsi, np.float64), np.asarray(deloc, np.float64)
+    W, P = dpsi.shape
+    der = np.ascontiguousarray(np.concatenate([dpsi, deloc], axis=1))
+    e = np.ascontiguousarray(eloc, np.float64)
+    out = np.zeros(3 + 3 * P + 3 * P * P)
+    lib().orc_accum(P, W, _d(der), _d(e), _d(out))
+    return split_accum(out, P)
+
+
+def split_accum(out, P):
+    """The accumulator layout of qmcw_accum_get / orc_accum as a dict."""
+    o = np.asarray(out)
+    PP = P * P
+    v = o[3:3 + 3 * P].reshape(3, P)
+    m = o[3 + 3 * P:].reshape(3, P, P)
+    return dict(n=o[0], se=o[1], se2=o[2], sd=v[0], sde=v[1], sl=v[2], dd=m[0], dde=m[1], dl=m[2])
 
 
 def matinv(a):

@@ -2714,3 +2714,37 @@ int32_t orc_dmc_get(const orc_dmc *d, double *x, int32_t *rng, double *weight, d
   }
   return d->nwalk;
 }
+
+/* ---------------------------------------------------------------------------
+ * Sums over W samples of the products the linear method averages, as CHAMP's
+ * per-sample objects define them: dpsi_dpsi(i,j) = dpsi(i) dpsi(j)
+ * (qmc/deriv_mod.f90:719), dpsi_dpsi_eloc = dpsi_dpsi * eloc (:924),
+ * dpsi_eloc = dpsi * eloc (dpsi_eloc_bld, :960-1000) and dpsi_deloc(i,j) =
+ * dpsi(i) deloc(j) (:1406-1410), with the plain sums of dpsi, deloc and eloc.
+ * der: [W][2P] rows dpsi[P], deloc[P]; out as the engine's accumulator:
+ * [0] W, [1] sum E, [2] sum E^2, sum dpsi[P], sum E dpsi[P], sum deloc[P],
+ * then row-major P x P sums of dpsi dpsi, E dpsi dpsi, dpsi deloc.
+ */
+void orc_accum(int32_t P, int32_t W, const double *der, const double *eloc, double *out) {
+  const int64_t PP = (int64_t)P * P;
+  double *sd = out + 3, *sde = sd + P, *sl = sde + P;
+  double *mdd = sl + P, *mdde = mdd + PP, *mdl = mdde + PP;
+  memset(out, 0, (size_t)(3 + 3 * (int64_t)P + 3 * PP) * sizeof(double));
+  for (int32_t w = 0; w < W; w++) {
+    const double *dp = der + (int64_t)w * 2 * P, *dl = dp + P, e = eloc[w];
+    out[0] += 1.0;
+    out[1] += e;
+    out[2] += e * e;
+    for (int32_t i = 0; i < P; i++) {
+      sd[i] += dp[i];
+      sde[i] += dp[i] * e;
+      sl[i] += dl[i];
+      for (int32_t j = 0; j < P; j++) {
+        const double dd = dp[i] * dp[j];
+        mdd[(int64_t)i * P + j] += dd;
+        mdde[(int64_t)i * P + j] += dd * e;
+        mdl[(int64_t)i * P + j] += dp[i] * dl[j];
+      }
+    }
+  }
+}

@@ -90,6 +90,8 @@ void orc_grad_1e(const orc_sys *o, orc_walker *w, int32_t ie, double *grad,
                  double *lap);
 /* derivatives at the last full evaluation: dpsi[P], deloc[P] */
 int orc_derivs(const orc_sys *o, orc_walker *w, double *dpsi, double *deloc);
+/* sums over samples of the linear method's derivative products (layout as qmcw_accum_get) */
+void orc_accum(int32_t P, int32_t W, const double *der, const double *eloc, double *out);
 /* walker accessors */
 void orc_walker_set_quad(orc_walker *w, const double *xq, const double *yq,
                          const double *zq, int nq);

@@ -496,3 +496,23 @@ def test_vmc_energy_matches_champ_transcript(iteration):
     sig = math.hypot(err, run["error"])
     assert abs(e.mean() - run["energy"]) < 3 * sig, (e.mean(), err, run)
     assert abs(e.std() - run["sigma"]) < 0.01
+
+
+def test_accum_products_against_numpy():
+    """orc_accum: sums over samples of dpsi dpsi^T, E dpsi dpsi^T, dpsi deloc^T
+    and the vector sums, against numpy's matrix products."""
+    r = np.random.default_rng(5)
+    W, P = 53, 17
+    dp, dl, e = r.normal(size=(W, P)), r.normal(size=(W, P)), r.normal(-5, 1, size=W)
+    a = oracle.accum(dp, dl, e)
+    assert a["n"] == W
+    assert np.allclose(a["se"], e.sum(), rtol=1e-14) and np.allclose(a["se2"], (e * e).sum(), rtol=1e-14)
+    assert np.allclose(a["sd"], dp.sum(0), rtol=1e-13, atol=1e-13)
+    assert np.allclose(a["sde"], dp.T @ e, rtol=1e-13, atol=1e-13)
+    assert np.allclose(a["sl"], dl.sum(0), rtol=1e-13, atol=1e-13)
+    assert np.allclose(a["dd"], dp.T @ dp, rtol=1e-13, atol=1e-12)
+    assert np.allclose(a["dde"], (dp * e[:, None]).T @ dp, rtol=1e-13, atol=1e-12)
+    assert np.allclose(a["dl"], dp.T @ dl, rtol=1e-13, atol=1e-12)
+    # the deloc product is not symmetric; dpsi dpsi is
+    assert np.array_equal(a["dd"], a["dd"].T)
+    assert not np.allclose(a["dl"], a["dl"].T)
