@@ -1594,9 +1594,6 @@ extern "C" int qmcw_get_branch_rng(const qmcw_handle *h, int32_t *seed) {
 
 extern "C" int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weight) {
   if (!h || nwalk <= 0 || nwalk > h->W) return fail(h, QMCW_ERR_ARG, "nwalk must be in 1..W");
-  if (h->A.S.nloc == 0)
-    return fail(h, QMCW_ERR_ARG,
-                "DMC is built for pseudopotential systems (the all-electron cusp mover is not)");
   int st = dmc_alloc(h);
   if (st) return st;
   h->nact = nwalk;

@@ -269,8 +269,10 @@ typedef struct qmcw_dmc_stats {
 /* DMC population (qmc/rundmc_mod.f90).  qmcw_dmc_begin makes the first
    nwalk walkers (positions set with qmcw_set_walkers, streams with
    qmcw_set_rng) the population, with weights (NULL: 1), and evaluates them;
-   the handle's W is the population's capacity (MWALK).  Pseudopotential
-   systems only (the reference's nloc > 0 mover). */
+   the handle's W is the population's capacity (MWALK).  The mover follows
+   sample_1e_move (qmc/green_functions_mod.f90:12-38): drift-diffusion for
+   pseudopotential systems (nloc > 0), drift-diffusion with the nucleus cusp
+   (:83-145, density :268-323) for all-electron ones. */
 int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weight);
 
 /* One DMC generation (qmc/rundmc_mod.f90:97-330): for every walker and

@@ -75,6 +75,7 @@ def lib():
                                       C.c_double, _pd, C.POINTER(C.c_int64), C.c_int32]),
             "orc_eval_batch": (C.c_int, [_vp, C.c_int32, _pd, _pd, _pd, _pd, _pd, _pd,
                                          C.c_int32]),
+            "orc_cusp_move": (C.c_double, [_vp, C.c_double, C.c_int32, _pd, _pd, _pi, _pd, _pd]),
             "orc_dmc_create": (_vp, [_vp, C.c_int32]),
             "orc_dmc_destroy": (None, [_vp]),
             "orc_dmc_set": (C.c_int, [_vp, C.c_int32, _pd, _pi, _pd]),
@@ -219,6 +220,15 @@ class Oracle:
         q = [np.zeros(12) for _ in range(4)]
         self.L.orc_quadrature(self.h, *[_d(a) for a in q])
         return q
+
+    def cusp_move(self, tau, g, xe, rng=None, pt=None, n_drift=2):
+        """The all-electron DMC mover at xe with gradient g: (xn or None,
+        proposal density at pt or None); rng [4] int32 advances in place."""
+        g, xe = (np.ascontiguousarray(a, np.float64) for a in (g, xe))
+        xn = np.zeros(3) if rng is not None else None
+        ptd = None if pt is None else np.ascontiguousarray(pt, np.float64)
+        p = self.L.orc_cusp_move(self.h, tau, n_drift, _d(g), _d(xe), _i(rng), _d(xn), _d(ptd))
+        return xn, (p if pt is not None else None)
 
     # ---- batched ----
     def eval_batch(self, x, derivs=False, nthreads=0):

@@ -2440,6 +2440,110 @@ static double dmc_prob(double tau, const double *vav, const double *xo, const do
   return exp(-d2 / (2 * tau)) / (nrm * nrm * nrm);
 }
 
+/* all-electron mover: geometry of the drift-diffusion move with the nucleus
+   cusp from an electron with e-n vectors rv[M][3] (electron minus centre),
+   distances r[M] and gradient g.  The local frame and the bent drift point
+   follow qmc/green_functions_mod.f90:93-131 (same as :276-302); the velocity
+   uses the nucleus adrift of averaged_1e_velocity (:616-622). */
+typedef struct {
+  double z[3], x[3], xdrift[3], c[3], pgaus, zeta;
+} cusp_geo;
+
+static void cusp_geo_at(const orc_sys *o, double tau, int ndrift, const double *g, const double *rv,
+                        const double *r, cusp_geo *cg) {
+  const double eps = 1e-10, adrift0 = 0.1;
+  int M = o->ncent, iw = 0;
+  for (int ic = 1; ic < M; ic++)
+    if (r[ic] < r[iw]) iw = ic;
+  const double *xe = rv + 3 * iw;
+  double zn = sqrt(xe[0] * xe[0] + xe[1] * xe[1] + xe[2] * xe[2]);
+  for (int k = 0; k < 3; k++) cg->z[k] = xe[k] / zn;
+  double gz = g[0] * cg->z[0] + g[1] * cg->z[1] + g[2] * cg->z[2];
+  for (int k = 0; k < 3; k++) cg->x[k] = g[k] - gz * cg->z[k];
+  if (sqrt(cg->x[0] * cg->x[0] + cg->x[1] * cg->x[1] + cg->x[2] * cg->x[2]) < eps) {
+    cg->x[0] = eps * (1 - cg->z[0] * cg->z[0]);
+    cg->x[1] = eps * (-cg->z[0] * cg->z[1]);
+    cg->x[2] = eps * (-cg->z[0] * cg->z[2]);
+  }
+  double xn = sqrt(cg->x[0] * cg->x[0] + cg->x[1] * cg->x[1] + cg->x[2] * cg->x[2]);
+  for (int k = 0; k < 3; k++) cg->x[k] /= xn;
+  double Z = o->znuc[o->iwctype[iw]];
+  double vn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  double hafzr2 = (Z * zn / 2) * (Z * zn / 2);
+  double adrift = (1 + eps + (xe[0] * g[0] + xe[1] * g[1] + xe[2] * g[2]) / (vn * zn)) / 2 +
+                  adrift0 * hafzr2 / (1 + hafzr2);
+  double v[3];
+  dmc_vav(tau, adrift, ndrift, g, v);
+  double zi = r[iw];
+  double zf = zi + (v[0] * cg->z[0] + v[1] * cg->z[1] + v[2] * cg->z[2]) * tau;
+  if (zf >= 0) cg->pgaus = 1 - erfc(zf / sqrt(2 * tau)) / 2;
+  else cg->pgaus = erfc(-zf / sqrt(2 * tau)) / 2;
+  zf = fmax(0.0, zf);
+  double xf = (v[0] * cg->x[0] + v[1] * cg->x[1] + v[2] * cg->x[2]) * tau * fmin(1.0, zf / ((zf + zi) / 2));
+  for (int k = 0; k < 3; k++) {
+    cg->c[k] = o->cent[3 * iw + k];
+    cg->xdrift[k] = cg->c[k] + xf * cg->x[k] + zf * cg->z[k];
+  }
+  cg->zeta = sqrt(1 / tau + Z * Z);
+}
+
+/* sample_1e_prob_drift_diffusion_with_cusp, qmc/green_functions_mod.f90:303-319 */
+static double cusp_prob(const cusp_geo *cg, double tau, const double *xn) {
+  const double pi = 3.141592653589793;
+  double d2 = 0, rn2 = 0;
+  for (int k = 0; k < 3; k++) {
+    d2 += (xn[k] - cg->xdrift[k]) * (xn[k] - cg->xdrift[k]);
+    rn2 += (xn[k] - cg->c[k]) * (xn[k] - cg->c[k]);
+  }
+  double nrm = sqrt(2 * pi * tau);
+  return cg->pgaus * exp(-d2 / (2 * tau)) / (nrm * nrm * nrm) +
+         (1 - cg->pgaus) * cg->zeta * cg->zeta * cg->zeta * exp(-2 * cg->zeta * sqrt(rn2)) / pi;
+}
+
+/* sample_1e_move_drift_diffusion_with_cusp, qmc/green_functions_mod.f90:132-144:
+   gaussian about the drift point with probability pgaus, otherwise the
+   exponential zeta^3 exp(-2 zeta r) / pi about the nearest nucleus */
+static void cusp_sample(const cusp_geo *cg, double tau, int32_t *rng, double *xn) {
+  const double pi = 3.141592653589793;
+  if (orc_rannyu(rng) < cg->pgaus) {
+    for (int k = 0; k < 3; k++) xn[k] = cg->xdrift[k] + sqrt(tau) * orc_gauss(rng);
+  } else {
+    double u1 = orc_rannyu(rng), u2 = orc_rannyu(rng), u3 = orc_rannyu(rng);
+    double r = (-0.5 / cg->zeta) * log(u1 * u2 * u3);
+    double costht = 2 * (orc_rannyu(rng) - 0.5);
+    double sintht = sqrt(1 - costht * costht);
+    double phi = 2 * pi * orc_rannyu(rng);
+    xn[0] = cg->c[0] + r * sintht * cos(phi);
+    xn[1] = cg->c[1] + r * sintht * sin(phi);
+    xn[2] = cg->c[2] + r * costht;
+  }
+}
+
+static void en_vectors(const orc_sys *o, const double *xe, double *rv, double *r) {
+  for (int ic = 0; ic < o->ncent; ic++) {
+    double s = 0;
+    for (int k = 0; k < 3; k++) {
+      rv[3 * ic + k] = xe[k] - o->cent[3 * ic + k];
+      s += rv[3 * ic + k] * rv[3 * ic + k];
+    }
+    r[ic] = sqrt(s);
+  }
+}
+
+/* the all-electron mover for an electron at xe with gradient g: one sample
+   (xn, or NULL) from the walker stream rng and the proposal density at pt
+   (or NULL; return value 0) */
+double orc_cusp_move(const orc_sys *o, double tau, int32_t ndrift, const double *g, const double *xe,
+                     int32_t *rng, double *xn, const double *pt) {
+  double rv[3 * 64], r[64];
+  if (o->ncent > 64) return -1;
+  en_vectors(o, xe, rv, r);
+  cusp_geo cg;
+  cusp_geo_at(o, tau, ndrift, g, rv, r, &cg);
+  if (xn) cusp_sample(&cg, tau, rng, xn);
+  return pt ? cusp_prob(&cg, tau, pt) : 0.0;
+}
+
 /* ene_int_rewt, qmc/green_functions_mod.f90:382-472 (ene_int_v9 or
    no_ene_int) */
 static double dmc_eint(const qmcw_dmc_params *p, int N, double tauint, double eo,
@@ -2507,10 +2611,11 @@ int orc_dmc_set(orc_dmc *d, int32_t nwalk, const double *x, const int32_t *rng,
   return err;
 }
 
-/* one walker's generation: qmc/rundmc_mod.f90:114-303 for the nloc > 0
-   mover (sample_1e_move_drift_diffusion, green_functions_mod.f90:70-80),
-   the walker's own rannyu stream for the grid rotation, the gaussians and
-   the Metropolis tests */
+/* one walker's generation: qmc/rundmc_mod.f90:114-303 with the mover of
+   sample_1e_move (green_functions_mod.f90:12-38): drift-diffusion
+   (:70-80) with pseudopotentials, drift-diffusion with the nucleus cusp
+   (:83-145) for all-electron systems; the walker's own rannyu stream for
+   the grid rotation, the mover and the Metropolis tests */
 static int dmc_walker(const orc_sys *o, orc_walker *w, int32_t *rng, double *weight,
                       const qmcw_dmc_params *p, double *res) {
   int N = o->nelec;
@@ -2521,22 +2626,38 @@ static int dmc_walker(const orc_sys *o, orc_walker *w, int32_t *rng, double *wei
   int l_accept = 0, st = 0;
   double rt = sqrt(p->tau);
   for (int ie = 0; ie < N; ie++) {
-    double gold[3], vo[3], vn[3], gnew[3], xo[3], xn[3], out4[4], lap;
+    double gold[3], vo[3], vn[3], gnew[3], xo[3], xn[3], out4[4], lap, po, pn, d2 = 0;
     orc_grad_1e(o, w, ie, gold, NULL);
-    dmc_vav(p->tau, p->adrift, p->n_drift, gold, vo);
-    double d2 = 0;
-    for (int k = 0; k < 3; k++) {
-      double xd = rt * orc_gauss(rng);
-      xo[k] = w->x[3 * ie + k];
-      xn[k] = xo[k] + vo[k] * p->tau + xd;
-      d2 += xd * xd;
+    for (int k = 0; k < 3; k++) xo[k] = w->x[3 * ie + k];
+    cusp_geo cg;
+    if (o->nloc == 0) {
+      cusp_geo_at(o, p->tau, p->n_drift, gold, w->rvec_en + 3 * ie * o->ncent,
+                  w->r_en + ie * o->ncent, &cg);
+      cusp_sample(&cg, p->tau, rng, xn);
+      for (int k = 0; k < 3; k++) d2 += (xn[k] - cg.xdrift[k]) * (xn[k] - cg.xdrift[k]);
+    } else {
+      dmc_vav(p->tau, p->adrift, p->n_drift, gold, vo);
+      for (int k = 0; k < 3; k++) {
+        double xd = rt * orc_gauss(rng);
+        xn[k] = xo[k] + vo[k] * p->tau + xd;
+        d2 += xd * xd;
+      }
     }
     dd[ie] = d2;
     int s1 = orc_ratio_1e(o, w, ie, xn, out4, gnew, &lap);
     if (s1) st = s1;
-    dmc_vav(p->tau, p->adrift, p->n_drift, gnew, vn);
-    double po = dmc_prob(p->tau, vo, xo, xn);
-    double pn = dmc_prob(p->tau, vn, xn, xo);
+    if (o->nloc == 0) {
+      /* green_functions_mod.f90:41-44 for both directions; the backward
+         move starts from the proposed point with its e-n vectors */
+      po = cusp_prob(&cg, p->tau, xn);
+      cusp_geo cgn;
+      cusp_geo_at(o, p->tau, p->n_drift, gnew, w->rvec_en_n, w->r_en_n, &cgn);
+      pn = cusp_prob(&cgn, p->tau, xo);
+    } else {
+      dmc_vav(p->tau, p->adrift, p->n_drift, gnew, vn);
+      po = dmc_prob(p->tau, vo, xo, xn);
+      pn = dmc_prob(p->tau, vn, xn, xo);
+    }
     double psirat = out4[0] * out4[0];
     double pa = fmin(1.0, psirat * pn / po);
     if (p->iacc_rej == 0) pa = 1.0;

@@ -111,10 +111,17 @@ int orc_eval_batch(const orc_sys *o, int32_t nw, const double *x,
                    double *out9, double *grad, double *divv, double *dpsi,
                    double *deloc, int32_t nthreads);
 
-/* DMC (qmc/rundmc_mod.f90, nloc > 0 mover): a population of up to cap
-   walkers with per-walker rannyu streams; one generation = moves of every
-   electron, full evaluation of walkers that moved, reweighting; branching =
-   split_join with its own stream. */
+/* DMC (qmc/rundmc_mod.f90; drift-diffusion mover with pseudopotentials,
+   drift-diffusion with the nucleus cusp for all-electron systems): a
+   population of up to cap walkers with per-walker rannyu streams; one
+   generation = moves of every electron, full evaluation of walkers that
+   moved, reweighting; branching = split_join with its own stream. */
+/* The all-electron mover (qmc/green_functions_mod.f90:83-145, 268-323) of an
+   electron at xe[3] with gradient g[3]: a sample xn[3] (NULL: none) drawn
+   from rng, and the proposal density at pt[3] (NULL: returns 0). */
+double orc_cusp_move(const orc_sys *o, double tau, int32_t ndrift, const double *g, const double *xe,
+                     int32_t *rng, double *xn, const double *pt);
+
 typedef struct orc_dmc orc_dmc;
 orc_dmc *orc_dmc_create(const orc_sys *o, int32_t cap);
 void orc_dmc_destroy(orc_dmc *d);

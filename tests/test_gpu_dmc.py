@@ -1,7 +1,9 @@
 """GPU parity of DMC generations (qmcw_dmc_step) against the oracle's DMC on
 the same population, random streams and branching stream.
 
-Per generation: accepted-move and node-crossing counts and the population
+Both movers of sample_1e_move are covered: drift-diffusion for the
+pseudopotential system and drift-diffusion with the nucleus cusp for the
+all-electron one.  Per generation: accepted-move and node-crossing counts and the population
 after split_join are identical; population sums, weights and positions agree
 to 1e-9 relative / 1e-11 absolute (the gaussian proposals go through cos and
 log, whose last bits may differ between the GPU and the host library);
@@ -25,8 +27,11 @@ def _setup(s, n0, cap, seed):
 
 
 @pytest.mark.parametrize("tau", [0.02, 0.1])
-def test_dmc_generations_match_oracle(tau):
-    s = system.c2_dimer(2)
+@pytest.mark.parametrize("name", ["c2_dimer", "be_atom"])
+def test_dmc_generations_match_oracle(name, tau):
+    """c2_dimer: pseudopotential mover; be_atom: all-electron mover with the
+    nucleus cusp (gaussian or exponential branch per proposal)."""
+    s = system.c2_dimer(2) if name == "c2_dimer" else system.be_atom(1)
     n0, cap = 20, 48
     x, seeds = _setup(s, n0, cap, seed=31)
     O = oracle.Oracle(s)
@@ -58,14 +63,6 @@ def test_dmc_generations_match_oracle(tau):
         assert np.allclose(wg, go["weight"], rtol=1e-9, atol=0)
         assert np.allclose(xg, go["x"], rtol=0, atol=1e-11)
         assert np.array_equal(e.get_branch_rng(), bo)
-
-
-def test_dmc_requires_pseudopotential():
-    s = system.be_atom(1)
-    e = Engine(s, 8)
-    e.set_walkers(system.random_walkers(s, 8, seed=1))
-    with pytest.raises(Exception):
-        e.dmc_begin(8)
 
 
 class _OracleStepper:

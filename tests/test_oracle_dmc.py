@@ -8,6 +8,13 @@
   split_join       total weight is conserved, no walker keeps weight zero,
                    population = n - kills - joins + splits, and every walker
                    after branching is a copy of one before
+  cusp mover       (all-electron systems) the proposal density is the
+                   density its sampler draws from (E_s[f/p] = 1 for
+                   normalized f); far from the nucleus and moving away it is
+                   the textbook drift-diffusion gaussian about x + v tau; when
+                   the drift crosses the nucleus it is centred on the nucleus
+                   (isotropic); it is invariant under rotations; and the
+                   Metropolis walk built on it samples |Psi|^2 (<r> = 3/(2Z))
 """
 import numpy as np
 import pytest
@@ -40,9 +47,18 @@ def _hydrogenic_pp(Z):
                          nquad=12, **barr)
 
 
-@pytest.mark.parametrize("Z", [1, 2])
-def test_exact_wavefunction_weights(Z):
+def _hydrogenic_ae(Z):
+    """The same exact 1s state as an all-electron system (V = -Z/r), which
+    the reference moves with the nucleus-cusp mover."""
     s = _hydrogenic_pp(Z)
+    s.nloc = 0
+    return s
+
+
+@pytest.mark.parametrize("Z", [1, 2])
+@pytest.mark.parametrize("ae", [False, True])
+def test_exact_wavefunction_weights(Z, ae):
+    s = _hydrogenic_ae(Z) if ae else _hydrogenic_pp(Z)
     W = 64
     x = system.random_walkers(s, W, seed=3, sigma=1.0 / Z)
     seeds = rng.walker_seeds("1837465927472523", W)
@@ -61,9 +77,10 @@ def test_exact_wavefunction_weights(Z):
         assert stats.nacc > 0 and stats.ntry == W
 
 
-def test_drift_diffusion_samples_the_ground_state():
+@pytest.mark.parametrize("ae", [False, True])
+def test_drift_diffusion_samples_the_ground_state(ae):
     Z = 1
-    s = _hydrogenic_pp(Z)
+    s = _hydrogenic_ae(Z) if ae else _hydrogenic_pp(Z)
     W = 2000
     x = system.random_walkers(s, W, seed=5, sigma=1.0)
     seeds = rng.walker_seeds("1122334455667789", W)
@@ -161,3 +178,95 @@ def test_dmc_recovers_the_exact_energy_of_a_nodeless_state():
     est, err = blocks.mean(), blocks.std() / np.sqrt(len(blocks))
     assert abs(est + 0.5) < max(4 * err, 0.004), (est, err, evmc)
     assert 0.3 * W0 < h[-1]["nwalk"] < 2 * W0      # population control holds
+
+
+# ---- the all-electron mover (qmc/green_functions_mod.f90:83-145, 268-323)
+
+def _stream(seed="1837465927472523"):
+    return np.asarray(rng.walker_seeds(seed, 1), np.int32)[0].copy()
+
+
+def _gauss3(d, var):
+    return np.exp(-np.dot(d, d) / (2 * var)) / (2 * np.pi * var) ** 1.5
+
+
+def test_cusp_mover_density_matches_its_sampler():
+    """Electron 0.1 bohr from a Z=2 nucleus with the drift pointing inward,
+    drift length ~ distance, so both branches are drawn (pgaus near 1/2).  For densities f normalized in
+    closed form, a gaussian about the electron narrower than sqrt(tau) and an
+    exponential about the nucleus sharper than the mover's, f/p is bounded
+    and E_s[f/p] = 1 when p is the density the sampler draws from."""
+    Z, tau = 2.0, 0.05
+    o = oracle.Oracle(_hydrogenic_ae(Z))
+    xe = np.array([0.03, -0.05, 0.08])
+    g = -Z * xe / np.linalg.norm(xe) + np.array([0.3, 0.1, -0.2])
+    st = _stream()
+    n, var, zf = 30000, 0.5 * tau, 8.0
+    r1, r2 = np.empty(n), np.empty(n)
+    for i in range(n):
+        xn, _ = o.cusp_move(tau, g, xe, rng=st)
+        _, p = o.cusp_move(tau, g, xe, pt=xn)
+        r = np.linalg.norm(xn)
+        r1[i] = _gauss3(xn - xe, var) / p
+        r2[i] = zf ** 3 * np.exp(-2 * zf * r) / np.pi / p
+    for est in (r1, r2):
+        m, err = est.mean(), est.std() / np.sqrt(n)
+        assert err < 0.01 and abs(m - 1) < 5 * err, (m, err)
+
+
+def test_cusp_mover_far_from_the_nucleus_is_drift_diffusion():
+    """Electron 2.4 bohr from the nucleus, moving away (g . xe > 0): zf > zi,
+    pgaus = 1 to machine precision and the bent drift point is x + v tau, so
+    the density is the plain drift-diffusion gaussian with the velocity of
+    averaged_1e_velocity (green_functions_mod.f90:616-630) for this frame:
+    adrift = (1 + 1e-10 + cos(g, xe)) / 2 + 0.1 h / (1 + h), h = (Z r / 2)^2,
+    |v| = (sqrt(1 + 2 adrift |g|^2 tau) - 1) / (adrift |g| tau)."""
+    Z, tau = 2.0, 0.01
+    o = oracle.Oracle(_hydrogenic_ae(Z))
+    xe = np.array([1.2, -0.7, 2.0])
+    g = 0.8 * xe / np.linalg.norm(xe) + np.array([0.5, 0.9, -0.3])
+    r, gn = np.linalg.norm(xe), np.linalg.norm(g)
+    assert g @ xe > 0
+    h = (Z * r / 2) ** 2
+    a = (1 + 1e-10 + (g @ xe) / (gn * r)) / 2 + 0.1 * h / (1 + h)
+    vav = (np.sqrt(1 + 2 * a * gn ** 2 * tau) - 1) / (a * gn * tau)
+    xd = xe + vav * g / gn * tau
+    rs = np.random.default_rng(4)
+    for _ in range(10):
+        pt = xd + rs.normal(size=3) * np.sqrt(tau)
+        _, p = o.cusp_move(tau, g, xe, pt=pt)
+        assert abs(p - _gauss3(pt - xd, tau)) <= 1e-12 * p
+
+
+def test_cusp_mover_toward_the_nucleus_is_centred_on_it():
+    """Drift past the nucleus (zf < 0): the electron cannot drift through
+    it, so the drift point is the nucleus and both branches are isotropic
+    about it: p(c + d u) is the same for every unit vector u."""
+    Z, tau = 2.0, 0.05
+    o = oracle.Oracle(_hydrogenic_ae(Z))
+    xe = np.array([0.01, 0.02, -0.015])
+    g = -3.0 * xe / np.linalg.norm(xe) + np.array([0.2, -0.4, 0.1])
+    rs = np.random.default_rng(5)
+    for d in (0.02, 0.1, 0.3):
+        ps = []
+        for _ in range(6):
+            u = rs.normal(size=3)
+            _, p = o.cusp_move(tau, g, xe, pt=d * u / np.linalg.norm(u))
+            ps.append(p)
+        assert np.ptp(ps) <= 1e-12 * max(ps), ps
+
+
+def test_cusp_mover_rotation_invariance():
+    """Rotating the electron, its gradient and the target point about the
+    nucleus leaves the density unchanged."""
+    Z, tau = 2.0, 0.05
+    o = oracle.Oracle(_hydrogenic_ae(Z))
+    rs = np.random.default_rng(6)
+    for _ in range(5):
+        q, _ = np.linalg.qr(rs.normal(size=(3, 3)))
+        xe = rs.normal(size=3) * 0.3
+        g = rs.normal(size=3) * 2
+        pt = xe + rs.normal(size=3) * 0.2
+        _, p0 = o.cusp_move(tau, g, xe, pt=pt)
+        _, p1 = o.cusp_move(tau, q @ g, q @ xe, pt=q @ pt)
+        assert abs(p1 - p0) <= 1e-12 * p0
