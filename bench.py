@@ -302,10 +302,15 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
     return dict(aggregate=agg, per_system=per, seconds=dt)
 
 
-def run_dmc_chain(W0, steps, warmup, precision, tau=0.01):
+def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20):
     """DMC of the 98-electron chain (16 Z_eff=4 centres with a
     pseudopotential, 34 H, 424 GTOs, one determinant): W0 walkers, capacity
-    2 W0, time step tau, under the rundmc driver (population control)."""
+    2 W0, time step tau, under the rundmc driver (population control),
+    started, as a DMC run starts from VMC configurations, from `equil` VMC
+    sweeps of the random initial walkers (step 0.3 bohr).  The synthetic
+    wavefunction (random orthonormal MOs) is far from the ground state, so
+    its DMC population drifts over hundreds of generations; the timed
+    generations are few and their population range is reported."""
     from champ_amd import rng, system
     from champ_amd.dmc import DmcDriver
     from champ_amd.engine import Engine
@@ -316,6 +321,8 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01):
     e.set_rng(rng.walker_seeds("1837465927472523", cap, stride=1 << 20))
     e.set_branch_rng(np.array([1, 2, 3, 5], np.int32))
     g = e.autotune()
+    if equil:
+        e.vmc_device(equil, 0.3)
     e.dmc_begin(W0)
     e0 = float(e.get_eloc()[:W0].mean())
     drv = DmcDriver(e, s.nelec, W0, tau=tau, etrial=e0)
@@ -333,7 +340,9 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01):
     prof = e.profile_get()
     e.profile(False)
     e.close()
+    nw = [x["nwalk"] for x in h]
     return dict(rate=moves / dt, G=g, nwalk_final=h[-1]["nwalk"], eave=h[-1]["eave"],
+                nwalk_range=[min(nw), round(float(np.mean(nw)), 1), max(nw)],
                 acc=float(np.mean([x["acc"] for x in h])), seconds=dt, prof=prof)
 
 
@@ -358,7 +367,8 @@ def copy_calibration(nbytes=256 << 20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 200 VMC steps; 20 DMC generations with --dmc)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--walkers", type=int, default=16384)
     ap.add_argument("--delta", type=float, default=0.6)
@@ -379,6 +389,8 @@ def main():
         print("bench.py measures one GPU: this build runs and reports a single GPU; "
               "multi-GPU runs (walker sharding) are not part of it.", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 20 if args.dmc else 200
     if args.dmc:
         prec = "mixed" if args.precision == "both" else args.precision
         d = run_dmc_chain(4096, args.steps, args.warmup, prec)
@@ -390,9 +402,11 @@ def main():
                           "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
                           "warmup": args.warmup, "higher_is_better": True,
                           "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
-                          "config": {"workload": "DMC, 98-electron chain, 4096 walkers, tau 0.01",
+                          "config": {"workload": "DMC, 98-electron chain, 4096 walkers (target), "
+                                                 "tau 0.01, from 20 VMC sweeps",
                                      "precision": prec, "group_width": d["G"]},
-                          "nwalk_final": d["nwalk_final"], "acceptance": d["acc"],
+                          "nwalk_final": d["nwalk_final"], "nwalk_min_mean_max": d["nwalk_range"],
+                          "acceptance": d["acc"],
                           "seconds": d["seconds"],
                           "kernel_avg_us": {k: (ms / n * 1e3 if n else None)
                                             for k, (ms, n) in d["prof"].items()}}), flush=True)

@@ -10,6 +10,9 @@ Builders:
   * :func:`from_champ_input` -- CHAMP's keyword-menu input (the subset used by
     finite molecules with analytic Slater/Gaussian bases), e.g.
     ``examples/be_vb1_jcas24_vmc_j_opt_lin.inp``;
+  * :func:`from_champ_old_input` -- CHAMP's old line-oriented input (finite
+    all-electron systems, Slater s/p basis), e.g.
+    ``examples/c2_ktt_hfj_vmc_emin_j_opt3_lin.inp``;
   * :func:`be_atom`, :func:`c2_dimer`, :func:`hchain` and
     :func:`random_molecule` -- the seeded synthetic workloads of the benchmark
     configurations.
@@ -557,6 +560,158 @@ def from_champ_input(text: str, name: str = "champ") -> System:
                   nocuspb=int(kvj.get("nocuspb", 0)), norda=norda, nordb=nordb,
                   nordc=nordc, scalek=float(kvj.get("scalek", 0.5)), a4=a4, b=b,
                   c=c, nloc=0, name=name, **barr)
+
+
+class _OldReader:
+    """Fortran list-directed records of CHAMP's old line-oriented input: each
+    read takes the numeric prefix of lines (the trailing text is the
+    variable-name comment) and continues on the next line while values are
+    missing."""
+
+    def __init__(self, lines, i=0):
+        self.lines, self.i = lines, i
+
+    def seek(self, key):
+        while key not in self.lines[self.i]:
+            self.i += 1
+        self.i += 1
+
+    def _numeric(self, ln):
+        vals = []
+        for tok in ln.split():
+            try:
+                vals.append(float(tok.lower().replace("d", "e")))
+            except ValueError:
+                break
+        return vals
+
+    def read(self, n=None):
+        """One record: n values (None: the numbers on the next non-empty line)."""
+        vals = []
+        while True:
+            ln = self.lines[self.i]
+            self.i += 1
+            if not ln.strip():
+                continue
+            vals.extend(self._numeric(ln))
+            if n is None or len(vals) >= n:
+                break
+        if n is not None:
+            if len(vals) < n:
+                raise ValueError(f"record of {n} values expected, got {len(vals)}: {ln!r}")
+            vals = vals[:n]          # a record's surplus values are skipped
+        return vals
+
+    def ints(self, n=None):
+        return [int(v) for v in self.read(n)]
+
+
+def from_champ_old_input(text: str, name: str = "champ") -> System:
+    """Build a System from CHAMP's old line-oriented input (read_input.f90;
+    e.g. examples/c2_ktt_hfj_vmc_emin_j_opt3_lin.inp).
+
+    Supported: finite all-electron systems (nloc = 0), analytic Slater basis
+    (ibasis = 1, positive counts) of s and p functions in the order of
+    orb_loc_ana_gamess_order (numr = -1: all s by n, then the px, py and pz
+    blocks by n; qmc/vmc/read_orb_loc.f90:843-1000), orbitals given by
+    coefficients, determinants/CSFs, ijas = 4 Jastrow (isc = 2 or 4).  The
+    dependent e-e-n coefficients are re-imposed with cuspexact4.
+    """
+    R = _OldReader(text.splitlines())
+    R.seek("iperiodic,ibasis")
+    R.i -= 1
+    iperiodic, ibasis = R.ints(2)
+    if iperiodic != 0 or ibasis != 1:
+        raise ValueError("finite systems with a localized analytic basis (iperiodic 0, ibasis 1) only")
+    R.seek("nfprod,tau")
+    nloc, numr = R.ints(4)[:2]
+    if nloc != 0:
+        raise ValueError("pseudopotential inputs are built with the synthetic builders")
+    if numr != -1:
+        raise ValueError("numr = -1 (analytic basis in the GAMESS order) only")
+    nelec, nup = R.ints(2)
+    R.seek("Geometry section")
+    if R.ints(1)[0] != 3:
+        raise ValueError("ndim = 3 only")
+    nctype, ncent = R.ints(2)
+    iwctype = np.array(R.ints(ncent), np.int32) - 1
+    znuc = np.array(R.read(nctype))
+    cent = np.array([R.read(3) for _ in range(ncent)])
+    R.seek("Determinantal section")
+    R.read()                                          # inum_orb
+    ndet, nbasis, norb = R.ints(3)
+    counts = []
+    for _ in range(nctype):
+        v = R.ints()
+        if len(v) < 18:
+            raise ValueError("basis count record too short")
+        if any(c < 0 for c in v):
+            raise ValueError("gaussian (negative) basis counts are not supported")
+        if any(v[9:14]) or any(v[18:]):
+            raise ValueError("only s and p basis functions are supported in the old format")
+        # n1s, n2s, n2p(x,y,z), n3s, n3p(x,y,z), n3d(5), n4s, n4p(x,y,z)
+        counts.append(dict(s={1: v[0], 2: v[1], 3: v[5], 4: v[14]},
+                           p={m: {2: v[2 + k], 3: v[6 + k], 4: v[15 + k]}
+                              for k, m in enumerate((1, -1, 0))}))
+    coef = np.array([R.read(nbasis) for _ in range(norb)])
+    zex = R.read(nbasis)
+    # functions (n, l, m) per centre type in the order of the coefficients
+    funcs = []
+    for cn in counts:
+        f = []
+        for n in (1, 2, 3, 4):
+            f += [(n, 0, 0)] * cn["s"][n]
+        for m in (1, -1, 0):
+            for n in (2, 3, 4):
+                f += [(n, 1, m)] * cn["p"][m][n]
+        funcs.append(f)
+    if sum(len(funcs[t]) for t in iwctype) != nbasis:
+        raise ValueError("basis counts do not add up to nbasis")
+    bb = _BasisBuilder(QMCW_BASIS_SLATER, nctype)
+    first = {}
+    ib = 0
+    for ic, t in enumerate(iwctype):
+        z = zex[ib:ib + len(funcs[t])]
+        ib += len(funcs[t])
+        if t in first:
+            if not np.allclose(first[t], z, rtol=0, atol=0):
+                raise ValueError("centres of one type must share their exponents")
+            continue
+        first[t] = z
+        for (n, l, m), zz in zip(funcs[t], z):
+            bb.add_function(t, bb.add_radial(n, [float(zz)], [1.0]), l, m)
+    barr, anorm = bb.arrays(iwctype)
+    coef = coef * anorm[None, :]
+    dets = [[x - 1 for x in R.ints(nelec)] for _ in range(ndet)]
+    det_up = [d[:nup] for d in dets]
+    det_dn = [d[nup:] for d in dets]
+    ncsf = R.ints(1)[0]
+    csf_coef = R.read(ncsf)
+    ndic = R.ints(ncsf)
+    csfs = []
+    for k in range(ncsf):
+        ds = [x - 1 for x in R.ints(ndic[k])]
+        cs = R.read(ndic[k])
+        csfs.append((csf_coef[k], list(zip(ds, cs))))
+    cc, off, cdet, cdc = _csf_arrays(csfs, False)
+    det_up, det_dn, cdc = sort_dets(det_up, det_dn, cdet, cdc)
+    R.seek("Jastrow section")
+    R.read()                                          # ianalyt_lap
+    ijas, isc, nspin1, nspin2 = R.ints(6)[:4]
+    if ijas != 4:
+        raise ValueError("only ijas=4 is supported")
+    norda, nordb, nordc = R.ints(3)
+    scalek = R.read(2)[0]
+    nspin2b = nspin2 - nspin1 + 1
+    a4 = np.array([R.read(norda + 1) for _ in range(nctype)])
+    b = np.array([R.read(nordb + 1) for _ in range(nspin2b)])
+    c = np.array([R.read(nterms4(nordc)) for _ in range(nctype)])
+    c = cusp_exact(c, nordc)
+    return System(nelec=nelec, nup=nup, cent=cent, iwctype=iwctype, znuc=znuc,
+                  coef=coef, det_up=det_up, det_dn=det_dn, csf_coef=cc,
+                  csf_det_off=off, csf_det=cdet, csf_det_coef=cdc, isc=isc,
+                  nspin2b=nspin2b, norda=norda, nordb=nordb, nordc=nordc,
+                  scalek=scalek, a4=a4, b=b, c=c, nloc=0, name=name, **barr)
 
 
 def jastrow_opt_list(sysm: System, vary_a1: bool = False):

@@ -25,8 +25,13 @@ def _systems():
     p = [r for r in fx["runs"] if r["iteration"] == 4][0]["params"]
     be.a4 = np.array([p["a"]]); be.b = np.array([p["b"]])
     be.c = system.cusp_exact(np.array([p["c"]]), be.nordc)
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c2_ktt_transcript.json")))
+    c2k = system.from_champ_old_input(fx["input"], "c2_ktt")
+    p = [r for r in fx["runs"] if r["iteration"] == 3][0]["params"]
+    c2k.a4 = np.array([p["a"]]); c2k.b = np.array([p["b"]])
+    c2k.c = system.cusp_exact(np.array([p["c"]]), c2k.nordc)
     return {"be": be, "be_atom": system.be_atom(1), "c2": system.c2_dimer(2),
-            "chain4": system.hchain(4, ncarbon=4)}
+            "chain4": system.hchain(4, ncarbon=4), "c2_ktt": c2k}
 
 
 SYS = _systems()

@@ -22,7 +22,9 @@ formula:
   parameter derivatives    central finite differences (h = 1e-5) of ln Psi
                            and E_L for CSF, Jastrow (incl. dependent c's) and
                            orbital-rotation parameters
-  VMC                      energies of CHAMP's Be transcript within 3 sigma
+  VMC                      energies of CHAMP's Be transcript within 3 sigma;
+                           energies and sigmas of its all-electron C2
+                           transcript (old-format input, isc = 2) likewise
 """
 import ctypes as C
 import json
@@ -496,6 +498,53 @@ def test_vmc_energy_matches_champ_transcript(iteration):
     sig = math.hypot(err, run["error"])
     assert abs(e.mean() - run["energy"]) < 3 * sig, (e.mean(), err, run)
     assert abs(e.std() - run["sigma"]) < 0.01
+
+
+def _c2_ktt(iteration):
+    fx = json.load(open(os.path.join(HERE, "golden", "c2_ktt_transcript.json")))
+    s = system.from_champ_old_input(fx["input"], "c2_ktt")
+    run = [r for r in fx["runs"] if r["iteration"] == iteration][0]
+    p = run["params"]
+    if p:
+        s.a4 = np.array([p["a"]])
+        s.b = np.array([p["b"]])
+        s.c = system.cusp_exact(np.array([p["c"]]), s.nordc)
+    return s, run
+
+
+def test_old_format_input_reads_the_c2_example():
+    """examples/c2_ktt_hfj_vmc_emin_j_opt3_lin.inp through the old-format
+    reader: sizes, the GAMESS-order basis (6 s, then 4 px, 4 py, 4 pz per
+    centre, as CHAMP's printout lists them) and the exponents it printed as
+    screening constants."""
+    s, _ = _c2_ktt(1)
+    assert (s.nelec, s.nup, s.ncent, s.nbasis, s.norb, s.isc) == (12, 6, 2, 36, 36, 2)
+    assert s.bas_l.tolist() == [0] * 6 + [1] * 12
+    assert s.bas_m.tolist() == [0] * 6 + [1] * 4 + [-1] * 4 + [0] * 4
+    assert s.rad_n.tolist() == [1, 1, 2, 2, 2, 3] + [2] * 12
+    assert np.allclose(s.prim_zeta[:10], [5.450236, 9.584061, 1.304429, 1.899079, 3.753271,
+                                          3.753486, 1.058381, 1.648879, 2.808762, 6.781327])
+    assert s.det_up.tolist() == [[0, 1, 2, 3, 4, 5]] and s.det_dn.tolist() == [[0, 1, 2, 3, 4, 5]]
+
+
+@pytest.mark.parametrize("iteration", [1, 2, 3])
+def test_vmc_energy_matches_champ_c2_transcript(iteration):
+    """All-electron C2 (12 electrons, 36 Slater functions, isc = 2 Jastrow)
+    from the old-format input: the oracle's VMC energy and sigma against the
+    runs of CHAMP's transcript (iteration 1: the input's Jastrow; 2, 3: the
+    optimised ones it printed), within 3 sigma of the combined errors."""
+    s, run = _c2_ktt(iteration)
+    O = oracle.Oracle(s)
+    nw, nstep = 256, 400
+    x = system.random_walkers(s, nw, seed=5)
+    seeds = rng.walker_seeds("1837465927472523", nw)
+    r = O.vmc_run(x, seeds, nstep, 1.0, nthreads=8)
+    e = r["eloc"][100:]
+    blk = e.reshape(-1, 20, nw).mean(axis=1).mean(axis=1)
+    err = blk.std(ddof=1) / math.sqrt(len(blk))
+    sig = math.hypot(err, run["error"])
+    assert abs(e.mean() - run["energy"]) < 3 * sig, (e.mean(), err, run)
+    assert abs(e.std() - run["sigma"]) < 0.1, (e.std(), run)
 
 
 def test_accum_products_against_numpy():
