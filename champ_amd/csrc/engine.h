@@ -115,6 +115,7 @@ struct SLayout {
   int64_t bt;                // [N][M] e-e-n variable 1 - k scaled(r_j,ic) (as the kernel's T)
   int64_t uq;                // [64][N] per-lane e-e-n variable of the pairs of a point
   int64_t plist;             // [N*M] ints: (electron, centre) pairs in pseudopotential range
+  int64_t prot;              // [N*M][9] DMC: axes of each in-range pair's grid rotation
   int64_t mv;                // pending one-electron move buffers
   int64_t stride;
 };

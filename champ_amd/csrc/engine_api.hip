@@ -967,6 +967,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SL.bt = take((int64_t)N * M);
   SL.uq = take((int64_t)kWave * N);
   SL.plist = take(((int64_t)N * M + 1) / 2);
+  SL.prot = take(S.nloc > 0 ? 9 * (int64_t)N * M : 1);
   SL.mv = take(ML.stride);
   if (S.nparm > 0) {
     DLayout &DL = h->A.DL;
@@ -1602,7 +1603,11 @@ extern "C" int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weigh
   std::vector<double> wt(h->Wp, 1.0);
   if (weight) std::copy(weight, weight + nwalk, wt.begin());
   HIPCHK(h, hipMemcpyAsync(h->dwt, wt.data(), h->Wp * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  // DMC-mode evaluation (per-pair grid rotations) of every walker
+  h->A.dmc_rot = h->A.S.nloc > 0 ? 1 : 0;
+  h->A.dmove = nullptr;
   launch_eval(h, false);
+  h->A.dmc_rot = 0;
   hipLaunchKernelGGL(k_dmc_init, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A, h->dm,
                      (const double *)h->dwt, nwalk);
   HIPCHK(h, hipGetLastError());
@@ -1709,7 +1714,13 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
     hipLaunchKernelGGL(k_dmc_decide, g1, b1, 0, h->stream, h->A, ie, h->dm, h->acc_buf, *p);
     DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
   }
+  // DMC-mode evaluation: walkers that moved draw a grid rotation per pair
+  h->A.dmc_rot = h->A.S.nloc > 0 ? 1 : 0;
+  h->A.dmove = h->dm + kDmNacc;
+  h->A.dmove_stride = kDmStride;
   launch_eval(h, false);
+  h->A.dmc_rot = 0;
+  h->A.dmove = nullptr;
   hipLaunchKernelGGL(k_dmc_finish, g1, b1, 0, h->stream, h->A, h->dm, *p);
   hipLaunchKernelGGL(k_dmc_sums, dim3(1), dim3(256), 0, h->stream, h->A, (const double *)h->dm,
                      h->nact, h->dstat);

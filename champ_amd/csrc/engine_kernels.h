@@ -29,6 +29,9 @@ struct KArgs {
   int lds_dnlpp;     // doubles per lane of k_deriv_nlpp's point scratch in LDS (0: global)
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
+  int dmc_rot;       // DMC evaluation: pseudopotential grid rotated per pair in range
+  const double *dmove;  // DMC: accepted moves of the generation at dmove[w * dmove_stride]
+  int dmove_stride;     // (null: every walker evaluates and draws)
 };
 
 // occupancy hint of the heavy per-walker kernels (waves per SIMD)
@@ -571,6 +574,43 @@ void k_full_det(KArgs A) {
   if (S.nloc > 0) group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
 }
 
+// rotqua (qmc/vmc/rotqua.f90): a random rotation drawn from the stream st4,
+// as the axes (x1, x2, x3), (y1, y2, y3), (z1, z2, z3) that take the
+// reference grid q0 to q = q0x x + q0y y + q0z z.
+__device__ __forceinline__ void rot_axes(int *st4, double *ax) {
+  double x1, x2, x3, xsum;
+  do {
+    x1 = 1.0 - 2.0 * rannyu(st4);
+    x2 = 1.0 - 2.0 * rannyu(st4);
+    xsum = x1 * x1 + x2 * x2;
+  } while (xsum >= 1.0);
+  double xsum2 = 2.0 * sqrt(fabs(1.0 - xsum));
+  x1 *= xsum2;
+  x2 *= xsum2;
+  x3 = 1.0 - 2.0 * xsum;
+  double theta = acos(x3), sthet = sin(theta), cfi, sfi;
+  if (sthet < 1e-5) {
+    cfi = 1; sfi = 0; x1 = 0; x2 = 0; x3 = 1; sthet = 0;
+  } else {
+    cfi = x1 / sthet;
+    sfi = x2 / sthet;
+  }
+  double yy1 = x3 * cfi, yy2 = x3 * sfi, yy3 = -sthet;
+  double zz1 = -sfi, zz2 = cfi, zz3 = 0;
+  double u1, u2, usum;
+  do {
+    u1 = rannyu(st4) * 2.0 - 1.0;
+    u2 = rannyu(st4) * 2.0 - 1.0;
+    usum = u1 * u1 + u2 * u2;
+  } while (usum >= 1.0);
+  double uu = sqrt(usum);
+  u1 /= uu;
+  u2 /= uu;
+  ax[0] = x1; ax[1] = x2; ax[2] = x3;
+  ax[3] = yy1 * u1 + zz1 * u2; ax[4] = yy2 * u1 + zz2 * u2; ax[5] = yy3 * u1 + zz3 * u2;
+  ax[6] = yy1 * u2 - zz1 * u1; ax[7] = yy2 * u2 - zz2 * u1; ax[8] = yy3 * u2 - zz3 * u1;
+}
+
 // Pseudopotential quadrature points of the in-range pairs plist[0..npair)
 // taken by this lane: sum of w exp(dJ) v P_l (Psi_D ratio).  rnew: 4M
 // doubles and uq: N values of per-lane scratch (LDS or global).
@@ -590,12 +630,23 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
   const int nq = S.nquad;
   double *rvnew = rnew + M;
   double vpsp = 0.0;
+  const double *prot = ss + SL.prot;
   for (int t = lane; t < npair * nq; t += G) {
     const int iq = t % nq, p = plist[t / nq], ic = p % M, i = p / M;
     const double *v = vnl + p * (QMCW_MAX_LPOT + 1);
     const int ict = cst(S.iwctype)[ic];
     const double rei = ren[p];
-    const double qx = quad[iq], qy = quad[nq + iq], qz = quad[2 * nq + iq];
+    double qx, qy, qz;
+    if (A.dmc_rot) {
+      // this pair's own rotation
+      const double *ax = prot + 9 * (t / nq);
+      const double a = cst(S.q0)[iq], b = cst(S.q0)[nq + iq], c = cst(S.q0)[2 * nq + iq];
+      qx = a * ax[0] + b * ax[3] + c * ax[6];
+      qy = a * ax[1] + b * ax[4] + c * ax[7];
+      qz = a * ax[2] + b * ax[5] + c * ax[8];
+    } else {
+      qx = quad[iq]; qy = quad[nq + iq]; qz = quad[2 * nq + iq];
+    }
     const double *rv0 = rven + 3 * p;
     const double costh = (rv0[0] * qx + rv0[1] * qy + rv0[2] * qz) / rei;
     const double xn[3] = {rei * qx + cst(S.cent)[3 * ic], rei * qy + cst(S.cent)[3 * ic + 1],
@@ -672,6 +723,18 @@ void k_full_nlpp(KArgs A) {
     // quadrature points only of the pairs in range, spread evenly over the lanes
     int *plist = reinterpret_cast<int *>(ss + SL.plist);
     const int npair = group_compact<G>(vnl + QMCW_MAX_LPOT, QMCW_MAX_LPOT + 1, N * M, plist, lane);
+    // DMC (qmc/vmc/nonloc.f90:89): every pair in range rotates the grid anew,
+    // drawing from the walker's stream in pair order.  rotqua's rejection
+    // loops make the number of draws vary, so one lane walks the stream and
+    // keeps each pair's axes; walkers that moved no electron are not
+    // evaluated by the reference and draw nothing.
+    if (A.dmc_rot && lane == 0 && (!A.dmove || A.dmove[(int64_t)w * A.dmove_stride] > 0.0)) {
+      int *rs = A.rng + 4 * w;
+      double *pax = ss + SL.prot;
+      int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+      for (int k = 0; k < npair; k++) rot_axes(st4, pax + 9 * k);
+      rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+    }
     __syncthreads();
     // per-lane distances of the point and U of the other electrons: LDS
     // when the launch reserved it
@@ -1522,43 +1585,15 @@ __global__ void k_rotqua(KArgs A) {
   const DevSys &S = A.S;
   int *rs = A.rng + 4 * w;
   int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
-  double x1, x2, x3, xsum;
-  do {
-    x1 = 1.0 - 2.0 * rannyu(st4);
-    x2 = 1.0 - 2.0 * rannyu(st4);
-    xsum = x1 * x1 + x2 * x2;
-  } while (xsum >= 1.0);
-  double xsum2 = 2.0 * sqrt(fabs(1.0 - xsum));
-  x1 *= xsum2;
-  x2 *= xsum2;
-  x3 = 1.0 - 2.0 * xsum;
-  double theta = acos(x3), sthet = sin(theta), cfi, sfi;
-  if (sthet < 1e-5) {
-    cfi = 1; sfi = 0; x1 = 0; x2 = 0; x3 = 1; sthet = 0;
-  } else {
-    cfi = x1 / sthet;
-    sfi = x2 / sthet;
-  }
-  double yy1 = x3 * cfi, yy2 = x3 * sfi, yy3 = -sthet;
-  double zz1 = -sfi, zz2 = cfi, zz3 = 0;
-  double u1, u2, usum;
-  do {
-    u1 = rannyu(st4) * 2.0 - 1.0;
-    u2 = rannyu(st4) * 2.0 - 1.0;
-    usum = u1 * u1 + u2 * u2;
-  } while (usum >= 1.0);
-  double uu = sqrt(usum);
-  u1 /= uu;
-  u2 /= uu;
-  double y1 = yy1 * u1 + zz1 * u2, y2 = yy2 * u1 + zz2 * u2, y3 = yy3 * u1 + zz3 * u2;
-  double z1 = yy1 * u2 - zz1 * u1, z2 = yy2 * u2 - zz2 * u1, z3 = yy3 * u2 - zz3 * u1;
+  double ax[9];
+  rot_axes(st4, ax);
   int nq = S.nquad;
   double *quad = A.st + (int64_t)w * A.L.stride + A.L.quad;
   for (int iq = 0; iq < nq; iq++) {
     double a = cst(S.q0)[iq], b = cst(S.q0)[nq + iq], c = cst(S.q0)[2 * nq + iq];
-    quad[iq] = a * x1 + b * y1 + c * z1;
-    quad[nq + iq] = a * x2 + b * y2 + c * z2;
-    quad[2 * nq + iq] = a * x3 + b * y3 + c * z3;
+    quad[iq] = a * ax[0] + b * ax[3] + c * ax[6];
+    quad[nq + iq] = a * ax[1] + b * ax[4] + c * ax[7];
+    quad[2 * nq + iq] = a * ax[2] + b * ax[5] + c * ax[8];
   }
   rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
 }

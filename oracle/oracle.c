@@ -1651,7 +1651,7 @@ static double jas_1e_value(const orc_sys *o, orc_walker *w, int iel,
    psp_nonloc_pot_bld (qmc/deriv_fast_mod.f90:841-855); with derivs, the
    Jastrow dpe of qmc/fit/deriv_nonloc_pot.f90:59-64 */
 static void nonloc_pot_w(const orc_sys *o, orc_walker *w, double *pe,
-                         int derivs) {
+                         int derivs, int32_t *prng) {
   int N = o->nelec, M = o->ncent, NO = o->norb, nup = o->nup, npj = o->npj;
   double vps[64][QMCW_MAX_LPOT];
   double *orbe = w->orbq, *gd = w->gd, *gtmp = w->scr_basis;
@@ -1684,14 +1684,22 @@ static void nonloc_pot_w(const orc_sys *o, orc_walker *w, double *pe,
         if (l != o->lpotp1[ict] && fabs(vps[ic][l - 1]) > 1e-4) iskip = 0;
       }
       if (iskip) continue;
+      /* DMC: a fresh rotation of the grid for every pair in range
+         (qmc/vmc/nonloc.f90:89, l_mode_dmc), drawn from the walker's stream */
+      double pxq[QMCW_MAX_NQUAD], pyq[QMCW_MAX_NQUAD], pzq[QMCW_MAX_NQUAD];
+      const double *qx = w->xq, *qy = w->yq, *qz = w->zq;
+      if (prng) {
+        orc_rotqua(prng, o->xq0, o->yq0, o->zq0, o->nquad, pxq, pyq, pzq);
+        qx = pxq; qy = pyq; qz = pzq;
+      }
       double rei = w->r_en[i * M + ic];
       double ri = 1.0 / rei;
       for (int iq = 0; iq < o->nquad; iq++) {
         const double *rv0 = w->rvec_en + (i * M + ic) * 3;
-        double costh = (rv0[0] * w->xq[iq] + rv0[1] * w->yq[iq] + rv0[2] * w->zq[iq]) * ri;
-        xn[0] = rei * w->xq[iq] + o->cent[3 * ic];
-        xn[1] = rei * w->yq[iq] + o->cent[3 * ic + 1];
-        xn[2] = rei * w->zq[iq] + o->cent[3 * ic + 2];
+        double costh = (rv0[0] * qx[iq] + rv0[1] * qy[iq] + rv0[2] * qz[iq]) * ri;
+        xn[0] = rei * qx[iq] + o->cent[3 * ic];
+        xn[1] = rei * qy[iq] + o->cent[3 * ic + 1];
+        xn[2] = rei * qz[iq] + o->cent[3 * ic + 2];
         for (int jc = 0; jc < M; jc++) {
           double s = 0;
           for (int k = 0; k < 3; k++) {
@@ -1729,14 +1737,15 @@ static void nonloc_pot_w(const orc_sys *o, orc_walker *w, double *pe,
 
 static int finite_or_err(double v) { return isfinite(v) ? 0 : QMCW_ERR_NAN; }
 
-/* hpsi: qmc/vmc/hpsi.f90:2-381 (ibasis != 3 branch) */
-static int eval_full_w(const orc_sys *o, orc_walker *w, int derivs) {
+/* hpsi: qmc/vmc/hpsi.f90:2-381 (ibasis != 3 branch); prng: DMC mode, the
+   pseudopotential grid rotated per pair from this stream */
+static int eval_full_w(const orc_sys *o, orc_walker *w, int derivs, int32_t *prng) {
   int N = o->nelec;
   double pe, pe_ee, d2lnd;
   distances(o, w, &pe, &pe_ee);
   jastrow_full_w(o, w, derivs);
   int st = determinant_w(o, w, &d2lnd);
-  if (o->nloc > 0) nonloc_pot_w(o, w, &pe, derivs);
+  if (o->nloc > 0) nonloc_pot_w(o, w, &pe, derivs, prng);
   else {
     w->eloc_pot_loc = pe;
     w->vpsp = 0;
@@ -1762,7 +1771,7 @@ int orc_eval_full(const orc_sys *o, orc_walker *w, const double *x,
                   double *out9, double *grad, double *divv) {
   int N = o->nelec;
   memcpy(w->x, x, sizeof(double) * 3 * N);
-  int st = eval_full_w(o, w, o->nparm > 0);
+  int st = eval_full_w(o, w, o->nparm > 0, NULL);
   if (out9) {
     out9[0] = w->logpsid; out9[1] = w->sgnpsid; out9[2] = w->psij; out9[3] = w->eloc;
     out9[4] = w->pe; out9[5] = w->d2lnpsi; out9[6] = w->vpsp; out9[7] = w->pe_ee;
@@ -2321,7 +2330,7 @@ static int vmc_walker(const orc_sys *o, orc_walker *w, double *x,
       orc_accept_1e(o, w, i, accept);
     }
     memcpy(x, w->x, sizeof(double) * 3 * N);
-    st = eval_full_w(o, w, 0);
+    st = eval_full_w(o, w, 0, NULL);
     if (etrace) etrace[(size_t)is * stride] = w->eloc;
     if (st) return st;
   }
@@ -2596,16 +2605,18 @@ void orc_dmc_destroy(orc_dmc *d) {
 }
 
 /* population of nwalk walkers at x with their streams and weights (NULL:
-   1); each gets a full evaluation (psi_at) */
+   1); each gets a full evaluation (psi_at) in DMC mode: the pseudopotential
+   grid is rotated per pair in range, from the walker's stream */
 int orc_dmc_set(orc_dmc *d, int32_t nwalk, const double *x, const int32_t *rng,
                 const double *weight) {
   int N = d->o->nelec, err = 0;
   if (nwalk > d->cap) return QMCW_ERR_ARG;
   d->nwalk = nwalk;
   for (int i = 0; i < nwalk; i++) {
-    int st = orc_eval_full(d->o, d->w[i], x + (size_t)i * 3 * N, NULL, NULL, NULL);
-    if (st) err = st;
     memcpy(d->rng + 4 * i, rng + 4 * i, 4 * sizeof(int32_t));
+    memcpy(d->w[i]->x, x + (size_t)i * 3 * N, sizeof(double) * 3 * N);
+    int st = eval_full_w(d->o, d->w[i], 0, d->rng + 4 * i);
+    if (st) err = st;
     d->weight[i] = weight ? weight[i] : 1.0;
   }
   return err;
@@ -2676,7 +2687,7 @@ static int dmc_walker(const orc_sys *o, orc_walker *w, int32_t *rng, double *wei
     sad += acc * d2;
   }
   if (l_accept) {
-    int s2 = eval_full_w(o, w, 0);
+    int s2 = eval_full_w(o, w, 0, rng);
     if (s2) st = s2;
   }
   double taunow = p->tau;

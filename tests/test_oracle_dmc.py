@@ -270,3 +270,49 @@ def test_cusp_mover_rotation_invariance():
         _, p0 = o.cusp_move(tau, g, xe, pt=pt)
         _, p1 = o.cusp_move(tau, q @ g, q @ xe, pt=q @ pt)
         assert abs(p1 - p0) <= 1e-12 * p0
+
+
+# ---- DMC-mode pseudopotential grid (qmc/vmc/nonloc.f90:89, l_mode_dmc)
+
+def _rotqua(st, q0):
+    """rotqua from the stream st (advanced in place): the rotated grid."""
+    L = oracle.lib()
+    xq, yq, zq = (np.zeros(12) for _ in range(3))
+    from oracle import _d, _i
+    L.orc_rotqua(_i(st), _d(q0[0]), _d(q0[1]), _d(q0[2]), 12, _d(xq), _d(yq), _d(zq))
+    return xq, yq, zq
+
+
+@pytest.mark.parametrize("near", [0, 1, 2])
+def test_dmc_evaluation_rotates_the_grid_per_pair_in_range(near):
+    """A DMC evaluation draws one rotqua per (electron, centre) pair within
+    the nonlocal range, in pair order, from the walker's stream: with `near`
+    electrons close to the first centre of the C2-like dimer and the others
+    far from both, the stream advances by exactly `near` rotations; with one,
+    E_L equals the evaluation on the grid rotated by that draw."""
+    s = system.c2_dimer(2)
+    assert np.allclose(s.cent[:, :2], 0) and abs(s.cent[0, 2]) < 1.5
+    N = s.nelec
+    rs = np.random.default_rng(3)
+    phi = 2 * np.pi * np.arange(N) / N                 # a ring 4 bohr from both centres
+    x = np.stack([4 * np.cos(phi), 4 * np.sin(phi), rs.normal(size=N) * 0.3], axis=1)
+    c0 = s.cent[0]
+    for i in range(near):
+        d = rs.normal(size=3)
+        x[i] = c0 + 0.35 * d / np.linalg.norm(d)
+    seed = np.asarray(rng.walker_seeds("1837465927472523", 1), np.int32)
+    d = oracle.OracleDMC(s, 1)
+    assert d.set(x[None], seed) == 0
+    g = d.get()
+    O = oracle.Oracle(s)
+    q0 = O.quadrature()
+    st = seed[0].copy()
+    grids = [_rotqua(st, q0) for _ in range(near)]
+    assert np.array_equal(g["rng"][0], st)
+    if near == 1:
+        O.set_quad(*grids[0])
+        r = O.eval_full(x)
+        assert r["status"] == 0
+        assert abs(r["eloc"] - g["eloc"][0]) <= 1e-12 * abs(r["eloc"])
+        O.set_quad(*q0[:3])
+        assert abs(O.eval_full(x)["eloc"] - g["eloc"][0]) > 1e-9   # the grid matters
