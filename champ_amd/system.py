@@ -614,8 +614,10 @@ def from_champ_old_input(text: str, name: str = "champ") -> System:
     (ibasis = 1, positive counts) of s and p functions in the order of
     orb_loc_ana_gamess_order (numr = -1: all s by n, then the px, py and pz
     blocks by n; qmc/vmc/read_orb_loc.f90:843-1000), orbitals given by
-    coefficients, determinants/CSFs, ijas = 4 Jastrow (isc = 2 or 4).  The
-    dependent e-e-n coefficients are re-imposed with cuspexact4.
+    coefficients, determinants/CSFs, ijas = 4 Jastrow (isc = 2 or 4) and,
+    for one centre type, the Jastrow parameters the optimisation section
+    varies (iwjasa, iwjasb, iwjasc).  The dependent e-e-n coefficients are
+    re-imposed with cuspexact4.
     """
     R = _OldReader(text.splitlines())
     R.seek("iperiodic,ibasis")
@@ -707,11 +709,23 @@ def from_champ_old_input(text: str, name: str = "champ") -> System:
     b = np.array([R.read(nordb + 1) for _ in range(nspin2b)])
     c = np.array([R.read(nterms4(nordc)) for _ in range(nctype)])
     c = cusp_exact(c, nordc)
+    # the Jastrow parameters the optimisation section varies (iwjasa, iwjasb,
+    # iwjasc; one centre type)
+    jas = {}
+    for key in ("iwjasa", "iwjasb", "iwjasc"):
+        for ln in R.lines[R.i:]:
+            if "(" + key in ln:
+                jas[key] = [int(v) for v in R._numeric(ln)]
+                break
+    ja = [(0, i) for i in jas.get("iwjasa", [])] if nctype == 1 else []
+    jb = [(0, i) for i in jas.get("iwjasb", [])] if nspin2b == 1 else []
+    jc = [(0, i) for i in jas.get("iwjasc", [])] if nctype == 1 else []
     return System(nelec=nelec, nup=nup, cent=cent, iwctype=iwctype, znuc=znuc,
                   coef=coef, det_up=det_up, det_dn=det_dn, csf_coef=cc,
                   csf_det_off=off, csf_det=cdet, csf_det_coef=cdc, isc=isc,
                   nspin2b=nspin2b, norda=norda, nordb=nordb, nordc=nordc,
-                  scalek=scalek, a4=a4, b=b, c=c, nloc=0, name=name, **barr)
+                  scalek=scalek, a4=a4, b=b, c=c, nloc=0, jas_a=ja, jas_b=jb, jas_c=jc,
+                  name=name, **barr)
 
 
 def jastrow_opt_list(sysm: System, vary_a1: bool = False):

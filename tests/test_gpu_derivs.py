@@ -24,13 +24,24 @@ def _rel(a, b, floor):
 def _derivs_system(name):
     if name == "c2":
         return system.c2_dimer(2, derivs=True)
+    if name == "c2_ktt":
+        # the all-electron C2 example: isc = 2 scaled distances, the 24
+        # Jastrow parameters its optimisation section varies
+        import json, os
+        fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                         "c2_ktt_transcript.json")))
+        s = system.from_champ_old_input(fx["input"], "c2_ktt")
+        p = [r for r in fx["runs"] if r["iteration"] == 3][0]["params"]
+        s.a4 = np.array([p["a"]]); s.b = np.array([p["b"]])
+        s.c = system.cusp_exact(np.array([p["c"]]), s.nordc)
+        return s
     s = system.be_atom(1)
     ja, jb, jc = system.jastrow_opt_list(s, vary_a1=False)
     s.jas_a, s.jas_b, s.jas_c = ja, jb, jc
     return s
 
 
-@pytest.mark.parametrize("name", ["c2", "be_jastrow"])
+@pytest.mark.parametrize("name", ["c2", "be_jastrow", "c2_ktt"])
 def test_derivs_match_oracle(name):
     s = _derivs_system(name)
     assert s.nparm > 0

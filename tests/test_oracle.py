@@ -547,6 +547,41 @@ def test_vmc_energy_matches_champ_c2_transcript(iteration):
     assert abs(e.std() - run["sigma"]) < 0.1, (e.std(), run)
 
 
+def test_c2_example_jastrow_derivatives_fd():
+    """The 24 Jastrow parameters the C2 example varies (isc = 2 scaled
+    distances, read from its optimisation section): d ln Psi / dp and
+    d E_L / dp against central finite differences (h = 1e-5), with the
+    dependent e-e-n coefficients re-imposed on each side."""
+    s, _ = _c2_ktt(3)
+    assert (len(s.jas_a), len(s.jas_b), len(s.jas_c)) == (4, 5, 15)
+    O = oracle.Oracle(s)
+    x = system.random_walkers(s, 1, seed=19)[0]
+    assert O.eval_full(x)["status"] == 0
+    dpsi, deloc = O.derivs()
+    assert len(dpsi) == 24
+    h = 1e-5
+
+    def lnpsi_el(s2):
+        q = oracle.Oracle(s2).eval_full(x)
+        return q["logpsid"] + q["psij"], q["eloc"]
+
+    p = 0
+    for blk, arr in (("a", s.jas_a), ("b", s.jas_b), ("c", s.jas_c)):
+        for (t, idx) in arr:
+            sp, sm = s.copy(), s.copy()
+            name = {"a": "a4", "b": "b", "c": "c"}[blk]
+            getattr(sp, name)[t, idx - 1] += h
+            getattr(sm, name)[t, idx - 1] -= h
+            if blk == "c":
+                sp.c = system.cusp_exact(sp.c, s.nordc)
+                sm.c = system.cusp_exact(sm.c, s.nordc)
+            lp, ep = lnpsi_el(sp)
+            lm, em = lnpsi_el(sm)
+            assert abs((lp - lm) / (2 * h) - dpsi[p]) < 1e-6 * max(1, abs(dpsi[p])), (blk, idx)
+            assert abs((ep - em) / (2 * h) - deloc[p]) < 2e-5 * max(1, abs(deloc[p])), (blk, idx)
+            p += 1
+
+
 def test_accum_products_against_numpy():
     """orc_accum: sums over samples of dpsi dpsi^T, E dpsi dpsi^T, dpsi deloc^T
     and the vector sums, against numpy's matrix products."""
