@@ -77,7 +77,7 @@ class DmcParams(C.Structure):
                 ("c_rewt", C.c_double), ("pow_rewt", C.c_double), ("limit_rewt", C.c_double),
                 ("adrift", C.c_double), ("n_drift", C.c_int32), ("itau_eff", C.c_int32),
                 ("itau_integ", C.c_int32), ("iacc_rej", C.c_int32), ("icross", C.c_int32),
-                ("ene_int", C.c_int32), ("idmc", C.c_int32), ("pad_", C.c_int32)]
+                ("ene_int", C.c_int32), ("idmc", C.c_int32), ("tmoves", C.c_int32)]
 
     def __init__(self, tau, etrial, eest=None, esigma=None, pop_ctrl=1.0, taueff=None, **kw):
         super().__init__()
@@ -89,6 +89,7 @@ class DmcParams(C.Structure):
         self.c_rewt, self.pow_rewt, self.limit_rewt, self.adrift = 3.5, 0.5, 10.0, 0.5
         self.n_drift, self.itau_eff, self.itau_integ = 2, 1, 1
         self.iacc_rej, self.icross, self.ene_int, self.idmc = 1, 1, 0, 2
+        self.tmoves = 0
         for k, v in kw.items():
             setattr(self, k, v)
 

@@ -248,7 +248,9 @@ typedef struct qmcw_dmc_params {
   int32_t icross;      /* [1] <= 0: reject node-crossing moves */
   int32_t ene_int;     /* [0] 0: ene_int_v9, 1: no_ene_int */
   int32_t idmc;        /* [2] > 0: reweight */
-  int32_t pad_;
+  int32_t tmoves;      /* [0] T-moves before each generation's moves
+                          (qmc/dmc/dmc_elec/tmove.f90): 0 off, 1
+                          det_balance1, 2 casula1 (pseudopotential systems) */
 } qmcw_dmc_params;
 
 /* Sums over the population of one generation (before branching) and the

@@ -76,6 +76,8 @@ def lib():
             "orc_eval_batch": (C.c_int, [_vp, C.c_int32, _pd, _pd, _pd, _pd, _pd, _pd,
                                          C.c_int32]),
             "orc_cusp_move": (C.c_double, [_vp, C.c_double, C.c_int32, _pd, _pd, _pi, _pd, _pd]),
+            "orc_tmove_run": (C.c_int, [_vp, _vp, _pd, _pi, C.c_double, C.c_int32, _pd]),
+            "orc_walker_tq": (C.c_int32, [_vp, _vp, C.c_int32, _pd, _pd]),
             "orc_dmc_create": (_vp, [_vp, C.c_int32]),
             "orc_dmc_destroy": (None, [_vp]),
             "orc_dmc_set": (C.c_int, [_vp, C.c_int32, _pd, _pi, _pd]),
@@ -229,6 +231,21 @@ class Oracle:
         ptd = None if pt is None else np.ascontiguousarray(pt, np.float64)
         p = self.L.orc_cusp_move(self.h, tau, n_drift, _d(g), _d(xe), _i(rng), _d(xn), _d(ptd))
         return xn, (p if pt is not None else None)
+
+    def tmove_run(self, x, rng, tau, casula=False, walker=None):
+        """DMC-mode evaluation at x, then T-moves; (accepted, x after);
+        rng [4] int32 advances in place."""
+        x = np.ascontiguousarray(x, np.float64)
+        xo = np.zeros_like(x)
+        a = self.L.orc_tmove_run(self.h, walker or self.w, _d(x), _i(rng), tau, int(casula), _d(xo))
+        return a, xo
+
+    def walker_tq(self, i, walker=None):
+        """(points [n][3], ratios [n]) kept for electron i by the last
+        DMC-mode evaluation."""
+        q, r = np.zeros((self.system.ncent * 12, 3)), np.zeros(self.system.ncent * 12)
+        n = self.L.orc_walker_tq(self.h, walker or self.w, i, _d(q), _d(r))
+        return q[:n], r[:n]
 
     # ---- batched ----
     def eval_batch(self, x, derivs=False, nthreads=0):

@@ -1246,6 +1246,11 @@ struct orc_walker {
   double *orbe, *dorbe, *ddorbe;
   double *ain, detan, *tn, *invn, *detn, chin, *yupn, *ydnn;
   double *work, *scr_basis, *orbq, *gd, *gold;
+  /* last DMC-mode evaluation, per electron: the quadrature points of its
+     pairs in range (unit vectors, pair order) and the Psi ratios there
+     (psi_t quadx / quadr, qmc/psi_type_mod.f90:34) */
+  double *tqx, *tqr;
+  int *tqn;
 };
 
 #define ALLOCD(n) ((double *)calloc((size_t)(n) + 1, sizeof(double)))
@@ -1293,6 +1298,9 @@ orc_walker *orc_walker_create(const orc_sys *o) {
   w->orbq = ALLOCD(NO);
   w->gd = ALLOCD(npj);
   w->gold = ALLOCD(npj);
+  w->tqx = ALLOCD(3 * N * M * QMCW_MAX_NQUAD);
+  w->tqr = ALLOCD(N * M * QMCW_MAX_NQUAD);
+  w->tqn = calloc((size_t)N + 1, sizeof(int));
   return w;
 }
 
@@ -1306,8 +1314,9 @@ void orc_walker_destroy(orc_walker *w) {
                  w->r_en_n, w->rvec_en_n, w->r_ee_n, w->rvec_ee_n, w->fsn, w->fijn_i,
                  w->fijn_j, w->lapn_i, w->lapn_j, w->fjn, w->orbe, w->dorbe, w->ddorbe,
                  w->ain, w->tn, w->invn, w->detn, w->yupn, w->ydnn, w->work,
-                 w->scr_basis, w->orbq, w->gd, w->gold};
+                 w->scr_basis, w->orbq, w->gd, w->gold, w->tqx, w->tqr};
   for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
+  free(w->tqn);
   free(w);
 }
 
@@ -1674,6 +1683,7 @@ static void nonloc_pot_w(const orc_sys *o, orc_walker *w, double *pe,
     int nocc = up ? o->noccup : o->noccdn;
     const int *occ = up ? o->occup : o->occdn;
     const double *G = up ? w->gup : w->gdn;
+    int npt = 0;
     for (int ic = 0; ic < M; ic++) {
       int ict = o->iwctype[ic];
       double r = fmax(1e-10, w->r_en[i * M + ic]);
@@ -1715,6 +1725,16 @@ static void nonloc_pot_w(const orc_sys *o, orc_walker *w, double *pe,
           if (l != o->lpotp1[ict]) vpot += vps[ic][l - 1] * yl0(l, costh);
         double factor = o->wq[iq] * exp(value);
         for (int q = 0; q < NO; q++) w->psp_nonloc_orb[i * NO + q] += factor * orbe[q] * vpot;
+        if (prng) {
+          /* DMC mode keeps the point and the Psi ratio for T-moves
+             (qmc/vmc/nonloc.f90:99-104, 166-174) */
+          double rat = 0;
+          for (int j = 0; j < nocc; j++) rat += G[j * ns + ie] * orbe[occ[j]];
+          size_t k = (size_t)i * M * QMCW_MAX_NQUAD + npt;
+          w->tqr[k] = rat * exp(value);
+          w->tqx[3 * k] = qx[iq]; w->tqx[3 * k + 1] = qy[iq]; w->tqx[3 * k + 2] = qz[iq];
+          npt++;
+        }
         if (derivs) {
           double rat = 0;
           for (int j = 0; j < nocc; j++) rat += G[j * ns + ie] * orbe[occ[j]];
@@ -1722,6 +1742,7 @@ static void nonloc_pot_w(const orc_sys *o, orc_walker *w, double *pe,
         }
       }
     }
+    if (prng) w->tqn[i] = npt;
   }
   for (int i = 0; i < N; i++) {
     int up = i < nup;
@@ -2408,6 +2429,8 @@ void orc_walker_copy(const orc_sys *o, orc_walker *d, const orc_walker *s) {
   CP(detup, o->ndetup); CP(detdn, o->ndetdn); CP(yup, o->noccup * nup);
   CP(ydn, o->noccdn * ndn); CP(gup, o->noccup * nup); CP(gdn, o->noccdn * ndn);
   CP(vj, 3 * N); CP(vd, 3 * N); CP(divvj, N); CP(divvd, N);
+  CP(tqx, 3 * N * M * QMCW_MAX_NQUAD); CP(tqr, N * M * QMCW_MAX_NQUAD);
+  memcpy(d->tqn, s->tqn, sizeof(int) * (size_t)N);
 #undef CP
   d->fsumo = s->fsumo; d->d2o = s->d2o; d->deta_up = s->deta_up; d->deta_dn = s->deta_dn;
   d->chi = s->chi; d->logpsid = s->logpsid; d->sgnpsid = s->sgnpsid; d->psij = s->psij;
@@ -2553,6 +2576,128 @@ double orc_cusp_move(const orc_sys *o, double tau, int32_t ndrift, const double 
   return pt ? cusp_prob(&cg, tau, pt) : 0.0;
 }
 
+/* T-moves of one walker, qmc/dmc/dmc_elec/tmove.f90:1-182 with
+   discrete_rand (:184-205) and quad_psi (:207-255): for each electron in
+   turn, the weights max(0, sum_l w_q (exp(-tau v_l) - 1) P_l(cos) Psi(x_q) /
+   Psi(x)) of moving it to the quadrature points of its pairs in range (the
+   points and ratios of the last DMC evaluation until a T-move is accepted,
+   then fresh rotations and ratios), against 1 for staying; a chosen move is
+   accepted with min(1, forward sum / backward sum) (det_balance1; the
+   backward points are a fresh rotation about the same centre, as the
+   reference, which assumes non-overlapping nonlocal ranges) or always
+   (casula1).  Returns 1 if any T-move was accepted. */
+static int tmove_w(const orc_sys *o, orc_walker *w, int32_t *rng, double tau, int casula) {
+  const int N = o->nelec, M = o->ncent, nq = o->nquad;
+  int tacc = 0;
+  double *gw = malloc(sizeof(double) * ((size_t)M * nq + 1));
+  double *xs = malloc(sizeof(double) * 3 * ((size_t)M * nq + 1));
+  double *pr = malloc(sizeof(double) * ((size_t)M * nq + 1));
+  int *wc = malloc(sizeof(int) * ((size_t)M * nq + 1));
+  double vps[QMCW_MAX_LPOT], gx[QMCW_MAX_NQUAD], gy[QMCW_MAX_NQUAD], gz[QMCW_MAX_NQUAD];
+  double out4[4], gn[3], lap;
+  for (int i = 0; i < N; i++) {
+    gw[0] = 1;
+    int n = 0;
+    for (int ic = 0; ic < M; ic++) {
+      int ict = o->iwctype[ic];
+      double r = fmax(1e-10, w->r_en[i * M + ic]);
+      int iskip = 1;
+      for (int l = 1; l <= o->npotd[ict]; l++) {
+        vps[l - 1] = gauss_pot(o, r, l - 1, ict);
+        if (l != o->lpotp1[ict] && fabs(vps[l - 1]) > 1e-4) iskip = 0;
+      }
+      if (iskip) continue;
+      if (tacc) orc_rotqua(rng, o->xq0, o->yq0, o->zq0, nq, gx, gy, gz);
+      double rei = w->r_en[i * M + ic];
+      const double *rv0 = w->rvec_en + (i * M + ic) * 3;
+      for (int iq = 0; iq < nq; iq++) {
+        double q[3];
+        size_t k = (size_t)i * M * QMCW_MAX_NQUAD + n;
+        if (tacc) { q[0] = gx[iq]; q[1] = gy[iq]; q[2] = gz[iq]; }
+        else { q[0] = w->tqx[3 * k]; q[1] = w->tqx[3 * k + 1]; q[2] = w->tqx[3 * k + 2]; }
+        double costh = (rv0[0] * q[0] + rv0[1] * q[1] + rv0[2] * q[2]) / rei;
+        double *xt = xs + 3 * n;
+        for (int d = 0; d < 3; d++) xt[d] = rei * q[d] + o->cent[3 * ic + d];
+        double ratio;
+        if (tacc) {
+          orc_ratio_1e(o, w, i, xt, out4, gn, &lap);
+          ratio = out4[0];
+        } else {
+          ratio = w->tqr[k];
+        }
+        double gpsp = 0;
+        for (int l = 1; l <= o->npotd[ict]; l++)
+          if (l != o->lpotp1[ict]) gpsp += o->wq[iq] * (exp(-tau * vps[l - 1]) - 1) * yl0(l, costh) * ratio;
+        pr[n] = ratio;
+        gw[1 + n] = fmax(0.0, gpsp);
+        wc[n] = ic;
+        n++;
+      }
+    }
+    /* discrete_rand over {stay, the n points} */
+    double tot = 0;
+    for (int k = 0; k <= n; k++) tot += gw[k];
+    double rnd = tot * orc_rannyu(rng), psum = 0;
+    int which = n;
+    for (int k = 0; k <= n; k++) {
+      psum += gw[k];
+      if (psum > rnd) { which = k; break; }
+    }
+    if (which == 0) continue;
+    int k = which - 1, ic = wc[k], ict = o->iwctype[ic];
+    double xnew[3] = {xs[3 * k], xs[3 * k + 1], xs[3 * k + 2]}, xen[3];
+    for (int d = 0; d < 3; d++) xen[d] = xnew[d] - o->cent[3 * ic + d];
+    double p = 1;
+    if (!casula) {
+      double bwd = 1;
+      double rei = w->r_en[i * M + ic], r = fmax(1e-10, rei);
+      for (int l = 1; l <= o->npotd[ict]; l++) vps[l - 1] = gauss_pot(o, r, l - 1, ict);
+      orc_rotqua(rng, o->xq0, o->yq0, o->zq0, nq, gx, gy, gz);
+      for (int jq = 0; jq < nq; jq++) {
+        double costh = (xen[0] * gx[jq] + xen[1] * gy[jq] + xen[2] * gz[jq]) / rei;
+        double xt[3] = {rei * gx[jq] + o->cent[3 * ic], rei * gy[jq] + o->cent[3 * ic + 1],
+                        rei * gz[jq] + o->cent[3 * ic + 2]};
+        orc_ratio_1e(o, w, i, xt, out4, gn, &lap);
+        double ratio = out4[0] / pr[k];
+        double gpsp = 0;
+        for (int l = 1; l <= o->npotd[ict]; l++)
+          if (l != o->lpotp1[ict]) gpsp += o->wq[jq] * (exp(-tau * vps[l - 1]) - 1) * yl0(l, costh) * ratio;
+        bwd += fmax(0.0, gpsp);
+      }
+      p = fmin(1.0, tot / bwd);
+    }
+    if (orc_rannyu(rng) < p || casula) {
+      tacc = 1;
+      orc_ratio_1e(o, w, i, xnew, out4, gn, &lap);
+      orc_accept_1e(o, w, i, 1);
+    }
+  }
+  free(gw); free(xs); free(pr); free(wc);
+  return tacc;
+}
+
+/* tests: a DMC-mode evaluation of walker w at x (grid rotations from rng),
+   then its T-moves; returns 1 if one was accepted, -1 on an evaluation
+   error; xout: the positions after the T-moves */
+int orc_tmove_run(const orc_sys *o, orc_walker *w, const double *x, int32_t *rng, double tau,
+                  int32_t casula, double *xout) {
+  memcpy(w->x, x, sizeof(double) * 3 * o->nelec);
+  if (eval_full_w(o, w, 0, rng)) return -1;
+  int a = tmove_w(o, w, rng, tau, casula);
+  if (xout) memcpy(xout, w->x, sizeof(double) * 3 * o->nelec);
+  return a;
+}
+
+/* tests: the quadrature points (unit vectors) and Psi ratios the last
+   DMC-mode evaluation kept for electron i; returns their number */
+int32_t orc_walker_tq(const orc_sys *o, const orc_walker *w, int32_t i, double *q, double *r) {
+  int n = w->tqn[i];
+  size_t k = (size_t)i * o->ncent * QMCW_MAX_NQUAD;
+  if (q) memcpy(q, w->tqx + 3 * k, sizeof(double) * 3 * n);
+  if (r) memcpy(r, w->tqr + k, sizeof(double) * n);
+  return n;
+}
+
 /* ene_int_rewt, qmc/green_functions_mod.f90:382-472 (ene_int_v9 or
    no_ene_int) */
 static double dmc_eint(const qmcw_dmc_params *p, int N, double tauint, double eo,
@@ -2635,6 +2780,9 @@ static int dmc_walker(const orc_sys *o, orc_walker *w, int32_t *rng, double *wei
   if (o->nloc > 0) orc_rotqua(rng, o->xq0, o->yq0, o->zq0, o->nquad, w->xq, w->yq, w->zq);
   double dd[128], pacc[128], sdd = 0, spd = 0, sad = 0, pa_sum = 0, nacc = 0, ncross = 0;
   int l_accept = 0, st = 0;
+  /* rundmc_mod.f90:137: T-moves first; any accepted one makes the walker
+     re-evaluated at the end */
+  if (p->tmoves && o->nloc > 0) l_accept = tmove_w(o, w, rng, p->tau, p->tmoves == 2);
   double rt = sqrt(p->tau);
   for (int ie = 0; ie < N; ie++) {
     double gold[3], vo[3], vn[3], gnew[3], xo[3], xn[3], out4[4], lap, po, pn, d2 = 0;

@@ -122,6 +122,15 @@ int orc_eval_batch(const orc_sys *o, int32_t nw, const double *x,
 double orc_cusp_move(const orc_sys *o, double tau, int32_t ndrift, const double *g, const double *xe,
                      int32_t *rng, double *xn, const double *pt);
 
+/* T-moves (qmc/dmc/dmc_elec/tmove.f90), for tests: a DMC-mode evaluation
+   of w at x (grid rotations drawn from rng) followed by the T-moves of every
+   electron (casula 0: det_balance1, 1: casula1); xout: positions after. */
+int orc_tmove_run(const orc_sys *o, orc_walker *w, const double *x, int32_t *rng, double tau,
+                  int32_t casula, double *xout);
+/* The quadrature points q[n][3] and Psi ratios r[n] kept for electron i by
+   the last DMC-mode evaluation; returns n. */
+int32_t orc_walker_tq(const orc_sys *o, const orc_walker *w, int32_t i, double *q, double *r);
+
 typedef struct orc_dmc orc_dmc;
 orc_dmc *orc_dmc_create(const orc_sys *o, int32_t cap);
 void orc_dmc_destroy(orc_dmc *d);

@@ -16,6 +16,8 @@
                    (isotropic); it is invariant under rotations; and the
                    Metropolis walk built on it samples |Psi|^2 (<r> = 3/(2Z))
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -316,3 +318,70 @@ def test_dmc_evaluation_rotates_the_grid_per_pair_in_range(near):
         assert abs(r["eloc"] - g["eloc"][0]) <= 1e-12 * abs(r["eloc"])
         O.set_quad(*q0[:3])
         assert abs(O.eval_full(x)["eloc"] - g["eloc"][0]) > 1e-9   # the grid matters
+
+
+# ---- T-moves (qmc/dmc/dmc_elec/tmove.f90)
+
+def _hydrogenic_nonlocal(Z, v0):
+    """One electron in the exact 1s state of -Z/r plus a nonlocal s channel
+    v_s(r) = v0 exp(-r^2): Psi is spherical, so every quadrature-point ratio
+    at the electron's radius is 1."""
+    s = _hydrogenic_pp(Z)
+    ecp = system._ecp_arrays([dict(local=[], l0=[(v0, 2, 1.0)], l1=[])])
+    s.npotd, s.lpotp1, s.ecp_off, s.ecp_coef, s.ecp_power, s.ecp_expo = ecp
+    return s
+
+
+@pytest.mark.parametrize("casula", [False, True])
+def test_tmove_frequency_of_a_spherical_state(casula):
+    """Spherical Psi and an s channel: each of the 12 points weighs
+    w_q (exp(-tau v_s) - 1) with sum w_q = 1, the backward sum equals the
+    forward one (det_balance1 accepts every proposal), so the electron moves
+    with probability S / (1 + S), S = exp(-tau v_s(r)) - 1, onto the sphere
+    of its radius."""
+    v0, tau, r = -2.0, 0.5, 0.8
+    s = _hydrogenic_nonlocal(1.0, v0)
+    O = oracle.Oracle(s)
+    S = math.exp(-tau * v0 * math.exp(-r * r)) - 1
+    st = _stream()
+    rs = np.random.default_rng(8)
+    n, moved = 3000, 0
+    for _ in range(n):
+        u = rs.normal(size=3)
+        x = (r * u / np.linalg.norm(u))[None]
+        a, xo = O.tmove_run(x, st, tau, casula)
+        assert a in (0, 1)
+        if a:
+            moved += 1
+            assert abs(np.linalg.norm(xo[0]) - r) < 1e-12
+            assert np.linalg.norm(xo[0] - x[0]) > 1e-3
+        else:
+            assert np.array_equal(xo, x)
+    p = S / (1 + S)
+    sd = math.sqrt(p * (1 - p) / n)
+    assert abs(moved / n - p) < 4 * sd, (moved / n, p, sd)
+
+
+def test_tmove_saved_ratios_are_the_one_electron_ratios():
+    """The ratios a DMC-mode evaluation keeps for T-moves (G^T phi times the
+    Jastrow factor at each rotated point) equal the one-electron move ratio
+    Psi(x_i -> c + r q) / Psi of the same point, for every electron of the
+    C2-like dimer in range of a pseudopotential."""
+    s = system.c2_dimer(2)
+    O = oracle.Oracle(s)
+    x = system.random_walkers(s, 1, seed=31)[0]
+    st = _stream()
+    O.tmove_run(x, st, 1e-12)            # evaluation; tau ~ 0 makes no T-move
+    npts = 0
+    for i in range(s.nelec):
+        q, rat = O.walker_tq(i)
+        assert len(rat) % 12 == 0
+        npts += len(rat)
+        for k in range(len(rat)):
+            # the point lies on the sphere of electron i about one of the centres
+            errs = []
+            for ic in range(s.ncent):
+                xn = s.cent[ic] + np.linalg.norm(x[i] - s.cent[ic]) * q[k]
+                errs.append(abs(O.ratio_1e(i, xn)["ratio"] - rat[k]) / max(1, abs(rat[k])))
+            assert min(errs) <= 1e-10, (i, k, errs)
+    assert npts > 0
