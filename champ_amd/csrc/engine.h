@@ -68,6 +68,7 @@ struct DevSys {
   const int *term_klm;       // [nterms][3]
   // pseudopotential
   int nloc, nquad;
+  int tm_kmax;               // most centres whose nonlocal ranges can hold one point (T-moves)
   const int *npotd, *lpotp1, *ecp_off, *ecp_power;
   const double *ecp_coef, *ecp_expo;
   const double *q0;          // [4][nquad] x0,y0,z0,w
@@ -115,7 +116,8 @@ struct SLayout {
   int64_t bt;                // [N][M] e-e-n variable 1 - k scaled(r_j,ic) (as the kernel's T)
   int64_t uq;                // [64][N] per-lane e-e-n variable of the pairs of a point
   int64_t plist;             // [N*M] ints: (electron, centre) pairs in pseudopotential range
-  int64_t prot;              // [N*M][9] DMC: axes of each in-range pair's grid rotation
+  int64_t prot;              // [N*M][10] DMC: axes of each in-range pair's grid rotation, then
+                             // the pair's rank among its electron's pairs
   int64_t mv;                // pending one-electron move buffers
   int64_t stride;
 };

@@ -73,6 +73,11 @@ struct qmcw_handle {
   // DMC population (qmcw_dmc_begin / qmcw_dmc_step)
   int dmc_on = 0, nact = 0;
   double *dm = nullptr, *dm2 = nullptr, *st2 = nullptr, *dstat = nullptr, *dwt = nullptr;
+  // T-moves: per-walker kept quadrature data (double-buffered like the
+  // state) and the point columns of one electron's step
+  double *tq2 = nullptr;
+  double *tmq = nullptr, *tmx = nullptr, *rcol = nullptr, *tmb = nullptr, *tmbq = nullptr,
+         *bcol = nullptr, *tmxin = nullptr;
   int *dorigin = nullptr;
   int32_t brng[4] = {0, 0, 0, 1};
   // event timing of the device driver
@@ -883,6 +888,35 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     std::vector<double> cf(s->ecp_coef, s->ecp_coef + nt), ex(s->ecp_expo, s->ecp_expo + nt);
     UP(npotd, S.npotd); UP(lpotp1, S.lpotp1); UP(off, S.ecp_off); UP(pw, S.ecp_power);
     UP(cf, S.ecp_coef); UP(ex, S.ecp_expo);
+    // T-moves keep the points of at most tm_kmax pairs per electron: the most
+    // centres whose nonlocal ranges (some |v_l| > 1e-4, scanned to 40 bohr
+    // in 1e-3 steps) can overlap at one point, with a 0.02 bohr margin
+    std::vector<double> rc(s->nctype, 0.0);
+    for (int t = 0; t < s->nctype; t++)
+      for (int k = 1; k <= 40000; k++) {
+        const double r = 1e-3 * k;
+        for (int l = 1; l <= npotd[t]; l++) {
+          if (l == lpotp1[t]) continue;
+          double v = 0.0;
+          for (int q = off[t * QMCW_MAX_LPOT + l - 1]; q < off[t * QMCW_MAX_LPOT + l]; q++)
+            v += (pw[q] != 2 ? std::pow(r, (double)(pw[q] - 2)) : 1.0) * cf[q] * std::exp(-ex[q] * r * r);
+          if (std::fabs(v) > 1e-4) rc[t] = r;
+        }
+      }
+    int km = 1;
+    for (int c = 0; c < S.M; c++) {
+      const double r1 = rc[s->iwctype[c]];
+      if (r1 <= 0.0) continue;
+      int n = 0;
+      for (int c2 = 0; c2 < S.M; c2++) {
+        const double r2 = rc[s->iwctype[c2]];
+        double d2 = 0.0;
+        for (int k = 0; k < 3; k++) d2 += (s->cent[3 * c + k] - s->cent[3 * c2 + k]) * (s->cent[3 * c + k] - s->cent[3 * c2 + k]);
+        if (r2 > 0.0 && std::sqrt(d2) < r1 + r2 + 0.02) n++;
+      }
+      km = std::max(km, n);
+    }
+    S.tm_kmax = km;
   }
   // ---- parameter counts (derivatives)
   S.npcsf = s->nparmcsf; S.npa = s->nparma_tot; S.npb = s->nparmb_tot; S.npc = s->nparmc_tot;
@@ -967,7 +1001,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SL.bt = take((int64_t)N * M);
   SL.uq = take((int64_t)kWave * N);
   SL.plist = take(((int64_t)N * M + 1) / 2);
-  SL.prot = take(S.nloc > 0 ? 9 * (int64_t)N * M : 1);
+  SL.prot = take(S.nloc > 0 ? 10 * (int64_t)N * M : 1);
   SL.mv = take(ML.stride);
   if (S.nparm > 0) {
     DLayout &DL = h->A.DL;
@@ -1180,6 +1214,7 @@ static int check_flags(qmcw_handle *h) {
   for (int f : fl) any |= f;
   if (any & 1) return fail(h, QMCW_ERR_SINGULAR, "a Slater matrix is singular");
   if (any & 2) return fail(h, QMCW_ERR_NAN, "non-finite local energy");
+  if (any & 4) return fail(h, QMCW_ERR_ARG, "an electron is in range of more pseudopotential centres than the T-move record holds");
   return QMCW_OK;
 }
 
@@ -1577,7 +1612,34 @@ static int dmc_alloc(qmcw_handle *h) {
   HIPCHK(h, hipMalloc(&p, 8 * sizeof(double))); h->allocs.push_back(p); h->dstat = (double *)p;
   HIPCHK(h, hipMalloc(&p, Wp * sizeof(double))); h->allocs.push_back(p); h->dwt = (double *)p;
   HIPCHK(h, hipMalloc(&p, Wp * sizeof(int))); h->allocs.push_back(p); h->dorigin = (int *)p;
+  if (h->A.S.nloc > 0) {
+    const int64_t ts = (int64_t)4 * h->A.S.N * h->A.S.tm_kmax * h->A.S.nquad;
+    HIPCHK(h, hipMalloc(&p, Wp * ts * sizeof(double))); h->allocs.push_back(p); h->A.tq = (double *)p;
+    HIPCHK(h, hipMalloc(&p, Wp * ts * sizeof(double))); h->allocs.push_back(p); h->tq2 = (double *)p;
+    h->A.tq_stride = ts;
+  }
   return QMCW_OK;
+}
+
+// point columns of the T-move steps (allocated at the first generation with T-moves)
+static int tm_alloc(qmcw_handle *h) {
+  if (h->tmq) return QMCW_OK;
+  void *p;
+  const size_t Wp = h->Wp, ncol = (size_t)h->A.S.tm_kmax * h->A.S.nquad, nb = h->A.S.nquad;
+  auto get = [&](size_t n, double *&dst) -> int {
+    HIPCHK(h, hipMalloc(&p, n * sizeof(double)));
+    h->allocs.push_back(p);
+    dst = (double *)p;
+    return QMCW_OK;
+  };
+  int st = get(ncol * Wp * 3, h->tmq);
+  if (!st) st = get(ncol * Wp * 3, h->tmx);
+  if (!st) st = get(ncol * Wp * 8, h->rcol);
+  if (!st) st = get(nb * Wp * 3, h->tmb);
+  if (!st) st = get(nb * Wp * 3, h->tmbq);
+  if (!st) st = get(nb * Wp * 8, h->bcol);
+  if (!st) st = get(Wp * 3, h->tmxin);
+  return st;
 }
 
 extern "C" int qmcw_set_branch_rng(qmcw_handle *h, const int32_t *seed) {
@@ -1701,11 +1763,38 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
   if (!h || !p) return QMCW_ERR_ARG;
   if (!h->dmc_on) return fail(h, QMCW_ERR_ARG, "qmcw_dmc_begin has not been called");
   if (!(p->tau > 0.0) || p->n_drift < 1) return fail(h, QMCW_ERR_ARG, "tau > 0 and n_drift >= 1 required");
+  if (p->tmoves < 0 || p->tmoves > 2) return fail(h, QMCW_ERR_ARG, "tmoves must be 0, 1 or 2");
   const int N = h->A.S.N;
   const dim3 g1((h->Wl + 63) / 64), b1(64);
   double *gbuf = h->io, *xin = h->io + (size_t)h->Wp * 8;
   if (h->A.S.nloc > 0) hipLaunchKernelGGL(k_rotqua, g1, b1, 0, h->stream, h->A);
   hipLaunchKernelGGL(k_dmc_gen_begin, g1, b1, 0, h->stream, h->A, h->dm);
+  if (p->tmoves && h->A.S.nloc > 0) {
+    // T-moves of every electron in turn (rundmc_mod.f90:137, tmove.f90):
+    // points, their ratios when fresh (one k_ratio_1e launch per point
+    // column; none is fresh for the first electron), the choice, the
+    // backward ratios (det_balance1), the decision and the commit
+    int st = tm_alloc(h);
+    if (st) return st;
+    const int cw = h->Wp, ncol = h->A.S.tm_kmax * h->A.S.nquad, nq = h->A.S.nquad;
+    for (int ie = 0; ie < N; ie++) {
+      hipLaunchKernelGGL(k_tm_points, g1, b1, 0, h->stream, h->A, ie, cw, h->tmq, h->tmx, h->dm);
+      if (ie > 0)
+        for (int c = 0; c < ncol; c++)
+          launch_ratio(h, ie, 0, (const double *)(h->tmx + (size_t)c * cw * 3), 1.0, 1.0,
+                       h->rcol + (size_t)c * cw * 8);
+      hipLaunchKernelGGL(k_tm_select, g1, b1, 0, h->stream, h->A, ie, cw, (const double *)h->tmq,
+                         (const double *)h->tmx, (const double *)h->rcol, h->tmb, h->tmbq, h->dm, *p);
+      if (p->tmoves == 1)
+        for (int jq = 0; jq < nq; jq++)
+          launch_ratio(h, ie, 0, (const double *)(h->tmb + (size_t)jq * cw * 3), 1.0, 1.0,
+                       h->bcol + (size_t)jq * cw * 8);
+      hipLaunchKernelGGL(k_tm_decide, g1, b1, 0, h->stream, h->A, ie, cw, (const double *)h->tmbq,
+                         (const double *)h->bcol, h->tmxin, h->acc_buf, h->dm, *p);
+      launch_ratio(h, ie, 0, (const double *)h->tmxin, 1.0, 1.0, (double *)nullptr);
+      DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+    }
+  }
   for (int ie = 0; ie < N; ie++) {
     DISPATCH_MOVE(h, launch_grad, ie, gbuf);
     hipLaunchKernelGGL(k_dmc_propose, g1, b1, 0, h->stream, h->A, ie, (const double *)gbuf, xin,
@@ -1716,7 +1805,7 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
   }
   // DMC-mode evaluation: walkers that moved draw a grid rotation per pair
   h->A.dmc_rot = h->A.S.nloc > 0 ? 1 : 0;
-  h->A.dmove = h->dm + kDmNacc;
+  h->A.dmove = h->dm + kDmAcc;
   h->A.dmove_stride = kDmStride;
   launch_eval(h, false);
   h->A.dmc_rot = 0;
@@ -1755,10 +1844,12 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
                            hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_dmc_gather, dim3(Wl2), dim3(256), 0, h->stream, (const double *)h->A.st,
                      h->st2, (int64_t)h->A.L.stride, (const double *)h->dm, h->dm2,
-                     (const int *)h->dorigin, (const double *)h->dwt, Wl2);
+                     (const int *)h->dorigin, (const double *)h->dwt, Wl2,
+                     (const double *)h->A.tq, h->tq2, h->A.tq_stride);
   HIPCHK(h, hipGetLastError());
   std::swap(h->A.st, h->st2);
   std::swap(h->dm, h->dm2);
+  std::swap(h->A.tq, h->tq2);
   h->nact = n2;
   h->Wl = Wl2;
   h->A.Wp = Wl2;

@@ -1,11 +1,12 @@
-// engine_dmc.h -- kernels of one DMC generation (qmc/rundmc_mod.f90:97-330)
-// for the pseudopotential mover: drift-diffusion proposal with the averaged
-// velocity (qmc/green_functions_mod.f90:70-80, 598-633), Metropolis test with
-// the forward and backward Green's functions (:257-265, rundmc_mod.f90:
-// 142-196), and reweighting with ene_int_rewt (green_functions_mod.f90:
-// 382-472).  The one-electron ratio and the commit reuse k_ratio_1e (given
-// position) and k_accept_1e (given decision).  Every walker draws from its
-// own rannyu stream.
+// engine_dmc.h -- kernels of one DMC generation (qmc/rundmc_mod.f90:97-330):
+// T-moves (qmc/dmc/dmc_elec/tmove.f90) when asked for, the drift-diffusion
+// proposal with the averaged velocity (qmc/green_functions_mod.f90:70-80,
+// 598-633; with the nucleus cusp for all-electron systems, :83-145),
+// Metropolis test with the forward and backward Green's functions (:257-323,
+// rundmc_mod.f90:142-196), and reweighting with ene_int_rewt
+// (green_functions_mod.f90:382-472).  The one-electron ratios and commits
+// reuse k_ratio_1e (given position) and k_accept_1e (given decision).
+// Every walker draws from its own rannyu stream.
 #pragma once
 #include "engine_kernels.h"
 
@@ -28,7 +29,14 @@ enum {
   kDmEpot = 14,   // potential energy at the start of the generation
   kDmDd = 15,     // dd of the current proposal
   kDmPo = 16,     // all-electron mover: forward probability of the current proposal
-  kDmStride = 20
+  kDmTacc = 17,   // a T-move was accepted in this generation
+  kDmTmSel = 18,  // T-move of the current electron proposed
+  kDmTmPr = 19,   // its Psi ratio
+  kDmTmTot = 20,  // forward sum of the T-move weights (with 1 for staying)
+  kDmTmIc = 21,   // centre of the proposed point
+  kDmTmX = 22,    // proposed position (3)
+  kDmTmN = 25,    // pairs in range of the current electron
+  kDmStride = 28
 };
 
 // averaged_1e_velocity with adrift = adrift_default
@@ -129,6 +137,7 @@ __global__ void k_dmc_gen_begin(KArgs A, double *dm) {
   d[kDmEloc] = ws[A.L.eloc];
   d[kDmEpot] = ws[A.L.epot];
   for (int q = kDmSdd; q <= kDmNcross; q++) d[q] = 0.0;
+  d[kDmTacc] = 0.0;
 }
 
 // proposal for electron ie; g[w][4] holds its current gradient
@@ -330,18 +339,230 @@ __global__ void k_dmc_weights(const double *dm, int nact, double *out) {
 // population after branching: record d is a copy of record origin[d]
 __global__ void k_dmc_gather(const double *st_src, double *st_dst, int64_t stride,
                              const double *dm_src, double *dm_dst, const int *origin,
-                             const double *wnew, int nl) {
+                             const double *wnew, int nl, const double *tq_src, double *tq_dst,
+                             int64_t tq_stride) {
   const int d = blockIdx.x;
   if (d >= nl) return;
   const int o = origin[d];
   const double *src = st_src + (int64_t)o * stride;
   double *dst = st_dst + (int64_t)d * stride;
   for (int64_t q = threadIdx.x; q < stride; q += blockDim.x) dst[q] = src[q];
+  if (tq_src)
+    for (int64_t q = threadIdx.x; q < tq_stride; q += blockDim.x)
+      tq_dst[(int64_t)d * tq_stride + q] = tq_src[(int64_t)o * tq_stride + q];
   if (threadIdx.x < kDmStride) {
     double v = dm_src[(int64_t)o * kDmStride + threadIdx.x];
     if (threadIdx.x == kDmW) v = wnew[d];
     dm_dst[(int64_t)d * kDmStride + threadIdx.x] = v;
   }
+}
+
+// ---------------------------------------------------------------------------
+// T-moves (qmc/dmc/dmc_elec/tmove.f90:1-182), one electron per step, one
+// thread per walker.  Point c of walker w sits at column c * cw + w of the
+// point buffers (cw: the column stride, at least the launch's walkers), so
+// that each column is one k_ratio_1e launch over all walkers.
+
+// nonlocal channel values of electron-centre distance r and whether one is
+// in range (the criterion of the pseudopotential evaluation)
+__device__ __forceinline__ bool tm_channels(const DevSys &S, double r_en, int ict, double *vps) {
+  const double r = fmax(1e-10, r_en);
+  bool inr = false;
+  for (int l = 1; l <= cst(S.npotd)[ict]; l++) {
+    vps[l - 1] = ecp_radial(S, r, l - 1, ict);
+    if (l != cst(S.lpotp1)[ict] && fabs(vps[l - 1]) > 1e-4) inr = true;
+  }
+  return inr;
+}
+
+// weight max(0, sum_l w_q (exp(-tau v_l) - 1) P_l(cos) ratio) of one point
+__device__ __forceinline__ double tm_weight(const DevSys &S, int ict, const double *vps, int iq,
+                                            double costh, double ratio, double tau) {
+  double g = 0.0;
+  const double wq = cst(S.q0)[3 * S.nquad + iq];
+  for (int l = 1; l <= cst(S.npotd)[ict]; l++)
+    if (l != cst(S.lpotp1)[ict]) g += wq * (exp(-tau * vps[l - 1]) - 1.0) * legendre_w(l, costh) * ratio;
+  return fmax(0.0, g);
+}
+
+// step 1 (tmove.f90:50-98): the points of electron ie's pairs in range, in
+// pair order: those kept by the walker's last evaluation while it has
+// accepted no T-move in this generation, else those of a fresh rotation
+// drawn per pair.  tmq: unit vectors; tmx: positions (the electron's own
+// position past the walker's points, so every column is a valid move).
+__global__ void k_tm_points(KArgs A, int ie, int cw, double *tmq, double *tmx, double *dm) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const DevSys &S = A.S;
+  const int M = S.M, nq = S.nquad, ncol = S.tm_kmax * nq;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  double *d = dm + (int64_t)w * kDmStride;
+  const double *tqw = A.tq + (int64_t)w * A.tq_stride;
+  const double *ren = ws + A.L.r_en + ie * M;
+  const double *xe = ws + A.L.x + 3 * ie;
+  const bool fresh = d[kDmTacc] != 0.0;
+  int *rs = A.rng + 4 * w;
+  int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+  double vps[QMCW_MAX_LPOT];
+  int j = 0;
+  for (int ic = 0; ic < M && j < S.tm_kmax; ic++) {
+    if (!tm_channels(S, ren[ic], cst(S.iwctype)[ic], vps)) continue;
+    double ax[9];
+    if (fresh) rot_axes(st4, ax);
+    for (int iq = 0; iq < nq; iq++) {
+      double q[3];
+      if (fresh) {
+        const double a = cst(S.q0)[iq], b = cst(S.q0)[nq + iq], c = cst(S.q0)[2 * nq + iq];
+        q[0] = a * ax[0] + b * ax[3] + c * ax[6];
+        q[1] = a * ax[1] + b * ax[4] + c * ax[7];
+        q[2] = a * ax[2] + b * ax[5] + c * ax[8];
+      } else {
+        const double *qp = tqw + tq_point(S, ie, j, iq);
+        q[0] = qp[0]; q[1] = qp[1]; q[2] = qp[2];
+      }
+      const int64_t col = (int64_t)(j * nq + iq) * cw + w;
+      for (int k = 0; k < 3; k++) {
+        tmq[3 * col + k] = q[k];
+        tmx[3 * col + k] = ren[ic] * q[k] + cst(S.cent)[3 * ic + k];
+      }
+    }
+    j++;
+  }
+  for (int c = j * nq; c < ncol; c++)
+    for (int k = 0; k < 3; k++) {
+      tmq[3 * ((int64_t)c * cw + w) + k] = 0.0;
+      tmx[3 * ((int64_t)c * cw + w) + k] = xe[k];
+    }
+  d[kDmTmN] = j;
+  rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+}
+
+// step 2 (tmove.f90:100-150, discrete_rand :184-205): the weights of the
+// points (kept ratios, or rcol: those of the fresh points), the choice
+// among staying and the points, and for det_balance1 the backward points of
+// a fresh rotation about the chosen point's centre (tmb, tmbq; the
+// electron's own position when nothing was chosen)
+__global__ void k_tm_select(KArgs A, int ie, int cw, const double *tmq, const double *tmx,
+                            const double *rcol, double *tmb, double *tmbq, double *dm,
+                            qmcw_dmc_params p) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const DevSys &S = A.S;
+  const int M = S.M, nq = S.nquad;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  double *d = dm + (int64_t)w * kDmStride;
+  const double *tqw = A.tq + (int64_t)w * A.tq_stride;
+  const double *ren = ws + A.L.r_en + ie * M, *rven = ws + A.L.rvec_en + (int64_t)ie * M * 3;
+  const double *xe = ws + A.L.x + 3 * ie;
+  const bool fresh = d[kDmTacc] != 0.0;
+  const int npair = (int)d[kDmTmN];
+  int *rs = A.rng + 4 * w;
+  int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+  double vps[QMCW_MAX_LPOT];
+  // two passes over the same weights: their sum, then the cumulative choice
+  double tot = 1.0, rnd = 0.0, psum = 1.0;
+  int which = -1, wic = -1;
+  double wpr = 0.0;
+  int64_t wcol = 0;
+  for (int pass = 0; pass < 2 && which < 0; pass++) {
+    if (pass == 1) {
+      rnd = tot * rannyu(st4);
+      if (psum > rnd) { which = 0; break; }
+    }
+    int j = 0, last = -1, lic = -1;
+    double lpr = 0.0;
+    for (int ic = 0; ic < M && j < npair; ic++) {
+      const int ict = cst(S.iwctype)[ic];
+      if (!tm_channels(S, ren[ic], ict, vps)) continue;
+      for (int iq = 0; iq < nq; iq++) {
+        const int64_t col = (int64_t)(j * nq + iq) * cw + w;
+        const double *q = tmq + 3 * col, *rv = rven + 3 * ic;
+        const double costh = (rv[0] * q[0] + rv[1] * q[1] + rv[2] * q[2]) / ren[ic];
+        const double ratio = fresh ? rcol[8 * col] : tqw[tq_ratio(S, ie, j, iq)];
+        const double g = tm_weight(S, ict, vps, iq, costh, ratio, p.tau);
+        if (pass == 0) {
+          tot += g;
+        } else {
+          psum += g;
+          last = j * nq + iq; lic = ic; lpr = ratio;
+          if (psum > rnd) { which = last + 1; wic = ic; wpr = ratio; wcol = col; break; }
+        }
+      }
+      if (which >= 0) break;
+      j++;
+    }
+    if (pass == 1 && which < 0 && last >= 0) {   // rounding: the last point
+      which = last + 1; wic = lic; wpr = lpr; wcol = (int64_t)last * cw + w;
+    }
+    if (pass == 1 && which < 0) which = 0;
+  }
+  d[kDmTmSel] = which > 0 ? 1.0 : 0.0;
+  if (which > 0) {
+    for (int k = 0; k < 3; k++) d[kDmTmX + k] = tmx[3 * wcol + k];
+    d[kDmTmPr] = wpr;
+    d[kDmTmTot] = tot;
+    d[kDmTmIc] = wic;
+  }
+  double ax[9];
+  const bool back = which > 0 && p.tmoves == 1;
+  if (back) rot_axes(st4, ax);
+  for (int jq = 0; jq < nq; jq++) {
+    const int64_t col = (int64_t)jq * cw + w;
+    if (back) {
+      const double a = cst(S.q0)[jq], b = cst(S.q0)[nq + jq], c = cst(S.q0)[2 * nq + jq];
+      const double q[3] = {a * ax[0] + b * ax[3] + c * ax[6], a * ax[1] + b * ax[4] + c * ax[7],
+                           a * ax[2] + b * ax[5] + c * ax[8]};
+      for (int k = 0; k < 3; k++) {
+        tmbq[3 * col + k] = q[k];
+        tmb[3 * col + k] = ren[wic] * q[k] + cst(S.cent)[3 * wic + k];
+      }
+    } else {
+      for (int k = 0; k < 3; k++) { tmbq[3 * col + k] = 0.0; tmb[3 * col + k] = xe[k]; }
+    }
+  }
+  rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+}
+
+// step 3 (tmove.f90:150-180): det_balance1 accepts with min(1, forward sum /
+// backward sum) (bcol: ratios at the backward points, relative to the
+// current state), casula1 always; the decision and the position to commit
+// (the current one for walkers that keep the electron)
+__global__ void k_tm_decide(KArgs A, int ie, int cw, const double *tmbq, const double *bcol,
+                            double *xin, uint8_t *acc, double *dm, qmcw_dmc_params p) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= A.Wp) return;
+  const DevSys &S = A.S;
+  const int M = S.M, nq = S.nquad;
+  const double *ws = A.st + (int64_t)w * A.L.stride;
+  double *d = dm + (int64_t)w * kDmStride;
+  const double *xe = ws + A.L.x + 3 * ie;
+  int a = 0;
+  if (d[kDmTmSel] != 0.0) {
+    double pacc = 1.0;
+    if (p.tmoves == 1) {
+      const int ic = (int)d[kDmTmIc], ict = cst(S.iwctype)[ic];
+      const double rei = ws[A.L.r_en + ie * M + ic];
+      double vps[QMCW_MAX_LPOT];
+      tm_channels(S, rei, ict, vps);
+      double xen[3];
+      for (int k = 0; k < 3; k++) xen[k] = d[kDmTmX + k] - cst(S.cent)[3 * ic + k];
+      double bwd = 1.0;
+      for (int jq = 0; jq < nq; jq++) {
+        const int64_t col = (int64_t)jq * cw + w;
+        const double *q = tmbq + 3 * col;
+        const double costh = (xen[0] * q[0] + xen[1] * q[1] + xen[2] * q[2]) / rei;
+        bwd += tm_weight(S, ict, vps, jq, costh, bcol[8 * col] / d[kDmTmPr], p.tau);
+      }
+      pacc = fmin(1.0, d[kDmTmTot] / bwd);
+    }
+    int *rs = A.rng + 4 * w;
+    int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+    a = (rannyu(st4) < pacc) || p.tmoves == 2;
+    rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+    if (a) { d[kDmTacc] = 1.0; d[kDmAcc] = 1.0; }
+  }
+  acc[w] = (uint8_t)a;
+  for (int k = 0; k < 3; k++) xin[3 * w + k] = a ? d[kDmTmX + k] : xe[k];
 }
 
 }  // namespace qmcw

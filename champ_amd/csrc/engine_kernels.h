@@ -30,9 +30,19 @@ struct KArgs {
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
   int dmc_rot;       // DMC evaluation: pseudopotential grid rotated per pair in range
-  const double *dmove;  // DMC: accepted moves of the generation at dmove[w * dmove_stride]
+  const double *dmove;  // DMC: nonzero at dmove[w * dmove_stride] if the walker moved
   int dmove_stride;     // (null: every walker evaluates and draws)
+  double *tq;           // DMC: per walker (tq_stride doubles) the quadrature-point ratios
+  int64_t tq_stride;    // [N][tm_kmax][nq] and points [N][tm_kmax][nq][3] kept for T-moves
 };
+
+// offsets in a walker's T-move record
+__host__ __device__ inline int64_t tq_ratio(const DevSys &S, int i, int j, int iq) {
+  return ((int64_t)i * S.tm_kmax + j) * S.nquad + iq;
+}
+__host__ __device__ inline int64_t tq_point(const DevSys &S, int i, int j, int iq) {
+  return (int64_t)S.N * S.tm_kmax * S.nquad + 3 * tq_ratio(S, i, j, iq);
+}
 
 // occupancy hint of the heavy per-walker kernels (waves per SIMD)
 #ifndef QMCW_WPE
@@ -617,7 +627,7 @@ __device__ __forceinline__ void rot_axes(int *st4, double *ax) {
 template <typename T, int G, int ORD>
 __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, const double *ss,
                                               int npair, const int *plist, double *rnew, T *uq,
-                                              int lane) {
+                                              double *tqw, int lane) {
   const DevSys &S = A.S;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -639,7 +649,7 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
     double qx, qy, qz;
     if (A.dmc_rot) {
       // this pair's own rotation
-      const double *ax = prot + 9 * (t / nq);
+      const double *ax = prot + 10 * (t / nq);
       const double a = cst(S.q0)[iq], b = cst(S.q0)[nq + iq], c = cst(S.q0)[2 * nq + iq];
       qx = a * ax[0] + b * ax[3] + c * ax[6];
       qy = a * ax[1] + b * ax[4] + c * ax[7];
@@ -674,6 +684,16 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
     for (int l = 1; l <= cst(S.npotd)[ict]; l++)
       if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
     vpsp += cst(S.q0)[3 * nq + iq] * exp(delta) * vpot * rd;
+    if (tqw) {
+      // kept for T-moves: the point and the Psi ratio there (rank j of the
+      // pair among electron i's pairs in range)
+      const int j = (int)prot[10 * (t / nq) + 9];
+      if (j >= 0) {
+        tqw[tq_ratio(S, i, j, iq)] = rd * exp(delta);
+        double *qp = tqw + tq_point(S, i, j, iq);
+        qp[0] = qx; qp[1] = qy; qp[2] = qz;
+      }
+    }
   }
   return vpsp;
 }
@@ -728,23 +748,34 @@ void k_full_nlpp(KArgs A) {
     // loops make the number of draws vary, so one lane walks the stream and
     // keeps each pair's axes; walkers that moved no electron are not
     // evaluated by the reference and draw nothing.
-    if (A.dmc_rot && lane == 0 && (!A.dmove || A.dmove[(int64_t)w * A.dmove_stride] > 0.0)) {
+    const bool mover = A.dmc_rot && (!A.dmove || A.dmove[(int64_t)w * A.dmove_stride] > 0.0);
+    if (mover && lane == 0) {
       int *rs = A.rng + 4 * w;
       double *pax = ss + SL.prot;
       int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
-      for (int k = 0; k < npair; k++) rot_axes(st4, pax + 9 * k);
+      int prev = -1, j = 0;
+      for (int k = 0; k < npair; k++) {
+        rot_axes(st4, pax + 10 * k);
+        const int i = plist[k] / M;
+        j = (i == prev) ? j + 1 : 0;
+        prev = i;
+        if (j >= S.tm_kmax) A.flags[w] |= 4;     // more pairs than the T-move record holds
+        pax[10 * k + 9] = j < S.tm_kmax ? (double)j : -1.0;
+      }
       rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
     }
     __syncthreads();
+    double *tqw = (mover && A.tq) ? A.tq + (int64_t)w * A.tq_stride : nullptr;
     // per-lane distances of the point and U of the other electrons: LDS
     // when the launch reserved it
     if (A.lds_nlpp) {
       extern __shared__ double dyn_lds[];
       double *lb = dyn_lds + (int64_t)threadIdx.x * A.lds_nlpp;
-      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, lb, reinterpret_cast<T *>(lb + 4 * M), lane);
+      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, lb, reinterpret_cast<T *>(lb + 4 * M),
+                                    tqw, lane);
     } else {
       vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, ss + SL.qd + (int64_t)lane * 4 * M,
-                                    reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N), lane);
+                                    reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N), tqw, lane);
     }
     vpsp = gsum<G>(vpsp);
   }

@@ -26,11 +26,14 @@ def _setup(s, n0, cap, seed):
     return x, seeds
 
 
-@pytest.mark.parametrize("tau", [0.02, 0.1])
+@pytest.mark.parametrize("tau,tmoves", [(0.02, 0), (0.1, 0), (0.1, 1), (0.1, 2), (0.3, 1)])
 @pytest.mark.parametrize("name", ["c2_dimer", "be_atom"])
-def test_dmc_generations_match_oracle(name, tau):
-    """c2_dimer: pseudopotential mover; be_atom: all-electron mover with the
-    nucleus cusp (gaussian or exponential branch per proposal)."""
+def test_dmc_generations_match_oracle(name, tau, tmoves):
+    """c2_dimer: pseudopotential mover, with T-moves (1: det_balance1, 2:
+    casula1) before the moves; be_atom: all-electron mover with the nucleus
+    cusp (gaussian or exponential branch per proposal)."""
+    if name == "be_atom" and tmoves:
+        pytest.skip("T-moves act on pseudopotentials")
     s = system.c2_dimer(2) if name == "c2_dimer" else system.be_atom(1)
     n0, cap = 20, 48
     x, seeds = _setup(s, n0, cap, seed=31)
@@ -46,7 +49,7 @@ def test_dmc_generations_match_oracle(name, tau):
     d = oracle.OracleDMC(s, cap)
     assert d.set(x[:n0], seeds[:n0]) == 0
     bo = brng.copy()
-    p = DmcParams(tau, e0, eest=e0, esigma=1.0)
+    p = DmcParams(tau, e0, eest=e0, esigma=1.0, tmoves=tmoves)
     for gen in range(4):
         sg = e.dmc_step(p)
         r, so = d.generation(p)
