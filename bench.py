@@ -302,7 +302,7 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
     return dict(aggregate=agg, per_system=per, seconds=dt)
 
 
-def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20):
+def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0):
     """DMC of the 98-electron chain (16 Z_eff=4 centres with a
     pseudopotential, 34 H, 424 GTOs, one determinant): W0 walkers, capacity
     2 W0, time step tau, under the rundmc driver (population control),
@@ -325,7 +325,7 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20):
         e.vmc_device(equil, 0.3)
     e.dmc_begin(W0)
     e0 = float(e.get_eloc()[:W0].mean())
-    drv = DmcDriver(e, s.nelec, W0, tau=tau, etrial=e0)
+    drv = DmcDriver(e, s.nelec, W0, tau=tau, etrial=e0, tmoves=tmoves)
     drv.run(max(1, warmup))
     e.sync()
     t0 = time.perf_counter()
@@ -382,6 +382,8 @@ def main():
     ap.add_argument("--no-derivs", action="store_true", help="skip the derivative samples/s measurement")
     ap.add_argument("--dmc", action="store_true",
                     help="only run DMC of the 98-electron chain (4096 walkers, tau 0.01)")
+    ap.add_argument("--tmoves", type=int, default=0, choices=[0, 1, 2],
+                    help="with --dmc: T-moves (1 det_balance1, 2 casula1)")
     ap.add_argument("--batch", type=int, default=0,
                     help="only run the multi-system batch with this many systems")
     args = ap.parse_args()
@@ -393,7 +395,7 @@ def main():
         args.steps = 20 if args.dmc else 200
     if args.dmc:
         prec = "mixed" if args.precision == "both" else args.precision
-        d = run_dmc_chain(4096, args.steps, args.warmup, prec)
+        d = run_dmc_chain(4096, args.steps, args.warmup, prec, tmoves=args.tmoves)
         for k, (ms, n) in d["prof"].items():
             if n:
                 print(f"[bench]   {k:12s} {n:5d} launches  avg {ms / n * 1e3:9.1f} us  "
@@ -403,7 +405,9 @@ def main():
                           "warmup": args.warmup, "higher_is_better": True,
                           "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
                           "config": {"workload": "DMC, 98-electron chain, 4096 walkers (target), "
-                                                 "tau 0.01, from 20 VMC sweeps",
+                                                 "tau 0.01, from 20 VMC sweeps"
+                                                 + (", T-moves" if args.tmoves else ""),
+                                     "tmoves": args.tmoves,
                                      "precision": prec, "group_width": d["G"]},
                           "nwalk_final": d["nwalk_final"], "nwalk_min_mean_max": d["nwalk_range"],
                           "acceptance": d["acc"],

@@ -78,6 +78,7 @@ struct qmcw_handle {
   double *tq2 = nullptr;
   double *tmq = nullptr, *tmx = nullptr, *rcol = nullptr, *tmb = nullptr, *tmbq = nullptr,
          *bcol = nullptr, *tmxin = nullptr;
+  int *tmint = nullptr;      // [2][Wp]: ratio columns a fresh walker needs; move selected
   int *dorigin = nullptr;
   int32_t brng[4] = {0, 0, 0, 1};
   // event timing of the device driver
@@ -1639,6 +1640,11 @@ static int tm_alloc(qmcw_handle *h) {
   if (!st) st = get(nb * Wp * 3, h->tmbq);
   if (!st) st = get(nb * Wp * 8, h->bcol);
   if (!st) st = get(Wp * 3, h->tmxin);
+  if (!st) {
+    HIPCHK(h, hipMalloc(&p, 2 * Wp * sizeof(int)));
+    h->allocs.push_back(p);
+    h->tmint = (int *)p;
+  }
   return st;
 }
 
@@ -1776,15 +1782,32 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
     // backward ratios (det_balance1), the decision and the commit
     int st = tm_alloc(h);
     if (st) return st;
-    const int cw = h->Wp, ncol = h->A.S.tm_kmax * h->A.S.nquad, nq = h->A.S.nquad;
+    // Host reads of two small flag arrays per electron skip the ratio
+    // launches nobody needs: columns past the most pairs of a walker with
+    // fresh points, and the backward and commit launches when no walker
+    // chose a move.
+    const int cw = h->Wp, nq = h->A.S.nquad;
+    std::vector<int> fl(h->Wl);
+    auto host_max = [&](const int *dflag, int &out) -> int {
+      HIPCHK(h, hipMemcpyAsync(fl.data(), dflag, h->Wl * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      out = 0;
+      for (int v : fl) out = std::max(out, v);
+      return QMCW_OK;
+    };
     for (int ie = 0; ie < N; ie++) {
-      hipLaunchKernelGGL(k_tm_points, g1, b1, 0, h->stream, h->A, ie, cw, h->tmq, h->tmx, h->dm);
-      if (ie > 0)
-        for (int c = 0; c < ncol; c++)
-          launch_ratio(h, ie, 0, (const double *)(h->tmx + (size_t)c * cw * 3), 1.0, 1.0,
-                       h->rcol + (size_t)c * cw * 8);
+      hipLaunchKernelGGL(k_tm_points, g1, b1, 0, h->stream, h->A, ie, cw, h->tmq, h->tmx, h->dm,
+                         h->tmint);
+      int need = 0, anysel = 0;
+      if (ie > 0 && (st = host_max(h->tmint, need))) return st;
+      for (int c = 0; c < need * nq; c++)
+        launch_ratio(h, ie, 0, (const double *)(h->tmx + (size_t)c * cw * 3), 1.0, 1.0,
+                     h->rcol + (size_t)c * cw * 8);
       hipLaunchKernelGGL(k_tm_select, g1, b1, 0, h->stream, h->A, ie, cw, (const double *)h->tmq,
-                         (const double *)h->tmx, (const double *)h->rcol, h->tmb, h->tmbq, h->dm, *p);
+                         (const double *)h->tmx, (const double *)h->rcol, h->tmb, h->tmbq, h->dm,
+                         h->tmint + cw, *p);
+      if ((st = host_max(h->tmint + cw, anysel))) return st;
+      if (!anysel) continue;
       if (p->tmoves == 1)
         for (int jq = 0; jq < nq; jq++)
           launch_ratio(h, ie, 0, (const double *)(h->tmb + (size_t)jq * cw * 3), 1.0, 1.0,

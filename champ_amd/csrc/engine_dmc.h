@@ -390,7 +390,8 @@ __device__ __forceinline__ double tm_weight(const DevSys &S, int ict, const doub
 // accepted no T-move in this generation, else those of a fresh rotation
 // drawn per pair.  tmq: unit vectors; tmx: positions (the electron's own
 // position past the walker's points, so every column is a valid move).
-__global__ void k_tm_points(KArgs A, int ie, int cw, double *tmq, double *tmx, double *dm) {
+__global__ void k_tm_points(KArgs A, int ie, int cw, double *tmq, double *tmx, double *dm,
+                            int *need) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= A.Wp) return;
   const DevSys &S = A.S;
@@ -434,6 +435,7 @@ __global__ void k_tm_points(KArgs A, int ie, int cw, double *tmq, double *tmx, d
       tmx[3 * ((int64_t)c * cw + w) + k] = xe[k];
     }
   d[kDmTmN] = j;
+  need[w] = fresh ? j : 0;     // pairs whose fresh points need ratio columns
   rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
 }
 
@@ -444,7 +446,7 @@ __global__ void k_tm_points(KArgs A, int ie, int cw, double *tmq, double *tmx, d
 // electron's own position when nothing was chosen)
 __global__ void k_tm_select(KArgs A, int ie, int cw, const double *tmq, const double *tmx,
                             const double *rcol, double *tmb, double *tmbq, double *dm,
-                            qmcw_dmc_params p) {
+                            int *sel, qmcw_dmc_params p) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= A.Wp) return;
   const DevSys &S = A.S;
@@ -497,6 +499,7 @@ __global__ void k_tm_select(KArgs A, int ie, int cw, const double *tmq, const do
     if (pass == 1 && which < 0) which = 0;
   }
   d[kDmTmSel] = which > 0 ? 1.0 : 0.0;
+  sel[w] = which > 0 ? 1 : 0;
   if (which > 0) {
     for (int k = 0; k < 3; k++) d[kDmTmX + k] = tmx[3 * wcol + k];
     d[kDmTmPr] = wpr;
