@@ -78,7 +78,8 @@ struct qmcw_handle {
   double *tq2 = nullptr;
   double *tmq = nullptr, *tmx = nullptr, *rcol = nullptr, *tmb = nullptr, *tmbq = nullptr,
          *bcol = nullptr, *tmxin = nullptr;
-  int *tmint = nullptr;      // [2][Wp]: ratio columns a fresh walker needs; move selected
+  int *tmint = nullptr;      // [3][Wp]: fresh points to evaluate, backward points, move chosen
+  double *tmstale = nullptr; // [Wp]: G, gp and bt of the slot must be rebuilt from the state
   int *dorigin = nullptr;
   int32_t brng[4] = {0, 0, 0, 1};
   // event timing of the device driver
@@ -1640,8 +1641,9 @@ static int tm_alloc(qmcw_handle *h) {
   if (!st) st = get(nb * Wp * 3, h->tmbq);
   if (!st) st = get(nb * Wp * 8, h->bcol);
   if (!st) st = get(Wp * 3, h->tmxin);
+  if (!st) st = get(Wp, h->tmstale);
   if (!st) {
-    HIPCHK(h, hipMalloc(&p, 2 * Wp * sizeof(int)));
+    HIPCHK(h, hipMalloc(&p, 3 * Wp * sizeof(int)));
     h->allocs.push_back(p);
     h->tmint = (int *)p;
   }
@@ -1777,43 +1779,37 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
   hipLaunchKernelGGL(k_dmc_gen_begin, g1, b1, 0, h->stream, h->A, h->dm);
   if (p->tmoves && h->A.S.nloc > 0) {
     // T-moves of every electron in turn (rundmc_mod.f90:137, tmove.f90):
-    // points, their ratios when fresh (one k_ratio_1e launch per point
-    // column; none is fresh for the first electron), the choice, the
-    // backward ratios (det_balance1), the decision and the commit
+    // points, their ratios when fresh (none is for the first electron), the
+    // choice, the backward ratios (det_balance1), the decision and the
+    // commit (k_ratio_1e at the chosen point, k_accept_1e)
     int st = tm_alloc(h);
     if (st) return st;
-    // Host reads of two small flag arrays per electron skip the ratio
-    // launches nobody needs: columns past the most pairs of a walker with
-    // fresh points, and the backward and commit launches when no walker
-    // chose a move.
-    const int cw = h->Wp, nq = h->A.S.nquad;
+    // Ratios at fresh and backward points come from one k_tm_ratios launch
+    // each (a walker with nothing to evaluate leaves at once); a host read
+    // of the choice flags skips the backward, decision and commit launches
+    // of an electron for which no walker chose a move.
+    const int cw = h->Wp;
+    const OrdKernels &K = ord_kernels(h->A.S.nordc);
+    const LaunchCtx ct = make_ctx(h, h->A, h->Gk[KC_NLPP]);
+    int *nfwd = h->tmint, *nbwd = h->tmint + cw, *sel = h->tmint + 2 * cw;
+    hipLaunchKernelGGL(k_fill_d, dim3((h->Wl + 255) / 256), dim3(256), 0, h->stream, h->tmstale, 1.0, h->Wl);
     std::vector<int> fl(h->Wl);
-    auto host_max = [&](const int *dflag, int &out) -> int {
-      HIPCHK(h, hipMemcpyAsync(fl.data(), dflag, h->Wl * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-      HIPCHK(h, hipStreamSynchronize(h->stream));
-      out = 0;
-      for (int v : fl) out = std::max(out, v);
-      return QMCW_OK;
-    };
     for (int ie = 0; ie < N; ie++) {
-      hipLaunchKernelGGL(k_tm_points, g1, b1, 0, h->stream, h->A, ie, cw, h->tmq, h->tmx, h->dm,
-                         h->tmint);
-      int need = 0, anysel = 0;
-      if (ie > 0 && (st = host_max(h->tmint, need))) return st;
-      for (int c = 0; c < need * nq; c++)
-        launch_ratio(h, ie, 0, (const double *)(h->tmx + (size_t)c * cw * 3), 1.0, 1.0,
-                     h->rcol + (size_t)c * cw * 8);
+      hipLaunchKernelGGL(k_tm_points, g1, b1, 0, h->stream, h->A, ie, cw, h->tmq, h->tmx, h->dm, nfwd);
+      if (ie > 0)
+        K.tm_ratios(ct, ie, cw, (const double *)h->tmx, (const int *)nfwd, h->rcol, h->tmstale);
       hipLaunchKernelGGL(k_tm_select, g1, b1, 0, h->stream, h->A, ie, cw, (const double *)h->tmq,
                          (const double *)h->tmx, (const double *)h->rcol, h->tmb, h->tmbq, h->dm,
-                         h->tmint + cw, *p);
-      if ((st = host_max(h->tmint + cw, anysel))) return st;
+                         nbwd, sel, *p);
+      HIPCHK(h, hipMemcpyAsync(fl.data(), sel, h->Wl * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      int anysel = 0;
+      for (int v : fl) anysel |= v;
       if (!anysel) continue;
       if (p->tmoves == 1)
-        for (int jq = 0; jq < nq; jq++)
-          launch_ratio(h, ie, 0, (const double *)(h->tmb + (size_t)jq * cw * 3), 1.0, 1.0,
-                       h->bcol + (size_t)jq * cw * 8);
+        K.tm_ratios(ct, ie, cw, (const double *)h->tmb, (const int *)nbwd, h->bcol, h->tmstale);
       hipLaunchKernelGGL(k_tm_decide, g1, b1, 0, h->stream, h->A, ie, cw, (const double *)h->tmbq,
-                         (const double *)h->bcol, h->tmxin, h->acc_buf, h->dm, *p);
+                         (const double *)h->bcol, h->tmxin, h->acc_buf, h->dm, h->tmstale, *p);
       launch_ratio(h, ie, 0, (const double *)h->tmxin, 1.0, 1.0, (double *)nullptr);
       DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
     }

@@ -435,7 +435,7 @@ __global__ void k_tm_points(KArgs A, int ie, int cw, double *tmq, double *tmx, d
       tmx[3 * ((int64_t)c * cw + w) + k] = xe[k];
     }
   d[kDmTmN] = j;
-  need[w] = fresh ? j : 0;     // pairs whose fresh points need ratio columns
+  need[w] = fresh ? j * nq : 0;     // fresh points that need ratios
   rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
 }
 
@@ -443,10 +443,11 @@ __global__ void k_tm_points(KArgs A, int ie, int cw, double *tmq, double *tmx, d
 // points (kept ratios, or rcol: those of the fresh points), the choice
 // among staying and the points, and for det_balance1 the backward points of
 // a fresh rotation about the chosen point's centre (tmb, tmbq; the
-// electron's own position when nothing was chosen)
+// electron's own position when nothing was chosen); nback[w]: backward
+// points to evaluate, sel[w]: a move was chosen
 __global__ void k_tm_select(KArgs A, int ie, int cw, const double *tmq, const double *tmx,
                             const double *rcol, double *tmb, double *tmbq, double *dm,
-                            int *sel, qmcw_dmc_params p) {
+                            int *nback, int *sel, qmcw_dmc_params p) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= A.Wp) return;
   const DevSys &S = A.S;
@@ -500,6 +501,7 @@ __global__ void k_tm_select(KArgs A, int ie, int cw, const double *tmq, const do
   }
   d[kDmTmSel] = which > 0 ? 1.0 : 0.0;
   sel[w] = which > 0 ? 1 : 0;
+  nback[w] = (which > 0 && p.tmoves == 1) ? nq : 0;
   if (which > 0) {
     for (int k = 0; k < 3; k++) d[kDmTmX + k] = tmx[3 * wcol + k];
     d[kDmTmPr] = wpr;
@@ -531,7 +533,8 @@ __global__ void k_tm_select(KArgs A, int ie, int cw, const double *tmq, const do
 // current state), casula1 always; the decision and the position to commit
 // (the current one for walkers that keep the electron)
 __global__ void k_tm_decide(KArgs A, int ie, int cw, const double *tmbq, const double *bcol,
-                            double *xin, uint8_t *acc, double *dm, qmcw_dmc_params p) {
+                            double *xin, uint8_t *acc, double *dm, double *stale,
+                            qmcw_dmc_params p) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= A.Wp) return;
   const DevSys &S = A.S;
@@ -562,10 +565,15 @@ __global__ void k_tm_decide(KArgs A, int ie, int cw, const double *tmbq, const d
     int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
     a = (rannyu(st4) < pacc) || p.tmoves == 2;
     rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
-    if (a) { d[kDmTacc] = 1.0; d[kDmAcc] = 1.0; }
+    if (a) { d[kDmTacc] = 1.0; d[kDmAcc] = 1.0; stale[w] = 1.0; }
   }
   acc[w] = (uint8_t)a;
   for (int k = 0; k < 3; k++) xin[3 * w + k] = a ? d[kDmTmX + k] : xe[k];
+}
+
+__global__ void k_fill_d(double *a, double v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 }  // namespace qmcw

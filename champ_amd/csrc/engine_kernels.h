@@ -782,6 +782,66 @@ void k_full_nlpp(KArgs A) {
   if (lane == 0) { red[55] = ploc; red[56] = vpsp; }
 }
 
+// T-moves: Psi ratios of moving electron ie to points of the walker's
+// current state, the linear form of the pseudopotential evaluation
+// (sum_b gp_i[b] phi_b(x) times the Jastrow factor; qmc/dmc/dmc_elec/
+// tmove.f90:207-255 computes the same ratio by a full one-electron update).
+// A walker's G matrices, their projection gp and the e-e-n variables bt are
+// rebuilt from its state first when its record is marked stale (after a
+// committed T-move, or after branching moved it to another scratch slot).
+// pts/out: column c of walker w at c * cw + w (3 and 8 doubles);
+// cnt[w]: the walker's number of points (0: nothing to do).
+template <typename T, int G, int ORD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(QMCW_WPE, 8)))
+void k_tm_ratios(KArgs A, int ie, int cw, const double *pts, const int *cnt, double *out,
+                 double *stale) {
+  QMCW_GROUP_SETUP
+  const int N = S.N, M = S.M, nq = S.nquad;
+  const WLayout &L = A.L;
+  const SLayout &SL = A.SL;
+  const int n = cnt[w];
+  if (n == 0) return;                  // the whole group leaves together
+  const double *ren = ws + L.r_en, *rven = ws + L.rvec_en, *fso = ws + L.fso, *x = ws + L.x;
+  T *bt = reinterpret_cast<T *>(ss + SL.bt);
+  if (stale[w] != 0.0) {
+    for (int sp = 0; sp < 2; sp++)
+      group_g<G>(S, sp, ws + L.ai[sp], ws + L.tocc[sp], ws + L.y[sp], ss + SL.g[sp], ss + SL.red, lane);
+    __syncthreads();
+    group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
+    for (int p = lane; p < N * M; p += G) {
+      T bu, t1, t2;
+      scaled<T>(S.isc, (T)S.scalek, (T)ren[p], bu, t1, t2);
+      bt[p] = T(1) - (T)S.scalek * bu;
+    }
+    __syncthreads();
+    if (lane == 0) stale[w] = 0.0;
+  }
+  double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
+  T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
+  const double *gpi = ss + SL.gproj + (int64_t)ie * S.NB;
+  double fold = fso[ie * N + ie];
+  for (int jj = 0; jj < N; jj++)
+    if (jj != ie) fold += fso[jj < ie ? ie * N + jj : jj * N + ie];
+  for (int c = lane; c < n; c += G) {
+    const double *xn = pts + 3 * ((int64_t)c * cw + w);
+    for (int jc = 0; jc < M; jc++) {
+      const double dx = xn[0] - cst(S.cent)[3 * jc], dy = xn[1] - cst(S.cent)[3 * jc + 1],
+                   dz = xn[2] - cst(S.cent)[3 * jc + 2];
+      rvnew[3 * jc] = dx; rvnew[3 * jc + 1] = dy; rvnew[3 * jc + 2] = dz;
+      rnew[jc] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    double rd = 0.0;
+    for (int sh = 0; sh < S.nshell; sh++) {
+      const int jc = cst(S.sh_cent)[sh];
+      rd += (double)shell_dot<T, double>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1],
+                                         (T)rvnew[3 * jc + 2], (T)rnew[jc], gpi);
+    }
+    const double delta = (double)jas_point_val<T, ORD>(S, ie, xn, x, rnew, bt, uq) - fold;
+    out[8 * ((int64_t)c * cw + w)] = rd * exp(delta);
+  }
+  (void)nq; (void)rven;
+}
+
 template <int G>
 __global__ __launch_bounds__(64) void k_full_fin(KArgs A) {
   QMCW_GROUP_SETUP

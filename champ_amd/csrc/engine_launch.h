@@ -28,6 +28,8 @@ struct OrdKernels {
   void (*deriv_jas)(const LaunchCtx &);
   void (*deriv_nlpp)(const LaunchCtx &);
   void (*move_a)(const LaunchCtx &, int ie, int mode, const double *xn, double delta, double fbias);
+  void (*tm_ratios)(const LaunchCtx &, int ie, int cw, const double *pts, const int *cnt,
+                    double *out, double *stale);
 };
 
 extern const OrdKernels kOrdKernels2, kOrdKernels3, kOrdKernels4, kOrdKernels5, kOrdKernels6,

@@ -83,6 +83,19 @@ void move_a_g(const LaunchCtx &c, int ie, int mode, const double *xn, double del
 void move_a(const LaunchCtx &c, int ie, int mode, const double *xn, double delta, double fbias) {
   QMCW_BY_G(move_a_g, c, ie, mode, xn, delta, fbias);
 }
+template <int G>
+void tm_ratios_g(const LaunchCtx &c, int ie, int cw, const double *pts, const int *cnt, double *out,
+                 double *stale) {
+  const dim3 grid(c.Wl / (kWave / G)), block(kWave);
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_tm_ratios<double, G, ORD>), grid, block, 0, c.stream, c.A, ie, cw, pts, cnt, out, stale);
+  else
+    hipLaunchKernelGGL((k_tm_ratios<float, G, ORD>), grid, block, 0, c.stream, c.A, ie, cw, pts, cnt, out, stale);
+}
+void tm_ratios(const LaunchCtx &c, int ie, int cw, const double *pts, const int *cnt, double *out,
+               double *stale) {
+  QMCW_BY_G(tm_ratios_g, c, ie, cw, pts, cnt, out, stale);
+}
 void deriv_jas(const LaunchCtx &c) { QMCW_BY_G(deriv_jas_g, c); }
 void deriv_nlpp(const LaunchCtx &c) { QMCW_BY_G(deriv_nlpp_g, c); }
 
@@ -90,7 +103,7 @@ void deriv_nlpp(const LaunchCtx &c) { QMCW_BY_G(deriv_nlpp_g, c); }
 
 #ifndef __HIP_DEVICE_COMPILE__   // a host table of host launchers
 extern const OrdKernels QMCW_CAT(kOrdKernels, QMCW_ORD) = {full_jas, full_nlpp, ratio, deriv_jas,
-                                                            deriv_nlpp, move_a};
+                                                            deriv_nlpp, move_a, tm_ratios};
 #endif
 
 }  // namespace qmcw
