@@ -277,8 +277,10 @@ typedef struct qmcw_dmc_stats {
    (:83-145, density :268-323) for all-electron ones. */
 int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weight);
 
-/* One DMC generation (qmc/rundmc_mod.f90:97-330): for every walker and
-   electron the drift-diffusion proposal (qmc/green_functions_mod.f90:70-80),
+/* One DMC generation (qmc/rundmc_mod.f90:97-330): with p->tmoves, first the
+   T-moves of every electron (qmc/dmc/dmc_elec/tmove.f90); then for every
+   walker and electron the drift-diffusion proposal
+   (qmc/green_functions_mod.f90:70-145),
    the one-electron ratio, the Metropolis test with the forward/backward
    Green's functions (rundmc_mod.f90:142-196) and the commit; the full
    evaluation of the walkers that moved; the effective time step and the

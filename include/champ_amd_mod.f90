@@ -45,6 +45,20 @@ module champ_amd_mod
     real(c_double) :: hb
   end type qmcw_system
 
+  ! struct qmcw_dmc_params: CHAMP's DMC controls of one generation
+  type, bind(C) :: qmcw_dmc_params
+    real(c_double) :: tau, etrial, eest, esigma, pop_ctrl, taueff
+    real(c_double) :: c_rewt, pow_rewt, limit_rewt, adrift
+    integer(c_int32_t) :: n_drift, itau_eff, itau_integ, iacc_rej, icross, ene_int, idmc
+    integer(c_int32_t) :: tmoves
+  end type qmcw_dmc_params
+
+  ! struct qmcw_dmc_stats: population sums of one generation
+  type, bind(C) :: qmcw_dmc_stats
+    real(c_double) :: wsum, wesum, we2sum, wtausum, pacc, nacc, ntry, ncross
+    integer(c_int32_t) :: nwalk, nbranch
+  end type qmcw_dmc_stats
+
   interface
     function qmcw_sizeof_system() bind(C, name="qmcw_sizeof_system") result(n)
       import :: c_int64_t
@@ -197,6 +211,46 @@ module champ_amd_mod
       import :: c_ptr
       type(c_ptr), value :: h
       type(c_ptr) :: msg
+    end function
+    function qmcw_autotune(h, chosen) bind(C, name="qmcw_autotune") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: chosen
+      integer(c_int) :: st
+    end function
+    ! DMC (qmc/rundmc_mod.f90): weight may be c_null_ptr (all 1)
+    function qmcw_dmc_begin(h, nwalk, weight) bind(C, name="qmcw_dmc_begin") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nwalk
+      type(c_ptr), value :: weight
+      integer(c_int) :: st
+    end function
+    function qmcw_dmc_step(h, p, stats) bind(C, name="qmcw_dmc_step") result(st)
+      import :: c_ptr, c_int, qmcw_dmc_params, qmcw_dmc_stats
+      type(c_ptr), value :: h
+      type(qmcw_dmc_params), intent(in) :: p
+      type(qmcw_dmc_stats), intent(out) :: stats
+      integer(c_int) :: st
+    end function
+    function qmcw_dmc_get(h, nwalk, weight) bind(C, name="qmcw_dmc_get") result(st)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: nwalk
+      real(c_double), intent(out) :: weight(*)
+      integer(c_int) :: st
+    end function
+    function qmcw_set_branch_rng(h, seed) bind(C, name="qmcw_set_branch_rng") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(in) :: seed(*)
+      integer(c_int) :: st
+    end function
+    function qmcw_get_branch_rng(h, seed) bind(C, name="qmcw_get_branch_rng") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: seed(*)
+      integer(c_int) :: st
     end function
   end interface
 end module champ_amd_mod

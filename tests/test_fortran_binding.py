@@ -17,7 +17,9 @@ program t
   use champ_amd_mod
   implicit none
   type(qmcw_system) :: s
-  print '(3i8)', c_sizeof(s), qmcw_sizeof_system(), qmcw_nterms4(5_c_int32_t)
+  type(qmcw_dmc_params) :: p
+  type(qmcw_dmc_stats) :: d
+  print '(5i8)', c_sizeof(s), qmcw_sizeof_system(), qmcw_nterms4(5_c_int32_t), c_sizeof(p), c_sizeof(d)
 end program
 """
 
@@ -34,6 +36,10 @@ def test_fortran_module_matches_the_c_abi(tmp_path):
     subprocess.run([FLANG, "t.f90", "mod.o", f"-L{libdir}", "-lchamp_amd", f"-Wl,-rpath,{libdir}",
                     "-o", "t"], cwd=tmp_path, check=True, capture_output=True)
     out = subprocess.run(["./t"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout
-    f_size, c_size, nterms = map(int, out.split())
+    f_size, c_size, nterms, p_size, d_size = map(int, out.split())
     assert f_size == c_size
     assert nterms == 23
+    # the DMC structs have the sizes of their C layout (mirrored by abi.py)
+    import ctypes
+    assert p_size == ctypes.sizeof(abi.DmcParams) == 10 * 8 + 8 * 4
+    assert d_size == ctypes.sizeof(abi.DmcStats) == 8 * 8 + 2 * 4
