@@ -61,6 +61,11 @@ struct qmcw_handle {
   double graph_delta = 0.0, graph_fbias = 0.0;
   const double *graph_st = nullptr;
   int use_graph = 1;
+  // every walker's occupied-orbital rows in the state match its positions
+  // (a full evaluation computed them and only committed moves changed
+  // positions since); cleared by new positions or orbital coefficients
+  int orb_valid = 0;
+  int no_keep_orb = 0;       // QMCW_NO_KEEP_ORB: recompute them in every full evaluation
   int use_lds = 1;           // LDS staging of the moved electron's basis values
   int chunk_override = 0;    // electrons per basis chunk (0: from the L2 budget)
   size_t det_lds_budget = 10240;  // LDS bytes per block for k_full_det's basis values
@@ -532,6 +537,10 @@ static void launch_derivs_asm(qmcw_handle *h, const KArgs &A) {
 static void launch_eval(qmcw_handle *h, bool derivs) {
   KArgs A = h->A;
   A.full_orb = derivs ? 1 : 0;
+  // the derivatives need all orbitals of every electron, the moves keep
+  // only the occupied ones
+  A.keep_orb = (h->orb_valid && !derivs && !h->no_keep_orb) ? 1 : 0;
+  h->orb_valid = 1;
   // electrons per basis chunk of k_full_det: a wavefront's basis values
   // (64/G walkers) stay near 16 KB so that they are read back from L2
   const int gd = h->Gk[KC_DET];
@@ -1061,6 +1070,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_DET_LDS")) h->det_lds_budget = (size_t)atol(e);
   if (const char *e = getenv("QMCW_SPLIT_MOVE")) { h->split_move = atoi(e) != 0; h->split_forced = 1; }
   if (const char *e = getenv("QMCW_NO_LDS_Y")) h->lds_y = atoi(e) == 0;
+  if (const char *e = getenv("QMCW_NO_KEEP_ORB")) h->no_keep_orb = atoi(e) != 0;
   {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1104,6 +1114,7 @@ extern "C" int qmcw_set_params(qmcw_handle *h, const double *a4, const double *b
                                const double *csf_coef, const double *coef) {
   if (!h) return QMCW_ERR_ARG;
   drop_graph(h);   // the captured kernel arguments hold the Jastrow asymptotes
+  h->orb_valid = 0;
   DevSys &S = h->A.S;
   if (a4) h->a4.assign(a4, a4 + h->a4.size());
   if (b) h->b.assign(b, b + h->b.size());
@@ -1166,6 +1177,7 @@ extern "C" int qmcw_set_walkers(qmcw_handle *h, const double *x) {
   if (!h || !x) return QMCW_ERR_ARG;
   int N = h->A.S.N;
   h->dmc_on = 0;            // a new configuration ends a DMC population
+  h->orb_valid = 0;
   h->Wl = h->Wp;
   h->A.Wp = h->Wp;
   HIPCHK(h, hipMemcpyAsync(h->io, x, (size_t)h->W * 3 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));

@@ -23,6 +23,7 @@ struct KArgs {
   int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
   int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
   int lds_det;       // k_full_det keeps a chunk's basis values in LDS
+  int keep_orb;      // k_full_det takes the occupied-orbital rows from the state (kept by the moves)
   int lds_y;         // doubles per group of the LDS staging of eval_y in the move kernels (0: none)
   int lds_y_off;     // its offset (doubles) in the dynamic LDS, after the basis values
   int lds_nlpp;      // doubles per lane of k_full_nlpp's point scratch in LDS (0: global)
@@ -506,7 +507,12 @@ void k_full_det(KArgs A) {
   extern __shared__ double dyn_lds[];
   double *oc = ws + L.oc;
   const int noccmax = S.noccmax;
-  if (A.lds_det)
+  // The rows of the occupied orbitals (value, gradient, laplacian) of every
+  // electron at its current position are already in the state when each
+  // position since the last full evaluation came from a committed move,
+  // whose orbitals are those of the new position: then they are kept.
+  if (A.keep_orb) {
+  } else if (A.lds_det)
     full_orbitals<T, G>(A, ws, ss, reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * A.chunk * S.NB * 5, lane);
   else
     full_orbitals<T, G>(A, ws, ss, reinterpret_cast<T *>(ss + SL.phi), lane);

@@ -1,7 +1,9 @@
 """The alternative code paths of the engine agree with the oracle as the
 default ones do: global scratch instead of LDS staging (basis values, the
 pseudopotential point scratch, eval_y's inputs), electron chunks in global
-scratch in the full evaluation, and the one-electron move as three launches.
+scratch in the full evaluation, the one-electron move as three launches, and
+full evaluations that recompute the occupied orbitals instead of keeping the
+rows the moves committed.
 Large systems take some of these paths by size; here they are forced on the
 C2-like system through the engine's environment switches, read when a
 handle is created."""
@@ -38,7 +40,8 @@ def env(monkeypatch):
     {"QMCW_NO_LDS_Y": "1"},
     {"QMCW_SPLIT_MOVE": "1"},
     {"QMCW_SPLIT_MOVE": "1", "QMCW_NO_LDS": "1"},
-], ids=["no_lds", "det_chunks", "no_lds_y", "split_move", "split_no_lds"])
+    {"QMCW_NO_KEEP_ORB": "1"},
+], ids=["no_lds", "det_chunks", "no_lds_y", "split_move", "split_no_lds", "recompute_orbitals"])
 def test_alternative_paths_match_oracle(env, switches):
     env(**switches)
     W = 17
