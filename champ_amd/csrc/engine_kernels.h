@@ -23,7 +23,8 @@ struct KArgs {
   int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
   int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
   int lds_det;       // k_full_det keeps a chunk's basis values in LDS
-  int keep_orb;      // k_full_det takes the occupied-orbital rows from the state (kept by the moves)
+  int keep_state;    // the full evaluation takes the rows the committed moves kept in the state
+                     // (orbitals, distances, Jastrow pairs) instead of recomputing them
   int lds_y;         // doubles per group of the LDS staging of eval_y in the move kernels (0: none)
   int lds_y_off;     // its offset (doubles) in the dynamic LDS, after the basis values
   int lds_nlpp;      // doubles per lane of k_full_nlpp's point scratch in LDS (0: global)
@@ -402,66 +403,81 @@ __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
   double *red = ss + SL.phs;
   (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
   (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
-  // ---- distances (qmc/vmc/distances.f90:1-233)
-  double pe_en = 0.0, pe_ee = 0.0;
-  for (int p = lane; p < N * M; p += G) {
-    int i = p / M, ic = p % M;
-    double s2 = 0.0;
-    for (int q = 0; q < 3; q++) {
-      double d = x[3 * i + q] - cst(S.cent)[3 * ic + q];
-      rven[3 * p + q] = d;
-      s2 += d * d;
+  double pe_en = 0.0, pe_ee = 0.0, fsum = 0.0;
+  if (A.keep_state) {
+    // Distances and Jastrow pair arrays as the committed moves left them: a
+    // move recomputes every pair of the moved electron, so they hold the
+    // values at the current positions; only the sums are formed here.
+    for (int p = lane; p < N * M; p += G)
+      if (S.nloc == 0) pe_en -= cst(S.znuc)[cst(S.iwctype)[p % M]] / ren[p];
+    for (int p = lane; p < N * N; p += G) {
+      const int i = p / N, j = p % N;
+      if (j < i) pe_ee += 1.0 / ree[p];
+      if (j <= i) fsum += fso[p];
     }
-    double r = sqrt(s2);
-    ren[p] = r;
-    if (S.nloc == 0) pe_en -= cst(S.znuc)[cst(S.iwctype)[ic]] / r;
-  }
-  for (int p = lane; p < N * N; p += G) {
-    int i = p / N, j = p % N;
-    if (j >= i) continue;
-    double s2 = 0.0;
-    for (int q = 0; q < 3; q++) {
-      double d = x[3 * i + q] - x[3 * j + q];
-      rvee[3 * p + q] = d;
-      s2 += d * d;
-    }
-    double r = sqrt(s2);
-    ree[p] = r;
-    pe_ee += 1.0 / r;
-  }
-  pe_en = gsum<G>(pe_en);
-  pe_ee = gsum<G>(pe_ee);
-  __syncthreads();
-  // ---- Jastrow pair arrays (qmc/vmc/jastrow4.f90)
-  double fsum = 0.0;
-  for (int p = lane; p < N * N; p += G) {
-    int i = p / N, j = p % N;
-    if (j > i) continue;
-    if (j == i) {
-      T f, g[3], lp;
-      jas_en<T>(S, ren + i * M, rven + i * M * 3, f, g, lp);
-      fso[p] = f;
-      for (int q = 0; q < 3; q++) fijo[3 * p + q] = g[q];
-      lapj[p] = lp;
-      fsum += f;
-    } else {
-      PairOut<T> o;
-      T rvij[3];
-      for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * p + q];
-      bool par = (i < S.nup) || (j >= S.nup);
-      jas_pair<T, ORD>(S, par, (T)ree[p], rvij, ren + i * M, ren + j * M, rven + i * M * 3,
-                  rven + j * M * 3, o);
-      fso[p] = o.f;
+    pe_en = gsum<G>(pe_en);
+    pe_ee = gsum<G>(pe_ee);
+    fsum = gsum<G>(fsum);
+  } else {
+    // ---- distances (qmc/vmc/distances.f90:1-233)
+    for (int p = lane; p < N * M; p += G) {
+      int i = p / M, ic = p % M;
+      double s2 = 0.0;
       for (int q = 0; q < 3; q++) {
-        fijo[3 * (i * N + j) + q] = o.gi[q];
-        fijo[3 * (j * N + i) + q] = o.gj[q];
+        double d = x[3 * i + q] - cst(S.cent)[3 * ic + q];
+        rven[3 * p + q] = d;
+        s2 += d * d;
       }
-      lapj[i * N + j] = o.li;
-      lapj[j * N + i] = o.lj;
-      fsum += o.f;
+      double r = sqrt(s2);
+      ren[p] = r;
+      if (S.nloc == 0) pe_en -= cst(S.znuc)[cst(S.iwctype)[ic]] / r;
     }
+    for (int p = lane; p < N * N; p += G) {
+      int i = p / N, j = p % N;
+      if (j >= i) continue;
+      double s2 = 0.0;
+      for (int q = 0; q < 3; q++) {
+        double d = x[3 * i + q] - x[3 * j + q];
+        rvee[3 * p + q] = d;
+        s2 += d * d;
+      }
+      double r = sqrt(s2);
+      ree[p] = r;
+      pe_ee += 1.0 / r;
+    }
+    pe_en = gsum<G>(pe_en);
+    pe_ee = gsum<G>(pe_ee);
+    __syncthreads();
+    // ---- Jastrow pair arrays (qmc/vmc/jastrow4.f90)
+    for (int p = lane; p < N * N; p += G) {
+      int i = p / N, j = p % N;
+      if (j > i) continue;
+      if (j == i) {
+        T f, g[3], lp;
+        jas_en<T>(S, ren + i * M, rven + i * M * 3, f, g, lp);
+        fso[p] = f;
+        for (int q = 0; q < 3; q++) fijo[3 * p + q] = g[q];
+        lapj[p] = lp;
+        fsum += f;
+      } else {
+        PairOut<T> o;
+        T rvij[3];
+        for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * p + q];
+        bool par = (i < S.nup) || (j >= S.nup);
+        jas_pair<T, ORD>(S, par, (T)ree[p], rvij, ren + i * M, ren + j * M, rven + i * M * 3,
+                    rven + j * M * 3, o);
+        fso[p] = o.f;
+        for (int q = 0; q < 3; q++) {
+          fijo[3 * (i * N + j) + q] = o.gi[q];
+          fijo[3 * (j * N + i) + q] = o.gj[q];
+        }
+        lapj[i * N + j] = o.li;
+        lapj[j * N + i] = o.lj;
+        fsum += o.f;
+      }
+    }
+    fsum = gsum<G>(fsum);
   }
-  fsum = gsum<G>(fsum);
   __syncthreads();
   double d2j = 0.0;
   for (int i = lane; i < N; i += G) {
@@ -511,7 +527,7 @@ void k_full_det(KArgs A) {
   // electron at its current position are already in the state when each
   // position since the last full evaluation came from a committed move,
   // whose orbitals are those of the new position: then they are kept.
-  if (A.keep_orb) {
+  if (A.keep_state) {
   } else if (A.lds_det)
     full_orbitals<T, G>(A, ws, ss, reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * A.chunk * S.NB * 5, lane);
   else
