@@ -484,8 +484,11 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         tf = json.load(open(TRAFFIC_FILE))
         ent = tf.get("kernels", {}).get(dom)
-        if ent and ent.get("walkers") == W and ent.get("precision") == main_prec \
-                and ent.get("group_widths") == list(r["G"]):
+        # the measurement holds for the same workload, precision and lane-group
+        # width of the kernel's own class (jastrow, det, nlpp, moves)
+        cls = {"k_full_jas": 0, "k_full_det": 1, "k_full_nlpp": 2, "k_ratio_1e": 3}.get(dom)
+        if ent and cls is not None and ent.get("walkers") == W and ent.get("precision") == main_prec \
+                and ent.get("group_widths", [None] * 4)[cls] == list(r["G"])[cls]:
             traffic = ent["hbm_bytes_per_launch"]
     out = {
         "metric": "walker-moves/s + dlnPsi/dp samples/s vs FP64 GPU path at <=0.1 mHa bias",
