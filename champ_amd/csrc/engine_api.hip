@@ -66,6 +66,7 @@ struct qmcw_handle {
   // positions since); cleared by new positions or orbital coefficients
   int orb_valid = 0;
   int no_keep_orb = 0;       // QMCW_NO_KEEP_ORB: recompute them in every full evaluation
+  int dmc_unfused = 0;       // QMCW_DMC_UNFUSED: DMC moves as five launches per electron
   int use_lds = 1;           // LDS staging of the moved electron's basis values
   int chunk_override = 0;    // electrons per basis chunk (0: from the L2 budget)
   size_t det_lds_budget = 10240;  // LDS bytes per block for k_full_det's basis values
@@ -1071,6 +1072,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_SPLIT_MOVE")) { h->split_move = atoi(e) != 0; h->split_forced = 1; }
   if (const char *e = getenv("QMCW_NO_LDS_Y")) h->lds_y = atoi(e) == 0;
   if (const char *e = getenv("QMCW_NO_KEEP_ORB")) h->no_keep_orb = atoi(e) != 0;
+  if (const char *e = getenv("QMCW_DMC_UNFUSED")) h->dmc_unfused = atoi(e) != 0;
   {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1822,17 +1824,33 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
         K.tm_ratios(ct, ie, cw, (const double *)h->tmb, (const int *)nbwd, h->bcol, h->tmstale);
       hipLaunchKernelGGL(k_tm_decide, g1, b1, 0, h->stream, h->A, ie, cw, (const double *)h->tmbq,
                          (const double *)h->bcol, h->tmxin, h->acc_buf, h->dm, h->tmstale, *p);
-      launch_ratio(h, ie, 0, (const double *)h->tmxin, 1.0, 1.0, (double *)nullptr);
-      DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+      if (!h->split_move && !h->dmc_unfused) {
+        // only the walkers that accepted evaluate and commit (mode 4)
+        h->A.acc_in = h->acc_buf;
+        launch_ratio(h, ie, 4, (const double *)h->tmxin, 1.0, 1.0, (double *)nullptr);
+        h->A.acc_in = nullptr;
+      } else {
+        launch_ratio(h, ie, 0, (const double *)h->tmxin, 1.0, 1.0, (double *)nullptr);
+        DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+      }
     }
   }
-  for (int ie = 0; ie < N; ie++) {
-    DISPATCH_MOVE(h, launch_grad, ie, gbuf);
-    hipLaunchKernelGGL(k_dmc_propose, g1, b1, 0, h->stream, h->A, ie, (const double *)gbuf, xin,
-                       h->dm, *p);
-    launch_ratio(h, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
-    hipLaunchKernelGGL(k_dmc_decide, g1, b1, 0, h->stream, h->A, ie, h->dm, h->acc_buf, *p);
-    DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+  if (!h->split_move && !h->dmc_unfused) {
+    // gradient, proposal, ratio, test and commit of each electron in one
+    // k_ratio_1e launch (mode 3)
+    h->A.dm = h->dm;
+    h->A.dmp = *p;
+    for (int ie = 0; ie < N; ie++) launch_ratio(h, ie, 3, nullptr, 1.0, 1.0, (double *)nullptr);
+    h->A.dm = nullptr;
+  } else {
+    for (int ie = 0; ie < N; ie++) {
+      DISPATCH_MOVE(h, launch_grad, ie, gbuf);
+      hipLaunchKernelGGL(k_dmc_propose, g1, b1, 0, h->stream, h->A, ie, (const double *)gbuf, xin,
+                         h->dm, *p);
+      launch_ratio(h, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
+      hipLaunchKernelGGL(k_dmc_decide, g1, b1, 0, h->stream, h->A, ie, h->dm, h->acc_buf, *p);
+      DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+    }
   }
   // DMC-mode evaluation: walkers that moved draw a grid rotation per pair
   h->A.dmc_rot = h->A.S.nloc > 0 ? 1 : 0;

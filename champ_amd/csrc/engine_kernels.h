@@ -36,6 +36,9 @@ struct KArgs {
   int dmove_stride;     // (null: every walker evaluates and draws)
   double *tq;           // DMC: per walker (tq_stride doubles) the quadrature-point ratios
   int64_t tq_stride;    // [N][tm_kmax][nq] and points [N][tm_kmax][nq][3] kept for T-moves
+  double *dm;           // DMC records (k_ratio_1e mode 3)
+  qmcw_dmc_params dmp;  // and the generation's parameters
+  const uint8_t *acc_in;  // k_ratio_1e mode 4: the walkers whose given move is committed
 };
 
 // offsets in a walker's T-move record
@@ -45,6 +48,10 @@ __host__ __device__ inline int64_t tq_ratio(const DevSys &S, int i, int j, int i
 __host__ __device__ inline int64_t tq_point(const DevSys &S, int i, int j, int iq) {
   return (int64_t)S.N * S.tm_kmax * S.nquad + 3 * tq_ratio(S, i, j, iq);
 }
+
+}  // namespace qmcw
+#include "engine_dmc_dev.h"
+namespace qmcw {
 
 // occupancy hint of the heavy per-walker kernels (waves per SIMD)
 #ifndef QMCW_WPE
@@ -265,14 +272,16 @@ __device__ __forceinline__ void full_orbitals(const KArgs &A, double *ws, double
   }
 }
 
-// gp[i][b] = sum_j G(j, i) coef(b, occ_j) for every electron (FP64; BF16
-// mode rounds the result), coef_occ holding the occupied columns contiguously
+// gp[i][b] = sum_j G(j, i) coef(b, occ_j) for every electron, or only for
+// electron `only` (FP64; BF16 mode rounds the result), coef_occ holding the
+// occupied columns contiguously
 template <int G>
 __device__ __forceinline__ void group_gproj(const DevSys &S, double *__restrict__ gp,
                                             const double *__restrict__ g0, const double *__restrict__ g1,
-                                            int bf16, int lane) {
+                                            int bf16, int lane, int only = -1) {
   const int NB = S.NB;
-  for (int t = lane; t < S.N * NB; t += G) {
+  const int t0 = only < 0 ? 0 : only * NB, t1 = only < 0 ? S.N * NB : (only + 1) * NB;
+  for (int t = t0 + lane; t < t1; t += G) {
     const int i = t / NB, b = t % NB;
     const int s = i < S.nup ? 0 : 1, ie = s ? i - S.nup : i, ns = S.ns[s], nocc = S.nocc[s];
     const double *__restrict__ gc = (s ? g1 : g0) + ie;
@@ -808,9 +817,10 @@ void k_full_nlpp(KArgs A) {
 // current state, the linear form of the pseudopotential evaluation
 // (sum_b gp_i[b] phi_b(x) times the Jastrow factor; qmc/dmc/dmc_elec/
 // tmove.f90:207-255 computes the same ratio by a full one-electron update).
-// A walker's G matrices, their projection gp and the e-e-n variables bt are
-// rebuilt from its state first when its record is marked stale (after a
-// committed T-move, or after branching moved it to another scratch slot).
+// A walker's G matrices and the e-e-n variables bt are rebuilt from its
+// state first when its record is marked stale (after a committed T-move, or
+// after branching moved it to another scratch slot); the projection gp of
+// electron ie is formed on every call.
 // pts/out: column c of walker w at c * cw + w (3 and 8 doubles);
 // cnt[w]: the walker's number of points (0: nothing to do).
 template <typename T, int G, int ORD>
@@ -828,8 +838,6 @@ void k_tm_ratios(KArgs A, int ie, int cw, const double *pts, const int *cnt, dou
   if (stale[w] != 0.0) {
     for (int sp = 0; sp < 2; sp++)
       group_g<G>(S, sp, ws + L.ai[sp], ws + L.tocc[sp], ws + L.y[sp], ss + SL.g[sp], ss + SL.red, lane);
-    __syncthreads();
-    group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
     for (int p = lane; p < N * M; p += G) {
       T bu, t1, t2;
       scaled<T>(S.isc, (T)S.scalek, (T)ren[p], bu, t1, t2);
@@ -838,6 +846,10 @@ void k_tm_ratios(KArgs A, int ie, int cw, const double *pts, const int *cnt, dou
     __syncthreads();
     if (lane == 0) stale[w] = 0.0;
   }
+  // the projected coefficients of electron ie only (the other rows are not
+  // read before the next full evaluation rebuilds them all)
+  group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane, ie);
+  __syncthreads();
   double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
   T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
   const double *gpi = ss + SL.gproj + (int64_t)ie * S.NB;
@@ -1028,6 +1040,9 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
                                                  double delta, double fbias,
                                                  double *out, long long *nacc) {
   QMCW_GROUP_SETUP
+  // mode 4: commit the given move of the walkers that accepted it; the
+  // others have nothing to do (a group leaves as a whole)
+  if (mode == 4 && !A.acc_in[w]) return;
   const int N = S.N, M = S.M, noccmax = S.noccmax;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -1038,15 +1053,19 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   const double *x = ws + L.x;
   const double *fso = ws + L.fso, *fijo = ws + L.fijo, *lapj = ws + L.lapj, *fjo = ws + L.fjo;
   double *xn = mv + ML.xn;
-  // ---- proposal (mode 1: ratio only; mode 2: ratio, test and commit)
-  if (mode >= 1) {
+  // ---- proposal (mode 1: ratio only; mode 2: ratio, test and commit;
+  // mode 3: DMC proposal, test and commit, with A.dm and A.dmp; modes 0
+  // and 4 take the position given)
+  if (mode >= 1 && mode <= 3) {
     double lapo_dummy;
     double vold[3];
     double lapjo = 0.0;  // not needed for the proposal
     group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, ws + L.ai[s],
                      ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lapjo, vold, nullptr, lane);
     (void)lapo_dummy;
-    if (lane == 0) {
+    if (mode == 3) {
+      if (lane == 0) dmc_propose_w(A, w, ie, vold, A.dm + (int64_t)w * kDmStride, A.dmp, xn);
+    } else if (lane == 0) {
       int *rs = A.rng + 4 * w;
       int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
       double deltfi = 2.0 * fbias / delta;
@@ -1232,12 +1251,15 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
       o[4] = gnew[0]; o[5] = gnew[1]; o[6] = gnew[2]; o[7] = lapn_tot;
     }
   }
-  // VMC sweep (mode 2): Metropolis test and commit in the same launch
-  if (mode == 2) {
+  // VMC sweep (mode 2), DMC (mode 3) or a given move (mode 4): test and
+  // commit in the same launch
+  if (mode >= 2) {
     __shared__ int s_acc[kWave / G];
     __syncthreads();
     if (lane == 0) {
-      const int acc = metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias);
+      const int acc = mode == 4   ? 1
+                      : mode == 3 ? dmc_decide_w(A, w, ie, mv, A.dm + (int64_t)w * kDmStride, A.dmp)
+                                  : metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias);
       s_acc[grp] = acc;
       if (acc && nacc) nacc[w] += 1;
     }

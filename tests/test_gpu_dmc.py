@@ -68,6 +68,15 @@ def test_dmc_generations_match_oracle(name, tau, tmoves):
         assert np.array_equal(e.get_branch_rng(), bo)
 
 
+@pytest.mark.parametrize("name,tmoves", [("c2_dimer", 1), ("be_atom", 0)])
+def test_dmc_five_launch_path_matches_oracle(monkeypatch, name, tmoves):
+    """The DMC move as separate launches per electron (gradient, proposal,
+    ratio, decision, commit; the path of the three-launch move) gives the
+    generations of the fused k_ratio_1e mode."""
+    monkeypatch.setenv("QMCW_DMC_UNFUSED", "1")
+    test_dmc_generations_match_oracle(name, 0.1, tmoves)
+
+
 class _OracleStepper:
     def __init__(self, d, brng):
         self.d, self.brng = d, brng
