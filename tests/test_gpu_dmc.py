@@ -115,3 +115,29 @@ def test_dmc_driver_run_matches_oracle():
         assert sg.nwalk == so.nwalk and sg.nacc == so.nacc
         assert abs(sg.wesum - so.wesum) <= 1e-9 * abs(so.wesum)
     assert abs(dg.eest - do.eest) <= 1e-9 * abs(do.eest)
+
+
+def test_gpu_dmc_recovers_the_exact_helium_energy():
+    """The GPU engine's all-electron DMC of He (1 + 1 electrons, nucleus-cusp
+    mover) under the rundmc driver: the exact -2.903724 Ha within 4 sigma
+    (or 5 mHa) at tau = 0.01."""
+    import os, sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle_dmc import he_atom
+    from champ_amd.dmc import DmcDriver
+    s = he_atom()
+    W0, cap = 1000, 2500
+    e = Engine(s, cap)
+    e.set_walkers(system.random_walkers(s, cap, seed=7, sigma=0.6))
+    e.set_rng(np.asarray(rng.walker_seeds("1837465927472523", cap), np.int32))
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    e.set_branch_rng(brng)
+    e.dmc_begin(W0)
+    drv = DmcDriver(e, 2, W0, tau=0.01, etrial=-2.9)
+    drv.run(200)
+    en = np.array([g["eave"] for g in drv.run(400)])
+    e.close()
+    blocks = en.reshape(20, 20).mean(axis=1)
+    est, err = blocks.mean(), blocks.std() / np.sqrt(len(blocks))
+    assert abs(est + 2.903724) < max(4 * err, 0.005), (est, err)

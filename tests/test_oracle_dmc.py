@@ -385,3 +385,42 @@ def test_tmove_saved_ratios_are_the_one_electron_ratios():
                 errs.append(abs(O.ratio_1e(i, xn)["ratio"] - rat[k]) / max(1, abs(rat[k])))
             assert min(errs) <= 1e-10, (i, k, errs)
     assert npts > 0
+
+
+# ---- the all-electron mover in a DMC run
+
+def he_atom(zeta=1.6875):
+    """He: Z = 2, one 1s Slater orbital per spin (exponent 27/16), e-e Jastrow
+    with the antiparallel cusp (b1 = 1/2), no e-n or e-e-n terms.  The
+    ground state is nodeless, so DMC gives the exact -2.903724 Ha."""
+    bb = system._BasisBuilder(system.QMCW_BASIS_SLATER, 1)
+    r = bb.add_radial(1, [zeta], [1.0])
+    bb.add_function(0, r, 0, 0)
+    barr, anorm = bb.arrays(np.array([0], np.int32))
+    cc, off, cdet, cdc = system._csf_arrays([(1.0, [(0, 1.0)])], False)
+    return system.System(nelec=2, nup=1, cent=np.zeros((1, 3)), iwctype=np.array([0], np.int32),
+                         znuc=np.array([2.0]), coef=np.array([[anorm[0]]]),
+                         det_up=np.array([[0]], np.int32), det_dn=np.array([[0]], np.int32),
+                         csf_coef=cc, csf_det_off=off, csf_det=cdet, csf_det_coef=cdc,
+                         isc=4, scalek=1.0, norda=4, nordb=4, nordc=4,
+                         a4=np.zeros((1, 5)), b=np.array([[0.5, 0.4, 0.0, 0.0, 0.0]]),
+                         c=np.zeros((1, system.nterms4(4))), name="he", **barr)
+
+
+def test_dmc_recovers_the_exact_helium_energy():
+    """All-electron DMC (the nucleus-cusp mover) of He from an approximate
+    wavefunction: -2.903724 Ha within 4 sigma (or 5 mHa) at tau = 0.01."""
+    from champ_amd.dmc import DmcDriver
+    s = he_atom()
+    W0, cap = 1000, 2500
+    x = system.random_walkers(s, W0, seed=7, sigma=0.6)
+    d = oracle.OracleDMC(s, cap)
+    assert d.set(x, rng.walker_seeds("1837465927472523", W0)) == 0
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    drv = DmcDriver(OracleStepper(d, brng), 2, W0, tau=0.01, etrial=-2.9)
+    drv.run(200)
+    e = np.array([g["eave"] for g in drv.run(400)])
+    blocks = e.reshape(20, 20).mean(axis=1)
+    est, err = blocks.mean(), blocks.std() / np.sqrt(len(blocks))
+    assert abs(est + 2.903724) < max(4 * err, 0.005), (est, err)
