@@ -148,3 +148,31 @@ def test_autotune_keeps_state_and_parity():
     o = oracle.Oracle(s).vmc_run(x, seeds, 2, 0.6)
     assert np.array_equal(e.get_walkers(), o["x"])
     assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
+
+
+def test_new_parameters_are_used_at_unchanged_positions():
+    """After moves, the full evaluation keeps the orbital rows and Jastrow
+    pairs the moves committed; qmcw_set_params must make it recompute them:
+    psi_at with new MO coefficients and Jastrow b at the walkers' current
+    positions against the oracle with the same parameters (an all-electron
+    system: the moves also rotated the walkers' pseudopotential grids, which
+    the oracle's batch evaluation does not share)."""
+    s = SYS["be_atom"]
+    W = 19
+    x = system.random_walkers(s, W, seed=71)
+    e = Engine(s, W)
+    e.set_walkers(x)
+    e.set_rng(rng.walker_seeds("1837465927472523", W))
+    e.vmc(2, delta=0.6)
+    s2 = s.copy()
+    r = np.random.default_rng(3)
+    s2.coef = s.coef + 0.01 * r.normal(size=s.coef.shape)
+    s2.b = s.b.copy()
+    s2.b[0, 2] += 0.03
+    e.set_params(b=s2.b, coef=s2.coef)
+    g = e.psi_at()
+    xw = e.get_walkers()
+    o = oracle.Oracle(s2).eval_batch(xw)
+    assert _close(g["logpsid"], o["out9"][:, 0]) < RTOL
+    assert _close(g["eloc"], o["out9"][:, 3], atol=1e-9) < 1e-9
+    e.close()
