@@ -538,9 +538,9 @@ static void launch_derivs_asm(qmcw_handle *h, const KArgs &A) {
 static void launch_eval(qmcw_handle *h, bool derivs) {
   KArgs A = h->A;
   A.full_orb = derivs ? 1 : 0;
-  // the derivatives need all orbitals of every electron, the moves keep
-  // only the occupied ones
-  A.keep_state = (h->orb_valid && !derivs && !h->no_keep_orb) ? 1 : 0;
+  // (the derivatives need all orbitals of every electron, the moves keep
+  // only the occupied ones: k_full_det recomputes them under full_orb)
+  A.keep_state = (h->orb_valid && !h->no_keep_orb) ? 1 : 0;
   h->orb_valid = 1;
   // electrons per basis chunk of k_full_det: a wavefront's basis values
   // (64/G walkers) stay near 16 KB so that they are read back from L2

@@ -24,7 +24,8 @@ struct KArgs {
   int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
   int lds_det;       // k_full_det keeps a chunk's basis values in LDS
   int keep_state;    // the full evaluation takes the rows the committed moves kept in the state
-                     // (orbitals, distances, Jastrow pairs) instead of recomputing them
+                     // (distances, Jastrow pairs and, without full_orb, the orbitals)
+                     // instead of recomputing them
   int lds_y;         // doubles per group of the LDS staging of eval_y in the move kernels (0: none)
   int lds_y_off;     // its offset (doubles) in the dynamic LDS, after the basis values
   int lds_nlpp;      // doubles per lane of k_full_nlpp's point scratch in LDS (0: global)
@@ -536,7 +537,7 @@ void k_full_det(KArgs A) {
   // electron at its current position are already in the state when each
   // position since the last full evaluation came from a committed move,
   // whose orbitals are those of the new position: then they are kept.
-  if (A.keep_state) {
+  if (A.keep_state && !A.full_orb) {
   } else if (A.lds_det)
     full_orbitals<T, G>(A, ws, ss, reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * A.chunk * S.NB * 5, lane);
   else

@@ -79,7 +79,7 @@ def test_mixed_precision_derivs_close_to_oracle():
     assert ede < 2e-3
 
 
-def test_sampling_mode_gives_last_step_derivatives():
+def test_sampling_mode_gives_last_step_derivatives(monkeypatch):
     s = system.c2_dimer(2, derivs=True)
     W = 11
     x = system.random_walkers(s, W, seed=22)
@@ -98,6 +98,18 @@ def test_sampling_mode_gives_last_step_derivatives():
     # evaluation at the same state gives the same numbers
     dp2, de2 = e.derivs()
     assert np.array_equal(dp, dp2) and np.array_equal(de, de2)
+    # those evaluations took the distances and Jastrow pairs the moves kept:
+    # the same walk with every evaluation recomputing them gives the same
+    # derivatives (the grid rotations come from the same streams)
+    monkeypatch.setenv("QMCW_NO_KEEP_ORB", "1")
+    e3 = Engine(s, W)
+    e3.set_walkers(x)
+    e3.set_rng(seeds)
+    e3.sample_derivs(True)
+    e3.vmc(2, delta=0.6)
+    dp3, de3 = e3.get_derivs()
+    assert np.array_equal(e3.get_walkers(), xs)
+    assert _rel(dp3, dp, 1e-9) < 1e-9 and _rel(de3, de, 1e-8) < 1e-8
     # the walk itself is unchanged by the derivative evaluation
     o = oracle.Oracle(s).vmc_run(x, seeds, 2, 0.6)
     assert np.array_equal(xs, o["x"])
