@@ -504,7 +504,7 @@ def main():
         "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "C2-like dimer VMC: 8 valence e, 58 GTO, ECP l=0,1 nonlocal (Q=12), "
-                               "24 CSF / 48 det, 16384 walkers x 200 steps",
+                               f"24 CSF / 48 det, {W} walkers, {args.steps} timed VMC steps",
                    "walkers": W, "delta": args.delta, "precision": main_prec,
                    "group_widths": r["G"]},
         "fp64_walker_moves_per_s": res.get("fp64", {}).get("rate"),
