@@ -928,6 +928,9 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
       }
       km = std::max(km, n);
     }
+    // QMCW_TM_KMAX lowers the bound (never raises it), to exercise the
+    // record-overflow status
+    if (const char *e = getenv("QMCW_TM_KMAX")) km = std::max(1, std::min(km, atoi(e)));
     S.tm_kmax = km;
   }
   // ---- parameter counts (derivatives)
@@ -1706,10 +1709,15 @@ extern "C" int qmcw_dmc_begin(qmcw_handle *h, int32_t nwalk, const double *weigh
 // stream.  Produces the new population as copies of old records: origin[d],
 // the new weights and the streams (a split copy continues on four 12-bit
 // limbs drawn from the branching stream, the last made odd).
-static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<int32_t> &rng,
-                          std::vector<int> &origin, int &n_out, int &nbranch) {
-  const int n = h->nact, MW = h->W;
-  origin.resize(MW);
+// The loops follow the reference's order (joins in walker order with one
+// rannyu per joined pair, the split threshold from the sorted weights,
+// splits into the holes last-first, then compaction); a split copy gets a
+// fresh stream of four 12-bit limbs drawn from the branching stream.
+extern "C" int qmcw_split_join(int32_t n, int32_t mwalk, double *w, int32_t *origin, int32_t *rng,
+                               int32_t *brng, int32_t *nwalk_out, int32_t *nbranch) {
+  if (n < 0 || mwalk < n || !w || !origin || !rng || !brng || !nwalk_out || !nbranch)
+    return fail(nullptr, QMCW_ERR_ARG, "qmcw_split_join: bad arguments");
+  const int MW = mwalk;
   for (int i = 0; i < MW; i++) origin[i] = i;
   std::vector<int> under;
   int ipair = 0, iw2 = 0, nbr = 0;
@@ -1725,7 +1733,7 @@ static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<in
       nbr++;
       ipair = 0;
       const double wt = w[iw] + w[iw2];
-      if (rannyu(h->brng) > w[iw] / wt) {
+      if (rannyu(brng) > w[iw] / wt) {
         w[iw2] = wt;
         under.push_back(iw);
       } else {
@@ -1735,9 +1743,9 @@ static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<in
     }
   }
   const int nafter = n - (int)under.size(), nmax_dupl = MW - nafter;
-  std::vector<double> srt(w.begin(), w.begin() + n);
+  std::vector<double> srt(w, w + n);
   std::sort(srt.begin(), srt.end());
-  auto at1 = [&](int i) { return srt[std::max(1, i) - 1]; };   // 1-based, clamped below
+  auto at1 = [&](int i) { return n > 0 ? srt[std::max(1, i) - 1] : 0.0; };   // 1-based, clamped below
   double wt_split = std::max(2.0, at1(n - nmax_dupl));
   int nmore = 0;
   for (int iw = std::max(1, n - nmax_dupl); iw <= n; iw++) {
@@ -1757,13 +1765,13 @@ static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<in
       dst = under.back();
       under.pop_back();
     } else {
-      if (n2 >= MW) return fail(h, QMCW_ERR_NOMEM, "DMC population exceeds the handle's walkers");
+      if (n2 >= MW) return fail(nullptr, QMCW_ERR_NOMEM, "DMC population exceeds the walker capacity");
       dst = n2++;
     }
     w[iw] *= 0.5;
     w[dst] = w[iw];
     origin[dst] = origin[iw];
-    for (int k = 0; k < 4; k++) rng[4 * dst + k] = (int32_t)(rannyu(h->brng) * 4096.0) & 4095;
+    for (int k = 0; k < 4; k++) rng[4 * dst + k] = (int32_t)(rannyu(brng) * 4096.0) & 4095;
     rng[4 * dst + 3] |= 1;
   }
   while (!under.empty()) {
@@ -1776,8 +1784,19 @@ static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<in
       for (int k = 0; k < 4; k++) rng[4 * dst + k] = rng[4 * src + k];
     }
   }
-  n_out = n2;
-  nbranch = nbr;
+  *nwalk_out = n2;
+  *nbranch = nbr;
+  return QMCW_OK;
+}
+
+static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<int32_t> &rng,
+                          std::vector<int> &origin, int &n_out, int &nbranch) {
+  origin.resize(h->W);
+  int32_t no = 0, nb = 0;
+  const int st = qmcw_split_join(h->nact, h->W, w.data(), origin.data(), rng.data(), h->brng, &no, &nb);
+  if (st) return fail(h, st, qmcw_last_error(nullptr));
+  n_out = no;
+  nbranch = nb;
   return QMCW_OK;
 }
 

@@ -414,6 +414,7 @@ __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
   (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
   (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
   double pe_en = 0.0, pe_ee = 0.0, fsum = 0.0;
+  if (lane == 0) A.flags[w] = 0;   // status of this evaluation (set by k_full_nlpp, k_full_fin)
   if (A.keep_state) {
     // Distances and Jastrow pair arrays as the committed moves left them: a
     // move recomputes every pair of the moved electron, so they hold the
@@ -799,8 +800,12 @@ void k_full_nlpp(KArgs A) {
     __syncthreads();
     double *tqw = (mover && A.tq) ? A.tq + (int64_t)w * A.tq_stride : nullptr;
     // per-lane distances of the point and U of the other electrons: LDS
-    // when the launch reserved it
-    if (A.lds_nlpp) {
+    // when the launch reserved it.  In DMC a walker that moved no electron
+    // drew no rotations (its local energy is the one kept from the start of
+    // the generation), so its points are not evaluated.
+    if (A.dmc_rot && !mover) {
+      vpsp = 0.0;
+    } else if (A.lds_nlpp) {
       extern __shared__ double dyn_lds[];
       double *lb = dyn_lds + (int64_t)threadIdx.x * A.lds_nlpp;
       vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, lb, reinterpret_cast<T *>(lb + 4 * M),
@@ -919,7 +924,7 @@ __global__ __launch_bounds__(64) void k_full_fin(KArgs A) {
     int fl = 0;
     if (psid == 0.0) fl |= 1;
     if (!isfinite(eloc)) fl |= 2;
-    A.flags[w] = fl;
+    A.flags[w] |= fl;           // k_full_jas cleared the word; k_full_nlpp may have set bit 4
   }
 }
 

@@ -299,6 +299,21 @@ int qmcw_dmc_get(qmcw_handle *h, int32_t *nwalk, double *weight);
 int qmcw_set_branch_rng(qmcw_handle *h, const int32_t *seed);
 int qmcw_get_branch_rng(const qmcw_handle *h, int32_t *seed);
 
+/* split_join (qmc/dmc/dmc_elec/split_join_mod.f90:3-219) over host arrays,
+   the branching qmcw_dmc_step applies between generations; host code only
+   (no handle, no GPU).  weight[mwalk]: in, the n walkers' weights; out,
+   the *nwalk_out weights after killing, joining and splitting (mwalk plays
+   MWALK's part in the split threshold).  origin[mwalk] (out): the index
+   among the n old walkers each new walker copies.  rng[mwalk][4]: the
+   walkers' streams, moved with them; a split copy gets four 12-bit limbs
+   drawn from the branching stream brng[4] (the last made odd), which the
+   reference, whose walkers share one stream, does not draw.  brng advances
+   by one rannyu per joined pair, as in the reference, plus four per split.
+   *nbranch: kills + joins + splits (the reference's nbrnch increment).
+   QMCW_ERR_NOMEM when the splits need more than mwalk walkers. */
+int qmcw_split_join(int32_t n, int32_t mwalk, double *weight, int32_t *origin, int32_t *rng,
+                    int32_t *brng, int32_t *nwalk_out, int32_t *nbranch);
+
 /* Read back per-walker local energies of the last evaluation. */
 int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
 

@@ -143,3 +143,65 @@ def test_gpu_dmc_recovers_the_exact_helium_energy():
     blocks = en.reshape(20, 20).mean(axis=1)
     est, err = blocks.mean(), blocks.std() / np.sqrt(len(blocks))
     assert abs(est + 2.903724) < max(4 * err, 0.005), (est, err)
+
+
+def test_dmc_walkers_that_move_nothing_match_oracle():
+    """At tau = 3 many proposals fail, so some walkers move no electron in a
+    generation (counted on the oracle) and, after branching, sit in scratch
+    slots other walkers used before.  Their evaluation draws no grid
+    rotations and keeps the local energy of the generation's start; counts,
+    streams, weights and positions still match the oracle."""
+    s = system.c2_dimer(2)
+    n0, cap = 40, 96
+    x, seeds = _setup(s, n0, cap, seed=31)
+    e0 = float(oracle.Oracle(s).eval_batch(x[:n0])["out9"][:, 3].mean())
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    e = Engine(s, cap)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    e.set_branch_rng(brng)
+    e.dmc_begin(n0)
+    d = oracle.OracleDMC(s, cap)
+    assert d.set(x[:n0], seeds[:n0]) == 0
+    bo = brng.copy()
+    p = DmcParams(3.0, e0, eest=e0, esigma=1.0, tmoves=0)
+    unmoved = 0
+    for gen in range(4):
+        x0 = d.get()["x"]
+        sg = e.dmc_step(p)
+        r, so = d.generation(p)
+        assert r == 0
+        unmoved += int(np.all(x0 == d.get()["x"], axis=(1, 2)).sum())
+        r, sb = d.branch(bo)
+        assert r == 0
+        assert sg.nacc == so.nacc and sg.ntry == so.ntry and sg.nwalk == sb.nwalk
+        assert abs(sg.wesum - so.wesum) <= 1e-9 * abs(so.wesum)
+        xg, wg, rg = e.dmc_population()
+        go = d.get()
+        assert np.array_equal(rg, go["rng"])
+        assert np.allclose(wg, go["weight"], rtol=1e-9, atol=0)
+        assert np.allclose(xg, go["x"], rtol=0, atol=1e-11)
+    assert unmoved > 0
+
+
+def test_tmove_record_overflow_is_reported(monkeypatch):
+    """An electron in range of more pseudopotential centres than the T-move
+    record holds is reported as an error, not dropped: QMCW_TM_KMAX=1 makes
+    the record hold one centre while the dimer's electrons between the
+    nuclei are in range of two."""
+    from champ_amd.engine import QmcwError
+    monkeypatch.setenv("QMCW_TM_KMAX", "1")
+    s = system.c2_dimer(2)
+    n0, cap = 20, 48
+    x, seeds = _setup(s, n0, cap, seed=31)
+    x[:, :, :] *= 0.2                      # every electron near the bond centre
+    e = Engine(s, cap)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    e.set_branch_rng(brng)
+    with pytest.raises(QmcwError, match="T-move record"):
+        e.dmc_begin(n0)                    # the first DMC-mode evaluation keeps the points
+        e.dmc_step(DmcParams(0.1, -11.0, eest=-11.0, esigma=1.0, tmoves=1))
