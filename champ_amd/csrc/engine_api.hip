@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1615,6 +1616,66 @@ extern "C" int qmcw_accum_get(qmcw_handle *h, double *out) {
   HIPCHK(h, hipMemcpyAsync(out, h->acc, accum_doubles(h->nparm) * sizeof(double), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return QMCW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Product sums with no allocation: the chunk partial sums live in a
+// file-scope device array (one copy per GPU, held until the process ends).
+// Its host-side name is not a device address: kernels get the address
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device.  The partial sums are overwritten in full by every call, so
+// nothing needs clearing; calls that overlap on one GPU share the array and
+// race.
+
+static __device__ __align__(256) double scratch[QMCW_ACCUM_STATIC_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[64];
+
+static int scratch_for_device(qmcw_handle *h, double **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return fail(h, QMCW_ERR_HIP, "hipGetDevice failed");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(h, QMCW_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_accum_add_static(qmcw_handle *h, double *acc) {
+  if (!h || !acc) return QMCW_ERR_ARG;
+  const int P = h->nparm;
+  if (P <= 0) return fail(h, QMCW_ERR_ARG, "the system has no parameters to differentiate");
+  const int nch = (h->W + kAccChunk - 1) / kAccChunk;
+  const int64_t need = (int64_t)nch * 3 * P * P + (int64_t)nch * (3 + 3 * (int64_t)P);
+  if (need > (int64_t)QMCW_ACCUM_STATIC_DOUBLES)
+    return fail(h, QMCW_ERR_ARG, "qmcw_accum_add_static: " + std::to_string(h->W) + " walkers x " +
+                std::to_string(P) + " parameters need " + std::to_string(need) +
+                " doubles of partial sums, more than the static array's " +
+                std::to_string((long long)QMCW_ACCUM_STATIC_DOUBLES));
+  if (!h->der_valid) return fail(h, QMCW_ERR_ARG, "no derivative evaluation has run on this handle");
+  double *part = nullptr;
+  int st = scratch_for_device(h, &part);
+  if (st) return st;
+  double *partv = part + (int64_t)nch * 3 * P * P;
+  const int nb = (P + kAccBlk - 1) / kAccBlk;
+  const int64_t n = accum_doubles(P);
+  hipLaunchKernelGGL(k_accum_gemm, dim3(nb * nb, nch), dim3(64), 0, h->stream, (const double *)h->der,
+                     2 * P, P, (const double *)h->A.st, (int64_t)h->A.L.stride, (int64_t)h->A.L.eloc,
+                     h->W, part);
+  hipLaunchKernelGGL(k_accum_vec, dim3((P + 1 + 63) / 64, nch), dim3(64), 0, h->stream,
+                     (const double *)h->der, 2 * P, P, (const double *)h->A.st,
+                     (int64_t)h->A.L.stride, (int64_t)h->A.L.eloc, h->W, partv);
+  hipLaunchKernelGGL(k_accum_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream,
+                     (const double *)part, (const double *)partv, nch, P, acc);
+  HIPCHK(h, hipGetLastError());
   return QMCW_OK;
 }
 

@@ -152,6 +152,12 @@ class Engine:
         """Add the last derivative sample of every walker."""
         self._chk(self.lib.qmcw_accum_add(self.h))
 
+    def accum_add_static(self, acc_device_ptr: int):
+        """qmcw_accum_add_static: add the last derivative sample of every
+        walker to a caller-owned device accumulator (QMCW_ACCUM_DOUBLES(P)
+        doubles at `acc_device_ptr`), with no device allocation."""
+        self._chk(self.lib.qmcw_accum_add_static(self.h, C.c_void_p(int(acc_device_ptr))))
+
     def accum_get(self):
         """dict: n, se, se2 (sums of 1, E_L, E_L^2); sd, sde, sl [P] (sums of
         dpsi, E_L dpsi, deloc); dd, dde, dl [P][P] (sums of dpsi_i dpsi_j,

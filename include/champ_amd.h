@@ -208,6 +208,22 @@ int qmcw_accum_reset(qmcw_handle *h);
 int qmcw_accum_add(qmcw_handle *h);
 int qmcw_accum_get(qmcw_handle *h, double *out);
 
+/* qmcw_accum_add with no device allocation.  It has no counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight sampling loops, harnesses that forbid
+   hipMalloc).  It adds the last derivative sample of every walker to the
+   caller's DEVICE accumulator acc (QMCW_ACCUM_DOUBLES(P) doubles, layout as
+   above; the caller clears it), enqueued on the handle's stream with no
+   host synchronisation.  Partial sums use a static device array of
+   QMCW_ACCUM_STATIC_DOUBLES doubles (32 MiB per GPU, held until the
+   process ends): ceil(W/256) * (3 P^2 + 3 P + 3) must fit it (P = 242: up to
+   5,888 walkers per handle), else QMCW_ERR_ARG.  Every handle on a GPU
+   shares the array, so calls that overlap on one GPU (different streams or
+   threads) race; calls in stream order do not.  The result equals
+   qmcw_accum_add's bit for bit (same kernels, same chunk order). */
+#define QMCW_ACCUM_STATIC_DOUBLES ((int64_t)4 << 20)
+int qmcw_accum_add_static(qmcw_handle *h, double *acc);
+
 /* Per-walker random streams (CHAMP's rannyu, qmc/vmc/rannyu.f90:53-74):
    seed[W][4] 12-bit limbs as in setrn (qmc/vmc/rannyu.f90:11-21). */
 int qmcw_set_rng(qmcw_handle *h, const int32_t *seed);
