@@ -84,8 +84,12 @@ __device__ __forceinline__ float bf16r(float x) {
 
 // orbital value, gradient and laplacian o[0..4] = sum_b coef[b * NO + lab] phi_b[0..4]
 // (coef: coefT with NO = all orbitals, or an occupied-orbital table with NO =
-// nocc; phi with stride 5); BF16: inputs rounded to bfloat16
-template <bool BF16, typename T>
+// nocc; phi with stride 5), accumulated in FP64.  These rows form the Slater
+// matrices, whose determinant can be orders of magnitude below the product
+// of its rows near a node: bfloat16 inputs there gave local-energy errors of
+// tens of Ha, so the BF16 mode does not round them (it rounds only the
+// pseudopotential projection, group_gproj).
+template <typename T>
 __device__ __forceinline__ void contract5(cptr<double> coefT, int NO, int NB, int lab,
                                           const T *__restrict__ ph, double *o) {
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
@@ -93,23 +97,10 @@ __device__ __forceinline__ void contract5(cptr<double> coefT, int NO, int NB, in
   for (int b = 0; b < NB; b++) {
     const double cf = coefT[(int64_t)b * NO + lab];
     const T *p = ph + 5 * b;
-    if (BF16) {
-      const float c = bf16r((float)cf);
-      a0 += (double)(c * bf16r((float)p[0])); a1 += (double)(c * bf16r((float)p[1]));
-      a2 += (double)(c * bf16r((float)p[2])); a3 += (double)(c * bf16r((float)p[3]));
-      a4 += (double)(c * bf16r((float)p[4]));
-    } else {
-      a0 += cf * (double)p[0]; a1 += cf * (double)p[1]; a2 += cf * (double)p[2];
-      a3 += cf * (double)p[3]; a4 += cf * (double)p[4];
-    }
+    a0 += cf * (double)p[0]; a1 += cf * (double)p[1]; a2 += cf * (double)p[2];
+    a3 += cf * (double)p[3]; a4 += cf * (double)p[4];
   }
   o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
-}
-template <typename T>
-__device__ __forceinline__ void contract5(int bf16, cptr<double> coefT, int NO, int NB, int lab,
-                                          const T *ph, double *o) {
-  if (bf16) contract5<true, T>(coefT, NO, NB, lab, ph, o);
-  else contract5<false, T>(coefT, NO, NB, lab, ph, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,13 +251,13 @@ __device__ __forceinline__ void full_orbitals(const KArgs &A, double *ws, double
       const int i = i0 + ii, s = (i < S.nup) ? 0 : 1;
       double *o = oc + ((int64_t)i * noccmax + j) * 5;
       if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
-      contract5<T>(A.bf16, cst(S.coef_occ[s]), S.nocc[s], S.NB, j, phi + (int64_t)ii * S.NB * 5, o);
+      contract5<T>(cst(S.coef_occ[s]), S.nocc[s], S.NB, j, phi + (int64_t)ii * S.NB * 5, o);
     }
     if (A.full_orb) {
       for (int t = lane; t < nch * NO; t += G) {
         const int ii = t / NO, lab = t % NO;
         const int i = i0 + ii;
-        contract5<T>(A.bf16, cst(S.coefT), NO, S.NB, lab, phi + (int64_t)ii * S.NB * 5,
+        contract5<T>(cst(S.coefT), NO, S.NB, lab, phi + (int64_t)ii * S.NB * 5,
                      orbf + ((int64_t)i * NO + lab) * 5);
       }
     }
@@ -1032,7 +1023,7 @@ __device__ __forceinline__ void orbitals_1e(const KArgs &A, int s, int nocc, con
   }
   __syncthreads();
   for (int j = lane; j < nocc; j += G)
-    contract5<T>(A.bf16, cst(S.coef_occ[s]), nocc, S.NB, j, phi, orbe + 5 * j);
+    contract5<T>(cst(S.coef_occ[s]), nocc, S.NB, j, phi, orbe + 5 * j);
 }
 
 // ---------------------------------------------------------------------------

@@ -935,7 +935,8 @@ def hchain(seed: int = 4, ncarbon: int = 16) -> System:
                   name="hchain", **barr)
 
 
-def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = None) -> System:
+def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = None,
+                    zeff: float | None = None, big_basis: bool = False) -> System:
     """Synthetic molecule of the multi-system batch: electron count uniform
     in [4, 128] (split evenly between spins, the extra one up), 1-24 nuclei
     with Z_eff in {1, 4, 6} placed by random packing at >= 2 bohr spacing,
@@ -943,11 +944,15 @@ def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = Non
     pseudopotentials on the Z_eff > 1 centres, random orthonormal MOs, one
     determinant, e-n / e-e / e-e-n Jastrow.  Charged systems are allowed:
     the electron count is drawn independently of the nuclei, then capped so
-    that the basis holds the occupied orbitals of each spin plus two."""
+    that the basis holds the occupied orbitals of each spin plus two.
+    zeff fixes every nucleus' charge; big_basis gives the Z_eff=6 centres
+    4s3p3d1f (35 functions), so that 23 of them reach ~800 functions."""
     rng = np.random.default_rng(seed)
     N = int(nelec if nelec is not None else rng.integers(4, 129))
     M = int(ncent if ncent is not None else rng.integers(1, 25))
     zs = rng.choice([1.0, 4.0, 6.0], size=M)
+    if zeff is not None:
+        zs[:] = float(zeff)
     if not (zs > 1).any():
         zs[0] = 4.0
     cent = []
@@ -963,7 +968,8 @@ def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = Non
     bb = _BasisBuilder(QMCW_BASIS_GAUSSIAN, 3)
     _gto_shells(bb, rng, 0, [(0, 2), (1, 1)])
     _gto_shells(bb, rng, 1, [(0, 3), (1, 2), (2, 1)])
-    _gto_shells(bb, rng, 2, [(0, 3), (1, 3), (2, 2), (3, 1)])
+    _gto_shells(bb, rng, 2, [(0, 4), (1, 3), (2, 3), (3, 1)] if big_basis else
+                [(0, 3), (1, 3), (2, 2), (3, 1)])
     barr, anorm = bb.arrays(iwctype)
     nb = len(anorm)
     N = max(2, min(N, 2 * (nb - 2)))

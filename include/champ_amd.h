@@ -30,7 +30,7 @@ extern "C" {
 
 #define QMCW_FP64 0          /* orbitals, Jastrow and determinants in FP64 */
 #define QMCW_MIXED 1         /* orbitals + Jastrow FP32, determinants/energies FP64 */
-#define QMCW_MIXED_BF16 2    /* as MIXED with BF16 orbital-contraction inputs */
+#define QMCW_MIXED_BF16 2    /* as MIXED with the pseudopotential projection rounded to BF16 */
 
 #define QMCW_BASIS_SLATER 1
 #define QMCW_BASIS_GAUSSIAN 2

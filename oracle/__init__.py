@@ -48,6 +48,7 @@ def lib():
             "orc_asymp": (None, [_vp, _pd, _pd]),
             "orc_cusp": (C.c_int32, [_vp, _pd, _pi]),
             "orc_cuspexact": (None, [_vp, _pd]),
+            "orc_set_emulation": (None, [_vp, C.c_int32]),
             "orc_set_params": (None, [_vp, _pd, _pd, _pd, _pd, _pd]),
             "orc_det_info": (None, [_vp, _pi]),
             "orc_setrn": (None, [_pi, _pi]),
@@ -140,6 +141,11 @@ class Oracle:
         info = np.zeros(5, np.int32)
         self.L.orc_det_info(self.h, _i(info))
         return dict(ndetup=info[0], ndetdn=info[1], noccup=info[2], noccdn=info[3], kmax=info[4])
+
+    def set_emulation(self, mode: int):
+        """Error studies only: bit 0 rounds the contraction inputs of the
+        full orbital rows to bfloat16, bit 1 those of the quadrature rows."""
+        self.L.orc_set_emulation(self.h, int(mode))
 
     def set_params(self, a4=None, b=None, c=None, csf_coef=None, coef=None):
         conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)

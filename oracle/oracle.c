@@ -74,6 +74,10 @@ struct orc_sys {
   int nparmc_t[64];      /* varied c per type */
   int pc_first[64];      /* index of the type's first varied c in the list */
   double hb;
+  /* precision emulation for error studies (orc_set_emulation): bit 0 rounds
+     the inputs of the full orbital rows (value, gradient, laplacian) to
+     bfloat16, bit 1 those of the value-only rows (quadrature points) */
+  int emul;
 };
 
 static void *dupa(const void *p, size_t n) {
@@ -887,6 +891,19 @@ void orc_basis(const orc_sys *o, const double *xe, double *phi, double *dphi,
   basis_at(o, rvec, r, phi, dphi, d2phi);
 }
 
+/* bfloat16 rounding (nearest, ties to even) of a double through FP32 */
+static double bf16_round(double x) {
+  float f = (float)x;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+void orc_set_emulation(orc_sys *o, int32_t mode) { o->emul = mode; }
+
 /* qmc/vmc/orbitals_loc_ana.f90:50-64 / _grade.f90:40-51 / _anae :76-102 */
 static void orbitals_at(const orc_sys *o, const double *rvec, const double *r,
                         double *orb, double *dorb, double *ddorb,
@@ -894,16 +911,27 @@ static void orbitals_at(const orc_sys *o, const double *rvec, const double *r,
   int nb = o->nbasis;
   double *phi = work, *dphi = work + nb, *d2phi = work + 4 * nb;
   basis_at(o, rvec, r, phi, dorb ? dphi : NULL, d2phi);
+  const int rnd = dorb ? (o->emul & 1) : (o->emul & 2);
+  if (rnd) {
+    for (int m = 0; m < nb; m++) {
+      phi[m] = bf16_round(phi[m]);
+      if (dorb) {
+        for (int k = 0; k < 3; k++) dphi[3 * m + k] = bf16_round(dphi[3 * m + k]);
+        d2phi[m] = bf16_round(d2phi[m]);
+      }
+    }
+  }
   for (int io = 0; io < o->norb; io++) {
     const double *cf = o->coef + (size_t)io * nb;
     double v = 0, g0 = 0, g1 = 0, g2 = 0, l = 0;
     for (int m = 0; m < nb; m++) {
-      v += cf[m] * phi[m];
+      const double c = rnd ? bf16_round(cf[m]) : cf[m];
+      v += c * phi[m];
       if (dorb) {
-        g0 += cf[m] * dphi[3 * m];
-        g1 += cf[m] * dphi[3 * m + 1];
-        g2 += cf[m] * dphi[3 * m + 2];
-        l += cf[m] * d2phi[m];
+        g0 += c * dphi[3 * m];
+        g1 += c * dphi[3 * m + 1];
+        g2 += c * dphi[3 * m + 2];
+        l += c * d2phi[m];
       }
     }
     orb[io] = v;

@@ -33,6 +33,9 @@ void orc_asymp(const orc_sys *o, double *asymp_jasa, double *asymp_jasb);
 int32_t orc_cusp(const orc_sys *o, double *d, int32_t *iwc4);
 /* apply cuspexact4 to c[nctype][nterms] in place */
 void orc_cuspexact(const orc_sys *o, double *c);
+/* error studies: bit 0 rounds the full orbital rows' contraction inputs to
+   bfloat16, bit 1 the value-only (quadrature point) rows' */
+void orc_set_emulation(orc_sys *o, int32_t mode);
 void orc_set_params(orc_sys *o, const double *a4, const double *b,
                     const double *c, const double *csf_coef,
                     const double *coef);

@@ -205,3 +205,40 @@ def test_tmove_record_overflow_is_reported(monkeypatch):
     with pytest.raises(QmcwError, match="T-move record"):
         e.dmc_begin(n0)                    # the first DMC-mode evaluation keeps the points
         e.dmc_step(DmcParams(0.1, -11.0, eest=-11.0, esigma=1.0, tmoves=1))
+
+
+def test_chain98_dmc_generations_match_oracle():
+    """The 98-electron chain (pseudopotential mover, 424 GTOs): a small
+    population (6 walkers, capacity 12), 2 generations at tau = 0.01 on the
+    GPU and in the oracle: counts, streams and population identical, weights
+    to 1e-9 relative, positions to 1e-10 bohr."""
+    s = system.hchain(4)
+    n0, cap = 6, 12
+    x, seeds = _setup(s, n0, cap, seed=43)
+    e0 = float(oracle.Oracle(s).eval_batch(x[:n0], nthreads=8)["out9"][:, 3].mean())
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    e = Engine(s, cap)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    e.set_branch_rng(brng)
+    e.dmc_begin(n0)
+    d = oracle.OracleDMC(s, cap)
+    assert d.set(x[:n0], seeds[:n0]) == 0
+    bo = brng.copy()
+    p = DmcParams(0.01, e0, eest=e0, esigma=1.0, tmoves=0)
+    for gen in range(2):
+        sg = e.dmc_step(p)
+        r, so = d.generation(p, nthreads=8)
+        assert r == 0
+        r, sb = d.branch(bo)
+        assert r == 0
+        assert sg.nacc == so.nacc and sg.ncross == so.ncross and sg.ntry == so.ntry
+        assert abs(sg.wsum - so.wsum) <= 1e-9 * abs(so.wsum)
+        assert sg.nwalk == sb.nwalk and sg.nbranch == sb.nbranch
+        xg, wg, rg = e.dmc_population()
+        go = d.get()
+        assert np.array_equal(rg, go["rng"])
+        assert np.allclose(wg, go["weight"], rtol=1e-9, atol=0)
+        assert np.allclose(xg, go["x"], rtol=0, atol=1e-10)
+        assert np.array_equal(e.get_branch_rng(), bo)
