@@ -149,6 +149,7 @@ struct MLayout {
   int64_t fjn;                            // [N][3]
   int64_t orbe;                           // [noccmax][5]
   int64_t ain, tn, invn, detn, yn, misc;  // candidate det state
+  int64_t slat;                           // accelerated mover: fxop, areao, r of the proposal [4]
   int64_t stride;
 };
 

@@ -977,6 +977,9 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   h->A.full_orb = 0;
   h->A.derivs = 0;
   h->A.bf16 = precision == QMCW_MIXED_BF16 ? 1 : 0;
+  h->A.imetro = 1;
+  h->A.deltar = 5.0;
+  h->A.deltat = 1.0;
   int nsm = std::max(nup, ndn), noccm = S.noccmax;
   int ndum = std::max(D.ndetu[0], D.ndetu[1]);
   WLayout &L = h->A.L;
@@ -1000,7 +1003,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   ML.fsn = take(N); ML.fijn_i = take(3 * N); ML.fijn_j = take(3 * N); ML.lapn_i = take(2 * N);
   ML.fjn = take(3 * N); ML.orbe = take(noccm * 5); ML.ain = take(nsm * nsm);
   ML.tn = take(nsm * noccm); ML.invn = take(ndum * S.kmax2); ML.detn = take(ndum);
-  ML.yn = take(D.nocc[0] * D.ns[0] + D.nocc[1] * D.ns[1]); ML.misc = take(24 + nsm + noccm);
+  ML.yn = take(D.nocc[0] * D.ns[0] + D.nocc[1] * D.ns[1]); ML.misc = take(24 + nsm + noccm); ML.slat = take(4);
   ML.stride = o;
   SLayout &SL = h->A.SL;
   o = 0;
@@ -1426,6 +1429,16 @@ extern "C" int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double 
                              h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return check_flags(h);
+}
+
+extern "C" int qmcw_set_vmc_mover(qmcw_handle *h, int32_t imetro, double deltar, double deltat) {
+  if (!h) return QMCW_ERR_ARG;
+  if (imetro != 1 && imetro != 6) return fail(h, QMCW_ERR_ARG, "imetro must be 1 or 6");
+  h->A.imetro = imetro;
+  h->A.deltar = deltar < 1.0 ? 2.0 : deltar;
+  h->A.deltat = (deltat < 0.0 || deltat > 2.0) ? 2.0 : deltat;
+  drop_graph(h);             // the captured sweep holds the kernel arguments
+  return QMCW_OK;
 }
 
 extern "C" int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta, double fbias) {

@@ -40,6 +40,8 @@ struct KArgs {
   double *dm;           // DMC records (k_ratio_1e mode 3)
   qmcw_dmc_params dmp;  // and the generation's parameters
   const uint8_t *acc_in;  // k_ratio_1e mode 4: the walkers whose given move is committed
+  int imetro;           // VMC mover: 1 metrop_mov1, 6 metrop_slat_mov1
+  double deltar, deltat;  // and the accelerated mover's radial and angular steps
 };
 
 // offsets in a walker's T-move record
@@ -52,6 +54,7 @@ __host__ __device__ inline int64_t tq_point(const DevSys &S, int i, int j, int i
 
 }  // namespace qmcw
 #include "engine_dmc_dev.h"
+#include "engine_slat_dev.h"
 namespace qmcw {
 
 // occupancy hint of the heavy per-walker kernels (waves per SIMD)
@@ -965,8 +968,11 @@ __device__ void group_grad_1e(const KArgs &A, const double *ws, int ie,
 }
 
 __device__ __forceinline__ int metrop_accept(const KArgs &A, const double *ws, const double *mv,
-                                             int w, int ns, int nocc, double delta,
+                                             int w, int ie, int ns, int nocc, double delta,
                                              double fbias);
+__device__ __forceinline__ void vmc_propose_w(const KArgs &A, int w, int ie, const double *ws,
+                                              const double *vold, double delta, double fbias,
+                                              double *xn, double *mv);
 template <int G>
 __device__ void group_commit(const KArgs &A, double *ws, const double *mv, int ie, int lane);
 
@@ -1063,28 +1069,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     if (mode == 3) {
       if (lane == 0) dmc_propose_w(A, w, ie, vold, A.dm + (int64_t)w * kDmStride, A.dmp, xn);
     } else if (lane == 0) {
-      int *rs = A.rng + 4 * w;
-      int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
-      double deltfi = 2.0 * fbias / delta;
-      double fxop = 1.0;
-      for (int k = 0; k < 3; k++) {
-        double fx = fmin(fabs(vold[k]), deltfi);
-        double ftest = fx * delta * 0.5;
-        fx = copysign(fx, vold[k]);
-        double dx;
-        if (ftest <= rannyu(st4)) {
-          dx = (rannyu(st4) - 0.5) * delta;
-        } else {
-          double r1 = rannyu(st4), r2 = rannyu(st4);
-          dx = (fmax(r1, r2) - 0.5) * delta;
-          if (vold[k] < 0) dx = -dx;
-        }
-        fxop *= (1.0 + fx * dx);
-        xn[k] = x[3 * ie + k] + dx;
-        mv[ML.misc + 2 + k] = dx;
-      }
-      mv[ML.misc + 1] = fxop;
-      rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+      vmc_propose_w(A, w, ie, ws, vold, delta, fbias, xn, mv);
     }
   } else {
     if (lane < 3) xn[lane] = xnew_in[3 * w + lane];
@@ -1256,7 +1241,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     if (lane == 0) {
       const int acc = mode == 4   ? 1
                       : mode == 3 ? dmc_decide_w(A, w, ie, mv, A.dm + (int64_t)w * kDmStride, A.dmp)
-                                  : metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias);
+                                  : metrop_accept(A, ws, mv, w, ie, ns, nocc, delta, fbias);
       s_acc[grp] = acc;
       if (acc && nacc) nacc[w] += 1;
     }
@@ -1294,30 +1279,7 @@ void k_move_a(KArgs A, int ie, int mode, const double *xnew_in, double delta, do
     group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, ws + L.ai[s],
                      ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lapjo, vold, nullptr, lane);
     (void)lapo_dummy;
-    if (lane == 0) {
-      int *rs = A.rng + 4 * w;
-      int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
-      double deltfi = 2.0 * fbias / delta;
-      double fxop = 1.0;
-      for (int k = 0; k < 3; k++) {
-        double fx = fmin(fabs(vold[k]), deltfi);
-        double ftest = fx * delta * 0.5;
-        fx = copysign(fx, vold[k]);
-        double dx;
-        if (ftest <= rannyu(st4)) {
-          dx = (rannyu(st4) - 0.5) * delta;
-        } else {
-          double r1 = rannyu(st4), r2 = rannyu(st4);
-          dx = (fmax(r1, r2) - 0.5) * delta;
-          if (vold[k] < 0) dx = -dx;
-        }
-        fxop *= (1.0 + fx * dx);
-        xn[k] = x[3 * ie + k] + dx;
-        mv[ML.misc + 2 + k] = dx;
-      }
-      mv[ML.misc + 1] = fxop;
-      rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
-    }
+    if (lane == 0) vmc_propose_w(A, w, ie, ws, vold, delta, fbias, xn, mv);
   } else {
     if (lane < 3) xn[lane] = xnew_in[3 * w + lane];
   }
@@ -1520,7 +1482,7 @@ __global__ QMCW_MOVE void k_move_c(KArgs A, int ie, int mode, double delta, doub
     __shared__ int s_acc[kWave / G];
     __syncthreads();
     if (lane == 0) {
-      const int acc = metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias);
+      const int acc = metrop_accept(A, ws, mv, w, ie, ns, nocc, delta, fbias);
       s_acc[grp] = acc;
       if (acc && nacc) nacc[w] += 1;
     }
@@ -1531,23 +1493,77 @@ __global__ QMCW_MOVE void k_move_c(KArgs A, int ie, int mode, double delta, doub
 
 // Metropolis test of metrop_mov1 (qmc/vmc/metrop_mov1.f90:90-99, 226-250)
 // for the pending move of electron ie, with the walker's rannyu (lane 0).
+// One walker's VMC proposal of electron ie (lane 0) from its current
+// velocity vold: metrop_mov1's box proposal with force bias (imetro 1,
+// qmc/vmc/metrop_mov1.f90:63-99) or the accelerated mover (imetro 6,
+// qmc/vmc/metrop_slat_mov1.f90:78-289).  The pending move buffer keeps what
+// the acceptance needs.
+__device__ __forceinline__ void vmc_propose_w(const KArgs &A, int w, int ie, const double *ws,
+                                              const double *vold, double delta, double fbias,
+                                              double *xn, double *mv) {
+  const MLayout &ML = A.ML;
+  int *rs = A.rng + 4 * w;
+  int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
+  if (A.imetro == 6) {
+    slat_propose(A.S, A.deltar, A.deltat, ws + A.L.rvec_en + (int64_t)ie * A.S.M * 3, vold, fbias, st4, xn,
+                 mv + ML.slat);
+  } else {
+    const double *x = ws + A.L.x;
+    const double deltfi = 2.0 * fbias / delta;
+    double fxop = 1.0;
+    for (int k = 0; k < 3; k++) {
+      double fx = fmin(fabs(vold[k]), deltfi);
+      double ftest = fx * delta * 0.5;
+      fx = copysign(fx, vold[k]);
+      double dx;
+      if (ftest <= rannyu(st4)) {
+        dx = (rannyu(st4) - 0.5) * delta;
+      } else {
+        double r1 = rannyu(st4), r2 = rannyu(st4);
+        dx = (fmax(r1, r2) - 0.5) * delta;
+        if (vold[k] < 0) dx = -dx;
+      }
+      fxop *= (1.0 + fx * dx);
+      xn[k] = x[3 * ie + k] + dx;
+      mv[ML.misc + 2 + k] = dx;
+    }
+    mv[ML.misc + 1] = fxop;
+  }
+  rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
+}
+
 __device__ __forceinline__ int metrop_accept(const KArgs &A, const double *ws, const double *mv,
-                                             int w, int ns, int nocc, double delta,
+                                             int w, int ie, int ns, int nocc, double delta,
                                              double fbias) {
   const WLayout &L = A.L;
   const MLayout &ML = A.ML;
   int *rs = A.rng + 4 * w;
   int st4[4] = {rs[0], rs[1], rs[2], rs[3]};
-  const double deltfi = 2.0 * fbias / delta;
   const double logn = mv[ML.misc + 8 + ns + nocc];
   const double psi2n = 2.0 * (logn + mv[ML.misc + 7]);
-  double fxnp = 1.0;
-  for (int k = 0; k < 3; k++) {
-    const double vn = mv[ML.misc + 10 + ns + nocc + k];
-    fxnp *= (1.0 - mv[ML.misc + 2 + k] * copysign(fmin(fabs(vn), deltfi), vn));
+  double p;
+  if (A.imetro == 6) {
+    // reverse move from the proposed point with its velocity
+    // (metrop_slat_mov1.f90:299-336, 376-481); 0 for a geometrical rejection
+    const double *rec = mv + ML.slat;
+    double fx, area, rto;
+    p = 0.0;
+    if (slat_density(A.S, A.deltar, A.deltat, mv + ML.rven, mv + ML.misc + 10 + ns + nocc,
+                     ws + L.rvec_en + (int64_t)ie * A.S.M * 3, fbias, fx, area, rto)) {
+      const double rr = rec[2] / rto;
+      p = rr * rr * exp(psi2n - ws[L.psi2o]) * fabs((fx * rec[1]) / (rec[0] * area));
+      p = p >= 0 ? fmin(1.0, p) : 0.0;     // NaN (an unsampled proposal): reject
+    }
+  } else {
+    const double deltfi = 2.0 * fbias / delta;
+    double fxnp = 1.0;
+    for (int k = 0; k < 3; k++) {
+      const double vn = mv[ML.misc + 10 + ns + nocc + k];
+      fxnp *= (1.0 - mv[ML.misc + 2 + k] * copysign(fmin(fabs(vn), deltfi), vn));
+    }
+    p = exp(psi2n - ws[L.psi2o]) * fxnp / mv[ML.misc + 1];
+    p = fmin(1.0, p);
   }
-  double p = exp(psi2n - ws[L.psi2o]) * fxnp / mv[ML.misc + 1];
-  p = fmin(1.0, p);
   const int acc = rannyu(st4) < p;
   rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
   return acc;
@@ -1680,7 +1696,7 @@ __global__ __launch_bounds__(64) void k_accept_1e(KArgs A, int ie, int mode,
   const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
   __shared__ int s_acc[kWave / G];
   if (lane == 0) {
-    const int acc = (mode == 1) ? metrop_accept(A, ws, mv, w, ns, nocc, delta, fbias)
+    const int acc = (mode == 1) ? metrop_accept(A, ws, mv, w, ie, ns, nocc, delta, fbias)
                                 : (accept_in[w] != 0);
     s_acc[grp] = acc;
     if (acc && naccept) naccept[w] += 1;

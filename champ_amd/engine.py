@@ -170,6 +170,11 @@ class Engine:
         return dict(n=o[0], se=o[1], se2=o[2], sd=v[0], sde=v[1], sl=v[2], dd=m[0], dde=m[1], dl=m[2])
 
     # ---- drivers
+    def set_vmc_mover(self, imetro: int, deltar: float = 5.0, deltat: float = 1.0):
+        """CHAMP's imetro: 1 metrop_mov1 (default), 6 metrop_slat_mov1 with
+        radial step multiplier deltar and cos(theta) step deltat."""
+        self._chk(self.lib.qmcw_set_vmc_mover(self.h, int(imetro), float(deltar), float(deltat)))
+
     def vmc(self, nstep: int, delta: float, fbias: float = 1.0, trace: bool = True):
         tr = np.zeros((nstep, self.W)) if trace else None
         nacc = np.zeros(self.W, np.int64)

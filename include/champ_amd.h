@@ -229,10 +229,20 @@ int qmcw_accum_add_static(qmcw_handle *h, double *acc);
 int qmcw_set_rng(qmcw_handle *h, const int32_t *seed);
 int qmcw_get_rng(const qmcw_handle *h, int32_t *seed);
 
+/* The mover of qmcw_vmc_run / qmcw_vmc_run_device (CHAMP's imetro,
+   qmc/control_mod.f90:78, 374-392): 1, metrop_mov1's box proposal
+   (qmc/vmc/metrop_mov1.f90:63-99, the handle's default); 6, CHAMP's
+   default accelerated Metropolis mover metrop_slat_mov1
+   (qmc/vmc/metrop_slat_mov1.f90:78-546): radial step multiplier deltar and
+   cos(theta) step deltat about the nearest nucleus, with the run's fbias.
+   Out-of-range steps are reset as CHAMP does (deltar < 1 -> 2; deltat
+   outside [0, 2] -> 2).  Other imetro: QMCW_ERR_ARG. */
+int qmcw_set_vmc_mover(qmcw_handle *h, int32_t imetro, double deltar, double deltat);
+
 /* Run nstep VMC steps on device.  One step = a Metropolis sweep over all
-   electrons with the proposal of qmc/vmc/metrop_mov1.f90:63-99 (box step
-   delta, force bias fbias) followed by a full evaluation, as in
-   qmc/vmc/metrop_mov1.f90:283-300.  Per walker and step the accumulated
+   electrons with the mover set by qmcw_set_vmc_mover (box step delta
+   and force bias fbias for imetro 1; fbias for imetro 6) followed by a full
+   evaluation, as in qmc/vmc/metrop_mov1.f90:283-300.  Per walker and step the accumulated
    local energy is written to eloc_trace[step][W] when non-NULL (host).
    naccept (host, may be NULL) receives the accepted move count per walker. */
 int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double fbias,

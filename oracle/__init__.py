@@ -49,6 +49,10 @@ def lib():
             "orc_cusp": (C.c_int32, [_vp, _pd, _pi]),
             "orc_cuspexact": (None, [_vp, _pd]),
             "orc_set_emulation": (None, [_vp, C.c_int32]),
+            "orc_set_mover": (None, [_vp, C.c_int32, C.c_double, C.c_double]),
+            "orc_slat_sample": (None, [_vp, _vp, C.c_int32, C.c_double, _pi, _pd, _pd]),
+            "orc_slat_density": (C.c_double, [_vp, _vp, C.c_int32, C.c_double, _pd]),
+            "orc_slat_area": (C.c_double, [C.c_double] * 5),
             "orc_set_params": (None, [_vp, _pd, _pd, _pd, _pd, _pd]),
             "orc_det_info": (None, [_vp, _pi]),
             "orc_setrn": (None, [_pi, _pi]),
@@ -146,6 +150,10 @@ class Oracle:
         """Error studies only: bit 0 rounds the contraction inputs of the
         full orbital rows to bfloat16, bit 1 those of the quadrature rows."""
         self.L.orc_set_emulation(self.h, int(mode))
+
+    def set_mover(self, imetro: int, deltar: float = 5.0, deltat: float = 1.0):
+        """VMC mover of vmc_run: 1 metrop_mov1, 6 metrop_slat_mov1."""
+        self.L.orc_set_mover(self.h, int(imetro), float(deltar), float(deltat))
 
     def set_params(self, a4=None, b=None, c=None, csf_coef=None, coef=None):
         conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)

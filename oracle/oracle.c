@@ -78,6 +78,9 @@ struct orc_sys {
      the inputs of the full orbital rows (value, gradient, laplacian) to
      bfloat16, bit 1 those of the value-only rows (quadrature points) */
   int emul;
+  /* VMC mover: 1 metrop_mov1, 6 metrop_slat_mov1 (orc_set_mover) */
+  int imetro;
+  double deltar, deltat;
 };
 
 static void *dupa(const void *p, size_t n) {
@@ -2335,6 +2338,286 @@ int orc_derivs(const orc_sys *o, orc_walker *w, double *dpsi, double *deloc) {
 }
 
 /* ------------------------------------------------------------------ */
+/* The accelerated Metropolis mover (imetro = 6), qmc/vmc/metrop_slat_mov1.f90:
+   the electron moves in local (r, theta, phi) coordinates about its nearest
+   nucleus: r from sqrt(r)|1 + co r|exp(-zeta r) on [r/deltar, r deltar] by
+   rejection, cos(theta) uniform on [1 - deltt, 1] about the electron's
+   direction, phi from 1 + term cos(phi) by rejection about the transverse
+   velocity.  Forward and reverse densities use the same formulas. */
+
+void orc_set_mover(orc_sys *o, int32_t imetro, double deltar, double deltat) {
+  o->imetro = imetro;
+  /* the reference resets out-of-range values (qmc/control_mod.f90:374-384) */
+  o->deltar = deltar < 1.0 ? 2.0 : deltar;
+  o->deltat = (deltat < 0.0 || deltat > 2.0) ? 2.0 : deltat;
+}
+
+/* incomplete gamma function gamma(a, x) for a = 3/2, 5/2 with xae = x^a e^-x
+   given; returns -gammac (iflag = -1) for x >= a + 4, qmc/vmc/gammai.f90:1-98 */
+static double slat_gammai(double a, double x, double xae, int *iflag) {
+  if (x < a + 4) {
+    *iflag = 1;
+    double ap = a, sum = 1 / a, del = sum;
+    int iter = (int)(21 + 3 * (x - a));
+    for (int n = 1; n <= iter; n++) {
+      ap = ap + 1;
+      del = del * x / ap;
+      sum = sum + del;
+    }
+    return sum * xae;
+  }
+  *iflag = -1;
+  double a0 = 1, a1 = x, b0 = 0, b1 = 1, fac = 1, g = 0;
+  int iter = 20 + (int)lround(a - x);
+  if (iter < 7) iter = 7;
+  for (int n = 1; n <= iter; n++) {
+    double an = (double)n, ana = an - a;
+    a0 = (a1 + a0 * ana) * fac;
+    b0 = (b1 + b0 * ana) * fac;
+    double anf = an * fac;
+    a1 = x * a0 + anf * a1;
+    b1 = x * b0 + anf * b1;
+    fac = 1 / a1;
+    g = b1 * fac;
+  }
+  return -(g * xae);
+}
+
+/* zeta and co of the radial function for radial velocity vr at distance r
+   (metrop_slat_mov1.f90:125-140, :398-412) */
+static void slat_zeta_co(double zcusp, double vr, double r, double *zeta, double *co) {
+  const double eps = 1e-10;
+  double root = (zcusp + vr) * (zcusp + vr - 4.0 / r), z;
+  if (root >= 0) {
+    root = sqrt(root);
+    z = 0.5 * (zcusp - vr - root);
+    if (z <= 0) z = 0.5 * (zcusp - vr + root);
+    if (z <= 0) z = eps;
+  } else {
+    z = vr < 0 ? -vr - eps : 1.0;
+  }
+  *zeta = z;
+  *co = (z + vr) / (1.0 - (z + vr) * r);
+}
+
+/* integral of sqrt(t)|1 + co t|exp(-zeta t) over [rbot, rtop] divided by
+   bot (metrop_slat_mov1.f90:198-232) */
+static double slat_area(double zeta, double co, double rbot, double rtop, double bot) {
+  const double d5b2 = 2.5, d2b3 = .666666666666667, g5b2 = 1.329340388179137;
+  int fb, ft, fz;
+  double rzero = -1.0 / co;
+  double zrbot = zeta * rbot, zrtop = zeta * rtop;
+  double zebot = sqrt(zrbot * zrbot * zrbot) * exp(-zrbot);
+  double zetop = sqrt(zrtop * zrtop * zrtop) * exp(-zrtop);
+  double g52bot = slat_gammai(d5b2, zrbot, zrbot * zebot, &fb);
+  double g52top = slat_gammai(d5b2, zrtop, zrtop * zetop, &ft);
+  if (rzero < rbot || rzero > rtop) {
+    double g52dif = (fb * ft == 1) ? g52top - g52bot : g52top - g52bot + g5b2;
+    double g32dif = d2b3 * (g52dif + zetop - zebot);
+    return fabs(g32dif + co * g52dif / zeta) / (bot * sqrt(zeta * zeta * zeta));
+  }
+  double zrzer = zeta * rzero;
+  double zezer = sqrt(zrzer * zrzer * zrzer) * exp(-zrzer);
+  double g52zer = slat_gammai(d5b2, zrzer, zrzer * zezer, &fz);
+  double g52dif1 = (fb * fz == 1) ? g52zer - g52bot : g52zer - g52bot + g5b2;
+  double g32dif1 = d2b3 * (g52dif1 + zezer - zebot);
+  double g52dif2 = (ft * fz == 1) ? g52top - g52zer : g52top - g52zer + g5b2;
+  double g32dif2 = d2b3 * (g52dif2 + zetop - zezer);
+  return (fabs(g32dif1 + co * g52dif1 / zeta) + fabs(g32dif2 + co * g52dif2 / zeta)) /
+         (bot * sqrt(zeta * zeta * zeta));
+}
+
+double orc_slat_area(double zeta, double co, double rbot, double rtop, double bot) {
+  return slat_area(zeta, co, rbot, rtop, bot);
+}
+
+static double slat_thetamx(const orc_sys *o, double r, double z) {
+  return o->deltat + (2.0 - o->deltat) / (1.0 + (z * r) * (z * r));
+}
+
+/* nearest nucleus of a point from its e-n vectors rv[M][3] (squared
+   distances, the first minimum; metrop_slat_mov1.f90:299-308) */
+static int slat_nearest(const orc_sys *o, const double *rv) {
+  double best = 99.e9;
+  int ic = 0;
+  for (int j = 0; j < o->ncent; j++) {
+    double d = 0;
+    for (int k = 0; k < 3; k++) d += rv[3 * j + k] * rv[3 * j + k];
+    if (d < best) { best = d; ic = j; }
+  }
+  return ic;
+}
+
+/* Proposal of electron i from the position whose e-n vectors are rv[M][3],
+   velocity v (grad ln Psi): xn and rec = {fxop, areao, rtry}
+   (metrop_slat_mov1.f90:78-289). */
+static void slat_propose(const orc_sys *o, const double *rv, const double *v, double fbias,
+                         int32_t *rng, double *xn, double *rec) {
+  const double eps = 1e-10, pi = 4.0 * atan(1.0), deltar = o->deltar, deltri = 1.0 / deltar;
+  const int nearo = slat_nearest(o, rv);
+  const double *rvm = rv + 3 * nearo;
+  const double zn = o->znuc[o->iwctype[nearo]];
+  const double zcusp = o->nloc == 0 ? zn : 0.0;
+  const double rmino = sqrt(rvm[0] * rvm[0] + rvm[1] * rvm[1] + rvm[2] * rvm[2]);
+  const double rbot = rmino * deltri, rtop = rmino * deltar;
+  double vr = 0, vp = 0, xa[3], ya[3], za[3];
+  for (int k = 0; k < 3; k++) vr += v[k] * rvm[k];
+  vr /= rmino;
+  for (int k = 0; k < 3; k++) {
+    za[k] = rvm[k] / rmino;
+    xa[k] = v[k] - vr * za[k];
+    vp += xa[k] * xa[k];
+  }
+  vp = sqrt(vp);
+  if (vp < eps) {
+    xa[0] = eps * (1 - za[0] * za[0]);
+    xa[1] = eps * (-za[0] * za[1]);
+    xa[2] = eps * (-za[0] * za[2]);
+    vp = eps * sqrt(1 - za[0] * za[0]);
+  }
+  for (int k = 0; k < 3; k++) xa[k] /= vp;
+  vr = fmax(vr, -2 * zn);
+  ya[0] = za[1] * xa[2] - za[2] * xa[1];
+  ya[1] = za[2] * xa[0] - za[0] * xa[2];
+  ya[2] = za[0] * xa[1] - za[1] * xa[0];
+  vp *= fbias;
+  double zeta, co;
+  slat_zeta_co(zcusp, vr, rmino, &zeta, &co);
+  /* maximum of the radial function on [rbot, rtop] for the rejection */
+  const double root = sqrt((1.5 * co - zeta) * (1.5 * co - zeta) + 2 * zeta * co);
+  double rmax1 = co != 0 ? (1.5 * co - zeta + root) / (2 * zeta * co) : rtop;
+  if (rmax1 < rbot) rmax1 = rbot;
+  if (rmax1 > rtop) rmax1 = rtop;
+  double gmax = sqrt(rmax1) * fabs(1 + co * rmax1) * exp(-zeta * rmax1);
+  if (co < 0) {
+    double rmax2 = (1.5 * co - zeta - root) / (2 * zeta * co);
+    if (rmax2 < rbot) rmax2 = rbot;
+    if (rmax2 > rtop) rmax2 = rtop;
+    gmax = fmax(gmax, sqrt(rmax2) * fabs(1 + co * rmax2) * exp(-zeta * rmax2));
+  }
+  const double bot = sqrt(rmino) * fabs(1 + co * rmino) * exp(-zeta * rmino);
+  double rtry = 0, top = 0;
+  /* the reference loops without bound; after 2^20 tries of either loop
+     (only non-finite input gets there) fxop = NaN and the move is rejected */
+  int ok = 0;
+  for (int t = 0; t < (1 << 20) && !ok; t++) {
+    rtry = ((deltar - deltri) * orc_rannyu(rng) + deltri) * rmino;
+    top = sqrt(rtry) * fabs(1 + co * rtry) * exp(-zeta * rtry);
+    ok = top / gmax > orc_rannyu(rng);
+  }
+  double fxop = ok ? top / bot : NAN;
+  double areao = slat_area(zeta, co, rbot, rtop, bot);
+  const double raver = 0.5 * (rmino + rtry);
+  const double deltt = slat_thetamx(o, raver, zn);
+  const double costht = 1 - deltt * orc_rannyu(rng);
+  const double zprime = rtry * costht;
+  const double sintht = sqrt(1 - costht * costht);
+  areao *= deltt;
+  const double term = fmin(vp * raver * sintht, 1.0);
+  const double fm = 1 + term;
+  double phitry = 0;
+  ok = 0;
+  for (int t = 0; t < (1 << 20) && !ok; t++) {
+    phitry = pi * orc_rannyu(rng);
+    top = 1 + term * cos(phitry);
+    ok = top > orc_rannyu(rng) * fm;
+  }
+  fxop = ok ? fxop * top : NAN;
+  const double xprime = rtry * sintht * cos(phitry);
+  double yprime = sqrt(fmax(0.0, rtry * rtry - xprime * xprime - zprime * zprime));
+  if (orc_rannyu(rng) < 0.5) yprime = -yprime;
+  for (int k = 0; k < 3; k++)
+    xn[k] = (xa[k] * xprime + ya[k] * yprime + za[k] * zprime) + o->cent[3 * nearo + k];
+  rec[0] = fxop;
+  rec[1] = areao;
+  rec[2] = rtry;
+}
+
+/* Density factors of the move from the point with e-n vectors rvf[M][3]
+   and velocity vf to the point with e-n vectors rvt[M][3]
+   (metrop_slat_mov1.f90:299-336, 376-475): fx and area (the density is
+   fx / (area 2 pi r_to^2), r_to the distance of the destination from the
+   start's nearest nucleus, returned in *rto).  Returns 0 when the
+   destination is outside what the start's sampler reaches (the reference's
+   geometrical rejection). */
+static int slat_density(const orc_sys *o, const double *rvf, const double *vf, const double *rvt,
+                        double fbias, double *fx, double *area, double *rto) {
+  const double eps = 1e-10;
+  const int nearn = slat_nearest(o, rvf);
+  const double *rn = rvf + 3 * nearn, *ro = rvt + 3 * nearn;
+  const double zn = o->znuc[o->iwctype[nearn]];
+  double rminn = 0, rminon = 0, dot = 0;
+  for (int k = 0; k < 3; k++) rminn += rn[k] * rn[k];
+  rminn = sqrt(rminn);
+  for (int k = 0; k < 3; k++) {
+    rminon += ro[k] * ro[k];
+    dot += rn[k] * ro[k];
+  }
+  rminon = sqrt(rminon);
+  dot = dot / (rminn * rminon);
+  const double costht = dot, sintht = sqrt(1 - costht * costht);
+  const double ravern = 0.5 * (rminn + rminon);
+  const double delttn = slat_thetamx(o, ravern, zn);
+  *rto = rminon;
+  if (rminon > rminn * o->deltar || dot < 1 - delttn) return 0;
+  double vr = 0, vp = 0, xa[3];
+  for (int k = 0; k < 3; k++) vr += vf[k] * rn[k];
+  vr /= rminn;
+  for (int k = 0; k < 3; k++) {
+    xa[k] = vf[k] - vr * rn[k] / rminn;
+    vp += xa[k] * xa[k];
+  }
+  vp = sqrt(vp);
+  if (vp < eps) {
+    xa[0] = eps * (1 - rn[0] * rn[0]);
+    xa[1] = eps * (-rn[0] * rn[1]);
+    xa[2] = eps * (-rn[0] * rn[2]);
+    vp = eps * sqrt(1 - rn[0] * rn[0] * (2 - rminn * rminn));
+  }
+  for (int k = 0; k < 3; k++) xa[k] /= vp;
+  const double zcusp = o->nloc == 0 ? zn : 0.0;
+  vr = fmax(vr, -2 * zn);
+  vp *= fbias;
+  double zeta, co;
+  slat_zeta_co(zcusp, vr, rminn, &zeta, &co);
+  const double bot = sqrt(rminn) * fabs(1 + co * rminn) * exp(-zeta * rminn);
+  const double top = sqrt(rminon) * fabs(1 + co * rminon) * exp(-zeta * rminon);
+  double f = top / bot;
+  double a = slat_area(zeta, co, rminn / o->deltar, rminn * o->deltar, bot) * delttn;
+  const double term = fmin(vp * ravern * sintht, 1.0);
+  double cosphi = 0, rnorm = 0;
+  for (int k = 0; k < 3; k++) {
+    const double t2 = ro[k] / rminon - costht * rn[k] / rminn;
+    rnorm += t2 * t2;
+    cosphi += t2 * xa[k];
+  }
+  cosphi = cosphi / sqrt(rnorm);
+  f *= (1 + term * cosphi);
+  *fx = f;
+  *area = a;
+  return 1;
+}
+
+/* test access: one proposal of electron i of walker w (its current
+   velocity), and the density of a move of electron i from its current
+   position to xt (in the same units: fx / (area 2 pi r_to^2)) */
+void orc_slat_sample(const orc_sys *o, orc_walker *w, int32_t i, double fbias, int32_t *rng,
+                     double *xn, double *rec) {
+  double v[3];
+  orc_grad_1e(o, w, i, v, NULL);
+  slat_propose(o, w->rvec_en + (size_t)i * o->ncent * 3, v, fbias, rng, xn, rec);
+}
+double orc_slat_density(const orc_sys *o, orc_walker *w, int32_t i, double fbias, const double *xt) {
+  double v[3], rvt[3 * 256], fx, area, rto;
+  orc_grad_1e(o, w, i, v, NULL);
+  for (int j = 0; j < o->ncent; j++)
+    for (int k = 0; k < 3; k++) rvt[3 * j + k] = xt[k] - o->cent[3 * j + k];
+  if (!slat_density(o, w->rvec_en + (size_t)i * o->ncent * 3, v, rvt, fbias, &fx, &area, &rto)) return 0.0;
+  const double pi = 4.0 * atan(1.0);
+  return fx / (area * 2 * pi * rto * rto);
+}
+
+/* ------------------------------------------------------------------ */
 /* VMC driver: qmc/vmc/vmc.f90:250-260 (rotqua before each step),
    proposal/acceptance of qmc/vmc/metrop_mov1.f90:63-99,226-250 with the
    current-state velocity of electron i (eval_grad + Jastrow, as in
@@ -2352,6 +2635,27 @@ static int vmc_walker(const orc_sys *o, orc_walker *w, double *x,
     for (int i = 0; i < N; i++) {
       double vold[3], vnew[3], dx[3], xn[3], fxop = 1, lap;
       orc_grad_1e(o, w, i, vold, NULL);
+      if (o->imetro == 6) {
+        /* metrop_slat_mov1.f90:78-546 */
+        double rec[3], out4[4], fx, area, rto;
+        const int M = o->ncent;
+        slat_propose(o, w->rvec_en + (size_t)i * M * 3, vold, fbias, rng, xn, rec);
+        double rvn[3 * 256];
+        for (int j = 0; j < M; j++)
+          for (int k = 0; k < 3; k++) rvn[3 * j + k] = xn[k] - o->cent[3 * j + k];
+        st = orc_ratio_1e(o, w, i, xn, out4, vnew, &lap);
+        double p = 0.0;
+        if (slat_density(o, rvn, vnew, w->rvec_en + (size_t)i * M * 3, fbias, &fx, &area, &rto)) {
+          const double psi2n = 2 * (out4[1] + out4[3]);
+          const double rratio = rec[2] / rto;
+          p = rratio * rratio * exp(psi2n - w->psi2o) * fabs((fx * rec[1]) / (rec[0] * area));
+          p = p >= 0 ? fmin(1.0, p) : 0.0;      /* NaN (an unsampled proposal): reject */
+        }
+        int accept = orc_rannyu(rng) < p;
+        acc += accept;
+        orc_accept_1e(o, w, i, accept);
+        continue;
+      }
       for (int k = 0; k < 3; k++) {
         double fx = fmin(fabs(vold[k]), deltfi);
         double ftest = fx * delta * 0.5;

@@ -36,6 +36,9 @@ void orc_cuspexact(const orc_sys *o, double *c);
 /* error studies: bit 0 rounds the full orbital rows' contraction inputs to
    bfloat16, bit 1 the value-only (quadrature point) rows' */
 void orc_set_emulation(orc_sys *o, int32_t mode);
+/* VMC mover of orc_vmc_run: imetro 1 (metrop_mov1) or 6 (metrop_slat_mov1,
+   with deltar, deltat) */
+void orc_set_mover(orc_sys *o, int32_t imetro, double deltar, double deltat);
 void orc_set_params(orc_sys *o, const double *a4, const double *b,
                     const double *c, const double *csf_coef,
                     const double *coef);
@@ -143,6 +146,14 @@ int orc_dmc_generation(orc_dmc *d, const qmcw_dmc_params *p, qmcw_dmc_stats *sta
                        int32_t nthreads);
 int orc_dmc_branch(orc_dmc *d, int32_t *brng, qmcw_dmc_stats *stats);
 int32_t orc_dmc_get(const orc_dmc *d, double *x, int32_t *rng, double *weight, double *eloc);
+
+/* accelerated Metropolis mover, test access: a proposal of electron i from
+   the walker's current state (rec: fxop, areao, r of the new point), and the
+   density of the move of electron i to xt */
+void orc_slat_sample(const orc_sys *o, orc_walker *w, int32_t i, double fbias, int32_t *rng,
+                     double *xn, double *rec);
+double orc_slat_density(const orc_sys *o, orc_walker *w, int32_t i, double fbias, const double *xt);
+double orc_slat_area(double zeta, double co, double rbot, double rtop, double bot);
 
 #ifdef __cplusplus
 }

@@ -5,7 +5,7 @@ tests/golden/be_transcript.json.
 Reads the reference's example files as text (data); nothing is executed.
 Each record: the Jastrow parameters CHAMP used for one VMC run ("Wave
 function for next iteration #k") and that run's "total E" line (energy,
-error, sigma, sigma error) plus the number of passes.
+error, sigma, sigma error), the number of passes and the acceptance ratio.
 """
 import json
 import os
@@ -13,6 +13,14 @@ import re
 
 REF = "/root/reference/examples"
 HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _acceptance(out, j):
+    """The run's "acceptance" line (the accelerated Metropolis mover, imetro
+    = 6, CHAMP's default) after its "total E" line."""
+    while not out[j].startswith("acceptance"):
+        j += 1
+    return float(out[j].split()[1])
 
 
 def main():
@@ -37,7 +45,8 @@ def main():
             v = out[j].split()
             e, err, sig, sigerr = float(v[3]), float(v[5]), float(v[8]), float(v[10])
             runs.append(dict(iteration=it, params=params, energy=e, error=err,
-                             sigma=sig, sigma_error=sigerr, passes=passes))
+                             sigma=sig, sigma_error=sigerr, passes=passes,
+                             acceptance=_acceptance(out, j)))
         if ln.startswith("Wave function for next iteration #"):
             j = i + 1
             while out[j].strip() != "parameters":

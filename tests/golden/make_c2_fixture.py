@@ -6,7 +6,7 @@ Reads the reference's example files as text (data); nothing is executed.
 Each record: the Jastrow parameters of one VMC run (iteration 1: the
 input's own; iteration k > 1: the "New wave function" block printed after
 iteration k - 1) and that run's "total E" line (energy, error, sigma, sigma
-error) plus the number of passes.
+error), the number of passes and the acceptance ratio.
 """
 import json
 import os
@@ -27,6 +27,14 @@ def _nums(line):
     return out
 
 
+def _acceptance(out, j):
+    """The run's "acceptance" line (the accelerated Metropolis mover, imetro
+    = 6, CHAMP's default) after its "total E" line."""
+    while not out[j].startswith("acceptance"):
+        j += 1
+    return float(out[j].split()[1])
+
+
 def main():
     inp = open(os.path.join(REF, STEM + ".inp")).read()
     out = open(os.path.join(REF, STEM + ".out")).read().splitlines()
@@ -43,7 +51,8 @@ def main():
                 j += 1
             v = out[j].split()
             runs.append(dict(iteration=it, params=params, energy=float(v[3]), error=float(v[5]),
-                             sigma=float(v[8]), sigma_error=float(v[10]), passes=passes))
+                             sigma=float(v[8]), sigma_error=float(v[10]), passes=passes,
+                             acceptance=_acceptance(out, j)))
         if ln.startswith("New wave function:"):
             j = i + 1
             while out[j].strip() != "Jastrow parameters:":

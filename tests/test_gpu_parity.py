@@ -116,6 +116,42 @@ def test_vmc_bit_exact_decisions(name):
     assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
 
 
+@pytest.mark.parametrize("name,path", [("be", "fused"), ("c2_ktt", "fused"), ("c2", "fused"),
+                                       ("chain4", "fused"), ("c2", "split"), ("be", "device")])
+def test_accelerated_mover_bit_exact_decisions(monkeypatch, name, path):
+    """CHAMP's default mover (imetro = 6, metrop_slat_mov1; deltar 5,
+    deltat 1, fbias 1 as in the example inputs) on the GPU and in the
+    oracle: identical decisions and streams (the rejection loops draw a
+    varying number of rannyu), positions to 1e-11 bohr, E_L to 1e-9.  `split`: the
+    three-launch move; `device`: the graph-captured device loop."""
+    if path == "split":
+        monkeypatch.setenv("QMCW_SPLIT_MOVE", "1")
+    s = SYS[name]
+    W = 24
+    x = system.random_walkers(s, W, seed=13)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    e = Engine(s, W)
+    e.set_walkers(x)
+    e.set_rng(seeds)
+    e.set_vmc_mover(6, 5.0, 1.0)
+    O = oracle.Oracle(s)
+    O.set_mover(6, 5.0, 1.0)
+    nstep = 4
+    o = O.vmc_run(x, seeds, nstep, 1.0)
+    if path == "device":
+        e.vmc_device(nstep, 1.0)
+        e.sync()
+        assert _close(e.get_eloc(), o["eloc"][-1], atol=1e-9) < 1e-9
+    else:
+        g = e.vmc(nstep, delta=1.0)
+        assert np.array_equal(g["naccept"], o["naccept"])
+        assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
+    # the proposal goes through cos, exp and sqrt, whose last bits may differ
+    # between the GPU and the host library
+    assert np.allclose(e.get_walkers(), o["x"], rtol=0, atol=1e-11)
+    assert np.array_equal(e.get_rng(), o["rng"])
+
+
 @pytest.mark.parametrize("G", [8, 16, 32, 64])
 def test_every_group_width_gives_identical_results(G):
     s = SYS["c2"]

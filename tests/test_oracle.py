@@ -547,6 +547,34 @@ def test_vmc_energy_matches_champ_c2_transcript(iteration):
     assert abs(e.std() - run["sigma"]) < 0.1, (e.std(), run)
 
 
+@pytest.mark.parametrize("case", [("be", 5), ("c2_ktt", 4)])
+def test_accelerated_mover_matches_champ_transcript_acceptance(case):
+    """CHAMP's default mover (imetro = 6, metrop_slat_mov1) with the
+    transcripts' settings (deltar = 5, deltat = 1, fbias = 1): the oracle's
+    acceptance ratio against the one each run printed (Be: 234,400 passes,
+    C2: 108,400), within 3.5 combined standard errors (binomial, doubled for
+    the walk's correlation), and the energy within 3 sigma."""
+    name, it = case
+    s, run = _be(it) if name == "be" else _c2_ktt(it)
+    O = oracle.Oracle(s)
+    O.set_mover(6, 5.0, 1.0)
+    nw, nstep, neq = (512, 300, 50) if name == "be" else (256, 250, 50)
+    x = system.random_walkers(s, nw, seed=5)
+    seeds = rng.walker_seeds("1837465927472523", nw)
+    r0 = O.vmc_run(x, seeds, neq, 1.0, nthreads=8)
+    r = O.vmc_run(r0["x"], r0["rng"], nstep, 1.0, nthreads=8)
+    nmove = nw * nstep * s.nelec
+    acc = float(r["naccept"].sum()) / nmove
+    pa = run["acceptance"]
+    se = 2 * math.hypot(math.sqrt(pa * (1 - pa) / nmove),
+                        math.sqrt(pa * (1 - pa) / (run["passes"] * s.nelec)))
+    assert abs(acc - pa) < 3.5 * se, (acc, pa, se)
+    e = r["eloc"]
+    blk = e.reshape(-1, 25, nw).mean(axis=1).mean(axis=1)
+    err = blk.std(ddof=1) / math.sqrt(len(blk))
+    assert abs(e.mean() - run["energy"]) < 3 * math.hypot(err, run["error"]), (e.mean(), err, run)
+
+
 def test_c2_example_jastrow_derivatives_fd():
     """The 24 Jastrow parameters the C2 example varies (isc = 2 scaled
     distances, read from its optimisation section): d ln Psi / dp and
