@@ -24,9 +24,18 @@ def _ref_sums(e):
 
 
 def _hip():
-    """The HIP runtime the library itself uses (torch ships another one,
-    which cannot initialise the GPU in the same process)."""
-    hip = C.CDLL("libamdhip64.so")
+    """The HIP runtime the library itself uses: the libamdhip64 file mapped
+    from ROCm's own directory (torch loads a second copy under the bare
+    name, which cannot initialise the GPU in the same process)."""
+    from champ_amd import abi
+    abi.load_library()
+    path = None
+    for ln in open("/proc/self/maps"):
+        f = ln.split()[-1]
+        if "libamdhip64.so" in f and "torch" not in f:
+            path = f
+            break
+    hip = C.CDLL(path or "/opt/rocm/lib/libamdhip64.so.7")
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
     hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
