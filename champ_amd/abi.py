@@ -141,6 +141,7 @@ SIGNATURES = {
     "qmcw_accum_get": (C.c_int, [_vp, _pd]),
     "qmcw_accum_add_static": (C.c_int, [_vp, C.c_void_p]),
     "qmcw_split_join": (C.c_int, [C.c_int32, C.c_int32, _pd, _pi, _pi, _pi, _pi, _pi]),
+    "qmcw_split_join_gpu": (C.c_int, [C.c_int32, C.c_int32, _pd, _pi, _pi, _pi, _pi, _pi]),
     "qmcw_set_rng": (C.c_int, [_vp, _pi]),
     "qmcw_get_rng": (C.c_int, [_vp, _pi]),
     "qmcw_vmc_run": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _pd,

@@ -88,6 +88,10 @@ struct qmcw_handle {
   int *tmint = nullptr;      // [3][Wp]: fresh points to evaluate, backward points, move chosen
   double *tmstale = nullptr; // [Wp]: G, gp and bt of the slot must be rebuilt from the state
   int *dorigin = nullptr;
+  double *dsort = nullptr;   // device split_join: sorted weights
+  int *dunder = nullptr, *dsj = nullptr;  // its hole stack; results [4] and branching stream [4]
+  int npow2 = 1;
+  int host_split_join = 0;    // QMCW_HOST_SPLIT_JOIN=1: split_join on the host
   int32_t brng[4] = {0, 0, 0, 1};
   // event timing of the device driver
   int prof = 0;
@@ -1080,6 +1084,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_NO_LDS_Y")) h->lds_y = atoi(e) == 0;
   if (const char *e = getenv("QMCW_NO_KEEP_ORB")) h->no_keep_orb = atoi(e) != 0;
   if (const char *e = getenv("QMCW_DMC_UNFUSED")) h->dmc_unfused = atoi(e) != 0;
+  if (const char *e = getenv("QMCW_HOST_SPLIT_JOIN")) h->host_split_join = atoi(e) != 0;
   {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1705,6 +1710,12 @@ static int dmc_alloc(qmcw_handle *h) {
   HIPCHK(h, hipMalloc(&p, 8 * sizeof(double))); h->allocs.push_back(p); h->dstat = (double *)p;
   HIPCHK(h, hipMalloc(&p, Wp * sizeof(double))); h->allocs.push_back(p); h->dwt = (double *)p;
   HIPCHK(h, hipMalloc(&p, Wp * sizeof(int))); h->allocs.push_back(p); h->dorigin = (int *)p;
+  // device split_join: sort buffer (next power of two), hole stack, results, branching stream
+  h->npow2 = 1;
+  while (h->npow2 < h->W) h->npow2 <<= 1;
+  HIPCHK(h, hipMalloc(&p, (size_t)h->npow2 * sizeof(double))); h->allocs.push_back(p); h->dsort = (double *)p;
+  HIPCHK(h, hipMalloc(&p, Wp * sizeof(int))); h->allocs.push_back(p); h->dunder = (int *)p;
+  HIPCHK(h, hipMalloc(&p, 8 * sizeof(int))); h->allocs.push_back(p); h->dsj = (int *)p;
   if (h->A.S.nloc > 0) {
     const int64_t ts = (int64_t)4 * h->A.S.N * h->A.S.tm_kmax * h->A.S.nquad;
     HIPCHK(h, hipMalloc(&p, Wp * ts * sizeof(double))); h->allocs.push_back(p); h->A.tq = (double *)p;
@@ -1863,6 +1874,59 @@ extern "C" int qmcw_split_join(int32_t n, int32_t mwalk, double *w, int32_t *ori
   return QMCW_OK;
 }
 
+// the device kernel on host arrays (test access; allocates and synchronises)
+extern "C" int qmcw_split_join_gpu(int32_t n, int32_t mwalk, double *w, int32_t *origin, int32_t *rng,
+                                   int32_t *brng, int32_t *nwalk_out, int32_t *nbranch) {
+  if (n < 0 || mwalk < n || !w || !origin || !rng || !brng || !nwalk_out || !nbranch)
+    return fail(nullptr, QMCW_ERR_ARG, "qmcw_split_join_gpu: bad arguments");
+  const int mp = (mwalk + 7) / 8 * 8;
+  int npow2 = 1;
+  while (npow2 < mwalk) npow2 <<= 1;
+  double *dw = nullptr, *ds = nullptr;
+  int *dr = nullptr, *dor = nullptr, *du = nullptr, *dsj = nullptr;
+  int st = QMCW_OK;
+  std::vector<double> wh(mp, 0.0);
+  std::vector<int32_t> rh((size_t)mp * 4, 0);
+  std::copy(w, w + n, wh.begin());
+  std::copy(rng, rng + (size_t)4 * n, rh.begin());
+  int res[8];
+  if (hipMalloc((void **)&dw, mp * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&ds, npow2 * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&dr, (size_t)mp * 4 * sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&dor, mp * sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&du, mp * sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&dsj, 8 * sizeof(int)) != hipSuccess) {
+    st = fail(nullptr, QMCW_ERR_NOMEM, "qmcw_split_join_gpu: hipMalloc");
+  } else if (hipMemcpy(dw, wh.data(), mp * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(dr, rh.data(), rh.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(dsj + 4, brng, 4 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    st = fail(nullptr, QMCW_ERR_HIP, "qmcw_split_join_gpu: hipMemcpy");
+  } else {
+    hipLaunchKernelGGL(k_split_join, dim3(1), dim3(1024), 0, 0, dw, n, mwalk, dsj + 4, dr, dor, ds, npow2,
+                       du, dsj);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(res, dsj, 8 * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(wh.data(), dw, mp * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(rh.data(), dr, rh.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+      st = fail(nullptr, QMCW_ERR_HIP, "qmcw_split_join_gpu: kernel or copy");
+    } else if (res[2]) {
+      st = fail(nullptr, QMCW_ERR_NOMEM, "DMC population exceeds the walker capacity");
+    } else {
+      std::vector<int> oh(mp);
+      (void)hipMemcpy(oh.data(), dor, mp * sizeof(int), hipMemcpyDeviceToHost);
+      std::copy(wh.begin(), wh.begin() + mwalk, w);
+      std::copy(oh.begin(), oh.begin() + mwalk, origin);
+      std::copy(rh.begin(), rh.begin() + (size_t)4 * mwalk, rng);
+      for (int k = 0; k < 4; k++) brng[k] = res[4 + k];
+      *nwalk_out = res[0];
+      *nbranch = res[1];
+    }
+  }
+  (void)hipFree(dw); (void)hipFree(ds); (void)hipFree(dr); (void)hipFree(dor); (void)hipFree(du);
+  (void)hipFree(dsj);
+  return st;
+}
+
 static int dmc_split_join(qmcw_handle *h, std::vector<double> &w, std::vector<int32_t> &rng,
                           std::vector<int> &origin, int &n_out, int &nbranch) {
   origin.resize(h->W);
@@ -1958,6 +2022,44 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
   hipLaunchKernelGGL(k_dmc_weights, g1, b1, 0, h->stream, (const double *)h->dm, h->nact, h->dwt);
   HIPCHK(h, hipGetLastError());
   double sums[8];
+  if (!h->host_split_join) {
+    // split_join and the gather on the device; one synchronisation
+    int res[8];
+    std::vector<int> fl(h->nact);
+    HIPCHK(h, hipMemcpyAsync(h->dsj + 4, h->brng, 4 * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_split_join, dim3(1), dim3(1024), 0, h->stream, h->dwt, h->nact, h->W, h->dsj + 4,
+                       h->A.rng, h->dorigin, h->dsort, h->npow2, h->dunder, h->dsj);
+    hipLaunchKernelGGL(k_dmc_gather, dim3(h->Wp), dim3(256), 0, h->stream, (const double *)h->A.st,
+                       h->st2, (int64_t)h->A.L.stride, (const double *)h->dm, h->dm2,
+                       (const int *)h->dorigin, (const double *)h->dwt, 0,
+                       (const double *)h->A.tq, h->tq2, h->A.tq_stride, (const int *)h->dsj);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(sums, h->dstat, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(fl.data(), h->A.flags, h->nact * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(res, h->dsj, 8 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int any = 0;
+    for (int f : fl) any |= f;
+    if (any & 1) return fail(h, QMCW_ERR_SINGULAR, "a Slater matrix is singular");
+    if (any & 2) return fail(h, QMCW_ERR_NAN, "non-finite local energy");
+    if (any & 4) return fail(h, QMCW_ERR_ARG, "an electron is in range of more pseudopotential centres than the T-move record holds");
+    if (res[2]) return fail(h, QMCW_ERR_NOMEM, "DMC population exceeds the handle's walkers");
+    for (int k = 0; k < 4; k++) h->brng[k] = res[4 + k];
+    std::swap(h->A.st, h->st2);
+    std::swap(h->dm, h->dm2);
+    std::swap(h->A.tq, h->tq2);
+    h->nact = res[0];
+    h->Wl = res[3];
+    h->A.Wp = res[3];
+    if (stats) {
+      stats->wsum = sums[0]; stats->wesum = sums[1]; stats->we2sum = sums[2];
+      stats->wtausum = sums[3]; stats->pacc = sums[4]; stats->nacc = sums[5];
+      stats->ncross = sums[6]; stats->ntry = sums[7];
+      stats->nwalk = res[0];
+      stats->nbranch = res[1];
+    }
+    return QMCW_OK;
+  }
   std::vector<double> w(h->Wp, 0.0);
   std::vector<int32_t> rng((size_t)h->Wp * 4, 0);   // the padding records reach Wp
   HIPCHK(h, hipMemcpyAsync(sums, h->dstat, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
@@ -1987,7 +2089,7 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
   hipLaunchKernelGGL(k_dmc_gather, dim3(Wl2), dim3(256), 0, h->stream, (const double *)h->A.st,
                      h->st2, (int64_t)h->A.L.stride, (const double *)h->dm, h->dm2,
                      (const int *)h->dorigin, (const double *)h->dwt, Wl2,
-                     (const double *)h->A.tq, h->tq2, h->A.tq_stride);
+                     (const double *)h->A.tq, h->tq2, h->A.tq_stride, (const int *)nullptr);
   HIPCHK(h, hipGetLastError());
   std::swap(h->A.st, h->st2);
   std::swap(h->dm, h->dm2);

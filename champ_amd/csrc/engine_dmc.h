@@ -140,12 +140,14 @@ __global__ void k_dmc_weights(const double *dm, int nact, double *out) {
 }
 
 // population after branching: record d is a copy of record origin[d]
+// nl_dev (may be null): the number of records, read on the device (the
+// device split_join leaves it there); else nl
 __global__ void k_dmc_gather(const double *st_src, double *st_dst, int64_t stride,
                              const double *dm_src, double *dm_dst, const int *origin,
                              const double *wnew, int nl, const double *tq_src, double *tq_dst,
-                             int64_t tq_stride) {
+                             int64_t tq_stride, const int *nl_dev) {
   const int d = blockIdx.x;
-  if (d >= nl) return;
+  if (d >= (nl_dev ? nl_dev[3] : nl)) return;
   const int o = origin[d];
   const double *src = st_src + (int64_t)o * stride;
   double *dst = st_dst + (int64_t)d * stride;
@@ -157,6 +159,121 @@ __global__ void k_dmc_gather(const double *st_src, double *st_dst, int64_t strid
     double v = dm_src[(int64_t)o * kDmStride + threadIdx.x];
     if (threadIdx.x == kDmW) v = wnew[d];
     dm_dst[(int64_t)d * kDmStride + threadIdx.x] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// split_join (qmc/dmc/dmc_elec/split_join_mod.f90:3-219) on the device, one
+// workgroup: the same loops and draws as the host routine qmcw_split_join
+// (joins in walker order with one rannyu per joined pair; the split
+// threshold from the sorted weights; splits into the holes last-first, each
+// copy with four 12-bit limbs drawn from the branching stream; compaction).
+// The joins, splits and compaction are sequential in the walker index and
+// the stream, so one lane runs them; the sort of the weights runs on every
+// lane (bitonic, over the next power of two, padded with +inf).
+// w[roundup(mwalk, 8)]: in, the n weights; out, the new population's
+// weights, padded with 0 to a multiple of 8.  rng[mwalk][4]: the walkers' streams, moved
+// with them.  origin[mwalk]: the old walker each new one copies.  sortbuf:
+// npow2 doubles; under: n ints.  res: {walkers, branching events, status
+// (0 ok, 1 more walkers than mwalk), walkers padded to a multiple of 8}.
+__global__ __launch_bounds__(1024) void k_split_join(double *w, int n, int mwalk, int *brng, int *rng,
+                                                     int *origin, double *sortbuf, int npow2, int *under,
+                                                     int *res) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  __shared__ int s_nunder, s_nbr;
+  for (int i = t; i < mwalk; i += nt) origin[i] = i;
+  if (t == 0) {
+    int st4[4] = {brng[0], brng[1], brng[2], brng[3]};
+    int nu = 0, ipair = 0, iw2 = 0, nbr = 0;
+    for (int iw = 0; iw < n; iw++) {
+      const double wi = w[iw];
+      if (!(wi < 0.5)) continue;
+      if (wi == 0.0) {
+        nbr++;
+        under[nu++] = iw;
+      } else if (ipair == 0) {
+        ipair = 1;
+        iw2 = iw;
+      } else {
+        nbr++;
+        ipair = 0;
+        const double wt = wi + w[iw2];
+        if (rannyu(st4) > wi / wt) {
+          w[iw2] = wt;
+          under[nu++] = iw;
+        } else {
+          w[iw] = wt;
+          under[nu++] = iw2;
+        }
+      }
+    }
+    brng[0] = st4[0]; brng[1] = st4[1]; brng[2] = st4[2]; brng[3] = st4[3];
+    s_nunder = nu;
+    s_nbr = nbr;
+  }
+  __syncthreads();
+  for (int i = t; i < npow2; i += nt) sortbuf[i] = i < n ? w[i] : __builtin_inf();
+  __syncthreads();
+  for (int k = 2; k <= npow2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < npow2; i += nt) {
+        const int l = i ^ j;
+        if (l > i) {
+          const double a = sortbuf[i], b = sortbuf[l];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) { sortbuf[i] = b; sortbuf[l] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  if (t == 0) {
+    int nu = s_nunder, nbr = s_nbr;
+    const int nafter = n - nu, nmax_dupl = mwalk - nafter;
+    auto at1 = [&](int i) { return n > 0 ? sortbuf[max(1, i) - 1] : 0.0; };   // 1-based, clamped below
+    double wt_split = fmax(2.0, at1(n - nmax_dupl));
+    int nmore = 0;
+    for (int iw = max(1, n - nmax_dupl); iw <= n; iw++) {
+      const double r = sortbuf[iw - 1] / wt_split;
+      if (r >= 8.0) nmore += 3;
+      else if (r >= 4.0) nmore += 2;
+      else if (r >= 2.0) nmore += 1;
+    }
+    wt_split = (n - nmax_dupl + nmore < n) ? fmax(2.0, at1(n - nmax_dupl + nmore)) : 1e99;
+    int st4[4] = {brng[0], brng[1], brng[2], brng[3]};
+    int n2 = n, status = 0;
+    for (int iw = 0; iw < n && !status; iw++) {
+      if (!(w[iw] > wt_split)) continue;
+      nbr++;
+      int dst;
+      if (nu > 0) {
+        dst = under[--nu];
+      } else {
+        if (n2 >= mwalk) { status = 1; break; }
+        dst = n2++;
+      }
+      w[iw] *= 0.5;
+      w[dst] = w[iw];
+      origin[dst] = origin[iw];
+      for (int k = 0; k < 4; k++) rng[4 * dst + k] = (int)(rannyu(st4) * 4096.0) & 4095;
+      rng[4 * dst + 3] |= 1;
+    }
+    while (!status && nu > 0) {
+      const int dst = under[--nu], src = n2 - 1;
+      n2--;
+      if (dst != src) {
+        w[dst] = w[src];
+        origin[dst] = origin[src];
+        for (int k = 0; k < 4; k++) rng[4 * dst + k] = rng[4 * src + k];
+      }
+    }
+    brng[0] = st4[0]; brng[1] = st4[1]; brng[2] = st4[2]; brng[3] = st4[3];
+    const int nl = (n2 + 7) / 8 * 8;     // the buffers hold mwalk rounded up to 8
+    for (int d = n2; d < nl; d++) {       // padding records: copies of the first walker, weight 0
+      origin[d] = origin[0];
+      w[d] = 0.0;
+      for (int k = 0; k < 4; k++) rng[4 * d + k] = rng[k];
+    }
+    res[0] = n2; res[1] = nbr; res[2] = status; res[3] = nl;
   }
 }
 

@@ -340,6 +340,14 @@ int qmcw_get_branch_rng(const qmcw_handle *h, int32_t *seed);
 int qmcw_split_join(int32_t n, int32_t mwalk, double *weight, int32_t *origin, int32_t *rng,
                     int32_t *brng, int32_t *nwalk_out, int32_t *nbranch);
 
+/* The same split_join by the device kernel qmcw_dmc_step runs (one
+   workgroup, k_split_join), on host arrays: test access, with the
+   arguments and results of qmcw_split_join (it allocates device memory and
+   synchronises).  QMCW_HOST_SPLIT_JOIN=1 makes qmcw_dmc_step branch on the
+   host instead. */
+int qmcw_split_join_gpu(int32_t n, int32_t mwalk, double *weight, int32_t *origin, int32_t *rng,
+                        int32_t *brng, int32_t *nwalk_out, int32_t *nbranch);
+
 /* Read back per-walker local energies of the last evaluation. */
 int qmcw_get_eloc(const qmcw_handle *h, double *eloc);
 
