@@ -302,7 +302,31 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
     return dict(aggregate=agg, per_system=per, seconds=dt)
 
 
-def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0):
+def cpu_baseline_dmc(s, seconds=10.0, nthreads=None, tau=0.01):
+    """The FP64 oracle's DMC generations (OpenMP over walkers) on the host
+    cores for the chain: batches of 16 walkers, one generation each, until
+    `seconds`."""
+    import oracle
+    from champ_amd import rng, system
+    from champ_amd.abi import DmcParams
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    nw, done, k = 16, 0, 0
+    t0 = time.time()
+    while time.time() - t0 < seconds:
+        x = system.random_walkers(s, nw, seed=200 + k)
+        d = oracle.OracleDMC(s, 2 * nw)
+        d.set(x, np.asarray(rng.walker_seeds("1234567890123457", nw, stride=(1 << 30) + k), np.int32))
+        d.generation(DmcParams(tau, -250.0, eest=-250.0, esigma=1.0), nthreads=nthreads)
+        d.close()
+        done += nw
+        k += 1
+    dt = time.time() - t0
+    return dict(value=done / dt, unit="walker-moves/s", cores=nthreads, kind="port",
+                sample=f"{k} batches of {nw} walkers x 1 DMC generation ({done} walker-moves, {dt:.1f} s)")
+
+
+def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0, reinv=64,
+                  keep_engine=False):
     """DMC of the 98-electron chain (16 Z_eff=4 centres with a
     pseudopotential, 34 H, 424 GTOs, one determinant): W0 walkers, capacity
     2 W0, time step tau, under the rundmc driver (population control),
@@ -320,6 +344,7 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0):
     e.set_walkers(system.random_walkers(s, cap, seed=9))
     e.set_rng(rng.walker_seeds("1837465927472523", cap, stride=1 << 20))
     e.set_branch_rng(np.array([1, 2, 3, 5], np.int32))
+    e.set_reinversion(reinv)
     g = e.autotune()
     if equil:
         e.vmc_device(equil, 0.3)
@@ -339,11 +364,15 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0):
     e.sync()
     prof = e.profile_get()
     e.profile(False)
-    e.close()
+    xs = e.get_walkers()[:h[-1]["nwalk"]]
+    if not keep_engine:
+        e.close()
+        e = None
     nw = [x["nwalk"] for x in h]
     return dict(rate=moves / dt, G=g, nwalk_final=h[-1]["nwalk"], eave=h[-1]["eave"],
                 nwalk_range=[min(nw), round(float(np.mean(nw)), 1), max(nw)],
-                acc=float(np.mean([x["acc"] for x in h])), seconds=dt, prof=prof)
+                acc=float(np.mean([x["acc"] for x in h])), seconds=dt, prof=prof, x=xs, system=s,
+                engine=e)
 
 
 def copy_calibration(nbytes=256 << 20):
@@ -384,6 +413,8 @@ def main():
                     help="only run DMC of the 98-electron chain (4096 walkers, tau 0.01)")
     ap.add_argument("--tmoves", type=int, default=0, choices=[0, 1, 2],
                     help="with --dmc: T-moves (1 det_balance1, 2 casula1)")
+    ap.add_argument("--reinversion", type=int, default=64,
+                    help="with --dmc: rebuild the Slater inverses every this many moves (0: per sweep)")
     ap.add_argument("--batch", type=int, default=0,
                     help="only run the multi-system batch with this many systems")
     args = ap.parse_args()
@@ -395,25 +426,66 @@ def main():
         args.steps = 20 if args.dmc else 200
     if args.dmc:
         prec = "mixed" if args.precision == "both" else args.precision
-        d = run_dmc_chain(4096, args.steps, args.warmup, prec, tmoves=args.tmoves)
+        precs = [prec] + ([q for q in ("fp64", "mixed_bf16") if q != prec] if args.precision == "both" else [])
+        runs = {}
+        for q in precs:
+            runs[q] = run_dmc_chain(4096, args.steps, args.warmup, q, tmoves=args.tmoves,
+                                    reinv=args.reinversion, keep_engine=(q == prec))
+            print(f"[bench] DMC chain {q}: {runs[q]['rate']:.4g} walker-moves/s "
+                  f"(population {runs[q]['nwalk_range']})", file=sys.stderr, flush=True)
+        d = runs[prec]
         for k, (ms, n) in d["prof"].items():
             if n:
                 print(f"[bench]   {k:12s} {n:5d} launches  avg {ms / n * 1e3:9.1f} us  "
                       f"total {ms:9.3f} ms", file=sys.stderr, flush=True)
-        print(json.dumps({"metric": "DMC walker-moves/s (98-electron chain)", "value": d["rate"],
-                          "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
-                          "warmup": args.warmup, "higher_is_better": True,
-                          "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
-                          "config": {"workload": "DMC, 98-electron chain, 4096 walkers (target), "
-                                                 "tau 0.01, from 20 VMC sweeps"
-                                                 + (", T-moves" if args.tmoves else ""),
-                                     "tmoves": args.tmoves,
-                                     "precision": prec, "group_width": d["G"]},
-                          "nwalk_final": d["nwalk_final"], "nwalk_min_mean_max": d["nwalk_range"],
-                          "acceptance": d["acc"],
-                          "seconds": d["seconds"],
-                          "kernel_avg_us": {k: (ms / n * 1e3 if n else None)
-                                            for k, (ms, n) in d["prof"].items()}}), flush=True)
+        s_ch, xs = d["system"], d["x"]
+        # energies of the final population at identical configurations
+        bias = {}
+        if len(precs) > 1:
+            from champ_amd.engine import Engine
+            el = {}
+            for q in precs:
+                eq = Engine(s_ch, len(xs), q)
+                eq.set_walkers(xs)
+                el[q] = eq.psi_at()["eloc"]
+                eq.close()
+            for q in precs:
+                if q != "fp64":
+                    dd = el[q] - el["fp64"]
+                    bias[q] = dict(mean_mHa=1e3 * float(dd.mean()),
+                                   sem_mHa=1e3 * float(dd.std() / np.sqrt(len(dd))),
+                                   max_abs_mHa=1e3 * float(np.abs(dd).max()))
+        if d["engine"] is not None:
+            d["engine"].close()
+        dom = max(d["prof"], key=lambda k: d["prof"][k][0])
+        ms_tot, nl = d["prof"][dom]
+        avg_ms = ms_tot / max(1, nl)
+        npop = d["nwalk_final"]
+        fl = kernel_flops(s_ch, dom, xs)
+        ach = fl / (avg_ms * 1e-3) / 1e12 if fl else None
+        roof = ({"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                 "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": dom,
+                 "avg_us": 1e3 * avg_ms, "launches_timed": nl, "flop_per_launch": fl,
+                 "walkers_per_launch": npop} if fl else None)
+        out = {"metric": "DMC walker-moves/s (98-electron chain)", "value": d["rate"],
+               "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
+               "config": {"workload": f"DMC, 98-electron chain (16 Z_eff=4 + 34 H, 424 GTOs), "
+                                      f"4096 walkers (target), tau 0.01, {args.steps} timed generations "
+                                      f"from 20 VMC sweeps, re-inversion every {args.reinversion} moves"
+                                      + (", T-moves" if args.tmoves else ""),
+                          "tmoves": args.tmoves, "reinversion_every": args.reinversion,
+                          "precision": prec, "group_width": d["G"]},
+               "walker_moves_per_s": {q: r["rate"] for q, r in runs.items()},
+               "energy_bias_vs_fp64": bias or None,
+               "nwalk_final": d["nwalk_final"], "nwalk_min_mean_max": d["nwalk_range"],
+               "acceptance": d["acc"], "seconds": d["seconds"],
+               "kernel_avg_us": {k: (ms / n * 1e3 if n else None) for k, (ms, n) in d["prof"].items()},
+               "roofline": roof}
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_baseline_dmc(s_ch, args.cpu_seconds)
+        print(json.dumps(out), flush=True)
         return
     if args.batch:
         prec = "mixed" if args.precision == "both" else args.precision

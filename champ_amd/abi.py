@@ -148,6 +148,8 @@ SIGNATURES = {
                                C.POINTER(C.c_int64)]),
     "qmcw_vmc_run_device": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double]),
     "qmcw_set_vmc_mover": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double]),
+    "qmcw_set_reinversion": (C.c_int, [_vp, C.c_int32]),
+    "qmcw_reinvert": (C.c_int, [_vp]),
     "qmcw_get_eloc": (C.c_int, [_vp, _pd]),
     "qmcw_sync": (C.c_int, [_vp]),
     "qmcw_autotune": (C.c_int, [_vp, C.POINTER(C.c_int32)]),

@@ -66,6 +66,8 @@ struct qmcw_handle {
   // (a full evaluation computed them and only committed moves changed
   // positions since); cleared by new positions or orbital coefficients
   int orb_valid = 0;
+  int reinv_every = 0;        // rebuild the Slater inverses after this many one-electron moves (0: only
+  int moves_since_inv = 0;    // at the full evaluation) and the moves since the last rebuild
   int no_keep_orb = 0;       // QMCW_NO_KEEP_ORB: recompute them in every full evaluation
   int dmc_unfused = 0;       // QMCW_DMC_UNFUSED: DMC moves as five launches per electron
   int use_lds = 1;           // LDS staging of the moved electron's basis values
@@ -547,6 +549,7 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
   // only the occupied ones: k_full_det recomputes them under full_orb)
   A.keep_state = (h->orb_valid && !h->no_keep_orb) ? 1 : 0;
   h->orb_valid = 1;
+  h->moves_since_inv = 0;
   // electrons per basis chunk of k_full_det: a wavefront's basis values
   // (64/G walkers) stay near 16 KB so that they are read back from L2
   const int gd = h->Gk[KC_DET];
@@ -1351,6 +1354,30 @@ static void launch_accum(qmcw_handle *h) {
   prof_end(h);
 }
 
+// The Slater inverses, T, the alpha inverses, Y and G rebuilt from the
+// orbital rows the committed moves kept (the determinant part of the full
+// evaluation, without its orbitals), which removes the Sherman-Morrison
+// updates' accumulated rounding.
+static void launch_reinvert(qmcw_handle *h) {
+  if (!h->orb_valid) return;
+  KArgs A = h->A;
+  A.full_orb = 0;
+  A.keep_state = 1;
+  A.lds_det = 0;
+  A.chunk = 1;
+  prof_begin(h, QMCW_PROF_FULL_DET);
+  DISPATCH_GW(h->Gk[KC_DET], launch_det, h, A, (size_t)0);
+  prof_end(h);
+  h->moves_since_inv = 0;
+}
+
+// one committed-or-rejected one-electron move of every walker: re-invert
+// when the interval set by qmcw_set_reinversion is reached (not after a
+// sweep's last move, which the full evaluation follows)
+static void count_move(qmcw_handle *h, bool last) {
+  if (h->reinv_every > 0 && ++h->moves_since_inv >= h->reinv_every && !last) launch_reinvert(h);
+}
+
 static void vmc_one_step(qmcw_handle *h, double delta, double fbias) {
   const int N = h->A.S.N;
   if (h->A.S.nloc > 0) {
@@ -1358,9 +1385,11 @@ static void vmc_one_step(qmcw_handle *h, double delta, double fbias) {
     hipLaunchKernelGGL(k_rotqua, dim3((h->Wl + 63) / 64), dim3(64), 0, h->stream, h->A);
     prof_end(h);
   }
-  for (int ie = 0; ie < N; ie++)   // proposal, ratio, Metropolis test and commit in one launch
+  for (int ie = 0; ie < N; ie++) {  // proposal, ratio, Metropolis test and commit in one launch
     launch_ratio(h, ie, 2, (const double *)nullptr, delta, fbias, (double *)nullptr,
                h->nacc);
+    count_move(h, ie + 1 == N);
+  }
   launch_eval(h, h->sample_derivs != 0);
   if (h->sample_derivs == 2) launch_accum(h);
 }
@@ -1434,6 +1463,21 @@ extern "C" int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double 
                              h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return check_flags(h);
+}
+
+extern "C" int qmcw_set_reinversion(qmcw_handle *h, int32_t every) {
+  if (!h || every < 0) return QMCW_ERR_ARG;
+  h->reinv_every = every;
+  drop_graph(h);             // the captured sweep holds the re-inversion launches
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_reinvert(qmcw_handle *h) {
+  if (!h) return QMCW_ERR_ARG;
+  if (!h->orb_valid) return fail(h, QMCW_ERR_ARG, "no full evaluation since the walkers were set");
+  launch_reinvert(h);
+  HIPCHK(h, hipGetLastError());
+  return QMCW_OK;
 }
 
 extern "C" int qmcw_set_vmc_mover(qmcw_handle *h, int32_t imetro, double deltar, double deltat) {
@@ -1997,7 +2041,10 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
     // k_ratio_1e launch (mode 3)
     h->A.dm = h->dm;
     h->A.dmp = *p;
-    for (int ie = 0; ie < N; ie++) launch_ratio(h, ie, 3, nullptr, 1.0, 1.0, (double *)nullptr);
+    for (int ie = 0; ie < N; ie++) {
+      launch_ratio(h, ie, 3, nullptr, 1.0, 1.0, (double *)nullptr);
+      count_move(h, ie + 1 == N);
+    }
     h->A.dm = nullptr;
   } else {
     for (int ie = 0; ie < N; ie++) {
@@ -2007,6 +2054,7 @@ extern "C" int qmcw_dmc_step(qmcw_handle *h, const qmcw_dmc_params *p, qmcw_dmc_
       launch_ratio(h, ie, 0, (const double *)xin, 1.0, 1.0, (double *)nullptr);
       hipLaunchKernelGGL(k_dmc_decide, g1, b1, 0, h->stream, h->A, ie, h->dm, h->acc_buf, *p);
       DISPATCH_MOVE(h, launch_accept, ie, 0, (const uint8_t *)h->acc_buf, 1.0, 1.0, (long long *)nullptr);
+      count_move(h, ie + 1 == N);
     }
   }
   // DMC-mode evaluation: walkers that moved draw a grid rotation per pair

@@ -170,6 +170,14 @@ class Engine:
         return dict(n=o[0], se=o[1], se2=o[2], sd=v[0], sde=v[1], sl=v[2], dd=m[0], dde=m[1], dl=m[2])
 
     # ---- drivers
+    def set_reinversion(self, every: int):
+        """Rebuild the Slater inverses after every `every` one-electron moves
+        (0: only at each step's full evaluation)."""
+        self._chk(self.lib.qmcw_set_reinversion(self.h, int(every)))
+
+    def reinvert(self):
+        self._chk(self.lib.qmcw_reinvert(self.h))
+
     def set_vmc_mover(self, imetro: int, deltar: float = 5.0, deltat: float = 1.0):
         """CHAMP's imetro: 1 metrop_mov1 (default), 6 metrop_slat_mov1 with
         radial step multiplier deltar and cos(theta) step deltat."""

@@ -229,6 +229,16 @@ int qmcw_accum_add_static(qmcw_handle *h, double *acc);
 int qmcw_set_rng(qmcw_handle *h, const int32_t *seed);
 int qmcw_get_rng(const qmcw_handle *h, int32_t *seed);
 
+/* Periodic re-inversion: after every `every` one-electron moves (counted
+   across sweeps; VMC and DMC moves, not T-moves) rebuild A^-1, T, the alpha
+   inverses and Y of every walker from its kept orbital rows, which removes
+   the rounding the Sherman-Morrison updates accumulate (dete_update,
+   qmc/dete_mod.f90:114-214).  0 (default): only at each step's full
+   evaluation, as CHAMP's hpsi does (deriv_fast_mod.f90:367-398).
+   qmcw_reinvert does it once, now. */
+int qmcw_set_reinversion(qmcw_handle *h, int32_t every);
+int qmcw_reinvert(qmcw_handle *h);
+
 /* The mover of qmcw_vmc_run / qmcw_vmc_run_device (CHAMP's imetro,
    qmc/control_mod.f90:78, 374-392): 1, metrop_mov1's box proposal
    (qmc/vmc/metrop_mov1.f90:63-99, the handle's default); 6, CHAMP's

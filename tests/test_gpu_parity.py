@@ -212,3 +212,37 @@ def test_new_parameters_are_used_at_unchanged_positions():
     assert _close(g["logpsid"], o["out9"][:, 0]) < RTOL
     assert _close(g["eloc"], o["out9"][:, 3], atol=1e-9) < 1e-9
     e.close()
+
+
+@pytest.mark.parametrize("name,every", [("c2", 3), ("chain4", 5)])
+def test_periodic_reinversion_keeps_the_walk(name, every):
+    """qmcw_set_reinversion: the Slater inverses rebuilt from the kept rows
+    every few moves within the sweeps (host loop and graph-captured device
+    loop) change them only by the Sherman-Morrison rounding, so decisions,
+    positions and streams still equal the oracle's (which never re-inverts
+    within a sweep) and E_L agrees to 1e-9."""
+    s = SYS[name]
+    W = 24
+    x = system.random_walkers(s, W, seed=13)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    o = oracle.Oracle(s).vmc_run(x, seeds, 3, 0.6)
+    for device in (False, True):
+        e = Engine(s, W)
+        e.set_walkers(x)
+        e.set_rng(seeds)
+        e.set_reinversion(every)
+        if device:
+            e.psi_at()
+            e.vmc_device(3, 0.6)
+            e.sync()
+            assert _close(e.get_eloc(), o["eloc"][-1], atol=1e-9) < 1e-9
+        else:
+            g = e.vmc(3, delta=0.6)
+            assert np.array_equal(g["naccept"], o["naccept"])
+            assert _close(g["eloc"], o["eloc"], atol=1e-9) < 1e-9
+        assert np.array_equal(e.get_walkers(), o["x"])
+        assert np.array_equal(e.get_rng(), o["rng"])
+        e.reinvert()
+        g2 = e.psi_at()
+        assert _close(g2["eloc"], o["eloc"][-1], atol=1e-9) < 1e-9
+        e.close()
