@@ -94,6 +94,7 @@ struct qmcw_handle {
   int *dunder = nullptr, *dsj = nullptr;  // its hole stack; results [4] and branching stream [4]
   int npow2 = 1;
   int host_split_join = 0;    // QMCW_HOST_SPLIT_JOIN=1: split_join on the host
+  int no_gproj_mfma = 0;      // QMCW_NO_GPROJ_MFMA=1: the projection on VALU in k_full_det
   int32_t brng[4] = {0, 0, 0, 1};
   // event timing of the device driver
   int prof = 0;
@@ -542,6 +543,26 @@ static void launch_derivs_asm(qmcw_handle *h, const KArgs &A) {
 // Full evaluation of every walker; with derivs also the parameter
 // derivatives into h->der (all orbitals, Jastrow slots, pseudopotential
 // pieces, assembly).  Each kernel class runs at its own lane-group width.
+// k_full_det stages A and the occupied-orbital values of large determinants
+// (at least 16 electrons of a spin) in LDS for the inversion and T, after
+// the basis values the launch may keep there, when a block's share stays
+// within 64 KB.  Small determinants keep the global path, which measured
+// faster for the 4 x 4 matrices of the C2-like system.
+static void det_lds_inv(qmcw_handle *h, KArgs &A, size_t &smem) {
+  A.lds_inv = 0;
+  A.lds_inv_off = 0;
+  const int nsm = std::max(A.S.ns[0], A.S.ns[1]);
+  const int noccm = std::max(A.S.nocc[0], A.S.nocc[1]);
+  if (!h->use_lds || nsm < 16) return;
+  const size_t per = (size_t)nsm * nsm + (size_t)nsm * noccm;
+  const size_t off = (smem + 7) / 8;
+  const size_t tot = (off + (size_t)(kWave / h->Gk[KC_DET]) * per) * 8;
+  if (tot > 64 * 1024) return;
+  A.lds_inv = (int)per;
+  A.lds_inv_off = (int)off;
+  smem = tot;
+}
+
 static void launch_eval(qmcw_handle *h, bool derivs) {
   KArgs A = h->A;
   A.full_orb = derivs ? 1 : 0;
@@ -567,6 +588,7 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     A.lds_det = 1;
     det_smem = per_e;
   }
+  det_lds_inv(h, A, det_smem);
   const OrdKernels &K = ord_kernels(A.S.nordc);
   const LaunchCtx cj = make_ctx(h, A, h->Gk[KC_JAS]);
   // k_full_nlpp's per-lane point scratch (4M distances, N values of U) in
@@ -599,8 +621,12 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     K.deriv_jas(cj);
     prof_end(h);
   }
+  // the pseudopotential projection of large determinants on FP64 MFMA
+  A.gproj_mfma = (A.S.nloc > 0 && std::max(A.S.ns[0], A.S.ns[1]) >= 16 && !h->no_gproj_mfma) ? 1 : 0;
   prof_begin(h, QMCW_PROF_FULL_DET);
   DISPATCH_GW(gd, launch_det, h, A, det_smem);
+  if (A.gproj_mfma)
+    hipLaunchKernelGGL(k_gproj_mfma, dim3(h->Wl, (A.S.NB + 63) / 64, 2), dim3(64), 0, h->stream, A);
   prof_end(h);
   if (A.S.nloc > 0) {
     prof_begin(h, QMCW_PROF_FULL_NLPP);
@@ -1088,6 +1114,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_NO_KEEP_ORB")) h->no_keep_orb = atoi(e) != 0;
   if (const char *e = getenv("QMCW_DMC_UNFUSED")) h->dmc_unfused = atoi(e) != 0;
   if (const char *e = getenv("QMCW_HOST_SPLIT_JOIN")) h->host_split_join = atoi(e) != 0;
+  if (const char *e = getenv("QMCW_NO_GPROJ_MFMA")) h->no_gproj_mfma = atoi(e) != 0;
   {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1365,8 +1392,11 @@ static void launch_reinvert(qmcw_handle *h) {
   A.keep_state = 1;
   A.lds_det = 0;
   A.chunk = 1;
+  A.gproj_mfma = 1;     // no projection: the full evaluation that follows forms it
+  size_t smem = 0;
+  det_lds_inv(h, A, smem);
   prof_begin(h, QMCW_PROF_FULL_DET);
-  DISPATCH_GW(h->Gk[KC_DET], launch_det, h, A, (size_t)0);
+  DISPATCH_GW(h->Gk[KC_DET], launch_det, h, A, smem);
   prof_end(h);
   h->moves_since_inv = 0;
 }

@@ -32,6 +32,10 @@ struct KArgs {
   int lds_dnlpp;     // doubles per lane of k_deriv_nlpp's point scratch in LDS (0: global)
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
+  int gproj_mfma;    // k_full_det leaves the pseudopotential projection out (k_gproj_mfma
+                     // forms it for large determinants; a re-inversion needs none)
+  int lds_inv;       // k_full_det: doubles per group of LDS for A and the orbital values (0: global)
+  int lds_inv_off;   // their offset (doubles) in the dynamic LDS
   int dmc_rot;       // DMC evaluation: pseudopotential grid rotated per pair in range
   const double *dmove;  // DMC: nonzero at dmove[w * dmove_stride] if the walker moved
   int dmove_stride;     // (null: every walker evaluates and draws)
@@ -172,6 +176,60 @@ __device__ double group_matinv(double *a, double *tmp, int n, int lane, int *ipi
     a[r + ipiv[c] * n] = tmp[r + c * n];
   }
   __syncthreads();
+  return singular ? 0.0 : det;
+}
+
+// group_matinv's elimination on a matrix staged in LDS, without the final
+// permutation: la (column-major n x n) ends as the eliminated matrix whose
+// row ipiv[r] and column c hold the inverse's element (r, ipiv[c]).  The
+// operations and their order are group_matinv's, so the values are the
+// same.  Returns det (0 and *singular set for a singular matrix).
+template <int G>
+__device__ double group_gj_lds(double *la, int n, int lane, int *ipiv, bool *singular_out) {
+  for (int i = lane; i < n; i += G) ipiv[i] = i;
+  __syncthreads();
+  double det = 1.0;
+  bool singular = false;
+  for (int i = 0; i < n; i++) {
+    const int col = i * n;
+    double am = -1.0, vm = 0.0;
+    int km = n;
+    for (int k = i + lane; k < n; k += G) {
+      double v = la[ipiv[k] + col];
+      if (fabs(v) > am) { am = fabs(v); vm = v; km = k; }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      double a2 = __shfl_xor(am, o, G);
+      int k2 = __shfl_xor(km, o, G);
+      double v2 = __shfl_xor(vm, o, G);
+      if (a2 > am || (a2 == am && k2 < km)) { am = a2; km = k2; vm = v2; }
+    }
+    __syncthreads();
+    if (!singular) {
+      if (lane == 0 && km != i) { int t = ipiv[i]; ipiv[i] = ipiv[km]; ipiv[km] = t; }
+      if (km != i) det = -det;
+      det *= vm;
+      if (vm == 0.0) singular = true;
+    }
+    __syncthreads();
+    const int p = ipiv[i];
+    const double inv = singular ? 0.0 : 1.0 / vm;
+    if (!singular && lane == 0) la[p + col] = 1.0;
+    __syncthreads();
+    if (!singular)
+      for (int q = lane; q < n; q += G) la[p + q * n] *= inv;
+    __syncthreads();
+    if (!singular)
+      for (int j = lane; j < n; j += G) {
+        if (j == p) continue;
+        double t = -la[j + col];
+        la[j + col] = 0.0;
+        for (int q = 0; q < n; q++) la[j + q * n] += t * la[p + q * n];
+      }
+    __syncthreads();
+  }
+  *singular_out = singular;
   return singular ? 0.0 : det;
 }
 
@@ -364,6 +422,16 @@ __device__ void group_g(const DevSys &S, int s, const double *ai,
                         double *tmp, int lane) {
   int ns = S.ns[s], nocc = S.nocc[s];
   if (ns == 0) return;
+  if (S.ndetu[s] == 1) {
+    // one determinant of this spin: Y = 0, so G is A^-1 in the rows of the
+    // reference orbitals (the products below would add exact zeros)
+    for (int e = lane; e < ns * ns; e += G) {
+      int k = e / ns, i = e % ns;
+      g[cst(S.iref[s])[k] * ns + i] = ai[k + ns * i];
+    }
+    __syncthreads();
+    return;
+  }
   for (int e = lane; e < nocc * ns; e += G) {
     int j = e / ns, i = e % ns;
     double acc = 0.0;
@@ -523,6 +591,7 @@ void k_full_det(KArgs A) {
   (void)NO; (void)x; (void)rven; (void)ren; (void)rvee; (void)ree; (void)fso; (void)fijo;
   (void)lapj; (void)fjo; (void)vj; (void)divv; (void)vd; (void)red;
   __shared__ int s_ipiv[kWave / G][128];
+  __shared__ int s_ipinv[kWave / G][128];
   // ---- orbitals (basis_fns + orbitals_loc_ana), chunks of electrons; the
   // chunk's basis values in LDS when the launch reserved it
   extern __shared__ double dyn_lds[];
@@ -548,6 +617,42 @@ void k_full_det(KArgs A) {
       __syncthreads();
       continue;
     }
+    double *tocc = ws + L.tocc[s];
+    if (A.lds_inv) {
+      // A and the occupied-orbital values in LDS: the elimination and
+      // T = A^-1 orb read them there (large determinants)
+      double *la = dyn_lds + A.lds_inv_off + (int64_t)grp * A.lds_inv;
+      double *lo = la + ns * ns;
+      int *ipv = s_ipiv[grp], *ipinv = s_ipinv[grp];
+      for (int e = lane; e < ns * ns; e += G) {
+        int i = e % ns, k = e / ns;
+        la[i + ns * k] = oc[((int64_t)(off + i) * noccmax + cst(S.iref[s])[k]) * 5];
+      }
+      for (int e = lane; e < ns * nocc; e += G) {
+        int i = e % ns, j = e / ns;
+        lo[e] = oc[((int64_t)(off + i) * noccmax + j) * 5];
+      }
+      __syncthreads();
+      bool singular;
+      double det = group_gj_lds<G>(la, ns, lane, ipv, &singular);
+      if (lane == 0) ws[L.deta[s]] = det;
+      for (int i = lane; i < ns; i += G) ipinv[ipv[i]] = i;
+      __syncthreads();
+      // the inverse: ai(r, ipiv[c]) = la(ipiv[r], c)
+      for (int q = lane; q < ns * ns; q += G) {
+        int r = q % ns, c = q / ns;
+        ai[r + ipv[c] * ns] = singular ? 0.0 : la[ipv[r] + c * ns];
+      }
+      // T(k, occ_j) = sum_i Ainv(k, i) orb(i, occ_j), Ainv(k, i) = la(ipiv[k], ipinv[i])
+      for (int e = lane; e < ns * nocc; e += G) {
+        int k = e / nocc, j = e % nocc;
+        double acc = 0.0;
+        if (!singular)
+          for (int i = 0; i < ns; i++) acc += la[ipv[k] + ns * ipinv[i]] * lo[i + ns * j];
+        tocc[e] = acc;
+      }
+      __syncthreads();
+    } else {
     // A(i, k) = orb(i, ref_k): position of ref_k in occ is iref[k]
     for (int e = lane; e < ns * ns; e += G) {
       int i = e % ns, k = e / ns;
@@ -557,7 +662,6 @@ void k_full_det(KArgs A) {
     double det = group_matinv<G>(ai, tmp, ns, lane, s_ipiv[grp]);
     if (lane == 0) ws[L.deta[s]] = det;
     // T(k, occ_j) = sum_i Ainv(k, i) orb(i, occ_j)
-    double *tocc = ws + L.tocc[s];
     for (int e = lane; e < ns * nocc; e += G) {
       int k = e / nocc, j = e % nocc;
       double acc = 0.0;
@@ -566,6 +670,7 @@ void k_full_det(KArgs A) {
       tocc[e] = acc;
     }
     __syncthreads();
+    }
     // alpha blocks of the unique dets
     double *inv = ws + L.inv[s], *dv = ws + L.detv[s];
     for (int u = lane; u < S.ndetu[s]; u += G) {
@@ -608,8 +713,70 @@ void k_full_det(KArgs A) {
   d2d = gsum<G>(d2d);
   if (lane == 0) red[54] = d2d;
   // projected coefficients gp_i = G(:,i)^T coef_occ of the pseudopotential
-  if (S.nloc > 0) group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
+  // (k_gproj_mfma instead, after this kernel, for large determinants)
+  if (S.nloc > 0 && !A.gproj_mfma) group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
 }
+
+#ifndef QMCW_ORD   // launched from the host translation unit only
+typedef double gp_d4 __attribute__((ext_vector_type(4)));
+
+// Projected coefficients of the pseudopotential on FP64 MFMA: for walker
+// blockIdx.x and spin blockIdx.z, gp[i][b] = sum_j G(j, i) coef_occ(b, j)
+// for the spin's electrons i and the basis functions b of column block
+// blockIdx.y (64 columns), i.e. [ns x nocc] x [nocc x 64] products in
+// v_mfma_f64_16x16x4f64 tiles, 16 electrons by 16 functions, k over the
+// occupied orbitals four at a time.  One wavefront per block.  The sums
+// run in the MFMA's order, not group_gproj's.
+__global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A) {
+  const DevSys &S = A.S;
+  const int w = blockIdx.x, cb = blockIdx.y, sp = blockIdx.z;
+  const int ns = S.ns[sp], nocc = S.nocc[sp], NB = S.NB, off = sp ? S.nup : 0;
+  if (ns == 0) return;
+  const double *ss = A.sc + (int64_t)w * A.SL.stride;
+  const double *__restrict__ g = ss + A.SL.g[sp];        // G(j, i) at g[j * ns + i]
+  double *__restrict__ gp = const_cast<double *>(ss) + A.SL.gproj;
+  const double *__restrict__ co = S.coef_occ[sp];        // coef(b, occ_j) at co[b * nocc + j]
+  // operand layout: lane l supplies A(row l % 16, k = l / 16) and B(k = l / 16,
+  // column l % 16); it receives D rows l / 16 + 4 q (q < 4) of column l % 16
+  const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
+  const int b0 = cb * 64;
+  for (int rt0 = 0; rt0 < ns; rt0 += 64) {              // up to 64 electrons per pass
+    gp_d4 acc[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc[t][u] = gp_d4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nocc; k0 += 4) {
+      const int kk = k0 + kq;
+      double a[4], bb[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int i = rt0 + 16 * t + r;
+        a[t] = (i < ns && kk < nocc) ? g[kk * ns + i] : 0.0;
+        const int b = b0 + 16 * t + r;
+        bb[t] = (b < NB && kk < nocc) ? co[(int64_t)b * nocc + kk] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bb[u], acc[t][u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int i = rt0 + 16 * t + kq + 4 * q, b = b0 + 16 * u + r;
+          if (i < ns && b < NB) {
+            const double v = acc[t][u][q];
+            gp[(int64_t)(off + i) * NB + b] = A.bf16 ? (double)bf16r((float)v) : v;
+          }
+        }
+  }
+}
+#endif
 
 // rotqua (qmc/vmc/rotqua.f90): a random rotation drawn from the stream st4,
 // as the axes (x1, x2, x3), (y1, y2, y3), (z1, z2, z3) that take the

@@ -40,3 +40,22 @@ def test_chain98_psi_at_and_vmc_step(G):
     assert np.array_equal(v["naccept"], o["naccept"])
     assert np.array_equal(e.get_walkers(), o["x"])
     assert _rel(v["eloc"], o["eloc"], 1e-8) < 1e-9
+
+
+def test_chain98_projection_on_mfma_matches_valu(monkeypatch):
+    """The pseudopotential projection gp = G^T coef_occ of the chain's
+    49 x 49 determinants on FP64 MFMA (k_gproj_mfma, the default for
+    determinants of 16 or more electrons) against the VALU loop of
+    k_full_det (QMCW_NO_GPROJ_MFMA=1): the local energies (whose nonlocal
+    part is linear in gp) agree to 1e-12 relative."""
+    W = 5
+    x = system.random_walkers(S, W, seed=42)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("QMCW_NO_GPROJ_MFMA", flag)
+        e = Engine(S, W)
+        e.set_walkers(x)
+        out[flag] = e.psi_at()
+        e.close()
+    assert _rel(out["0"]["eloc"], out["1"]["eloc"], 1e-8) < 1e-12
+    assert np.array_equal(out["0"]["logpsid"], out["1"]["logpsid"])
