@@ -67,6 +67,9 @@ class QmcwSystem(C.Structure):
         ("nparmorb", C.c_int32), ("orb_parm_i", _pi), ("orb_parm_j", _pi),
         ("orb_parm_rev", _pi),
         ("hb", C.c_double),
+        ("rad_num", _pi), ("nnum", C.c_int32), ("num_igrid", _pi), ("num_nr", _pi),
+        ("num_h", _pd), ("num_r0", _pd), ("num_off", _pi), ("num_rwf", _pd), ("num_d2", _pd),
+        ("num_ce", _pd), ("num_ae", _pd),
     ]
 
 

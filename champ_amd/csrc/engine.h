@@ -15,6 +15,10 @@ constexpr int kMaxCtype = 16;
 constexpr int kMaxTerms = 64;
 constexpr int kMaxKmax = 8;        // excitation order of an AMF alpha block
 constexpr int kMaxNs = 512;        // electrons of one spin
+// per tabulated radial function: igrid, nr, r0, 1/h (igrid 1) or 1/ln h,
+// ce[5] (small-r polynomial), ae[2] (tail a0 exp(-a1 r)), offset of its
+// grid points in num_tab
+constexpr int kNumPar = 12;
 
 // Flattened system in device memory.  Pointers are device pointers.
 struct DevSys {
@@ -32,6 +36,12 @@ struct DevSys {
   const int *sh_m;           // [nshell][9]
   const int *sh_bas;         // [nshell][9] basis index of each component
   const double *prim_zeta, *prim_coef;
+  // tabulated radial functions: a shell with sh_num[s] = k >= 0 has the
+  // radial part R_k(r) r^-l (R_k tabulated) in place of r^a g(r)
+  int has_num;               // any tabulated shell (sh_num valid)
+  const int *sh_num;         // [nshell] table or -1
+  const double *num_par;     // [nnum][kNumPar]
+  const double *num_tab;     // [sum of nr][4] grid point x, R, R'' (spline), 0
   int maxshell_per_cent;
   const double *coefT;       // [NB][NO] (normalisation folded in)
   const double *coef_occ[2]; // [NB][nocc] coefT of the occupied orbitals of each spin

@@ -437,6 +437,15 @@ static int set_asymp(qmcw_handle *h, const qmcw_system *s, DevSys &S) {
   return QMCW_OK;
 }
 
+// 6-point octahedral rule: +-x, +-y, +-z, weights 1/6 (qmc/vmc/rotqua.f90:396-428)
+static void gesqua6(double *q) {
+  double *x = q, *y = q + 6, *z = q + 12, *w = q + 18;
+  for (int j = 0; j < 6; j++) { x[j] = y[j] = z[j] = 0; w[j] = 1.0 / 6.0; }
+  x[0] = 1.0; x[1] = -1.0;
+  y[2] = 1.0; y[3] = -1.0;
+  z[4] = 1.0; z[5] = -1.0;
+}
+
 // 12-point icosahedral rule with CHAMP's value of pi (qmc/vmc/rotqua.f90:300-500)
 static void gesqua12(double *q) {
   const double pi = 3.14159265359;
@@ -717,8 +726,19 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (s->isc != 2 && s->isc != 4) return fail(nullptr, QMCW_ERR_ARG, "isc must be 2 or 4");
   if (s->nordc > 7 || s->norda > QMCW_MAX_ORD || s->nordb > QMCW_MAX_ORD)
     return fail(nullptr, QMCW_ERR_ARG, "Jastrow order too large (nordc <= 7)");
-  if (s->nloc != 0 && (s->nloc != 6 || s->nquad != 12))
-    return fail(nullptr, QMCW_ERR_ARG, "pseudopotential must be nloc=6 with nquad=12");
+  if (s->nloc != 0 && (s->nloc != 6 || (s->nquad != 12 && s->nquad != 6)))
+    return fail(nullptr, QMCW_ERR_ARG, "pseudopotential must be nloc=6 with nquad=6 or 12");
+  if (s->rad_num && s->nnum > 0) {
+    for (int k = 0; k < s->nnum; k++) {
+      const int ig = s->num_igrid[k], nr = s->num_nr[k];
+      if ((ig != 1 && ig != 2 && ig != 3) || nr < 5 || s->num_off[k + 1] - s->num_off[k] != nr ||
+          !(s->num_h[k] > 0) || (ig != 1 && !(s->num_h[k] > 1)) || (ig != 1 && !(s->num_r0[k] > 0)))
+        return fail(nullptr, QMCW_ERR_ARG, "invalid tabulated radial function (grid)");
+    }
+    for (int r = 0; r < s->nrad; r++)
+      if (s->rad_num[r] >= s->nnum)
+        return fail(nullptr, QMCW_ERR_ARG, "rad_num names a missing tabulated function");
+  }
   if (precision < QMCW_FP64 || precision > QMCW_MIXED_BF16)
     return fail(nullptr, QMCW_ERR_ARG, "unknown precision mode");
   int nup = s->nup, ndn = s->nelec - s->nup;
@@ -756,7 +776,9 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   // ---- basis shells: group each type's functions by (radial function
   // content, l); the components of one shell share the radial part
   std::vector<int> sh_cent, sh_l, sh_a, sh_nm, sh_prim_off{0}, sh_m, sh_bas, cent_sh_off{0};
+  std::vector<int> sh_num;   // tabulated radial function of the shell or -1
   std::vector<double> pz, pc;
+  const bool has_num = s->rad_num && s->nnum > 0;
   int nbas = 0, maxsh = 0;
   for (int ic = 0; ic < s->ncent; ic++) {
     int t = s->iwctype[ic];
@@ -765,20 +787,23 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     for (int q = b0; q < b1; q++) {
       int r = s->bas_rad[q], l = s->bas_l[q], m = s->bas_m[q];
       int n = s->rad_n[r];
+      const int kn = has_num ? s->rad_num[r] : -1;
       if (l < 0 || l > QMCW_MAX_L || m < -l || m > l) {
         delete h;
         return fail(nullptr, QMCW_ERR_ARG, "basis function with l > 4 or |m| > l");
       }
+      if (kn >= 0) n = l + 1;   // tabulated R(r) Y_lm: radial part R r^-l, no r^a factor
       if (n - 1 - l < 0) {
         delete h;
         return fail(nullptr, QMCW_ERR_ARG, "basis function with n < l + 1");
       }
       int p0 = s->rad_prim_off[r], p1 = s->rad_prim_off[r + 1];
+      if (kn >= 0) p1 = p0;
       // slot of m in a complete shell: m = 0, 1, -1, 2, -2, ...
       const int slot = m > 0 ? 2 * m - 1 : (m < 0 ? -2 * m : 0);
       int found = -1;
       for (int sh = first_shell; sh < (int)sh_l.size() && found < 0; sh++) {
-        if (sh_l[sh] != l || sh_a[sh] != n - 1 - l) continue;
+        if (sh_l[sh] != l || sh_a[sh] != n - 1 - l || sh_num[sh] != kn) continue;
         if (sh_bas[sh * kMaxShellM + slot] >= 0) continue;
         int np = sh_prim_off[sh + 1] - sh_prim_off[sh];
         if (np != p1 - p0) continue;
@@ -790,6 +815,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
       }
       if (found < 0) {
         sh_cent.push_back(ic); sh_l.push_back(l); sh_a.push_back(n - 1 - l); sh_nm.push_back(0);
+        sh_num.push_back(kn);
         for (int p = p0; p < p1; p++) { pz.push_back(s->prim_zeta[p]); pc.push_back(s->prim_coef[p]); }
         sh_prim_off.push_back((int)pz.size());
         for (int qq = 0; qq < kMaxShellM; qq++) { sh_m.push_back(0); sh_bas.push_back(-1); }
@@ -811,6 +837,31 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   UP(sh_cent, S.sh_cent); UP(sh_l, S.sh_l); UP(sh_a, S.sh_a); UP(sh_nm, S.sh_nm);
   UP(sh_prim_off, S.sh_prim_off); UP(sh_m, S.sh_m); UP(sh_bas, S.sh_bas);
   UP(cent_sh_off, S.cent_sh_off); UP(pz, S.prim_zeta); UP(pc, S.prim_coef);
+  S.has_num = 0;
+  for (int v : sh_num) S.has_num |= (v >= 0);
+  if (S.has_num) {
+    // per table: igrid, nr, r0, 1/h or 1/ln h, ce[5], ae[2], offset; the grid
+    // points as qmc/vmc/splfit.f90:13-34 places them
+    std::vector<double> par((size_t)s->nnum * kNumPar), tab((size_t)4 * s->num_off[s->nnum]);
+    for (int k = 0; k < s->nnum; k++) {
+      const int ig = s->num_igrid[k], nr = s->num_nr[k], o = s->num_off[k];
+      const double hb = s->num_h[k], r0 = s->num_r0[k];
+      double *pk = par.data() + (size_t)k * kNumPar;
+      pk[0] = ig; pk[1] = nr; pk[2] = r0;
+      pk[3] = ig == 1 ? 1.0 / hb : 1.0 / std::log(hb);
+      for (int c = 0; c < 5; c++) pk[4 + c] = s->num_ce[5 * k + c];
+      pk[9] = s->num_ae[2 * k]; pk[10] = s->num_ae[2 * k + 1];
+      pk[11] = o;
+      for (int j = 0; j < nr; j++) {
+        double xj = ig == 1 ? r0 + hb * j : (ig == 2 ? r0 * std::pow(hb, j) : r0 * std::pow(hb, j) - r0);
+        tab[4 * (size_t)(o + j) + 0] = xj;
+        tab[4 * (size_t)(o + j) + 1] = s->num_rwf[o + j];
+        tab[4 * (size_t)(o + j) + 2] = s->num_d2[o + j];
+        tab[4 * (size_t)(o + j) + 3] = 0.0;
+      }
+    }
+    UP(sh_num, S.sh_num); UP(par, S.num_par); UP(tab, S.num_tab);
+  }
   h->coef.assign(s->coef, s->coef + (size_t)s->norb * s->nbasis);
   std::vector<double> coefT((size_t)nbas * s->norb);
   for (int o = 0; o < s->norb; o++)
@@ -922,9 +973,12 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   UP(klm, S.term_klm);
   // ---- pseudopotential
   S.nloc = s->nloc;
-  S.nquad = 12;
-  std::vector<double> q0(48);
-  gesqua12(q0.data());
+  S.nquad = (s->nloc > 0 && s->nquad == 6) ? 6 : 12;
+  std::vector<double> q0(4 * S.nquad);
+  if (S.nquad == 6)
+    gesqua6(q0.data());
+  else
+    gesqua12(q0.data());
   UP(q0, S.q0);
   if (s->nloc > 0) {
     std::vector<int> npotd(s->npotd, s->npotd + s->nctype), lpotp1(s->lpotp1, s->lpotp1 + s->nctype);

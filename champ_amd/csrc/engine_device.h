@@ -534,6 +534,64 @@ __device__ __forceinline__ void shell_write(T x, T y, T z, T f, T fr, T lapfac, 
   }
 }
 
+// Tabulated radial function k at r: R, and with want R' and R''
+// (qmc/vmc/splfit.f90:13-76).  xr is the 1-based grid coordinate; below the
+// first point the polynomial ce, at or beyond the last point the tail
+// a0 exp(-a1 r), else the cubic spline between points jx and jx+1, whose
+// positions are read from the table.  jx = int(xr) lies in [1, nr-1] on
+// the spline branch, so both points are inside the function's table.
+template <typename T>
+__device__ __forceinline__ void num_radial(const DevSys &S, int k, T r, int want, T &R, T &R1,
+                                           T &R2) {
+  const cptr<double> p = cst(S.num_par) + k * kNumPar;
+  const int igrid = (int)p[0], nr = (int)p[1];
+  const T r0 = T(p[2]), ih = T(p[3]);
+  T xr;
+  if (igrid == 1)
+    xr = (r - r0) * ih + T(1);
+  else if (igrid == 2)
+    xr = log(r / r0) * ih + T(1);
+  else
+    xr = log((r + r0) / r0) * ih + T(1);
+  if (!(xr >= T(1))) {
+    const T c1 = T(p[5]), c2 = T(p[6]), c3 = T(p[7]), c4 = T(p[8]);
+    R = T(p[4]) + r * (c1 + r * (c2 + r * (c3 + r * c4)));
+    R1 = c1 + r * (T(2) * c2 + r * (T(3) * c3 + r * T(4) * c4));
+    R2 = T(2) * c2 + r * (T(6) * c3 + r * T(12) * c4);
+    return;
+  }
+  if (xr >= T(nr)) {
+    const T a1 = T(p[10]);
+    R = T(p[9]) * exp(-a1 * r);
+    R1 = -a1 * R;
+    R2 = a1 * a1 * R;
+    return;
+  }
+  const int jx = (int)xr;
+  const cptr<double> t = cst(S.num_tab) + 4 * ((int64_t)p[11] + jx - 1);
+  const T ref0 = T(t[0]), y0 = T(t[1]), d0 = T(t[2]);
+  const T ref1 = T(t[4]), y1 = T(t[5]), d1 = T(t[6]);
+  const T delh = ref1 - ref0, idh = T(1) / delh;
+  const T bb = (r - ref0) * idh, aa = (ref1 - r) * idh;
+  const T h6 = delh * delh * T(1.0 / 6.0);
+  R = aa * y0 + bb * y1 + (aa * (aa * aa - T(1)) * d0 + bb * (bb * bb - T(1)) * d1) * h6;
+  if (want) {
+    R1 = (y1 - y0) * idh + ((T(3) * bb * bb - T(1)) * d1 - (T(3) * aa * aa - T(1)) * d0) * delh *
+                               T(1.0 / 6.0);
+    R2 = aa * d0 + bb * d1;
+  }
+}
+
+// Radial part f = R_k(r) r^-l of a tabulated shell (value only).
+template <typename T>
+__device__ __forceinline__ T num_radial_f(const DevSys &S, int k, int l, T r) {
+  T R, R1, R2;
+  num_radial<T>(S, k, r, 0, R, R1, R2);
+  const T ri = T(1) / r;
+  for (int q = 0; q < l; q++) R *= ri;
+  return R;
+}
+
 // Evaluate shell s at displacement (x, y, z) from its centre, distance r.
 // want: 0 value only, 1 value + gradient + laplacian.  ST is the stride of
 // phi per basis function (5: value, gradient, laplacian; 1: value only).
@@ -542,6 +600,33 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
                            T *phi) {
   int l = cst(S.sh_l)[s], a = cst(S.sh_a)[s];
   if (r == T(0)) { r = T(1e-30); x = T(1e-30); }
+  const int kn = S.has_num ? cst(S.sh_num)[s] : -1;
+  if (kn >= 0) {
+    // f = R r^-l: f' = (R' - l R / r) r^-l,
+    // f'' = (R'' - 2 l R' / r + l (l + 1) R / r^2) r^-l
+    T R, R1, R2;
+    num_radial<T>(S, kn, r, want, R, R1, R2);
+    const T ri = T(1) / r;
+    T ril = T(1);
+    for (int q = 0; q < l; q++) ril *= ri;
+    const T f = R * ril;
+    T fr = 0, lapfac = 0;
+    if (want) {
+      const T f1 = (R1 - T(l) * R * ri) * ril;
+      const T f2 = (R2 - T(2 * l) * R1 * ri + T(l * (l + 1)) * R * ri * ri) * ril;
+      fr = f1 * ri;
+      lapfac = f2 + T(2 * (l + 1)) * fr;
+    }
+    const cptr<int> bas = cst(S.sh_bas) + s * kMaxShellM;
+    switch (l) {
+      case 0: shell_write<T, 0, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+      case 1: shell_write<T, 1, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+      case 2: shell_write<T, 2, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+      case 3: shell_write<T, 3, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+      default: shell_write<T, 4, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    }
+    return;
+  }
   // radial f = r^a g(r), g = sum d exp(-z r^q); need f, f'/r, f''
   T g = 0, g1 = 0, g2 = 0;  // g, dg/dr, d2g/dr2
   const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
@@ -606,12 +691,17 @@ __device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const GP *gp)
   int l = cst(S.sh_l)[s], a = cst(S.sh_a)[s];
   const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
   T g = 0;
-  const cptr<double> pz = cst(S.prim_zeta), pc = cst(S.prim_coef);
-  for (int p = cst(S.sh_prim_off)[s]; p < cst(S.sh_prim_off)[s + 1]; p++) {
-    T zeta = T(pz[p]), d = T(pc[p]);
-    g += d * exp(gauss ? -zeta * r * r : -zeta * r);
+  const int kn = S.has_num ? cst(S.sh_num)[s] : -1;
+  if (kn >= 0) {
+    g = num_radial_f<T>(S, kn, l, r);
+  } else {
+    const cptr<double> pz = cst(S.prim_zeta), pc = cst(S.prim_coef);
+    for (int p = cst(S.sh_prim_off)[s]; p < cst(S.sh_prim_off)[s + 1]; p++) {
+      T zeta = T(pz[p]), d = T(pc[p]);
+      g += d * exp(gauss ? -zeta * r * r : -zeta * r);
+    }
+    for (int q = 0; q < a; q++) g *= r;
   }
-  for (int q = 0; q < a; q++) g *= r;
   const cptr<int> bas = cst(S.sh_bas) + s * kMaxShellM;
   T acc;
   switch (l) {
@@ -652,12 +742,17 @@ __device__ __forceinline__ void shell_val_axpy(const DevSys &S, int s, T x, T y,
   const int l = cst(S.sh_l)[s], a = cst(S.sh_a)[s];
   const bool gauss = (S.basis_type == QMCW_BASIS_GAUSSIAN);
   T g = 0;
-  const cptr<double> pz = cst(S.prim_zeta), pc = cst(S.prim_coef);
-  for (int p = cst(S.sh_prim_off)[s]; p < cst(S.sh_prim_off)[s + 1]; p++) {
-    const T zeta = T(pz[p]), d = T(pc[p]);
-    g += d * exp(gauss ? -zeta * r * r : -zeta * r);
+  const int kn = S.has_num ? cst(S.sh_num)[s] : -1;
+  if (kn >= 0) {
+    g = num_radial_f<T>(S, kn, l, r);
+  } else {
+    const cptr<double> pz = cst(S.prim_zeta), pc = cst(S.prim_coef);
+    for (int p = cst(S.sh_prim_off)[s]; p < cst(S.sh_prim_off)[s + 1]; p++) {
+      const T zeta = T(pz[p]), d = T(pc[p]);
+      g += d * exp(gauss ? -zeta * r * r : -zeta * r);
+    }
+    for (int q = 0; q < a; q++) g *= r;
   }
-  for (int q = 0; q < a; q++) g *= r;
   const cptr<int> bas = cst(S.sh_bas) + s * kMaxShellM;
   switch (l) {
     case 0: shell_axpy_l<T, 0>(x, y, z, g, bas, gp, fac, pb, rat); break;

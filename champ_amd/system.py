@@ -171,6 +171,18 @@ class System:
     csf_parm: list = field(default_factory=list)
     orb_parm: list = field(default_factory=list)  # [(i, j, rev)]
     hb: float = 0.5
+    # tabulated radial functions (numerical basis): rad_num[r] >= 0 selects
+    # table k for radial function r; the tables as in qmcw_system
+    rad_num: np.ndarray = None
+    num_igrid: np.ndarray = None
+    num_nr: np.ndarray = None
+    num_h: np.ndarray = None
+    num_r0: np.ndarray = None
+    num_off: np.ndarray = None
+    num_rwf: np.ndarray = None
+    num_d2: np.ndarray = None
+    num_ce: np.ndarray = None
+    num_ae: np.ndarray = None
     name: str = ""
     _keep: list = field(default_factory=list, repr=False)
 
@@ -292,6 +304,18 @@ class System:
         s.orb_parm_j = I("op_j", op[:, 1])
         s.orb_parm_rev = I("op_r", op[:, 2])
         s.hb = self.hb
+        if self.rad_num is not None and len(self.num_nr):
+            s.rad_num = I("rad_num", self.rad_num)
+            s.nnum = len(self.num_nr)
+            s.num_igrid = I("num_igrid", self.num_igrid)
+            s.num_nr = I("num_nr", self.num_nr)
+            s.num_h = D("num_h", self.num_h)
+            s.num_r0 = D("num_r0", self.num_r0)
+            s.num_off = I("num_off", self.num_off)
+            s.num_rwf = D("num_rwf", self.num_rwf)
+            s.num_d2 = D("num_d2", self.num_d2)
+            s.num_ce = D("num_ce", np.asarray(self.num_ce).reshape(-1))
+            s.num_ae = D("num_ae", np.asarray(self.num_ae).reshape(-1))
         s._arrays = arrs          # keep the buffers alive with the struct
         self._keep.append(arrs)
         return s
@@ -312,6 +336,8 @@ class _BasisBuilder:
         self.basis_type = basis_type
         self.rad_n, self.prim_zeta, self.prim_coef = [], [], []
         self.rad_off = [0]
+        self.rad_num = []                              # table index or -1
+        self.tables = []                               # numbas.RadialTable
         self.per_type = [[] for _ in range(nctype)]   # (rad, l, m, norm)
 
     def add_radial(self, n, zetas, coefs):
@@ -319,9 +345,26 @@ class _BasisBuilder:
         self.prim_zeta.extend(zetas)
         self.prim_coef.extend(coefs)
         self.rad_off.append(len(self.prim_zeta))
+        self.rad_num.append(-1)
+        return len(self.rad_n) - 1
+
+    def add_table(self, table):
+        """Register a tabulated radial function; returns its table index."""
+        self.tables.append(table)
+        return len(self.tables) - 1
+
+    def add_numeric_radial(self, k):
+        """A radial function that is table k (no primitives)."""
+        self.rad_n.append(0)
+        self.rad_off.append(len(self.prim_zeta))
+        self.rad_num.append(k)
         return len(self.rad_n) - 1
 
     def add_function(self, ictype, rad, l, m):
+        if self.rad_num[rad] >= 0:
+            # tabulated R(r) Y_lm: qmc/vmc/basis_norm.f90:38-39
+            self.per_type[ictype].append((rad, l, m, math.sqrt((2 * l + 1) / (4 * math.pi))))
+            return
         n = self.rad_n[rad]
         z = self.prim_zeta[self.rad_off[rad]:self.rad_off[rad + 1]]
         d = self.prim_coef[self.rad_off[rad]:self.rad_off[rad + 1]]
@@ -339,7 +382,18 @@ class _BasisBuilder:
         anorm = []
         for t in iwctype:
             anorm.extend(a for (_, _, _, a) in self.per_type[t])
-        return dict(basis_type=self.basis_type,
+        num = {}
+        if self.tables:
+            T = self.tables
+            num = dict(rad_num=np.array(self.rad_num, np.int32),
+                       num_igrid=np.array([t.igrid for t in T], np.int32),
+                       num_nr=np.array([t.nr for t in T], np.int32),
+                       num_h=np.array([t.h for t in T]), num_r0=np.array([t.r0 for t in T]),
+                       num_off=np.cumsum([0] + [t.nr for t in T]).astype(np.int32),
+                       num_rwf=np.concatenate([t.rwf for t in T]),
+                       num_d2=np.concatenate([t.d2 for t in T]),
+                       num_ce=np.array([t.ce for t in T]), num_ae=np.array([t.ae for t in T]))
+        return dict(basis_type=self.basis_type, **num,
                     rad_n=np.array(self.rad_n, np.int32),
                     rad_prim_off=np.array(self.rad_off, np.int32),
                     prim_zeta=np.array(self.prim_zeta, np.float64),
@@ -606,31 +660,103 @@ class _OldReader:
         return [int(v) for v in self.read(n)]
 
 
-def from_champ_old_input(text: str, name: str = "champ") -> System:
-    """Build a System from CHAMP's old line-oriented input (read_input.f90;
-    e.g. examples/c2_ktt_hfj_vmc_emin_j_opt3_lin.inp).
+# m of each value of the count record of numerical-basis inputs
+# (qmc/vmc/read_orb_loc.f90:241-245): s; p 1, -1, 0; d 0, 2, -2, 1, -1;
+# f 3, -3, 2, -2, 1, -1, 0; g 4, -4, ..., 0; h 5, -5, ..., 0.  The basis
+# functions are listed in the same order (orb_loc_ana_original_order,
+# read_orb_loc.f90:449-840).
+_NUMR1_LM = ([(0, 0)] + [(1, m) for m in (1, -1, 0)] + [(2, m) for m in (0, 2, -2, 1, -1)] +
+             [(l, m) for l in (3, 4, 5) for m in
+              [v for mm in range(l, 0, -1) for v in (mm, -mm)] + [0]])
 
-    Supported: finite all-electron systems (nloc = 0), analytic Slater basis
-    (ibasis = 1, positive counts) of s and p functions in the order of
+
+def read_gauss_pseudopotential(text: str, lpotp1: int):
+    """One pseudopot<ict> file in the format of qmc/vmc/readps_gauss.f90:
+    a label line, npotd, then npotd blocks `nterms` + nterms rows `coef power
+    exponent`; the first block is the local component (stored at lpotp1),
+    block l >= 2 is component l - 1.  Returns (npotd, lpotp1, components),
+    components[i] the (coef, power, exponent) terms of CHAMP's component
+    i + 1 (an l = i nonlocal channel unless i + 1 == lpotp1)."""
+    rows = [ln.split() for ln in text.splitlines()[1:] if ln.strip()]
+    npotd = int(rows[0][0])
+    if lpotp1 < 0:
+        lpotp1 = npotd
+    if lpotp1 != npotd:
+        # the file lists the local component first and the l = 0, 1, ...
+        # channels after it, which fills components 1..npotd only when the
+        # local one is the last (readps_gauss.f90:46-53)
+        raise ValueError("the local component must be the last one (lpotp1 = npotd)")
+    if npotd > QMCW_MAX_LPOT:
+        raise ValueError(f"more than {QMCW_MAX_LPOT} pseudopotential components")
+    comps = [[] for _ in range(npotd)]
+    i = 1
+    for l in range(1, npotd + 1):
+        idx = lpotp1 if l == 1 else l - 1
+        nt = int(rows[i][0])
+        i += 1
+        for _ in range(nt):
+            c, pw, ex = rows[i][:3]
+            comps[idx - 1].append((float(c.lower().replace("d", "e")), int(pw),
+                                   float(ex.lower().replace("d", "e"))))
+            i += 1
+    return npotd, lpotp1, comps
+
+
+def _gauss_ecp_arrays(per_type):
+    """per_type: (npotd, lpotp1, components) of each centre type."""
+    npotd, lpotp1, off, coef, power, expo = [], [], [0], [], [], []
+    for (nd, lp, comps) in per_type:
+        npotd.append(nd)
+        lpotp1.append(lp)
+        for i in range(QMCW_MAX_LPOT):
+            for (cf, pw, ex) in (comps[i] if i < nd else []):
+                coef.append(cf)
+                power.append(pw)
+                expo.append(ex)
+            off.append(len(coef))
+    return (np.array(npotd, np.int32), np.array(lpotp1, np.int32),
+            np.array(off, np.int32), np.array(coef), np.array(power, np.int32),
+            np.array(expo))
+
+
+def from_champ_old_input(text: str, name: str = "champ", files: dict | None = None) -> System:
+    """Build a System from CHAMP's old line-oriented input (read_input.f90;
+    e.g. examples/c2_ktt_hfj_vmc_emin_j_opt3_lin.inp and
+    examples/c2_bfd_vtz_rhf_vmc.inp).
+
+    Supported: finite systems, all-electron (nloc = 0) or with Gaussian
+    pseudopotentials (nloc = 6, readps_gauss files pseudopot<ict>, nquad 6
+    or 12); the analytic Slater basis of s and p functions in the order of
     orb_loc_ana_gamess_order (numr = -1: all s by n, then the px, py and pz
-    blocks by n; qmc/vmc/read_orb_loc.f90:843-1000), orbitals given by
-    coefficients, determinants/CSFs, ijas = 4 Jastrow (isc = 2 or 4) and,
-    for one centre type, the Jastrow parameters the optimisation section
-    varies (iwjasa, iwjasb, iwjasc).  The dependent e-e-n coefficients are
-    re-imposed with cuspexact4.
+    blocks by n; qmc/vmc/read_orb_loc.f90:843-1000) or numerical radial
+    functions (numr = 1: files basis.<ict>, count record and function order
+    of orb_loc_ana_original_order, functions up to g, the radial function of
+    each from the iwrwf record); orbitals given by coefficients,
+    determinants/CSFs, ijas = 4 Jastrow (isc = 2 or 4) and, for one centre
+    type, the Jastrow parameters the optimisation section varies (iwjasa,
+    iwjasb, iwjasc).  The dependent e-e-n coefficients are re-imposed with
+    cuspexact4.  ``files`` maps file names (basis.1, pseudopot1, ...) to
+    their text.
     """
+    from . import numbas
+    files = files or {}
     R = _OldReader(text.splitlines())
     R.seek("iperiodic,ibasis")
     R.i -= 1
     iperiodic, ibasis = R.ints(2)
     if iperiodic != 0 or ibasis != 1:
-        raise ValueError("finite systems with a localized analytic basis (iperiodic 0, ibasis 1) only")
+        raise ValueError("finite systems with a localized basis (iperiodic 0, ibasis 1) only")
     R.seek("nfprod,tau")
     nloc, numr = R.ints(4)[:2]
-    if nloc != 0:
-        raise ValueError("pseudopotential inputs are built with the synthetic builders")
-    if numr != -1:
-        raise ValueError("numr = -1 (analytic basis in the GAMESS order) only")
+    if nloc not in (0, 6):
+        raise ValueError("nloc = 0 (all-electron) or 6 (Gaussian pseudopotential) only")
+    if numr not in (-1, 1):
+        raise ValueError("numr = -1 (analytic basis in the GAMESS order) or 1 (numerical) only")
+    nquad = 12
+    if nloc == 6:
+        nquad = R.ints(1)[0]
+        if nquad not in (6, 12):
+            raise ValueError("nquad = 6 or 12 only")
     nelec, nup = R.ints(2)
     R.seek("Geometry section")
     if R.ints(1)[0] != 3:
@@ -638,13 +764,25 @@ def from_champ_old_input(text: str, name: str = "champ") -> System:
     nctype, ncent = R.ints(2)
     iwctype = np.array(R.ints(ncent), np.int32) - 1
     znuc = np.array(R.read(nctype))
+    lpotp1 = R.ints(nctype) if nloc > 0 else None
     cent = np.array([R.read(3) for _ in range(ncent)])
     R.seek("Determinantal section")
     R.read()                                          # inum_orb
     ndet, nbasis, norb = R.ints(3)
-    counts = []
+    funcs, iwrwf = [], []
     for _ in range(nctype):
         v = R.ints()
+        if numr == 1:
+            if len(v) < len(_NUMR1_LM):
+                raise ValueError("numerical-basis count record too short")
+            f = []
+            for (l, m), cnt in zip(_NUMR1_LM, v):
+                if cnt and l > 4:
+                    raise ValueError("basis functions with l > 4 are not supported")
+                f += [(0, l, m)] * abs(cnt)
+            funcs.append(f)
+            iwrwf.append([k - 1 for k in R.ints(len(f))])
+            continue
         if len(v) < 18:
             raise ValueError("basis count record too short")
         if any(c < 0 for c in v):
@@ -652,14 +790,8 @@ def from_champ_old_input(text: str, name: str = "champ") -> System:
         if any(v[9:14]) or any(v[18:]):
             raise ValueError("only s and p basis functions are supported in the old format")
         # n1s, n2s, n2p(x,y,z), n3s, n3p(x,y,z), n3d(5), n4s, n4p(x,y,z)
-        counts.append(dict(s={1: v[0], 2: v[1], 3: v[5], 4: v[14]},
-                           p={m: {2: v[2 + k], 3: v[6 + k], 4: v[15 + k]}
-                              for k, m in enumerate((1, -1, 0))}))
-    coef = np.array([R.read(nbasis) for _ in range(norb)])
-    zex = R.read(nbasis)
-    # functions (n, l, m) per centre type in the order of the coefficients
-    funcs = []
-    for cn in counts:
+        cn = dict(s={1: v[0], 2: v[1], 3: v[5], 4: v[14]},
+                  p={m: {2: v[2 + k], 3: v[6 + k], 4: v[15 + k]} for k, m in enumerate((1, -1, 0))})
         f = []
         for n in (1, 2, 3, 4):
             f += [(n, 0, 0)] * cn["s"][n]
@@ -667,6 +799,8 @@ def from_champ_old_input(text: str, name: str = "champ") -> System:
             for n in (2, 3, 4):
                 f += [(n, 1, m)] * cn["p"][m][n]
         funcs.append(f)
+    coef = np.array([R.read(nbasis) for _ in range(norb)])
+    zex = R.read(nbasis)
     if sum(len(funcs[t]) for t in iwctype) != nbasis:
         raise ValueError("basis counts do not add up to nbasis")
     bb = _BasisBuilder(QMCW_BASIS_SLATER, nctype)
@@ -680,6 +814,24 @@ def from_champ_old_input(text: str, name: str = "champ") -> System:
                 raise ValueError("centres of one type must share their exponents")
             continue
         first[t] = z
+        if numr == 1:
+            # numerical radial functions: zex 0 (read_orb_loc.f90:428-436),
+            # or 1 in inputs written before that convention
+            if any(zz not in (0.0, 1.0) for zz in z):
+                raise ValueError("mixed analytic and numerical radial functions are not supported")
+            fname = f"basis.{t + 1}"
+            if fname not in files:
+                raise ValueError(f"numerical basis file {fname} not given")
+            bf = numbas.read_basis_file(files[fname], nloc)
+            tabs = numbas.radial_tables(bf, nloc)
+            rad = {}
+            for (_, l, m), k in zip(funcs[t], iwrwf[t]):
+                if not 0 <= k < len(tabs):
+                    raise ValueError(f"{fname}: radial function {k + 1} not in the file")
+                if k not in rad:
+                    rad[k] = bb.add_numeric_radial(bb.add_table(tabs[k]))
+                bb.add_function(t, rad[k], l, m)
+            continue
         for (n, l, m), zz in zip(funcs[t], z):
             bb.add_function(t, bb.add_radial(n, [float(zz)], [1.0]), l, m)
     barr, anorm = bb.arrays(iwctype)
@@ -720,12 +872,23 @@ def from_champ_old_input(text: str, name: str = "champ") -> System:
     ja = [(0, i) for i in jas.get("iwjasa", [])] if nctype == 1 else []
     jb = [(0, i) for i in jas.get("iwjasb", [])] if nspin2b == 1 else []
     jc = [(0, i) for i in jas.get("iwjasc", [])] if nctype == 1 else []
+    psp = {}
+    if nloc == 6:
+        per = []
+        for t in range(nctype):
+            fname = f"pseudopot{t + 1}"
+            if fname not in files:
+                raise ValueError(f"pseudopotential file {fname} not given")
+            per.append(read_gauss_pseudopotential(files[fname], lpotp1[t]))
+        ecp = _gauss_ecp_arrays(per)
+        psp = dict(npotd=ecp[0], lpotp1=ecp[1], ecp_off=ecp[2], ecp_coef=ecp[3],
+                   ecp_power=ecp[4], ecp_expo=ecp[5], nquad=nquad)
     return System(nelec=nelec, nup=nup, cent=cent, iwctype=iwctype, znuc=znuc,
                   coef=coef, det_up=det_up, det_dn=det_dn, csf_coef=cc,
                   csf_det_off=off, csf_det=cdet, csf_det_coef=cdc, isc=isc,
                   nspin2b=nspin2b, norda=norda, nordb=nordb, nordc=nordc,
-                  scalek=scalek, a4=a4, b=b, c=c, nloc=0, jas_a=ja, jas_b=jb, jas_c=jc,
-                  name=name, **barr)
+                  scalek=scalek, a4=a4, b=b, c=c, nloc=nloc, jas_a=ja, jas_b=jb, jas_c=jc,
+                  name=name, **psp, **barr)
 
 
 def jastrow_opt_list(sysm: System, vary_a1: bool = False):

@@ -98,7 +98,7 @@ typedef struct qmcw_system {
   const double *ecp_coef;   /* [nterm] */
   const int32_t *ecp_power; /* [nterm] */
   const double *ecp_expo;   /* [nterm] */
-  int32_t nquad;            /* 12 (icosahedron) */
+  int32_t nquad;            /* 6 (octahedron) or 12 (icosahedron) */
   /* parameters whose derivatives are wanted (qmc/optim_mod, optimization menu) */
   int32_t nparma_tot;       /* Jastrow a's varied, over types */
   const int32_t *jas_a_type;/* [nparma_tot] centre type */
@@ -116,6 +116,25 @@ typedef struct qmcw_system {
   const int32_t *orb_parm_j;/* [nparmorb] second orbital label */
   const int32_t *orb_parm_rev; /* [nparmorb] 1: include the reverse excitation */
   double hb;                /* hbar^2/2m = 0.5 */
+  /* tabulated radial functions (qmc/vmc/read_bas_num.f90, splfit.f90):
+     rad_num[r] = k >= 0 makes radial function r the tabulated R_k(r), and
+     its basis functions R_k(r) Y_lm (Y_lm = r^-l S_lm, the solid harmonic
+     above; rad_n and the primitives of r are then unused); -1, or rad_num
+     NULL, keeps r analytic.  R_k between grid points is the cubic spline
+     through num_rwf with second derivatives num_d2; below the first
+     interval the polynomial num_ce[k][0..4] in r, beyond the grid
+     num_ae[k][0] exp(-num_ae[k][1] r). */
+  const int32_t *rad_num;   /* [nrad] or NULL */
+  int32_t nnum;             /* tabulated functions */
+  const int32_t *num_igrid; /* [nnum] grid: 1 linear, 2 exponential, 3 shifted exponential */
+  const int32_t *num_nr;    /* [nnum] points */
+  const double *num_h;      /* [nnum] exp_h_bas (the step for igrid 1) */
+  const double *num_r0;     /* [nnum] r0_bas as splfit_bas uses it */
+  const int32_t *num_off;   /* [nnum+1] offsets of each function's grid values */
+  const double *num_rwf;    /* values at the grid points */
+  const double *num_d2;     /* spline second derivatives at the grid points */
+  const double *num_ce;     /* [nnum][5] */
+  const double *num_ae;     /* [nnum][2] */
 } qmcw_system;
 
 /* sizeof(qmcw_system) as compiled into the library (ABI check) */

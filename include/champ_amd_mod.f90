@@ -43,6 +43,9 @@ module champ_amd_mod
     integer(c_int32_t) :: nparmorb
     type(c_ptr) :: orb_parm_i, orb_parm_j, orb_parm_rev
     real(c_double) :: hb
+    type(c_ptr) :: rad_num
+    integer(c_int32_t) :: nnum
+    type(c_ptr) :: num_igrid, num_nr, num_h, num_r0, num_off, num_rwf, num_d2, num_ce, num_ae
   end type qmcw_system
 
   ! struct qmcw_dmc_params: CHAMP's DMC controls of one generation

@@ -59,6 +59,7 @@ def lib():
             "orc_rannyu": (C.c_double, [_pi]),
             "orc_gauss": (C.c_double, [_pi]),
             "orc_quadrature": (None, [_vp, _pd, _pd, _pd, _pd]),
+            "orc_num_radial": (None, [_vp, C.c_int32, C.c_int32, _pd, _pd]),
             "orc_rotqua": (None, [_pi, _pd, _pd, _pd, C.c_int32, _pd, _pd, _pd]),
             "orc_matinv": (C.c_double, [_pd, C.c_int32]),
             "orc_basis": (None, [_vp, _pd, _pd, _pd, _pd]),
@@ -233,9 +234,18 @@ class Oracle:
         self.L.orc_walker_set_quad(walker or self.w, _d(xq), _d(yq), _d(zq), len(xq))
 
     def quadrature(self):
+        """The unrotated grid (x, y, z, w) of the nonlocal quadrature."""
         q = [np.zeros(12) for _ in range(4)]
         self.L.orc_quadrature(self.h, *[_d(a) for a in q])
-        return q
+        n = int((q[3] > 0).sum())
+        return [a[:n] for a in q]
+
+    def num_radial(self, k, r):
+        """(R, R', R'') of tabulated radial function k at the distances r, [n][3]."""
+        r = np.ascontiguousarray(r, np.float64)
+        out = np.zeros((len(r), 3))
+        self.L.orc_num_radial(self.h, k, len(r), _d(r), _d(out))
+        return out
 
     def cusp_move(self, tau, g, xe, rng=None, pt=None, n_drift=2):
         """The all-electron DMC mover at xe with gradient g: (xn or None,

@@ -38,6 +38,9 @@ struct orc_sys {
   int *iwctype;
   int *rad_n, *rad_prim_off, *ctype_bas_off, *bas_rad, *bas_l, *bas_m;
   double *prim_zeta, *prim_coef;
+  /* tabulated radial functions (rad_num[r] >= 0: table of radial fn r) */
+  int *rad_num, nnum, *num_igrid, *num_nr, *num_off;
+  double *num_h, *num_r0, *num_rwf, *num_d2, *num_ce, *num_ae;
   double *coef; /* [norb][nbasis] */
   /* dets */
   int ndet, ncsf;
@@ -450,6 +453,15 @@ void orc_det_info(const orc_sys *o, int32_t *info) {
   info[4] = o->kmax;
 }
 
+/* qmc/vmc/rotqua.f90:396-428 (gesqua, octahedron), the 6-point rule:
+   +-x, +-y, +-z with weights 1/6 */
+static void gesqua6(double *xq, double *yq, double *zq, double *wq) {
+  for (int j = 0; j < 6; j++) { xq[j] = yq[j] = zq[j] = 0; wq[j] = 1.0 / 6.0; }
+  xq[0] = 1.0; xq[1] = -1.0;
+  yq[2] = 1.0; yq[3] = -1.0;
+  zq[4] = 1.0; zq[5] = -1.0;
+}
+
 /* qmc/vmc/rotqua.f90:300-500 (gesqua), 12-point icosahedral rule with the
    reference's value of pi (3.14159265359d0, rotqua.f90:325) */
 static void gesqua12(double *xq, double *yq, double *zq, double *wq) {
@@ -513,6 +525,19 @@ orc_sys *orc_create(const qmcw_system *s) {
   int nprim = s->rad_prim_off[s->nrad];
   o->prim_zeta = DUPD(s->prim_zeta, nprim);
   o->prim_coef = DUPD(s->prim_coef, nprim);
+  if (s->rad_num && s->nnum > 0) {
+    o->rad_num = DUPI(s->rad_num, s->nrad);
+    o->nnum = s->nnum;
+    o->num_igrid = DUPI(s->num_igrid, s->nnum);
+    o->num_nr = DUPI(s->num_nr, s->nnum);
+    o->num_off = DUPI(s->num_off, s->nnum + 1);
+    o->num_h = DUPD(s->num_h, s->nnum);
+    o->num_r0 = DUPD(s->num_r0, s->nnum);
+    o->num_rwf = DUPD(s->num_rwf, s->num_off[s->nnum]);
+    o->num_d2 = DUPD(s->num_d2, s->num_off[s->nnum]);
+    o->num_ce = DUPD(s->num_ce, 5 * s->nnum);
+    o->num_ae = DUPD(s->num_ae, 2 * s->nnum);
+  }
   o->ctype_bas_off = DUPI(s->ctype_bas_off, s->nctype + 1);
   int nbt = s->ctype_bas_off[s->nctype];
   o->bas_rad = DUPI(s->bas_rad, nbt);
@@ -537,8 +562,8 @@ orc_sys *orc_create(const qmcw_system *s) {
   o->nloc = s->nloc;
   o->nquad = s->nloc > 0 ? s->nquad : 0;
   if (s->nloc > 0) {
-    if (s->nquad != 12) {
-      snprintf(g_err, sizeof g_err, "only nquad = 12 is supported");
+    if (s->nquad != 12 && s->nquad != 6) {
+      snprintf(g_err, sizeof g_err, "only nquad = 6 or 12 is supported");
       free(o);
       return NULL;
     }
@@ -551,7 +576,10 @@ orc_sys *orc_create(const qmcw_system *s) {
     o->ecp_coef = DUPD(s->ecp_coef, ntermecp);
     o->ecp_power = DUPI(s->ecp_power, ntermecp);
     o->ecp_expo = DUPD(s->ecp_expo, ntermecp);
-    gesqua12(o->xq0, o->yq0, o->zq0, o->wq);
+    if (s->nquad == 6)
+      gesqua6(o->xq0, o->yq0, o->zq0, o->wq);
+    else
+      gesqua12(o->xq0, o->yq0, o->zq0, o->wq);
   }
   o->npa = s->nparma_tot; o->npb = s->nparmb_tot; o->npc = s->nparmc_tot;
   o->pa_type = DUPI(s->jas_a_type, o->npa); o->pa_idx = DUPI(s->jas_a_idx, o->npa);
@@ -603,7 +631,9 @@ void orc_destroy(orc_sys *o) {
                   o->refup, o->refdn, o->excup, o->excdn, o->a4, o->b, o->c,
                   o->ecp_off, o->ecp_power, o->ecp_coef, o->ecp_expo,
                   o->pa_type, o->pa_idx, o->pb_spin, o->pb_idx, o->pc_type,
-                  o->pc_idx, o->pcsf, o->porb_i, o->porb_j, o->porb_rev};
+                  o->pc_idx, o->pcsf, o->porb_i, o->porb_j, o->porb_rev,
+                  o->rad_num, o->num_igrid, o->num_nr, o->num_off, o->num_h,
+                  o->num_r0, o->num_rwf, o->num_d2, o->num_ce, o->num_ae};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(o);
 }
@@ -824,9 +854,74 @@ static void ylm_fill(double xx, double yy, double zz, double r, ylm_tab *T) {
   dth[4][4][2] = th[4][4] * (-4 * ri2 * zz);
 }
 
+/* tabulated radial function k at r: value and first two derivatives
+   (qmc/vmc/splfit.f90:13-76).  The grid index jx = int(xr) decides the
+   piece: jx < 1 the small-r polynomial ce, jx >= nr the exponential tail
+   ae0 exp(-ae1 r), else the cubic spline on the 1-based points jx, jx+1. */
+static void num_radial(const orc_sys *o, int k, double r, double *w) {
+  const int igrid = o->num_igrid[k], nr = o->num_nr[k];
+  const double hb = o->num_h[k], r0 = o->num_r0[k];
+  double xr;
+  if (igrid == 1)
+    xr = (r - r0) / hb + 1;
+  else if (igrid == 2)
+    xr = log(r / r0) / log(hb) + 1;
+  else
+    xr = log((r + r0) / r0) / log(hb) + 1.0;
+  if (!(xr >= 1.0)) {
+    const double *ce = o->num_ce + 5 * k;
+    w[0] = ce[0];
+    w[1] = w[2] = 0;
+    for (int i = 2; i <= 5; i++) {
+      w[0] += ce[i - 1] * pow(r, i - 1);
+      w[1] += (i - 1) * ce[i - 1] * pow(r, i - 2);
+      w[2] += (i - 1) * (i - 2) * ce[i - 1] * pow(r, i - 3);
+    }
+    return;
+  }
+  if (xr >= nr) {
+    const double *ae = o->num_ae + 2 * k;
+    w[0] = ae[0] * exp(-ae[1] * r);
+    w[1] = -ae[1] * w[0];
+    w[2] = -ae[1] * w[1];
+    return;
+  }
+  const int jx = (int)xr;
+  double ref0, ref1, delh;
+  if (igrid == 1) {
+    ref0 = r0 + hb * (jx - 1);
+    ref1 = ref0 + hb;
+    delh = hb;
+  } else if (igrid == 2) {
+    ref0 = r0 * pow(hb, jx - 1);
+    ref1 = ref0 * hb;
+    delh = ref1 - ref0;
+  } else {
+    ref0 = r0 * pow(hb, jx - 1) - r0;
+    ref1 = (ref0 + r0) * hb - r0;
+    delh = ref1 - ref0;
+  }
+  const double *y = o->num_rwf + o->num_off[k], *y2 = o->num_d2 + o->num_off[k];
+  const double bb = (r - ref0) / delh, aa = (ref1 - r) / delh;
+  const double cc = aa * (aa * aa - 1) * delh * delh / 6, dd = bb * (bb * bb - 1) * delh * delh / 6;
+  w[0] = aa * y[jx - 1] + bb * y[jx] + cc * y2[jx - 1] + dd * y2[jx];
+  w[1] = (y[jx] - y[jx - 1]) / delh +
+         (-(3 * aa * aa - 1) * y2[jx - 1] + (3 * bb * bb - 1) * y2[jx]) * delh / 6;
+  w[2] = aa * y2[jx - 1] + bb * y2[jx];
+}
+
+void orc_num_radial(const orc_sys *o, int32_t k, int32_t n, const double *r, double *out) {
+  for (int i = 0; i < n; i++) num_radial(o, k, r[i], out + 3 * i);
+}
+
 /* radial value and first two derivatives: basis_fns.f90:118-157, summed
-   over the primitives of a contracted function */
+   over the primitives of a contracted function; a tabulated function
+   through num_radial (basis_fns.f90:161-165) */
 static void radial(const orc_sys *o, int irb, double r, double *w) {
+  if (o->rad_num && o->rad_num[irb] >= 0) {
+    num_radial(o, o->rad_num[irb], r, w);
+    return;
+  }
   int n = o->rad_n[irb];
   double rn = abs(n);
   double rm3 = pow(r, rn - 3), rm2 = rm3 * r, rm1 = rm2 * r;

@@ -51,6 +51,10 @@ double orc_rannyu(int32_t *state);
 double orc_gauss(int32_t *state);
 void orc_quadrature(const orc_sys *o, double *xq, double *yq, double *zq,
                     double *wq);
+/* tabulated radial function k (value, first and second derivative) at n
+   distances: out[3 * i + 0..2] */
+void orc_num_radial(const orc_sys *o, int32_t k, int32_t n, const double *r,
+                    double *out);
 void orc_rotqua(int32_t *state, const double *xq0, const double *yq0,
                 const double *zq0, int32_t nq, double *xq, double *yq,
                 double *zq);

@@ -35,13 +35,20 @@ def _derivs_system(name):
         s.a4 = np.array([p["a"]]); s.b = np.array([p["b"]])
         s.c = system.cusp_exact(np.array([p["c"]]), s.nordc)
         return s
+    if name == "c2_bfd":
+        # the pseudopotential C2 example (tabulated radial basis, 6-point
+        # quadrature): the 24 Jastrow parameters its optimisation section varies
+        import os, sys
+        sys.path.insert(0, os.path.dirname(__file__))
+        from transcripts import c2_bfd
+        return c2_bfd(3)[0]
     s = system.be_atom(1)
     ja, jb, jc = system.jastrow_opt_list(s, vary_a1=False)
     s.jas_a, s.jas_b, s.jas_c = ja, jb, jc
     return s
 
 
-@pytest.mark.parametrize("name", ["c2", "be_jastrow", "c2_ktt"])
+@pytest.mark.parametrize("name", ["c2", "be_jastrow", "c2_ktt", "c2_bfd"])
 def test_derivs_match_oracle(name):
     s = _derivs_system(name)
     assert s.nparm > 0

@@ -28,15 +28,26 @@ def _setup(s, n0, cap, seed):
 
 # T-moves act on pseudopotentials, so the all-electron atom runs without them.
 _CASES = ([("c2_dimer", t, m) for t, m in [(0.02, 0), (0.1, 0), (0.1, 1), (0.1, 2), (0.3, 1)]]
-          + [("be_atom", t, 0) for t in (0.02, 0.1)])
+          + [("be_atom", t, 0) for t in (0.02, 0.1)] + [("c2_bfd", 0.1, 0), ("c2_bfd", 0.1, 1)])
+
+
+def _system(name):
+    if name == "c2_bfd":
+        # CHAMP's pseudopotential C2 example: tabulated radial basis, 6-point quadrature
+        import os, sys
+        sys.path.insert(0, os.path.dirname(__file__))
+        from transcripts import c2_bfd
+        return c2_bfd(3)[0]
+    return system.c2_dimer(2) if name == "c2_dimer" else system.be_atom(1)
 
 
 @pytest.mark.parametrize("name,tau,tmoves", _CASES)
 def test_dmc_generations_match_oracle(name, tau, tmoves):
     """c2_dimer: pseudopotential mover, with T-moves (1: det_balance1, 2:
     casula1) before the moves; be_atom: all-electron mover with the nucleus
-    cusp (gaussian or exponential branch per proposal)."""
-    s = system.c2_dimer(2) if name == "c2_dimer" else system.be_atom(1)
+    cusp (gaussian or exponential branch per proposal); c2_bfd: CHAMP's
+    pseudopotential example with and without T-moves."""
+    s = _system(name)
     n0, cap = 20, 48
     x, seeds = _setup(s, n0, cap, seed=31)
     O = oracle.Oracle(s)

@@ -1,6 +1,7 @@
 """Mixed-precision engines against the FP64 oracle on identical walker
 configurations (sampled by a few FP64 VMC steps from random positions), for
-the C2-like dimer and the 98-electron chain.
+the C2-like dimer, the 98-electron chain and CHAMP's pseudopotential C2
+example (tabulated radial basis).
 
 QMCW_MIXED evaluates the basis functions and the Jastrow in FP32 and keeps
 the orbital accumulation, determinants, inverses, E_L and reductions in
@@ -32,8 +33,16 @@ from champ_amd.engine import Engine
 
 pytestmark = pytest.mark.gpu
 
+def _c2_bfd():
+    import os, sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from transcripts import c2_bfd
+    return c2_bfd(3)[0]
+
+
 CASES = {"c2": (lambda: system.c2_dimer(2), 512, 0.6),
-         "chain": (lambda: system.hchain(4), 48, 0.3)}
+         "chain": (lambda: system.hchain(4), 48, 0.3),
+         "c2_bfd": (_c2_bfd, 512, 0.6)}
 
 
 def _configs(s, W, delta):

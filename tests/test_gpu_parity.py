@@ -30,8 +30,11 @@ def _systems():
     p = [r for r in fx["runs"] if r["iteration"] == 3][0]["params"]
     c2k.a4 = np.array([p["a"]]); c2k.b = np.array([p["b"]])
     c2k.c = system.cusp_exact(np.array([p["c"]]), c2k.nordc)
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from transcripts import c2_bfd
     return {"be": be, "be_atom": system.be_atom(1), "c2": system.c2_dimer(2),
-            "chain4": system.hchain(4, ncarbon=4), "c2_ktt": c2k}
+            "chain4": system.hchain(4, ncarbon=4), "c2_ktt": c2k, "c2_bfd": c2_bfd(3)[0]}
 
 
 SYS = _systems()
@@ -98,7 +101,7 @@ def test_move_accept_sequence_matches_oracle(name):
         O.L.orc_walker_destroy(w)
 
 
-@pytest.mark.parametrize("name", ["be", "c2", "chain4"])
+@pytest.mark.parametrize("name", ["be", "c2", "chain4", "c2_bfd"])
 def test_vmc_bit_exact_decisions(name):
     s = SYS[name]
     W = 24
@@ -117,7 +120,8 @@ def test_vmc_bit_exact_decisions(name):
 
 
 @pytest.mark.parametrize("name,path", [("be", "fused"), ("c2_ktt", "fused"), ("c2", "fused"),
-                                       ("chain4", "fused"), ("c2", "split"), ("be", "device")])
+                                       ("chain4", "fused"), ("c2", "split"), ("be", "device"),
+                                       ("c2_bfd", "fused"), ("c2_bfd", "device")])
 def test_accelerated_mover_bit_exact_decisions(monkeypatch, name, path):
     """CHAMP's default mover (imetro = 6, metrop_slat_mov1; deltar 5,
     deltat 1, fbias 1 as in the example inputs) on the GPU and in the
