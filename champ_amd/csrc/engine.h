@@ -67,6 +67,9 @@ struct DevSys {
   const double *ucf[2];      // per udet_d entry: cdd of the det
   const int *uiu[2], *uid[2];  // per udet_d entry: its up and down unique dets
   int kmax2;                 // kmax*kmax (>= 1)
+  int single_det;            // one determinant, no orbital beyond the reference ones: the
+                             // one-electron move needs neither T nor Y (Y is zero, T the
+                             // fixed permutation ref -> occ) and does not update them
   // Jastrow ijas=4
   int isc, nspin2b, nocuspb, norda, nordb, nordc, nterms;
   double scalek, asymp_r;

@@ -876,6 +876,9 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   }
   S.ndet = s->ndet; S.kmax = D.kmax; S.kmax2 = std::max(1, D.kmax * D.kmax);
   S.noccmax = std::max(D.nocc[0], D.nocc[1]);
+  S.single_det = (s->ndet == 1 && D.kmax == 0 && D.ndetu[0] <= 1 && D.ndetu[1] <= 1 &&
+                  D.nocc[0] == D.ns[0] && D.nocc[1] == D.ns[1]) ? 1 : 0;
+  if (const char *e = getenv("QMCW_NO_SINGLE_DET")) if (atoi(e)) S.single_det = 0;
   for (int sp = 0; sp < 2; sp++) {
     S.ns[sp] = D.ns[sp]; S.nocc[sp] = D.nocc[sp]; S.ndetu[sp] = D.ndetu[sp];
     UP(D.ref[sp], S.ref[sp]); UP(D.occ[sp], S.occ[sp]); UP(D.iref[sp], S.iref[sp]);

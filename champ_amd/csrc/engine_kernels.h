@@ -1112,9 +1112,11 @@ __device__ void group_grad_1e(const KArgs &A, const double *ws, int ie,
     for (int c = 0; c < 4; c++) acc[c] += a * o[1 + c];
   }
   // w_i = ai(i, e) ; z_j = sum_i y(j,i) w_i ; the four components of v(j)
-  // in one pass over the reference orbitals (each keeps its own order)
+  // in one pass over the reference orbitals (each keeps its own order).
+  // A single determinant has Y = 0: every z_j is zero and the loop adds
+  // nothing.
   const double *aie = ai + ns * e;
-  for (int j = lane; j < nocc; j += G) {
+  for (int j = S.single_det ? nocc : lane; j < nocc; j += G) {
     double z = 0.0;
 #pragma unroll 4
     for (int i = 0; i < ns; i++) z += y[j * ns + i] * aie[i];
@@ -1150,13 +1152,14 @@ template <int G>
 __device__ __forceinline__ void sm_vectors(double *__restrict__ vi1, double *__restrict__ vi2,
                                            const double *__restrict__ orbe, const double *__restrict__ ai,
                                            const double *__restrict__ tocc, const int *__restrict__ iref,
-                                           int ns, int nocc, int e, int lane) {
+                                           int ns, int nocc, int e, int lane, bool no_t = false) {
   for (int i = lane; i < ns; i += G) {
     double acc = 0.0;
 #pragma unroll 4
     for (int k = 0; k < ns; k++) acc += orbe[5 * iref[k]] * ai[k + ns * i];
     vi1[i] = acc - (i == e ? 1.0 : 0.0);
   }
+  if (no_t) return;
   for (int j = lane; j < nocc; j += G) {
     double acc = 0.0;
 #pragma unroll 4
@@ -1171,12 +1174,13 @@ template <int G>
 __device__ __forceinline__ void sm_rank1(double *__restrict__ ain, double *__restrict__ tn,
                                          const double *__restrict__ ai, const double *__restrict__ tocc,
                                          const double *__restrict__ vi1, const double *__restrict__ vi2,
-                                         int ns, int nocc, int e, double rat, int lane) {
+                                         int ns, int nocc, int e, double rat, int lane, bool no_t = false) {
 #pragma unroll 4
   for (int q = lane; q < ns * ns; q += G) {
     const int k = q % ns, i = q / ns;
     ain[q] = ai[q] + (-ai[k + ns * e] / rat) * vi1[i];
   }
+  if (no_t) return;
 #pragma unroll 4
   for (int q = lane; q < ns * nocc; q += G) {
     const int k = q / nocc, j = q % nocc;
@@ -1349,12 +1353,15 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   // vi1(i) = sum_k orbe(ref_k) ai(k, i) - delta(i, e); ui(k) = -ai(k,e)/rat
   double *vi1 = mv + ML.misc + 8;            // [ns]
   double *vi2 = vi1 + ns;                    // [nocc]
-  sm_vectors<G>(vi1, vi2, orbe, ai, tocc, S.iref[s], ns, nocc, e, lane);
+  // a single determinant needs only A^-1: T stays the fixed permutation
+  // and the excited-determinant blocks, Y and chi do not change
+  const bool single = S.single_det;
+  sm_vectors<G>(vi1, vi2, orbe, ai, tocc, S.iref[s], ns, nocc, e, lane, single);
   __syncthreads();
-  sm_rank1<G>(ain, tn, ai, tocc, vi1, vi2, ns, nocc, e, rat, lane);
+  sm_rank1<G>(ain, tn, ai, tocc, vi1, vi2, ns, nocc, e, rat, lane, single);
   __syncthreads();
   double *invn = mv + ML.invn, *detn = mv + ML.detn;
-  for (int u = lane; u < S.ndetu[s]; u += G) {
+  for (int u = single ? S.ndetu[s] : lane; u < S.ndetu[s]; u += G) {
     int k = cst(S.ord[s])[u];
     double *blk = invn + u * S.kmax2;
     for (int ii = 0; ii < k; ii++)
@@ -1362,9 +1369,12 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
         blk[jj + k * ii] = tn[cst(S.rf[s])[u * ns + jj] * nocc + cst(S.exo[s])[u * ns + ii]];
     detn[u] = small_matinv(blk, k);
   }
-  __syncthreads();
+  if (!single) __syncthreads();
   double chin;
-  {
+  if (single) {
+    // chi = cdd * 1 * 1 (the empty alpha blocks have determinant 1), as eval_y forms it
+    chin = cst(S.cdd)[0] * ws[L.detv[0]] * ws[L.detv[1]];
+  } else {
     const double *invs[2], *dvs[2];
     double *ys[2];
     invs[s] = invn; dvs[s] = detn; ys[s] = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
@@ -1377,7 +1387,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
       chin = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
     }
   }
-  __syncthreads();
+  if (!single) __syncthreads();
   const double *yn = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
   double deta_n = ws[L.deta[s]] * rat;
   double gnew[3], lapn_tot;
@@ -1821,9 +1831,12 @@ __device__ void group_commit(const KArgs &A, double *ws, const double *mv, int i
   const WLayout &L = A.L;
   const MLayout &ML = A.ML;
   const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
-  const int len[kCommitSeg] = {3, M, 3 * M, 3 * N, nocc * 5, ns * ns, ns * nocc,
-                               S.ndetu[s] * S.kmax2, S.ndetu[s], S.nocc[0] * S.ns[0],
-                               S.nocc[1] * S.ns[1]};
+  // a single determinant's move leaves T, the alpha blocks, their
+  // determinants and Y unchanged: only A^-1 is copied
+  const int sd = S.single_det;
+  const int len[kCommitSeg] = {3, M, 3 * M, 3 * N, nocc * 5, ns * ns, sd ? 0 : ns * nocc,
+                               sd ? 0 : S.ndetu[s] * S.kmax2, sd ? 0 : S.ndetu[s],
+                               sd ? 0 : S.nocc[0] * S.ns[0], sd ? 0 : S.nocc[1] * S.ns[1]};
   const int dso[kCommitSeg] = {(int)L.x + 3 * ie, (int)L.r_en + ie * M, (int)L.rvec_en + ie * M * 3,
                                (int)L.fjo, (int)L.oc + ie * noccmax * 5, (int)L.ai[s],
                                (int)L.tocc[s], (int)L.inv[s], (int)L.detv[s], (int)L.y[0],

@@ -59,3 +59,39 @@ def test_chain98_projection_on_mfma_matches_valu(monkeypatch):
         e.close()
     assert _rel(out["0"]["eloc"], out["1"]["eloc"], 1e-8) < 1e-12
     assert np.array_equal(out["0"]["logpsid"], out["1"]["logpsid"])
+
+
+@pytest.mark.parametrize("mode", ["vmc", "dmc"])
+def test_single_determinant_move_equals_the_general_path(monkeypatch, mode):
+    """One-determinant systems take a shorter move (A^-1 only: T, the alpha
+    blocks and Y stay as the full evaluation left them).  It leaves every
+    observable unchanged: against the general path (QMCW_NO_SINGLE_DET=1) the
+    walk, its random streams, weights and local energies are bit-identical."""
+    from champ_amd.abi import DmcParams
+    W = 24
+    x = system.random_walkers(S, W, seed=43)
+    seeds = rng.walker_seeds("1837465927472523", W)
+    out = []
+    for general in (False, True):
+        if general:
+            monkeypatch.setenv("QMCW_NO_SINGLE_DET", "1")
+        e = Engine(S, W, "mixed")
+        e.set_walkers(x)
+        e.set_rng(seeds)
+        if mode == "vmc":
+            v = e.vmc(3, delta=0.3)
+            out.append((v["naccept"], v["eloc"], e.get_walkers(), e.get_rng()))
+        else:
+            brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+            brng[3] |= 1
+            e.set_branch_rng(brng)
+            e0 = float(np.mean(e.psi_at()["eloc"][:16]))
+            e.dmc_begin(16)
+            p = DmcParams(0.01, e0, eest=e0, esigma=1.0)
+            for _ in range(2):
+                e.dmc_step(p)
+            xg, wg, rg = e.dmc_population()
+            out.append((xg, wg, rg, e.get_eloc()))
+        e.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
