@@ -255,5 +255,77 @@ module champ_amd_mod
       integer(c_int32_t), intent(out) :: seed(*)
       integer(c_int) :: st
     end function
+    ! DMC branching of a host population (split_join_mod.f90:3-219), on the
+    ! host or on the device in one workgroup
+    function qmcw_split_join(n, mwalk, weight, origin, rng, brng, nwalk_out, nbranch) &
+        bind(C, name="qmcw_split_join") result(st)
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: n, mwalk
+      real(c_double), intent(inout) :: weight(*)
+      integer(c_int32_t), intent(inout) :: origin(*), rng(*), brng(*)
+      integer(c_int32_t), intent(out) :: nwalk_out, nbranch
+      integer(c_int) :: st
+    end function
+    function qmcw_split_join_gpu(n, mwalk, weight, origin, rng, brng, nwalk_out, nbranch) &
+        bind(C, name="qmcw_split_join_gpu") result(st)
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: n, mwalk
+      real(c_double), intent(inout) :: weight(*)
+      integer(c_int32_t), intent(inout) :: origin(*), rng(*), brng(*)
+      integer(c_int32_t), intent(out) :: nwalk_out, nbranch
+      integer(c_int) :: st
+    end function
+    ! VMC mover (imetro 1: metrop_mov1, 6: metrop_slat_mov1) and its steps
+    function qmcw_set_vmc_mover(h, imetro, deltar, deltat) bind(C, name="qmcw_set_vmc_mover") result(st)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: imetro
+      real(c_double), value :: deltar, deltat
+      integer(c_int) :: st
+    end function
+    ! full re-inversion of the Slater matrices every `every` moves (0: never)
+    function qmcw_set_reinversion(h, every) bind(C, name="qmcw_set_reinversion") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: every
+      integer(c_int) :: st
+    end function
+    function qmcw_reinvert(h) bind(C, name="qmcw_reinvert") result(st)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int) :: st
+    end function
+    ! derivative-product sums into a caller-held device buffer, no allocation
+    function qmcw_accum_add_static(h, acc) bind(C, name="qmcw_accum_add_static") result(st)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h, acc
+      integer(c_int) :: st
+    end function
+    ! lane-group widths and per-kernel timing
+    function qmcw_set_group_width(h, g) bind(C, name="qmcw_set_group_width") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: g
+      integer(c_int) :: st
+    end function
+    function qmcw_get_group_widths(h, widths) bind(C, name="qmcw_get_group_widths") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: widths(*)
+      integer(c_int) :: st
+    end function
+    function qmcw_profile(h, enable) bind(C, name="qmcw_profile") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: enable
+      integer(c_int) :: st
+    end function
+    function qmcw_profile_get(h, ms, count) bind(C, name="qmcw_profile_get") result(st)
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: ms(*)
+      integer(c_int64_t), intent(out) :: count(*)
+      integer(c_int) :: st
+    end function
   end interface
 end module champ_amd_mod

@@ -43,3 +43,16 @@ def test_fortran_module_matches_the_c_abi(tmp_path):
     import ctypes
     assert p_size == ctypes.sizeof(abi.DmcParams) == 10 * 8 + 8 * 4
     assert d_size == ctypes.sizeof(abi.DmcStats) == 8 * 8 + 2 * 4
+
+
+def test_fortran_module_binds_every_entry_point():
+    """Every function include/champ_amd.h declares has a bind(C) interface
+    of the same name in include/champ_amd_mod.f90."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "champ_amd.h")).read()
+    mod = open(os.path.join(ROOT, "include", "champ_amd_mod.f90")).read()
+    c_names = set(re.findall(r"^(?:int|int32_t|int64_t|double|const char \*|void)\s*\*?\s*(qmcw_\w+)\(",
+                             hdr, re.M))
+    f_names = set(re.findall(r'bind\(C, name="(qmcw_\w+)"\)', mod))
+    assert len(c_names) > 40
+    assert c_names - f_names == set()
