@@ -3,6 +3,7 @@ transcript (examples/be_vb1_jcas24_vmc_j_opt_lin.{inp,out}) into
 tests/golden/be_transcript.json.
 
 Reads the reference's example files as text (data); nothing is executed.
+The first "sites:" line (the starting configuration from the seed) is kept.
 Each record: the Jastrow parameters CHAMP used for one VMC run ("Wave
 function for next iteration #k") and that run's "total E" line (energy,
 error, sigma, sigma error), the number of passes and the acceptance ratio.
@@ -21,6 +22,16 @@ def _acceptance(out, j):
     while not out[j].startswith("acceptance"):
         j += 1
     return float(out[j].split()[1])
+
+
+def _sites(out):
+    """The first configuration CHAMP generated with its sites routine (the
+    "sites:" line, printed to four significant digits), from the seed of
+    the input."""
+    for ln in out:
+        if ln.startswith("sites:"):
+            return [float(t.replace("D", "E")) for t in ln.split()[1:]]
+    return None
 
 
 def main():
@@ -62,7 +73,7 @@ def main():
                 rows.append(nums)
             params = dict(a=rows[0], b=rows[1], c=rows[2])
         i += 1
-    json.dump(dict(input=inp, runs=runs), open(os.path.join(HERE, "be_transcript.json"), "w"),
+    json.dump(dict(input=inp, runs=runs, sites=_sites(out)), open(os.path.join(HERE, "be_transcript.json"), "w"),
               indent=1)
     for r in runs:
         print(r["iteration"], r["energy"], r["error"], r["sigma"], r["passes"],

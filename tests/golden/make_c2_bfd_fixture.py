@@ -5,6 +5,7 @@ input, numerical radial basis basis.1, Gaussian pseudopotential pseudopot1,
 and tests/golden/c2_bfd_basis.1.gz.
 
 Reads the reference's example files as text (data); nothing is executed.
+The first "sites:" line (the starting configuration from the seed) is kept.
 Each record: the Jastrow parameters of one VMC run (iteration 1: the input's
 own; iteration k > 1: the "New wave function" block printed after iteration
 k - 1), that run's "total E" line (energy, error, sigma, sigma error), its
@@ -38,6 +39,16 @@ def _after(out, j, prefix):
     return j
 
 
+def _sites(out):
+    """The first configuration CHAMP generated with its sites routine (the
+    "sites:" line, printed to four significant digits), from the seed of
+    the input."""
+    for ln in out:
+        if ln.startswith("sites:"):
+            return [float(t.replace("D", "E")) for t in ln.split()[1:]]
+    return None
+
+
 def main():
     inp = open(os.path.join(REF, STEM + ".inp")).read()
     out = open(os.path.join(REF, STEM + ".out")).read().splitlines()
@@ -65,7 +76,7 @@ def main():
             rows = [_nums(out[j + 1 + k]) for k in range(3)]
             params = dict(a=rows[0], b=rows[1], c=rows[2])
     psp = open(os.path.join(REF, "pseudopot1")).read()
-    json.dump(dict(input=inp, pseudopot1=psp, runs=runs),
+    json.dump(dict(input=inp, pseudopot1=psp, runs=runs, sites=_sites(out)),
               open(os.path.join(HERE, "c2_bfd_transcript.json"), "w"), indent=1)
     with open(os.path.join(REF, "basis.1"), "rb") as f, \
             gzip.GzipFile(os.path.join(HERE, "c2_bfd_basis.1.gz"), "wb", mtime=0) as g:
