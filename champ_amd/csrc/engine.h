@@ -108,7 +108,12 @@ struct WLayout {
   int64_t x, rvec_en, r_en, rvec_ee, r_ee;
   int64_t fso, fijo, lapj, fjo, fsumo;
   int64_t oc;                // [N][noccmax][5] occupied-orbital values
-  int64_t ai[2], tocc[2], inv[2], detv[2], y[2], deta[2], chi;
+  // determinant blocks in two slots each, [spin][slot]: A^-1 (slot number
+  // at ws[par + s]), T, the alpha inverses and their determinants (slot at
+  // ws[par + 2 + s]) and Y (both spins, slot at ws[par + 4]).  A one-electron
+  // move writes its candidates into the other slots; its commit flips the
+  // slot numbers instead of copying them back.
+  int64_t ai[2][2], tocc[2][2], inv[2][2], detv[2][2], y[2][2], par, deta[2], chi;
   int64_t psi2o, logpsid, sgn, psij, eloc, epot;
   int64_t quad;              // [3][nquad] rotated quadrature grid
   int64_t stride;
@@ -161,7 +166,7 @@ struct MLayout {
   int64_t fsn, fijn_i, fijn_j, lapn_i;    // pair rows [N]...
   int64_t fjn;                            // [N][3]
   int64_t orbe;                           // [noccmax][5]
-  int64_t ain, tn, invn, detn, yn, misc;  // candidate det state
+  int64_t misc;                           // ratio, proposal and Sherman-Morrison vectors
   int64_t slat;                           // accelerated mover: fxop, areao, r of the proposal [4]
   int64_t stride;
 };

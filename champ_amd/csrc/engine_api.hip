@@ -473,6 +473,7 @@ __global__ void k_init_walkers(KArgs A, int W) {
   double *ws = A.st + (int64_t)w * A.L.stride;
   int nq = A.S.nquad;
   for (int q = 0; q < 3 * nq; q++) ws[A.L.quad + q] = A.S.q0[q];
+  for (int q = 0; q < 5; q++) ws[A.L.par + q] = 0.0;   // determinant blocks in slot 0
   if (w >= W) {  // padded walkers copy walker 0
     const double *w0 = A.st + A.L.x;
     for (int q = 0; q < 3 * A.S.N; q++) ws[A.L.x + q] = w0[q];
@@ -1071,7 +1072,6 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   h->A.deltar = 5.0;
   h->A.deltat = 1.0;
   int nsm = std::max(nup, ndn), noccm = S.noccmax;
-  int ndum = std::max(D.ndetu[0], D.ndetu[1]);
   WLayout &L = h->A.L;
   int64_t o = 0;
   auto take = [&o](int64_t n) { int64_t r = o; o += std::max<int64_t>(n, 1); return r; };
@@ -1080,10 +1080,14 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   L.fso = take(N * N); L.fijo = take(3 * N * N); L.lapj = take(N * N); L.fjo = take(3 * N);
   L.fsumo = take(1); L.oc = take((int64_t)N * noccm * 5);
   for (int sp = 0; sp < 2; sp++) {
-    L.ai[sp] = take(D.ns[sp] * D.ns[sp]); L.tocc[sp] = take(D.ns[sp] * D.nocc[sp]);
-    L.inv[sp] = take(D.ndetu[sp] * S.kmax2); L.detv[sp] = take(D.ndetu[sp]);
-    L.y[sp] = take(D.nocc[sp] * D.ns[sp]); L.deta[sp] = take(1);
+    for (int sl = 0; sl < 2; sl++) {
+      L.ai[sp][sl] = take(D.ns[sp] * D.ns[sp]); L.tocc[sp][sl] = take(D.ns[sp] * D.nocc[sp]);
+      L.inv[sp][sl] = take(D.ndetu[sp] * S.kmax2); L.detv[sp][sl] = take(D.ndetu[sp]);
+      L.y[sp][sl] = take(D.nocc[sp] * D.ns[sp]);
+    }
+    L.deta[sp] = take(1);
   }
+  L.par = take(5);
   L.chi = take(1); L.psi2o = take(1); L.logpsid = take(1); L.sgn = take(1); L.psij = take(1);
   L.eloc = take(1); L.epot = take(1); L.quad = take(36);
   L.stride = (o + 7) / 8 * 8;
@@ -1091,9 +1095,8 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   o = 0;
   ML.xn = take(3); ML.ren = take(M); ML.rven = take(3 * M); ML.ree = take(N); ML.rvee = take(3 * N);
   ML.fsn = take(N); ML.fijn_i = take(3 * N); ML.fijn_j = take(3 * N); ML.lapn_i = take(2 * N);
-  ML.fjn = take(3 * N); ML.orbe = take(noccm * 5); ML.ain = take(nsm * nsm);
-  ML.tn = take(nsm * noccm); ML.invn = take(ndum * S.kmax2); ML.detn = take(ndum);
-  ML.yn = take(D.nocc[0] * D.ns[0] + D.nocc[1] * D.ns[1]); ML.misc = take(24 + nsm + noccm); ML.slat = take(4);
+  ML.fjn = take(3 * N); ML.orbe = take(noccm * 5);
+  ML.misc = take(24 + nsm + noccm); ML.slat = take(4);
   ML.stride = o;
   SLayout &SL = h->A.SL;
   o = 0;

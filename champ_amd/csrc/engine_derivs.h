@@ -558,7 +558,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
   for (int s = 0; s < 2; s++) {
     const int ns = S.ns[s], off = s ? S.nup : 0;
     if (ns == 0) continue;
-    const double *ai = ws + L.ai[s];
+    const double *ai = w_ai(L, ws, s);
     double *tf = ss + DL.tfull[s], *dt = ss + DL.dt[s], *aib = ss + DL.aib;
     for (int e = lane; e < ns * NO; e += G) {
       const int k = e / NO, q = e % NO;
@@ -579,7 +579,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
       dt[e] = aib[e] - a;
     }
     __syncthreads();
-    const double *inv = ws + L.inv[s];
+    const double *inv = w_inv(L, ws, s);
     double *del = ss + DL.del[s], *idti = ss + DL.idti[s];
     for (int u = lane; u < S.ndetu[s]; u += G) {
       const int k = cst(S.ord[s])[u];
@@ -606,7 +606,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
   }
   // ---- coefficient sums per unique det and esum
   const double chii = 1.0 / ws[L.chi];
-  const double *dv0 = ws + L.detv[0], *dv1 = ws + L.detv[1];
+  const double *dv0 = w_detv(L, ws, 0), *dv1 = w_detv(L, ws, 1);
   const double *del0 = ss + DL.del[0], *del1 = ss + DL.del[1];
   // over the dets that contain unique det u (the CSR lists of eval_y, in
   // ascending det order)
@@ -629,7 +629,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
   double es = 0.0;
   for (int s = 0; s < 2; s++) {
     const int ns = S.ns[s], nocc = S.nocc[s];
-    const double *y = ws + L.y[s], *dt = ss + DL.dt[s];
+    const double *y = w_y(L, ws, s), *dt = ss + DL.dt[s];
     for (int e = lane; e < nocc * ns; e += G) {
       const int j = e / ns, i = e % ns;
       es += y[e] * dt[i * NO + cst(S.occ[s])[j]];
@@ -640,7 +640,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
   // ---- dy_bld
   for (int s = 0; s < 2; s++) {
     const int ns = S.ns[s], nocc = S.nocc[s];
-    const double *inv = ws + L.inv[s], *idti = ss + DL.idti[s], *y = ws + L.y[s];
+    const double *inv = w_inv(L, ws, s), *idti = ss + DL.idti[s], *y = w_y(L, ws, s);
     const double *c0 = ss + DL.c0[s], *c1 = ss + DL.c1[s];
     double *dy = ss + DL.dy[s];
     // contributions to entry e (the Y-entry CSR lists of eval_y, ascending u)
@@ -662,7 +662,7 @@ __global__ QMCW_HEAVY void k_derivs(KArgs A, double *out) {
     const int ns = S.ns[s], nocc = S.nocc[s], off = s ? S.nup : 0;
     if (ns == 0) continue;
     const double *tf = ss + DL.tfull[s], *dt = ss + DL.dt[s], *dy = ss + DL.dy[s];
-    const double *y = ws + L.y[s], *g = ss + SL.g[s], *ai = ws + L.ai[s];
+    const double *y = w_y(L, ws, s), *g = ss + SL.g[s], *ai = w_ai(L, ws, s);
     double *aux = ss + DL.aux4, *dg = ss + DL.dg[s];
     for (int e = lane; e < nocc * ns; e += G) aux[e] = 0.0;
     __syncthreads();

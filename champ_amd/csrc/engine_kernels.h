@@ -82,6 +82,31 @@ constexpr int kMvJas = 16;
   double *ss = A.sc + (int64_t)w * A.SL.stride;                  \
   (void)ss;
 
+// The walker's current (o = 0) or candidate (o = 1) slot of a determinant
+// block (WLayout: A^-1, then T / alpha inverses / their determinants, then
+// Y, each with its own slot number in the state)
+__device__ __forceinline__ int wslot(const double *ws, int64_t at) { return ws[at] != 0.0 ? 1 : 0; }
+template <typename P>
+__device__ __forceinline__ P *w_ai(const WLayout &L, P *ws, int s, int o = 0) {
+  return ws + L.ai[s][wslot(ws, L.par + s) ^ o];
+}
+template <typename P>
+__device__ __forceinline__ P *w_tocc(const WLayout &L, P *ws, int s, int o = 0) {
+  return ws + L.tocc[s][wslot(ws, L.par + 2 + s) ^ o];
+}
+template <typename P>
+__device__ __forceinline__ P *w_inv(const WLayout &L, P *ws, int s, int o = 0) {
+  return ws + L.inv[s][wslot(ws, L.par + 2 + s) ^ o];
+}
+template <typename P>
+__device__ __forceinline__ P *w_detv(const WLayout &L, P *ws, int s, int o = 0) {
+  return ws + L.detv[s][wslot(ws, L.par + 2 + s) ^ o];
+}
+template <typename P>
+__device__ __forceinline__ P *w_y(const WLayout &L, P *ws, int s, int o = 0) {
+  return ws + L.y[s][wslot(ws, L.par + 4) ^ o];
+}
+
 // bfloat16 rounding (nearest, ties to even) of an FP32 value, kept in FP32
 __device__ __forceinline__ float bf16r(float x) {
   unsigned u = __float_as_uint(x);
@@ -611,13 +636,13 @@ void k_full_det(KArgs A) {
   double *tmp = ss + SL.red;  // >= max(ns^2, nocc*ns) doubles
   for (int s = 0; s < 2; s++) {
     int ns = S.ns[s], nocc = S.nocc[s], off = s ? S.nup : 0;
-    double *ai = ws + L.ai[s];
+    double *ai = w_ai(L, ws, s);
     if (ns == 0) {
-      if (lane == 0) { ws[L.deta[s]] = 1.0; ws[L.detv[s]] = 1.0; }
+      if (lane == 0) { ws[L.deta[s]] = 1.0; w_detv(L, ws, s)[0] = 1.0; }
       __syncthreads();
       continue;
     }
-    double *tocc = ws + L.tocc[s];
+    double *tocc = w_tocc(L, ws, s);
     if (A.lds_inv) {
       // A and the occupied-orbital values in LDS: the elimination and
       // T = A^-1 orb read them there (large determinants)
@@ -672,7 +697,7 @@ void k_full_det(KArgs A) {
     __syncthreads();
     }
     // alpha blocks of the unique dets
-    double *inv = ws + L.inv[s], *dv = ws + L.detv[s];
+    double *inv = w_inv(L, ws, s), *dv = w_detv(L, ws, s);
     for (int u = lane; u < S.ndetu[s]; u += G) {
       int k = cst(S.ord[s])[u];
       double *blk = inv + u * S.kmax2;
@@ -684,14 +709,14 @@ void k_full_det(KArgs A) {
     __syncthreads();
   }
   {
-    const double *invs[2] = {ws + L.inv[0], ws + L.inv[1]};
-    const double *dvs[2] = {ws + L.detv[0], ws + L.detv[1]};
-    double *ys[2] = {ws + L.y[0], ws + L.y[1]};
+    const double *invs[2] = {w_inv(L, ws, 0), w_inv(L, ws, 1)};
+    const double *dvs[2] = {w_detv(L, ws, 0), w_detv(L, ws, 1)};
+    double *ys[2] = {w_y(L, ws, 0), w_y(L, ws, 1)};
     double chi = group_eval_y<G>(S, invs, dvs, ys, ss + SL.cup, lane);
     if (lane == 0) ws[L.chi] = chi;
     __syncthreads();
     for (int s = 0; s < 2; s++)
-      group_g<G>(S, s, ws + L.ai[s], ws + L.tocc[s], ws + L.y[s], ss + SL.g[s], tmp, lane);
+      group_g<G>(S, s, w_ai(L, ws, s), w_tocc(L, ws, s), w_y(L, ws, s), ss + SL.g[s], tmp, lane);
   }
   // ---- grad / laplacian of ln D (qmc/vmc/determinant.f90:99-152)
   double d2d = 0.0;
@@ -1004,7 +1029,7 @@ void k_tm_ratios(KArgs A, int ie, int cw, const double *pts, const int *cnt, dou
   T *bt = reinterpret_cast<T *>(ss + SL.bt);
   if (stale[w] != 0.0) {
     for (int sp = 0; sp < 2; sp++)
-      group_g<G>(S, sp, ws + L.ai[sp], ws + L.tocc[sp], ws + L.y[sp], ss + SL.g[sp], ss + SL.red, lane);
+      group_g<G>(S, sp, w_ai(L, ws, sp), w_tocc(L, ws, sp), w_y(L, ws, sp), ss + SL.g[sp], ss + SL.red, lane);
     for (int p = lane; p < N * M; p += G) {
       T bu, t1, t2;
       scaled<T>(S.isc, (T)S.scalek, (T)ren[p], bu, t1, t2);
@@ -1234,8 +1259,8 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     double lapo_dummy;
     double vold[3];
     double lapjo = 0.0;  // not needed for the proposal
-    group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, ws + L.ai[s],
-                     ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lapjo, vold, nullptr, lane);
+    group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, w_ai(L, ws, s),
+                     w_tocc(L, ws, s), w_y(L, ws, s), fjo + 3 * ie, lapjo, vold, nullptr, lane);
     (void)lapo_dummy;
     if (mode == 3) {
       if (lane == 0) dmc_propose_w(A, w, ie, vold, A.dm + (int64_t)w * kDmStride, A.dmp, xn);
@@ -1344,9 +1369,10 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(ss + SL.phi), orbe, lane);
   __syncthreads();
   // ---- dete_update (qmc/dete_mod.f90:114-214)
-  const double *ai = ws + L.ai[s];
-  const double *tocc = ws + L.tocc[s];
-  double *ain = mv + ML.ain, *tn = mv + ML.tn;
+  const double *ai = w_ai(L, ws, s);
+  const double *tocc = w_tocc(L, ws, s);
+  // candidates into the other slots of the state (the commit flips them)
+  double *ain = w_ai(L, ws, s, 1), *tn = w_tocc(L, ws, s, 1);
   double rat = 0.0;
   for (int k = lane; k < ns; k += G) rat += orbe[5 * cst(S.iref[s])[k]] * ai[k + ns * e];
   rat = gsum<G>(rat);
@@ -1360,7 +1386,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   __syncthreads();
   sm_rank1<G>(ain, tn, ai, tocc, vi1, vi2, ns, nocc, e, rat, lane, single);
   __syncthreads();
-  double *invn = mv + ML.invn, *detn = mv + ML.detn;
+  double *invn = w_inv(L, ws, s, 1), *detn = w_detv(L, ws, s, 1);
   for (int u = single ? S.ndetu[s] : lane; u < S.ndetu[s]; u += G) {
     int k = cst(S.ord[s])[u];
     double *blk = invn + u * S.kmax2;
@@ -1373,13 +1399,13 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   double chin;
   if (single) {
     // chi = cdd * 1 * 1 (the empty alpha blocks have determinant 1), as eval_y forms it
-    chin = cst(S.cdd)[0] * ws[L.detv[0]] * ws[L.detv[1]];
+    chin = cst(S.cdd)[0] * w_detv(L, ws, 0)[0] * w_detv(L, ws, 1)[0];
   } else {
     const double *invs[2], *dvs[2];
     double *ys[2];
-    invs[s] = invn; dvs[s] = detn; ys[s] = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
-    invs[1 - s] = ws + L.inv[1 - s]; dvs[1 - s] = ws + L.detv[1 - s];
-    ys[1 - s] = mv + ML.yn + (s ? 0 : S.nocc[0] * S.ns[0]);
+    invs[s] = invn; dvs[s] = detn; ys[s] = w_y(L, ws, s, 1);
+    invs[1 - s] = w_inv(L, ws, 1 - s); dvs[1 - s] = w_detv(L, ws, 1 - s);
+    ys[1 - s] = w_y(L, ws, 1 - s, 1);
     if (A.lds_y) {
       extern __shared__ double dyn_lds[];
       chin = group_eval_y_lds<G>(S, invs, dvs, ys, dyn_lds + A.lds_y_off + (int64_t)grp * A.lds_y, lane);
@@ -1388,7 +1414,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     }
   }
   if (!single) __syncthreads();
-  const double *yn = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
+  const double *yn = w_y(L, ws, s, 1);
   double deta_n = ws[L.deta[s]] * rat;
   double gnew[3], lapn_tot;
   group_grad_1e<G>(A, ws, ie, orbe, ain, tn, yn, fji, lapjn, gnew, &lapn_tot, lane);
@@ -1453,8 +1479,8 @@ void k_move_a(KArgs A, int ie, int mode, const double *xnew_in, double delta, do
     double lapo_dummy;
     double vold[3];
     double lapjo = 0.0;  // not needed for the proposal
-    group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, ws + L.ai[s],
-                     ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lapjo, vold, nullptr, lane);
+    group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * noccmax * 5, w_ai(L, ws, s),
+                     w_tocc(L, ws, s), w_y(L, ws, s), fjo + 3 * ie, lapjo, vold, nullptr, lane);
     (void)lapo_dummy;
     if (lane == 0) vmc_propose_w(A, w, ie, ws, vold, delta, fbias, xn, mv);
   } else {
@@ -1576,9 +1602,10 @@ __global__ QMCW_MOVE void k_move_b(KArgs A, int ie) {
     orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(ss + SL.phi), orbe, lane);
   __syncthreads();
   // ---- dete_update (qmc/dete_mod.f90:114-214)
-  const double *ai = ws + L.ai[s];
-  const double *tocc = ws + L.tocc[s];
-  double *ain = mv + ML.ain, *tn = mv + ML.tn;
+  const double *ai = w_ai(L, ws, s);
+  const double *tocc = w_tocc(L, ws, s);
+  // candidates into the other slots of the state (the commit flips them)
+  double *ain = w_ai(L, ws, s, 1), *tn = w_tocc(L, ws, s, 1);
   double rat = 0.0;
   for (int k = lane; k < ns; k += G) rat += orbe[5 * cst(S.iref[s])[k]] * ai[k + ns * e];
   rat = gsum<G>(rat);
@@ -1589,7 +1616,7 @@ __global__ QMCW_MOVE void k_move_b(KArgs A, int ie) {
   __syncthreads();
   sm_rank1<G>(ain, tn, ai, tocc, vi1, vi2, ns, nocc, e, rat, lane);
   __syncthreads();
-  double *invn = mv + ML.invn, *detn = mv + ML.detn;
+  double *invn = w_inv(L, ws, s, 1), *detn = w_detv(L, ws, s, 1);
   for (int u = lane; u < S.ndetu[s]; u += G) {
     int k = cst(S.ord[s])[u];
     double *blk = invn + u * S.kmax2;
@@ -1611,8 +1638,8 @@ __global__ QMCW_MOVE void k_move_c(KArgs A, int ie, int mode, double delta, doub
   const MLayout &ML = A.ML;
   double *mv = ss + SL.mv;
   const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
-  double *orbe = mv + ML.orbe, *ain = mv + ML.ain, *tn = mv + ML.tn;
-  double *invn = mv + ML.invn, *detn = mv + ML.detn;
+  double *orbe = mv + ML.orbe, *ain = w_ai(L, ws, s, 1), *tn = w_tocc(L, ws, s, 1);
+  double *invn = w_inv(L, ws, s, 1), *detn = w_detv(L, ws, s, 1);
   const double rat = mv[ML.misc + 0], fsumn = mv[ML.misc + 7];
   const int oj = ML.misc + kMvJas + ns + nocc;
   const double fji[3] = {mv[oj], mv[oj + 1], mv[oj + 2]};
@@ -1621,9 +1648,9 @@ __global__ QMCW_MOVE void k_move_c(KArgs A, int ie, int mode, double delta, doub
   {
     const double *invs[2], *dvs[2];
     double *ys[2];
-    invs[s] = invn; dvs[s] = detn; ys[s] = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
-    invs[1 - s] = ws + L.inv[1 - s]; dvs[1 - s] = ws + L.detv[1 - s];
-    ys[1 - s] = mv + ML.yn + (s ? 0 : S.nocc[0] * S.ns[0]);
+    invs[s] = invn; dvs[s] = detn; ys[s] = w_y(L, ws, s, 1);
+    invs[1 - s] = w_inv(L, ws, 1 - s); dvs[1 - s] = w_detv(L, ws, 1 - s);
+    ys[1 - s] = w_y(L, ws, 1 - s, 1);
     if (A.lds_y) {
       extern __shared__ double dyn_lds[];
       chin = group_eval_y_lds<G>(S, invs, dvs, ys, dyn_lds + A.lds_y_off + (int64_t)grp * A.lds_y, lane);
@@ -1632,7 +1659,7 @@ __global__ QMCW_MOVE void k_move_c(KArgs A, int ie, int mode, double delta, doub
     }
   }
   __syncthreads();
-  const double *yn = mv + ML.yn + (s ? S.nocc[0] * S.ns[0] : 0);
+  const double *yn = w_y(L, ws, s, 1);
   double deta_n = ws[L.deta[s]] * rat;
   double gnew[3], lapn_tot;
   group_grad_1e<G>(A, ws, ie, orbe, ain, tn, yn, fji, lapjn, gnew, &lapn_tot, lane);
@@ -1752,7 +1779,7 @@ __device__ __forceinline__ int metrop_accept(const KArgs &A, const double *ws, c
 // rows, gradients, A^-1, T, alpha inverses and determinants, Y) are copied
 // as one flat list of segments, up to kCommitBatch loads in flight per lane
 // before the stores; the pair-array rows follow, one electron per lane.
-constexpr int kCommitSeg = 11;
+constexpr int kCommitSeg = 5;
 constexpr int kCommitBatch = 16;
 
 template <int G>
@@ -1831,19 +1858,10 @@ __device__ void group_commit(const KArgs &A, double *ws, const double *mv, int i
   const WLayout &L = A.L;
   const MLayout &ML = A.ML;
   const int s = ie < S.nup ? 0 : 1, ns = S.ns[s], nocc = S.nocc[s];
-  // a single determinant's move leaves T, the alpha blocks, their
-  // determinants and Y unchanged: only A^-1 is copied
-  const int sd = S.single_det;
-  const int len[kCommitSeg] = {3, M, 3 * M, 3 * N, nocc * 5, ns * ns, sd ? 0 : ns * nocc,
-                               sd ? 0 : S.ndetu[s] * S.kmax2, sd ? 0 : S.ndetu[s],
-                               sd ? 0 : S.nocc[0] * S.ns[0], sd ? 0 : S.nocc[1] * S.ns[1]};
+  const int len[kCommitSeg] = {3, M, 3 * M, 3 * N, nocc * 5};
   const int dso[kCommitSeg] = {(int)L.x + 3 * ie, (int)L.r_en + ie * M, (int)L.rvec_en + ie * M * 3,
-                               (int)L.fjo, (int)L.oc + ie * noccmax * 5, (int)L.ai[s],
-                               (int)L.tocc[s], (int)L.inv[s], (int)L.detv[s], (int)L.y[0],
-                               (int)L.y[1]};
-  const int sso[kCommitSeg] = {(int)ML.xn, (int)ML.ren, (int)ML.rven, (int)ML.fjn, (int)ML.orbe,
-                               (int)ML.ain, (int)ML.tn, (int)ML.invn, (int)ML.detn, (int)ML.yn,
-                               (int)ML.yn + S.nocc[0] * S.ns[0]};
+                               (int)L.fjo, (int)L.oc + ie * noccmax * 5};
+  const int sso[kCommitSeg] = {(int)ML.xn, (int)ML.ren, (int)ML.rven, (int)ML.fjn, (int)ML.orbe};
   int st[kCommitSeg + 1];
   st[0] = 0;
 #pragma unroll
@@ -1851,6 +1869,14 @@ __device__ void group_commit(const KArgs &A, double *ws, const double *mv, int i
   commit_blocks<G>(st, dso, sso, ws, mv, lane);
   commit_pairs<G>(S, L, ML, ws, mv, ie, lane);
   if (lane == 0) {
+    // the candidate slots become the current ones: A^-1 of the spin, and
+    // unless the system has a single determinant (whose move leaves them
+    // unchanged) its T, alpha blocks and determinants and Y of both spins
+    ws[L.par + s] = wslot(ws, L.par + s) ? 0.0 : 1.0;
+    if (!S.single_det) {
+      ws[L.par + 2 + s] = wslot(ws, L.par + 2 + s) ? 0.0 : 1.0;
+      ws[L.par + 4] = wslot(ws, L.par + 4) ? 0.0 : 1.0;
+    }
     const double chi = mv[ML.misc + 5], deta = mv[ML.misc + 6], fs = mv[ML.misc + 7];
     const double logn = mv[ML.misc + 8 + ns + nocc];
     ws[L.chi] = chi;
@@ -1898,8 +1924,8 @@ __global__ __launch_bounds__(64) void k_grad_1e(KArgs A, int ie, double *out) {
   lj = gsum<G>(lj);
   lj += fjo[3 * ie] * fjo[3 * ie] + fjo[3 * ie + 1] * fjo[3 * ie + 1] + fjo[3 * ie + 2] * fjo[3 * ie + 2];
   double g[3], lap;
-  group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * S.noccmax * 5, ws + L.ai[s],
-                   ws + L.tocc[s], ws + L.y[s], fjo + 3 * ie, lj, g, &lap, lane);
+  group_grad_1e<G>(A, ws, ie, ws + L.oc + (int64_t)ie * S.noccmax * 5, w_ai(L, ws, s),
+                   w_tocc(L, ws, s), w_y(L, ws, s), fjo + 3 * ie, lj, g, &lap, lane);
   if (lane == 0) {
     out[4 * w] = g[0]; out[4 * w + 1] = g[1]; out[4 * w + 2] = g[2]; out[4 * w + 3] = lap;
   }
