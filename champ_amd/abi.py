@@ -30,7 +30,7 @@ PROF_KERNELS = ("k_full_jas", "k_full_det", "k_full_nlpp", "k_full_fin",
 
 QMCW_BASIS_SLATER = 1
 QMCW_BASIS_GAUSSIAN = 2
-QMCW_MAX_L = 4
+QMCW_MAX_L = 5
 QMCW_MAX_ORD = 8
 QMCW_MAX_LPOT = 5
 QMCW_MAX_NQUAD = 12

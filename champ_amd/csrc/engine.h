@@ -10,7 +10,7 @@
 namespace qmcw {
 
 constexpr int kWave = 64;          // CDNA wavefront width
-constexpr int kMaxShellM = 9;      // 2l+1 for l <= 4
+constexpr int kMaxShellM = 11;     // 2l+1 for l <= 5
 constexpr int kMaxCtype = 16;
 constexpr int kMaxTerms = 64;
 constexpr int kMaxKmax = 8;        // excitation order of an AMF alpha block
@@ -33,8 +33,8 @@ struct DevSys {
   int nshell;                // over all centres
   const int *cent_sh_off;    // [M+1]
   const int *sh_cent, *sh_l, *sh_a, *sh_nm, *sh_prim_off;
-  const int *sh_m;           // [nshell][9]
-  const int *sh_bas;         // [nshell][9] basis index of each component
+  const int *sh_m;           // [nshell][kMaxShellM]
+  const int *sh_bas;         // [nshell][kMaxShellM] basis index of each component
   const double *prim_zeta, *prim_coef;
   // tabulated radial functions: a shell with sh_num[s] = k >= 0 has the
   // radial part R_k(r) r^-l (R_k tabulated) in place of r^a g(r)

@@ -791,7 +791,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
       const int kn = has_num ? s->rad_num[r] : -1;
       if (l < 0 || l > QMCW_MAX_L || m < -l || m > l) {
         delete h;
-        return fail(nullptr, QMCW_ERR_ARG, "basis function with l > 4 or |m| > l");
+        return fail(nullptr, QMCW_ERR_ARG, "basis function with l > 5 or |m| > l");
       }
       if (kn >= 0) n = l + 1;   // tabulated R(r) Y_lm: radial part R r^-l, no r^a factor
       if (n - 1 - l < 0) {

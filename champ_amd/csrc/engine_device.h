@@ -469,7 +469,7 @@ __device__ __forceinline__ void solid_harm(int l, T x, T y, T z, T *Sv, T (*Sg)[
       Sg[6][0] = T(6) * c3 * x * y; Sg[6][1] = c3 * (T(3) * x2 - T(3) * y2); Sg[6][2] = 0;
       break;
     }
-    default: {  // l = 4
+    case 4: {
       const T c0 = T(0.125), c1 = T(0.7905694150420949), c2 = T(0.5590169943749475),
               c3 = T(2.0916500663351889), c4 = T(0.7395099728874520);
       T r2 = x2 + y2 + z2;
@@ -507,6 +507,51 @@ __device__ __forceinline__ void solid_harm(int l, T x, T y, T z, T *Sv, T (*Sg)[
       Sv[8] = c4 * q4;
       Sg[8][0] = c4 * (T(12) * x2 * y - T(4) * y2 * y); Sg[8][1] = c4 * (T(4) * x2 * x - T(12) * x * y2);
       Sg[8][2] = 0;
+      break;
+    }
+    default: {  // l = 5: S_5m = f_m(z, rho) {A_m, B_m}, A_m + i B_m = (x + i y)^|m|
+      const T rho = x2 + y2 + z2;
+      // f_m = N_m Pi_5^m with its partials in z and rho; N_m Pi_5^m's
+      // constants: sqrt(15)/8, sqrt(105)/4, sqrt(70)/16, 3 sqrt(35)/8, 3 sqrt(14)/16
+      const T c1 = T(0.4841229182759271), c2 = T(2.5617376914898995), c3 = T(0.5229125165837972),
+              c4 = T(2.2185299186623560), c5 = T(0.7015607600201140);
+      T f[6], fz[6], fr[6];
+      f[0] = T(0.125) * z * (T(63) * z2 * z2 - T(70) * rho * z2 + T(15) * rho * rho);
+      fz[0] = T(0.125) * (T(315) * z2 * z2 - T(210) * rho * z2 + T(15) * rho * rho);
+      fr[0] = T(0.125) * z * (T(-70) * z2 + T(30) * rho);
+      f[1] = c1 * (T(21) * z2 * z2 - T(14) * rho * z2 + rho * rho);
+      fz[1] = c1 * (T(84) * z2 * z - T(28) * rho * z);
+      fr[1] = c1 * (T(-14) * z2 + T(2) * rho);
+      f[2] = c2 * z * (T(3) * z2 - rho);
+      fz[2] = c2 * (T(9) * z2 - rho);
+      fr[2] = -c2 * z;
+      f[3] = c3 * (T(9) * z2 - rho);
+      fz[3] = c3 * T(18) * z;
+      fr[3] = -c3;
+      f[4] = c4 * z; fz[4] = c4; fr[4] = T(0);
+      f[5] = c5; fz[5] = T(0); fr[5] = T(0);
+      T A[6], B[6];
+      A[0] = T(1); B[0] = T(0);
+#pragma unroll
+      for (int m = 1; m <= 5; m++) {
+        A[m] = A[m - 1] * x - B[m - 1] * y;
+        B[m] = A[m - 1] * y + B[m - 1] * x;
+      }
+      Sv[0] = f[0];
+      Sg[0][0] = T(2) * x * fr[0]; Sg[0][1] = T(2) * y * fr[0]; Sg[0][2] = fz[0] + T(2) * z * fr[0];
+#pragma unroll
+      for (int m = 1; m <= 5; m++) {
+        // d(x+iy)^m/dx = m (x+iy)^(m-1), d/dy = i m (x+iy)^(m-1)
+        const T gx = T(2) * x * fr[m], gy = T(2) * y * fr[m], gz = fz[m] + T(2) * z * fr[m];
+        Sv[2 * m - 1] = f[m] * A[m];
+        Sg[2 * m - 1][0] = gx * A[m] + f[m] * T(m) * A[m - 1];
+        Sg[2 * m - 1][1] = gy * A[m] - f[m] * T(m) * B[m - 1];
+        Sg[2 * m - 1][2] = gz * A[m];
+        Sv[2 * m] = f[m] * B[m];
+        Sg[2 * m][0] = gx * B[m] + f[m] * T(m) * B[m - 1];
+        Sg[2 * m][1] = gy * B[m] + f[m] * T(m) * A[m - 1];
+        Sg[2 * m][2] = gz * B[m];
+      }
       break;
     }
   }
@@ -623,7 +668,8 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
       case 1: shell_write<T, 1, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
       case 2: shell_write<T, 2, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
       case 3: shell_write<T, 3, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
-      default: shell_write<T, 4, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+      case 4: shell_write<T, 4, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    default: shell_write<T, 5, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
     }
     return;
   }
@@ -667,7 +713,8 @@ __device__ void eval_shell(const DevSys &S, int s, T x, T y, T z, T r, int want,
     case 1: shell_write<T, 1, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
     case 2: shell_write<T, 2, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
     case 3: shell_write<T, 3, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
-    default: shell_write<T, 4, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    case 4: shell_write<T, 4, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
+    default: shell_write<T, 5, ST>(x, y, z, f, fr, lapfac, want, bas, phi); break;
   }
 }
 
@@ -709,7 +756,8 @@ __device__ T shell_dot(const DevSys &S, int s, T x, T y, T z, T r, const GP *gp)
     case 1: acc = shell_dot_l<T, 1, GP>(x, y, z, bas, gp); break;
     case 2: acc = shell_dot_l<T, 2, GP>(x, y, z, bas, gp); break;
     case 3: acc = shell_dot_l<T, 3, GP>(x, y, z, bas, gp); break;
-    default: acc = shell_dot_l<T, 4, GP>(x, y, z, bas, gp); break;
+    case 4: acc = shell_dot_l<T, 4, GP>(x, y, z, bas, gp); break;
+    default: acc = shell_dot_l<T, 5, GP>(x, y, z, bas, gp); break;
   }
   return acc * g;
 }
@@ -759,7 +807,8 @@ __device__ __forceinline__ void shell_val_axpy(const DevSys &S, int s, T x, T y,
     case 1: shell_axpy_l<T, 1>(x, y, z, g, bas, gp, fac, pb, rat); break;
     case 2: shell_axpy_l<T, 2>(x, y, z, g, bas, gp, fac, pb, rat); break;
     case 3: shell_axpy_l<T, 3>(x, y, z, g, bas, gp, fac, pb, rat); break;
-    default: shell_axpy_l<T, 4>(x, y, z, g, bas, gp, fac, pb, rat); break;
+    case 4: shell_axpy_l<T, 4>(x, y, z, g, bas, gp, fac, pb, rat); break;
+    default: shell_axpy_l<T, 5>(x, y, z, g, bas, gp, fac, pb, rat); break;
   }
 }
 

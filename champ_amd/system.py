@@ -26,7 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import (QMCW_BASIS_GAUSSIAN, QMCW_BASIS_SLATER, QMCW_MAX_LPOT,
+from .abi import (QMCW_BASIS_GAUSSIAN, QMCW_BASIS_SLATER, QMCW_MAX_L, QMCW_MAX_LPOT,
                   QmcwSystem, dptr, iptr)
 
 
@@ -111,11 +111,11 @@ _LABEL_LM = {"s": (0, [0]), "px": (1, [1]), "py": (1, [-1]), "pz": (1, [0])}
 def parse_label(lab: str):
     """CHAMP basis label (qmc/basis_mod.f90:400-470) -> (n, l, m)."""
     lab = lab.lower()
-    mt = re.fullmatch(r"(\d)([spdfg])(x|y|z|[+-]?\d)?", lab)
+    mt = re.fullmatch(r"(\d)([spdfgh])(x|y|z|[+-]?\d)?", lab)
     if not mt:
         raise ValueError(f"unknown basis function label {lab!r}")
     n = int(mt.group(1))
-    l = "spdfg".index(mt.group(2))
+    l = "spdfgh".index(mt.group(2))
     sub = mt.group(3)
     if l == 0:
         return n, 0, 0
@@ -730,7 +730,7 @@ def from_champ_old_input(text: str, name: str = "champ", files: dict | None = No
     orb_loc_ana_gamess_order (numr = -1: all s by n, then the px, py and pz
     blocks by n; qmc/vmc/read_orb_loc.f90:843-1000) or numerical radial
     functions (numr = 1: files basis.<ict>, count record and function order
-    of orb_loc_ana_original_order, functions up to g, the radial function of
+    of orb_loc_ana_original_order, functions up to h, the radial function of
     each from the iwrwf record); orbitals given by coefficients,
     determinants/CSFs, ijas = 4 Jastrow (isc = 2 or 4) and, for one centre
     type, the Jastrow parameters the optimisation section varies (iwjasa,
@@ -777,8 +777,8 @@ def from_champ_old_input(text: str, name: str = "champ", files: dict | None = No
                 raise ValueError("numerical-basis count record too short")
             f = []
             for (l, m), cnt in zip(_NUMR1_LM, v):
-                if cnt and l > 4:
-                    raise ValueError("basis functions with l > 4 are not supported")
+                if cnt and l > QMCW_MAX_L:
+                    raise ValueError(f"basis functions with l > {QMCW_MAX_L} are not supported")
                 f += [(0, l, m)] * abs(cnt)
             funcs.append(f)
             iwrwf.append([k - 1 for k in R.ints(len(f))])
@@ -982,7 +982,7 @@ def _ecp_arrays(types):
 
 
 def _gto_shells(bb, rng, ictype, shells, nprim=2):
-    """Add contracted spherical GTO shells ('s', 'p', 'd', 'f' counts) with
+    """Add contracted spherical GTO shells ((l, count) pairs, l <= 5) with
     exponents log-uniform in [0.1, 20]."""
     for l, count in shells:
         for _ in range(count):
@@ -1099,7 +1099,8 @@ def hchain(seed: int = 4, ncarbon: int = 16) -> System:
 
 
 def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = None,
-                    zeff: float | None = None, big_basis: bool = False) -> System:
+                    zeff: float | None = None, big_basis: bool = False,
+                    gh_shells: bool = False) -> System:
     """Synthetic molecule of the multi-system batch: electron count uniform
     in [4, 128] (split evenly between spins, the extra one up), 1-24 nuclei
     with Z_eff in {1, 4, 6} placed by random packing at >= 2 bohr spacing,
@@ -1109,7 +1110,8 @@ def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = Non
     the electron count is drawn independently of the nuclei, then capped so
     that the basis holds the occupied orbitals of each spin plus two.
     zeff fixes every nucleus' charge; big_basis gives the Z_eff=6 centres
-    4s3p3d1f (35 functions), so that 23 of them reach ~800 functions."""
+    4s3p3d1f (35 functions), so that 23 of them reach ~800 functions;
+    gh_shells adds one g and one h shell to every centre type."""
     rng = np.random.default_rng(seed)
     N = int(nelec if nelec is not None else rng.integers(4, 129))
     M = int(ncent if ncent is not None else rng.integers(1, 25))
@@ -1133,6 +1135,9 @@ def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = Non
     _gto_shells(bb, rng, 1, [(0, 3), (1, 2), (2, 1)])
     _gto_shells(bb, rng, 2, [(0, 4), (1, 3), (2, 3), (3, 1)] if big_basis else
                 [(0, 3), (1, 3), (2, 2), (3, 1)])
+    if gh_shells:
+        for t in range(3):
+            _gto_shells(bb, rng, t, [(4, 1), (5, 1)])
     barr, anorm = bb.arrays(iwctype)
     nb = len(anorm)
     N = max(2, min(N, 2 * (nb - 2)))

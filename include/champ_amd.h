@@ -35,7 +35,7 @@ extern "C" {
 #define QMCW_BASIS_SLATER 1
 #define QMCW_BASIS_GAUSSIAN 2
 
-#define QMCW_MAX_L 4         /* explicit real solid harmonics up to g */
+#define QMCW_MAX_L 5         /* real solid harmonics up to h */
 #define QMCW_MAX_ORD 8       /* maximum Jastrow polynomial order */
 #define QMCW_MAX_LPOT 5      /* maximum pseudopotential l components */
 #define QMCW_MAX_NQUAD 12

@@ -60,6 +60,7 @@ def lib():
             "orc_gauss": (C.c_double, [_pi]),
             "orc_quadrature": (None, [_vp, _pd, _pd, _pd, _pd]),
             "orc_num_radial": (None, [_vp, C.c_int32, C.c_int32, _pd, _pd]),
+            "orc_solid_harmonic": (None, [C.c_int32, C.c_int32, _pd, _pd]),
             "orc_rotqua": (None, [_pi, _pd, _pd, _pd, C.c_int32, _pd, _pd, _pd]),
             "orc_matinv": (C.c_double, [_pd, C.c_int32]),
             "orc_basis": (None, [_vp, _pd, _pd, _pd, _pd]),
@@ -292,6 +293,14 @@ class Oracle:
         st = self.L.orc_vmc_run(self.h, nw, _d(x), _i(rng), nstep, delta, fbias, _d(tr),
                                 nacc.ctypes.data_as(C.POINTER(C.c_int64)), nthreads)
         return dict(x=x, rng=rng, eloc=tr, naccept=nacc, status=st)
+
+
+def solid_harmonic(l, m, x):
+    """(S_lm, grad S_lm) at x from the oracle's closed form."""
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros(4)
+    lib().orc_solid_harmonic(int(l), int(m), _d(x), _d(out))
+    return out[0], out[1:]
 
 
 def rng_stream(seed4, n):

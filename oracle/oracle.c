@@ -943,6 +943,51 @@ static void radial(const orc_sys *o, int irb, double r, double *w) {
   }
 }
 
+/* Real solid harmonic S_lm(x, y, z) = r^l Y_lm with <Y_lm^2> = 4 pi/(2l+1)
+   and its gradient, from the closed form (Helgaker, Jorgensen and Olsen,
+   Molecular Electronic-Structure Theory, eqs. 6.4.47-6.4.50):
+   S_lm = N_m Pi_l^|m|(z, r^2) {Re, Im}(x + i y)^|m| (Re for m >= 0),
+   Pi_l^m = sum_k (-1)^k 2^-l C(l,k) C(2l-2k,l) (l-2k)!/(l-2k-m)! r^2k z^(l-2k-m),
+   N_0 = 1, N_m = sqrt(2 (l-m)!/(l+m)!).  For l >= 5 CHAMP generates these
+   by recursion (qmc/vmc/basis_fns.f90:1962-2444); for l <= 4 it has the
+   explicit forms of ylm_fill, which agree with both. */
+static double fact_d(int n) { double f = 1; for (int i = 2; i <= n; i++) f *= i; return f; }
+static double binom_d(int n, int k) { return fact_d(n) / (fact_d(k) * fact_d(n - k)); }
+static void solid_harmonic(int l, int m, double x, double y, double z, double *S, double *g) {
+  const int am = abs(m);
+  const double rho = x * x + y * y + z * z;
+  /* Pi and its partials in z and rho */
+  double pi = 0, piz = 0, pir = 0;
+  for (int k = 0; 2 * k <= l - am; k++) {
+    const int e = l - 2 * k - am;
+    const double c = ((k & 1) ? -1.0 : 1.0) * binom_d(l, k) * binom_d(2 * l - 2 * k, l) *
+                     fact_d(l - 2 * k) / (fact_d(e) * pow(2.0, l));
+    pi += c * pow(rho, k) * pow(z, e);
+    if (e > 0) piz += c * pow(rho, k) * e * pow(z, e - 1);
+    if (k > 0) pir += c * k * pow(rho, k - 1) * pow(z, e);
+  }
+  /* (x + i y)^am and (x + i y)^(am-1) */
+  double re = 1, im = 0, re1 = 0, im1 = 0;
+  for (int q = 0; q < am; q++) {
+    re1 = re; im1 = im;
+    const double t = re * x - im * y;
+    im = re * y + im * x;
+    re = t;
+  }
+  double a, ax, ay;
+  if (m >= 0) { a = re; ax = am * re1; ay = -am * im1; }
+  else { a = im; ax = am * im1; ay = am * re1; }
+  const double nm = am == 0 ? 1.0 : sqrt(2.0 * fact_d(l - am) / fact_d(l + am));
+  *S = nm * pi * a;
+  g[0] = nm * (2 * x * pir * a + pi * ax);
+  g[1] = nm * (2 * y * pir * a + pi * ay);
+  g[2] = nm * ((piz + 2 * z * pir) * a);
+}
+
+void orc_solid_harmonic(int32_t l, int32_t m, const double *x, double *out) {
+  solid_harmonic(l, m, x[0], x[1], x[2], out, out + 1);
+}
+
 static void basis_at(const orc_sys *o, const double *rvec, const double *r_en,
                      double *phi, double *dphi, double *d2phi) {
   int ib = 0;
@@ -962,6 +1007,20 @@ static void basis_at(const orc_sys *o, const double *rvec, const double *r_en,
     for (int q = b0; q < b1; q++, ib++) {
       int l = o->bas_l[q], m = o->bas_m[q], mabs = abs(m);
       const double *w = wfv[q - b0];
+      if (l >= 5) {
+        /* Y = S / r^l, grad Y = grad S / r^l - l S x / r^(l+2) */
+        double Sv, gS[3];
+        solid_harmonic(l, m, xc[0], xc[1], xc[2], &Sv, gS);
+        const double ril = pow(ri, l), ylm = Sv * ril;
+        phi[ib] = ylm * w[0];
+        if (dphi) {
+          const double xh[3] = {xhat, yhat, zhat};
+          for (int k = 0; k < 3; k++)
+            dphi[3 * ib + k] = ylm * xh[k] * w[1] + (gS[k] * ril - l * ylm * ri * xh[k]) * w[0];
+          d2phi[ib] = ylm * (w[2] + 2 * ri * w[1] - l * (l + 1) * ri2 * w[0]);
+        }
+        continue;
+      }
       double th = T.th[l][mabs], ph = T.ph[m + 4];
       double ylm = th * ph;
       phi[ib] = ylm * w[0];

@@ -51,6 +51,9 @@ double orc_rannyu(int32_t *state);
 double orc_gauss(int32_t *state);
 void orc_quadrature(const orc_sys *o, double *xq, double *yq, double *zq,
                     double *wq);
+/* real solid harmonic S_lm (r^l Y_lm, <Y^2> = 4 pi/(2l+1)) at x: out[0]
+   the value, out[1..3] the gradient */
+void orc_solid_harmonic(int32_t l, int32_t m, const double *x, double *out);
 /* tabulated radial function k (value, first and second derivative) at n
    distances: out[3 * i + 0..2] */
 void orc_num_radial(const orc_sys *o, int32_t k, int32_t n, const double *r,

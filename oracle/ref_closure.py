@@ -1,6 +1,9 @@
-"""Recipe: build the reference's split_join (qmc/dmc/dmc_elec/split_join_mod.f90)
-into oracle/_ref/libref_branch.so with AMD flang, compiling each reference
-file where it lies under /root/reference (never copied).
+"""Recipe: build reference routines with their module closures into
+oracle/_ref with AMD flang, compiling each reference file where it lies
+under /root/reference (never copied): `python3 ref_closure.py branch`
+(split_join, qmc/dmc/dmc_elec/split_join_mod.f90 -> _ref/libref_branch.so)
+or `python3 ref_closure.py ylm` (the recursive real spherical harmonics of
+qmc/vmc/basis_fns.f90 -> _ref/libref_ylm.so, with ref_ylm_shim.f90).
 
 TEST INFRASTRUCTURE (the oracle's checker), run by `make ref`.  The module
 closure of split_join_mod is found from the `use` statements (118 files),
@@ -22,12 +25,26 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = os.environ.get("REF", "/root/reference/qmc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
-OUT = os.path.join(HERE, "_ref", "branch")
-SO = os.path.join(HERE, "_ref", "libref_branch.so")
-ROOTS = ["dmc/dmc_elec/split_join_mod.f90"]
-EXTRA = [(os.path.join(REF, "vmc/rannyu.f90"), "free"),
-         (os.path.join(os.path.dirname(REF), "lib/lib/shell.f"), "fixed"),
-         (os.path.join(HERE, "ref_branch_shim.f90"), "free")]
+# target: (build folder, library, root files of the closure, extra sources)
+TARGETS = {
+    "branch": ("branch", "libref_branch.so", ["dmc/dmc_elec/split_join_mod.f90"],
+               [(os.path.join(REF, "vmc/rannyu.f90"), "free"),
+                (os.path.join(os.path.dirname(REF), "lib/lib/shell.f"), "fixed"),
+                (os.path.join(HERE, "ref_branch_shim.f90"), "free")]),
+    # the recursive real spherical harmonics of qmc/vmc/basis_fns.f90 (its
+    # module closure; basis_fns's own routines are never called)
+    "ylm": ("ylm", "libref_ylm.so", ["vmc/basis_fns.f90"],
+            [(os.path.join(HERE, "ref_ylm_shim.f90"), "free")]),
+}
+OUT = SO = ROOTS = EXTRA = None
+
+
+def _select(name):
+    global OUT, SO, ROOTS, EXTRA
+    d, so, roots, extra = TARGETS[name]
+    OUT = os.path.join(HERE, "_ref", d)
+    SO = os.path.join(HERE, "_ref", so)
+    ROOTS, EXTRA = roots, extra
 
 
 def _sources():
@@ -95,7 +112,9 @@ def build(jobs: int = 8) -> str:
 
 
 if __name__ == "__main__":
+    target = sys.argv[1] if len(sys.argv) > 1 else "branch"
+    _select(target)
     if not os.path.isdir(REF):
-        print("reference sources absent: _ref/libref_branch.so not built")
+        print(f"reference sources absent: {os.path.basename(SO)} not built")
         sys.exit(0)
     print(build())

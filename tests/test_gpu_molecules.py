@@ -1,7 +1,7 @@
 """GPU parity on synthetic molecules of the multi-system batch (uneven
 electron counts, H centres without and Z_eff = 4, 6 centres with
-pseudopotentials, s-f shells): full evaluation and one VMC step against the
-oracle."""
+pseudopotentials, s-f shells, and g and h shells in the last case): full
+evaluation and one VMC step against the oracle."""
 import numpy as np
 import pytest
 
@@ -17,9 +17,13 @@ def _rel(a, b, floor):
     return float((np.abs(a - b) / (np.abs(b) + floor)).max())
 
 
-@pytest.mark.parametrize("seed,nelec,ncent", [(101, 5, 1), (102, 12, 3), (103, 24, 6), (104, 37, 9)])
-def test_molecule_parity(seed, nelec, ncent):
-    s = system.random_molecule(seed, nelec=nelec, ncent=ncent)
+@pytest.mark.parametrize("seed,nelec,ncent,gh", [(101, 5, 1, False), (102, 12, 3, False),
+                                                 (103, 24, 6, False), (104, 37, 9, False),
+                                                 (105, 14, 3, True)])
+def test_molecule_parity(seed, nelec, ncent, gh):
+    s = system.random_molecule(seed, nelec=nelec, ncent=ncent, gh_shells=gh)
+    if gh:
+        assert 5 in set(int(v) for v in s.bas_l)
     W = 9
     x = system.random_walkers(s, W, seed=seed)
     e = Engine(s, W)
