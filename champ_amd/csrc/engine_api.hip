@@ -631,12 +631,15 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     K.deriv_jas(cj);
     prof_end(h);
   }
-  // the pseudopotential projection of large determinants on FP64 MFMA
-  A.gproj_mfma = (A.S.nloc > 0 && std::max(A.S.ns[0], A.S.ns[1]) >= 16 && !h->no_gproj_mfma) ? 1 : 0;
+  // the pseudopotential projection on FP64 MFMA, rows = (walker, electron)
+  A.gproj_mfma = (A.S.nloc > 0 && !h->no_gproj_mfma) ? 1 : 0;
   prof_begin(h, QMCW_PROF_FULL_DET);
   DISPATCH_GW(gd, launch_det, h, A, det_smem);
   if (A.gproj_mfma)
-    hipLaunchKernelGGL(k_gproj_mfma, dim3(h->Wl, (A.S.NB + 63) / 64, 2), dim3(64), 0, h->stream, A);
+    hipLaunchKernelGGL(k_gproj_mfma,
+                       dim3((unsigned)(((int64_t)h->Wl * std::max(A.S.ns[0], A.S.ns[1]) + 63) / 64),
+                            (A.S.NB + 63) / 64, 2),
+                       dim3(64), 0, h->stream, A, h->Wl);
   prof_end(h);
   if (A.S.nloc > 0) {
     prof_begin(h, QMCW_PROF_FULL_NLPP);

@@ -745,61 +745,77 @@ void k_full_det(KArgs A) {
 #ifndef QMCW_ORD   // launched from the host translation unit only
 typedef double gp_d4 __attribute__((ext_vector_type(4)));
 
-// Projected coefficients of the pseudopotential on FP64 MFMA: for walker
-// blockIdx.x and spin blockIdx.z, gp[i][b] = sum_j G(j, i) coef_occ(b, j)
-// for the spin's electrons i and the basis functions b of column block
-// blockIdx.y (64 columns), i.e. [ns x nocc] x [nocc x 64] products in
-// v_mfma_f64_16x16x4f64 tiles, 16 electrons by 16 functions, k over the
-// occupied orbitals four at a time.  One wavefront per block.  The sums
-// run in the MFMA's order, not group_gproj's.
-__global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A) {
+// Projected coefficients of the pseudopotential on FP64 MFMA:
+// gp[i][b] = sum_j G(j, i) coef_occ(b, j) for every walker w < nw and
+// electron i of spin blockIdx.z, as one GEMM [nw*ns x nocc] x [nocc x NB]
+// whose rows are the (walker, electron) pairs in walker order: block
+// blockIdx.x takes 64 consecutive rows (16 walkers of a 4-electron spin, or
+// one walker and part of the next for 49 electrons) and blockIdx.y a block of
+// 64 basis functions, in v_mfma_f64_16x16x4f64 tiles, 16 rows by 16
+// functions, k over the occupied orbitals four at a time.  The coefficients
+// (the B operand) are shared by every walker.  One wavefront per block.
+// Each sum runs in the MFMA's order, not group_gproj's, and is the same
+// whatever rows share its wavefront.
+__global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A, int nw) {
   const DevSys &S = A.S;
-  const int w = blockIdx.x, cb = blockIdx.y, sp = blockIdx.z;
+  const int cb = blockIdx.y, sp = blockIdx.z;
   const int ns = S.ns[sp], nocc = S.nocc[sp], NB = S.NB, off = sp ? S.nup : 0;
-  if (ns == 0) return;
-  const double *ss = A.sc + (int64_t)w * A.SL.stride;
-  const double *__restrict__ g = ss + A.SL.g[sp];        // G(j, i) at g[j * ns + i]
-  double *__restrict__ gp = const_cast<double *>(ss) + A.SL.gproj;
+  const int64_t rows = (int64_t)nw * ns, row0 = (int64_t)blockIdx.x * 64;
+  if (ns == 0 || row0 >= rows) return;
+  const int64_t stride = A.SL.stride;
   const double *__restrict__ co = S.coef_occ[sp];        // coef(b, occ_j) at co[b * nocc + j]
   // operand layout: lane l supplies A(row l % 16, k = l / 16) and B(k = l / 16,
   // column l % 16); it receives D rows l / 16 + 4 q (q < 4) of column l % 16
   const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
   const int b0 = cb * 64;
-  for (int rt0 = 0; rt0 < ns; rt0 += 64) {              // up to 64 electrons per pass
-    gp_d4 acc[4][4];
+  // the G column of each of this lane's four A rows: G(j, i) of walker w at
+  // sc[w] + SL.g[sp] + j * ns + i
+  const double *__restrict__ ga[4];
+  bool va[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+  for (int t = 0; t < 4; t++) {
+    const int64_t rho = row0 + 16 * t + r;
+    va[t] = rho < rows;
+    const int64_t w = va[t] ? rho / ns : 0, i = va[t] ? rho % ns : 0;
+    ga[t] = A.sc + w * stride + A.SL.g[sp] + i;
+  }
+  gp_d4 acc[4][4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) acc[t][u] = gp_d4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < nocc; k0 += 4) {
-      const int kk = k0 + kq;
-      double a[4], bb[4];
+  for (int t = 0; t < 4; t++)
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const int i = rt0 + 16 * t + r;
-        a[t] = (i < ns && kk < nocc) ? g[kk * ns + i] : 0.0;
-        const int b = b0 + 16 * t + r;
-        bb[t] = (b < NB && kk < nocc) ? co[(int64_t)b * nocc + kk] : 0.0;
-      }
+    for (int u = 0; u < 4; u++) acc[t][u] = gp_d4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < nocc; k0 += 4) {
+    const int kk = k0 + kq;
+    double a[4], bb[4];
 #pragma unroll
-      for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bb[u], acc[t][u], 0, 0, 0);
+    for (int t = 0; t < 4; t++) {
+      a[t] = (va[t] && kk < nocc) ? ga[t][(int64_t)kk * ns] : 0.0;
+      const int b = b0 + 16 * t + r;
+      bb[t] = (b < NB && kk < nocc) ? co[(int64_t)b * nocc + kk] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int i = rt0 + 16 * t + kq + 4 * q, b = b0 + 16 * u + r;
-          if (i < ns && b < NB) {
-            const double v = acc[t][u][q];
-            gp[(int64_t)(off + i) * NB + b] = A.bf16 ? (double)bf16r((float)v) : v;
-          }
-        }
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bb[u], acc[t][u], 0, 0, 0);
   }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int64_t rho = row0 + 16 * t + kq + 4 * q;
+      if (rho >= rows) continue;
+      const int64_t w = rho / ns, i = rho % ns;
+      double *__restrict__ gpr = A.sc + w * stride + A.SL.gproj + (off + i) * NB;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int b = b0 + 16 * u + r;
+        if (b < NB) {
+          const double v = acc[t][u][q];
+          gpr[b] = A.bf16 ? (double)bf16r((float)v) : v;
+        }
+      }
+    }
 }
 #endif
 

@@ -44,8 +44,9 @@ def test_chain98_psi_at_and_vmc_step(G):
 
 def test_chain98_projection_on_mfma_matches_valu(monkeypatch):
     """The pseudopotential projection gp = G^T coef_occ of the chain's
-    49 x 49 determinants on FP64 MFMA (k_gproj_mfma, the default for
-    determinants of 16 or more electrons) against the VALU loop of
+    49 x 49 determinants on FP64 MFMA (k_gproj_mfma, the default; a block of
+    64 (walker, electron) rows holds one walker and part of the next here)
+    against the VALU loop of
     k_full_det (QMCW_NO_GPROJ_MFMA=1): the local energies (whose nonlocal
     part is linear in gp) agree to 1e-12 relative."""
     W = 5

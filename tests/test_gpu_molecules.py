@@ -42,3 +42,31 @@ def test_molecule_parity(seed, nelec, ncent, gh):
     assert np.array_equal(v["naccept"], o["naccept"])
     assert np.array_equal(e.get_walkers(), o["x"])
     assert _rel(v["eloc"], o["eloc"], 1e-8) < 1e-9
+
+
+@pytest.mark.parametrize("which", ["c2", "mol5", "mol1"])
+def test_projection_on_mfma_matches_valu(monkeypatch, which):
+    """The pseudopotential projection gp = G^T coef_occ on FP64 MFMA
+    (k_gproj_mfma: rows = (walker, electron) pairs of many walkers, 16 walkers
+    of a 4-electron spin per 64-row block) against the VALU loop of
+    k_full_det (QMCW_NO_GPROJ_MFMA=1), on walker counts that leave the last
+    block partly empty, spins of 4 + 4, 3 + 2 and 1 + 1 electrons: the local
+    energies (whose nonlocal part is linear in gp) agree to 1e-12 relative
+    and equal the oracle's to 1e-9."""
+    s = {"c2": lambda: system.c2_dimer(),
+         "mol5": lambda: system.random_molecule(101, nelec=5, ncent=1),
+         "mol1": lambda: system.random_molecule(106, nelec=1, ncent=2)}[which]()
+    assert s.nloc > 0
+    W = 37
+    x = system.random_walkers(s, W, seed=44)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("QMCW_NO_GPROJ_MFMA", flag)
+        e = Engine(s, W)
+        e.set_walkers(x)
+        out[flag] = e.psi_at()
+        e.close()
+    assert _rel(out["0"]["eloc"], out["1"]["eloc"], 1e-8) < 1e-12
+    assert np.array_equal(out["0"]["logpsid"], out["1"]["logpsid"])
+    r = oracle.Oracle(s).eval_batch(x)
+    assert _rel(out["0"]["eloc"], r["out9"][:, 3], 1e-8) < 1e-9
