@@ -263,8 +263,20 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
   }
 }
 
-template <typename T>
-__device__ T jas_en_val(const DevSys &S, const double *ri_v) {
+// A per-lane array stored lane-interleaved: element e of this lane at
+// p[e * s] (s = the lanes sharing the region), so that the lanes' accesses
+// to one element are contiguous
+template <typename V>
+struct LaneArr {
+  V *p;
+  int s;
+  __device__ __forceinline__ V &operator[](int e) const { return p[(int64_t)e * s]; }
+  __device__ __forceinline__ LaneArr<V> operator+(int e) const { return {p + (int64_t)e * s, s}; }
+};
+
+// ri_v: a pointer or a LaneArr of the M distances
+template <typename T, typename RA>
+__device__ T jas_en_val(const DevSys &S, RA ri_v) {
   const T k = T(S.scalek);
   T f = 0;
   for (int ic = 0; ic < S.M; ic++) {
@@ -352,9 +364,9 @@ __device__ __forceinline__ T jas_b_val(const DevSys &S, bool parallel_spin, T u)
 // with every other electron where it is: its e-n terms plus the e-e and
 // e-e-n terms of all pairs (i, j).  bt[j * M + ic] = 1 - k scaled(r_j,ic)
 // of the current configuration; uq is per-lane scratch of N values.
-template <typename T, int ORD>
+template <typename T, int ORD, typename RA, typename UA>
 __device__ __forceinline__ T jas_point_val(const DevSys &S, int i, const double *xn, const double *x,
-                           const double *rnew, const T *bt, T *uq) {
+                                           RA rnew, const T *bt, UA uq) {
   const int N = S.N, M = S.M;
   const T k = T(S.scalek);
   T f = jas_en_val<T>(S, rnew);

@@ -858,11 +858,12 @@ __device__ __forceinline__ void rot_axes(int *st4, double *ax) {
 
 // Pseudopotential quadrature points of the in-range pairs plist[0..npair)
 // taken by this lane: sum of w exp(dJ) v P_l (Psi_D ratio).  rnew: 4M
-// doubles and uq: N values of per-lane scratch (LDS or global).
+// doubles and uq: N values of per-lane scratch (LDS or global), stored
+// lane-interleaved.
 template <typename T, int G, int ORD>
 __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, const double *ss,
-                                              int npair, const int *plist, double *rnew, T *uq,
-                                              double *tqw, int lane) {
+                                              int npair, const int *plist, LaneArr<double> rnew,
+                                              LaneArr<T> uq, double *tqw, int lane) {
   const DevSys &S = A.S;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -873,7 +874,7 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
   const double *gp = ss + SL.gproj;
   const double *quad = ws + L.quad;
   const int nq = S.nquad;
-  double *rvnew = rnew + M;
+  const LaneArr<double> rvnew = rnew + M;
   double vpsp = 0.0;
   const double *prot = ss + SL.prot;
   for (int t = lane; t < npair * nq; t += G) {
@@ -1008,13 +1009,16 @@ void k_full_nlpp(KArgs A) {
     if (A.dmc_rot && !mover) {
       vpsp = 0.0;
     } else if (A.lds_nlpp) {
+      // the block's 64 lanes interleaved: 4M doubles, then N values
       extern __shared__ double dyn_lds[];
-      double *lb = dyn_lds + (int64_t)threadIdx.x * A.lds_nlpp;
-      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, lb, reinterpret_cast<T *>(lb + 4 * M),
-                                    tqw, lane);
+      const LaneArr<double> rq{dyn_lds + threadIdx.x, kWave};
+      const LaneArr<T> uq{reinterpret_cast<T *>(dyn_lds + 4 * M * kWave) + threadIdx.x, kWave};
+      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, rq, uq, tqw, lane);
     } else {
-      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, ss + SL.qd + (int64_t)lane * 4 * M,
-                                    reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N), tqw, lane);
+      // the walker's G lanes interleaved in its scratch
+      const LaneArr<double> rq{ss + SL.qd + lane, G};
+      const LaneArr<T> uq{reinterpret_cast<T *>(ss + SL.uq) + lane, G};
+      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, rq, uq, tqw, lane);
     }
     vpsp = gsum<G>(vpsp);
   }
@@ -1058,8 +1062,8 @@ void k_tm_ratios(KArgs A, int ie, int cw, const double *pts, const int *cnt, dou
   // read before the next full evaluation rebuilds them all)
   group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane, ie);
   __syncthreads();
-  double *rnew = ss + SL.qd + (int64_t)lane * 4 * M, *rvnew = rnew + M;
-  T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
+  const LaneArr<double> rnew{ss + SL.qd + lane, G}, rvnew = rnew + M;
+  const LaneArr<T> uq{reinterpret_cast<T *>(ss + SL.uq) + lane, G};
   const double *gpi = ss + SL.gproj + (int64_t)ie * S.NB;
   double fold = fso[ie * N + ie];
   for (int jj = 0; jj < N; jj++)
