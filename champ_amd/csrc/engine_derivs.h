@@ -27,9 +27,8 @@ struct SlotAcc {
 // a-coefficient pieces of the e-n terms of one electron (distances r[M],
 // vectors rv[3M]), qmc/vmc/deriv_jastrow4.f90:880-1090.  cv scales the value
 // pieces (0: none); FULL adds the gradient and the laplacian.
-template <bool FULL>
-__device__ void en_slots(const DevSys &S, const double *r, const double *rv, double cv,
-                         const SlotAcc &acc) {
+template <bool FULL, typename RA, typename RVA>
+__device__ void en_slots(const DevSys &S, RA r, RVA rv, double cv, const SlotAcc &acc) {
   const double k = S.scalek, ar = S.asymp_r;
   const int na = S.norda + 1;
   for (int ic = 0; ic < S.M; ic++) {
@@ -210,10 +209,9 @@ __device__ void pair_slots(const DevSys &S, bool par, double rij, const double *
 // uqd: per-lane scratch of N values).  Summed over j, the term values of a
 // centre are A^(l+m) S(k, m) + A^m S(k, l+m) with S(k, n) = sum_j U_j^k B_j^n,
 // so each slot takes one addition per call.
-template <int ORD>
+template <int ORD, typename RA, typename UA>
 __device__ void point_pair_slots(const DevSys &S, int i, const double *xn, const double *x,
-                                 const double *ri_v, const double *btd, double *uqd, double cv,
-                                 const SlotAcc &acc) {
+                                 RA ri_v, const double *btd, UA uqd, double cv, const SlotAcc &acc) {
   const int N = S.N, M = S.M;
   const double k = S.scalek, ar = S.asymp_r;
   // b-coefficient pieces summed over the other electrons in registers
@@ -375,11 +373,12 @@ __global__ QMCW_HEAVY void k_deriv_jas(KArgs A) {
 // The (electron, centre) pairs plist[kp] taken by this lane, whole: the
 // weighted basis sums (pbc rows) and the slot pieces of dpe (dpe_raw rows)
 // over the pair's quadrature points.  rnew: 4M doubles, uq: N values of the
-// kernel's T, uqd: N doubles of per-lane scratch (LDS or global).
+// kernel's T, uqd: N doubles of per-lane scratch (LDS or global), stored
+// lane-interleaved.
 template <typename T, int G, int ORD>
 __device__ __forceinline__ void deriv_nlpp_pairs(const KArgs &A, const double *ws, double *ss, int npair,
-                                                 const int *plist, double *rnew, T *uq, double *uqd,
-                                                 int lane) {
+                                                 const int *plist, LaneArr<double> rnew, LaneArr<T> uq,
+                                                 LaneArr<double> uqd, int lane) {
   const DevSys &S = A.S;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -389,7 +388,7 @@ __device__ __forceinline__ void deriv_nlpp_pairs(const KArgs &A, const double *w
   const double *vnl = ss + SL.vnl, *quad = ws + L.quad, *gp = ss + SL.gproj, *btd = ss + DL.btd;
   const T *bt = reinterpret_cast<const T *>(ss + SL.bt);    // filled by k_full_nlpp
   double *pbc = ss + DL.pbc, *draw = ss + DL.dpe_raw;
-  double *rvnew = rnew + M;
+  const LaneArr<double> rvnew = rnew + M;
   for (int kp = lane; kp < npair; kp += G) {
     const int p = plist[kp];
     double *pbrow = pbc + (int64_t)p * NB;
@@ -461,9 +460,8 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   const DLayout &DL = A.DL;
   const double *ren = ws + L.r_en;
   const double *vnl = ss + SL.vnl;
-  T *uq = reinterpret_cast<T *>(ss + SL.uq + (int64_t)lane * N);
   double *pbc = ss + DL.pbc, *draw = ss + DL.dpe_raw;
-  double *btd = ss + DL.btd, *uqd = ss + DL.uqd + (int64_t)lane * N;
+  double *btd = ss + DL.btd;
   for (int p = lane; p < N * M; p += G) {
     double bu, t1, t2;
     scaled<double>(S.isc, S.scalek, ren[p], bu, t1, t2);
@@ -478,12 +476,18 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   // per-lane distances of the points and e-e-n variables of the other
   // electrons: LDS when the launch reserved it
   if (A.lds_dnlpp) {
+    // the block's 64 lanes interleaved: 4M doubles, N values of T, N doubles
     extern __shared__ double dyn_lds[];
-    double *lb = dyn_lds + (int64_t)threadIdx.x * A.lds_dnlpp;
-    T *uql = reinterpret_cast<T *>(lb + 4 * M);
-    deriv_nlpp_pairs<T, G, ORD>(A, ws, ss, npair, plist, lb, uql, lb + A.lds_dnlpp - N, lane);
+    const LaneArr<double> rq{dyn_lds + threadIdx.x, kWave};
+    const LaneArr<T> uq{reinterpret_cast<T *>(dyn_lds + 4 * M * kWave) + threadIdx.x, kWave};
+    const LaneArr<double> uqd{dyn_lds + (int64_t)(A.lds_dnlpp - N) * kWave + threadIdx.x, kWave};
+    deriv_nlpp_pairs<T, G, ORD>(A, ws, ss, npair, plist, rq, uq, uqd, lane);
   } else {
-    deriv_nlpp_pairs<T, G, ORD>(A, ws, ss, npair, plist, ss + SL.qd + (int64_t)lane * 4 * M, uq, uqd, lane);
+    // the walker's G lanes interleaved in its scratch
+    const LaneArr<double> rq{ss + SL.qd + lane, G};
+    const LaneArr<T> uq{reinterpret_cast<T *>(ss + SL.uq) + lane, G};
+    const LaneArr<double> uqd{ss + DL.uqd + lane, G};
+    deriv_nlpp_pairs<T, G, ORD>(A, ws, ss, npair, plist, rq, uq, uqd, lane);
   }
   __syncthreads();
   // per-electron basis sums over the centres in range, kept in the row of
