@@ -183,10 +183,36 @@ struct PairOut {
   T f, gi[3], gj[3], li, lj;
 };
 
-template <typename T, int ORD>
-__device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
-                         const T *rvij, const double *ri_v, const double *rj_v,
-                         const double *rvi, const double *rvj, PairOut<T> &o) {
+// An electron's distances to the centres for jas_pair: read from its rows
+// of the state (EnRows) or formed from its position the way the moves and
+// the full evaluation form the rows (EnFromX: the same differences and the
+// same square root, so the same values), which spares the lanes of a group
+// a strided read of each other electron's rows.
+struct EnRows {
+  const double *r, *rv;
+  __device__ __forceinline__ void get(const DevSys &, int ic, double &ri, double *rvi) const {
+    ri = r[ic];
+    rvi[0] = rv[3 * ic]; rvi[1] = rv[3 * ic + 1]; rvi[2] = rv[3 * ic + 2];
+  }
+};
+struct EnFromX {
+  double x0, x1, x2;
+  __device__ __forceinline__ void get(const DevSys &S, int ic, double &ri, double *rvi) const {
+    const double p[3] = {x0, x1, x2};
+    double s2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const double d = p[q] - cst(S.cent)[3 * ic + q];
+      rvi[q] = d;
+      s2 += d * d;
+    }
+    ri = sqrt(s2);
+  }
+};
+
+template <typename T, int ORD, typename PI, typename PJ>
+__device__ void jas_pair_g(const DevSys &S, bool parallel_spin, T rij, const T *rvij, const PI &pi,
+                           const PJ &pj, PairOut<T> &o) {
   const T k = T(S.scalek);
   T sspinn = T(1);
   int isb = 0;
@@ -233,7 +259,10 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
   Ud2 = -k * Ud2;
   for (int ic = 0; ic < S.M; ic++) {
     int it = cst(S.iwctype)[ic];
-    T ri = (T)ri_v[ic], rj = (T)rj_v[ic];
+    double rid, rjd, rvi[3], rvj[3];
+    pi.get(S, ic, rid, rvi);
+    pj.get(S, ic, rjd, rvj);
+    T ri = (T)rid, rj = (T)rjd;
     T A, Ad1, Ad2, B, Bd1, Bd2;
     scaled<T>(S.isc, k, ri, A, Ad1, Ad2);
     scaled<T>(S.isc, k, rj, B, Bd1, Bd2);
@@ -252,8 +281,8 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
     o.f += fc;
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-      o.gi[q] += gA * (T)rvi[3 * ic + q] + gU * rvij[q];
-      o.gj[q] += gB * (T)rvj[3 * ic + q] - gU * rvij[q];
+      o.gi[q] += gA * (T)rvi[q] + gU * rvij[q];
+      o.gj[q] += gB * (T)rvj[q] - gU * rvij[q];
     }
     // cos of the angle between r_ij and r_i: (rij^2 + ri^2 - rj^2)/(2 rij ri)
     T cI = (rij * rij + ri * ri - rj * rj) / (rij * ri);
@@ -261,6 +290,13 @@ __device__ void jas_pair(const DevSys &S, bool parallel_spin, T rij,
     o.li += T(2) * (gU + gA) + gUU + gAA + gUA * cI;
     o.lj += T(2) * (gU + gB) + gUU + gBB + gUB * cJ;
   }
+}
+
+template <typename T, int ORD>
+__device__ __forceinline__ void jas_pair(const DevSys &S, bool parallel_spin, T rij, const T *rvij,
+                                         const double *ri_v, const double *rj_v, const double *rvi,
+                                         const double *rvj, PairOut<T> &o) {
+  jas_pair_g<T, ORD>(S, parallel_spin, rij, rvij, EnRows{ri_v, rvi}, EnRows{rj_v, rvj}, o);
 }
 
 // A per-lane array stored lane-interleaved: element e of this lane at

@@ -1318,7 +1318,6 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   // ---- jastrow4e (+ per-electron laplacian)
   double *fsn = mv + ML.fsn, *fijn_i = mv + ML.fijn_i, *fijn_j = mv + ML.fijn_j,
          *lapn = mv + ML.lapn_i, *fjn = mv + ML.fjn;
-  const double *ren_o = ws + L.r_en, *rven_o = ws + L.rvec_en;
   double dsum = 0.0, gix = 0, giy = 0, giz = 0, lapi = 0;
   for (int j = lane; j < N; j += G) {
     if (j == ie) {  // e-n diagonal term, on the lane the pair list leaves free
@@ -1337,12 +1336,15 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
     T rvij[3];
     for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * j + q];
-    const double *rI = ren, *rvI = rven, *rJ = ren_o + j * M, *rvJ = rven_o + j * M * 3;
+    const double *rI = ren, *rvI = rven;
     PairOut<T> po;
     bool par = (hi < S.nup) || (lo >= S.nup);
     // moved electron's arrays go to whichever slot (hi or lo) it occupies
-    if (hi == ie) jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rI, rJ, rvI, rvJ, po);
-    else jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rJ, rI, rvJ, rvI, po);
+    // the other electron's distances formed from its position (see EnFromX)
+    const EnRows pI{rI, rvI};
+    const EnFromX pJ{x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+    if (hi == ie) jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pI, pJ, po);
+    else jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pJ, pI, po);
     // old pair values before the stores to the pending buffers below
     const double f_old = fso[hi * N + lo];
     const double gi_old[3] = {fijo[3 * (ie * N + j)], fijo[3 * (ie * N + j) + 1], fijo[3 * (ie * N + j) + 2]};
@@ -1534,7 +1536,6 @@ void k_move_a(KArgs A, int ie, int mode, const double *xnew_in, double delta, do
   // ---- jastrow4e (+ per-electron laplacian)
   double *fsn = mv + ML.fsn, *fijn_i = mv + ML.fijn_i, *fijn_j = mv + ML.fijn_j,
          *lapn = mv + ML.lapn_i, *fjn = mv + ML.fjn;
-  const double *ren_o = ws + L.r_en, *rven_o = ws + L.rvec_en;
   double dsum = 0.0, gix = 0, giy = 0, giz = 0, lapi = 0;
   for (int j = lane; j < N; j += G) {
     if (j == ie) {  // e-n diagonal term, on the lane the pair list leaves free
@@ -1553,12 +1554,15 @@ void k_move_a(KArgs A, int ie, int mode, const double *xnew_in, double delta, do
     int hi = j < ie ? ie : j, lo = j < ie ? j : ie;
     T rvij[3];
     for (int q = 0; q < 3; q++) rvij[q] = (T)rvee[3 * j + q];
-    const double *rI = ren, *rvI = rven, *rJ = ren_o + j * M, *rvJ = rven_o + j * M * 3;
+    const double *rI = ren, *rvI = rven;
     PairOut<T> po;
     bool par = (hi < S.nup) || (lo >= S.nup);
     // moved electron's arrays go to whichever slot (hi or lo) it occupies
-    if (hi == ie) jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rI, rJ, rvI, rvJ, po);
-    else jas_pair<T, ORD>(S, par, (T)ree[j], rvij, rJ, rI, rvJ, rvI, po);
+    // the other electron's distances formed from its position (see EnFromX)
+    const EnRows pI{rI, rvI};
+    const EnFromX pJ{x[3 * j], x[3 * j + 1], x[3 * j + 2]};
+    if (hi == ie) jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pI, pJ, po);
+    else jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pJ, pI, po);
     // old pair values before the stores to the pending buffers below
     const double f_old = fso[hi * N + lo];
     const double gi_old[3] = {fijo[3 * (ie * N + j)], fijo[3 * (ie * N + j) + 1], fijo[3 * (ie * N + j) + 2]};
