@@ -345,7 +345,8 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0, re
     e.set_rng(rng.walker_seeds("1837465927472523", cap, stride=1 << 20))
     e.set_branch_rng(np.array([1, 2, 3, 5], np.int32))
     e.set_reinversion(reinv)
-    g = e.autotune()
+    e.autotune()
+    g = [int(v) for v in e.group_widths()]
     if equil:
         e.vmc_device(equil, 0.3)
     e.dmc_begin(W0)
@@ -476,7 +477,7 @@ def main():
                                       f"from 20 VMC sweeps, re-inversion every {args.reinversion} moves"
                                       + (", T-moves" if args.tmoves else ""),
                           "tmoves": args.tmoves, "reinversion_every": args.reinversion,
-                          "precision": prec, "group_width": d["G"]},
+                          "precision": prec, "group_widths": d["G"]},
                "walker_moves_per_s": {q: r["rate"] for q, r in runs.items()},
                "energy_bias_vs_fp64": bias or None,
                "nwalk_final": d["nwalk_final"], "nwalk_min_mean_max": d["nwalk_range"],

@@ -136,6 +136,7 @@ struct SLayout {
   int64_t plist;             // [N*M] ints: (electron, centre) pairs in pseudopotential range
   int64_t prot;              // [N*M][10] DMC: axes of each in-range pair's grid rotation, then
                              // the pair's rank among its electron's pairs
+  int64_t fold;              // [N] Jastrow pair sum of each electron at its current position
   int64_t mv;                // pending one-electron move buffers
   int64_t stride;
 };

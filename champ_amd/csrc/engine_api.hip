@@ -1117,6 +1117,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   SL.uq = take((int64_t)kWave * N);
   SL.plist = take(((int64_t)N * M + 1) / 2);
   SL.prot = take(S.nloc > 0 ? 10 * (int64_t)N * M : 1);
+  SL.fold = take(N);
   SL.mv = take(ML.stride);
   if (S.nparm > 0) {
     DLayout &DL = h->A.DL;

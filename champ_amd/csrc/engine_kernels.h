@@ -869,7 +869,7 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
   const SLayout &SL = A.SL;
   const int N = S.N, M = S.M;
   const double *x = ws + L.x, *rven = ws + L.rvec_en, *ren = ws + L.r_en, *fso = ws + L.fso;
-  const double *vnl = ss + SL.vnl;
+  const double *vnl = ss + SL.vnl, *foldv = ss + SL.fold;
   const T *bt = reinterpret_cast<const T *>(ss + SL.bt);
   const double *gp = ss + SL.gproj;
   const double *quad = ws + L.quad;
@@ -911,10 +911,17 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
       rd += (double)shell_dot<T, double>(S, sh, (T)rvnew[3 * jc], (T)rvnew[3 * jc + 1],
                                          (T)rvnew[3 * jc + 2], (T)rnew[jc], gpi);
     }
-    // Jastrow value change
-    double fold = fso[i * N + i];
-    for (int jj = 0; jj < N; jj++)
-      if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
+    // Jastrow value change against the electron's current pair sum: kept
+    // per electron for small N; summed here for large N, where the sum's
+    // loads overlap the point's other work (measured faster on the chain)
+    double fold;
+    if (N <= 32) {
+      fold = foldv[i];
+    } else {
+      fold = fso[i * N + i];
+      for (int jj = 0; jj < N; jj++)
+        if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
+    }
     const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
     double vpot = 0.0;
     for (int l = 1; l <= cst(S.npotd)[ict]; l++)
@@ -975,6 +982,24 @@ void k_full_nlpp(KArgs A) {
       bt[p] = T(1) - (T)S.scalek * bu;
     }
     ploc = gsum<G>(ploc);
+    // the pair sum of the Jastrow of each electron at its current position
+    // (what a quadrature point's value is compared with): one pass over the
+    // lower triangle of the pair values, row by row with the lanes along
+    // the row, so every load is contiguous.  Lane jj % G owns foldv[jj]:
+    // it adds the column terms as the rows pass and the row total of row
+    // jj (summed over the lanes) when its row comes.
+    double *foldv = ss + SL.fold;
+    for (int jj = lane; jj < N; jj += G) foldv[jj] = fso[jj * N + jj];
+    for (int i = 1; i < N && N <= 32; i++) {
+      double rs = 0.0;
+      for (int jj = lane; jj < i; jj += G) {
+        const double v = fso[i * N + jj];
+        foldv[jj] += v;
+        rs += v;
+      }
+      rs = gsum<G>(rs);
+      if (lane == i % G) foldv[i] += rs;
+    }
     __syncthreads();
     // quadrature points only of the pairs in range, spread evenly over the lanes
     int *plist = reinterpret_cast<int *>(ss + SL.plist);
