@@ -95,6 +95,7 @@ struct qmcw_handle {
   int npow2 = 1;
   int host_split_join = 0;    // QMCW_HOST_SPLIT_JOIN=1: split_join on the host
   int no_gproj_mfma = 0;      // QMCW_NO_GPROJ_MFMA=1: the projection on VALU in k_full_det
+  int no_een_tab = 0;         // QMCW_NO_EEN_TAB=1: the move's e-e-n terms summed per pair
   int32_t brng[4] = {0, 0, 0, 1};
   // event timing of the device driver
   int prof = 0;
@@ -675,12 +676,20 @@ static void launch_move_bc(qmcw_handle *h, const KArgs &A, size_t smem_b, size_t
 static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
                          double fbias, double *out, long long *nacc = nullptr) {
   const int G = h->Gk[KC_MOVE];
-  // the moved electron's basis values in LDS when they fit (<= 48 KB per block)
-  const size_t lds = (size_t)(kWave / G) * h->A.S.NB * 5 *
-                     (h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float));
   KArgs A = h->A;
+  // the moved electron's basis values in LDS when they fit (<= 48 KB per
+  // block); in the single-kernel move the same region first holds the
+  // e-e-n tables of the Jastrow row (one per centre)
+  const size_t tsz = h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float);
+  const int ordk = std::max(2, A.S.nordc);
+  const int tab = (A.S.nordc > 1 && !h->split_move && !h->no_een_tab)
+                      ? A.S.M * (3 * ((ordk + 1) * (ordk + 2) / 2) + 4) : 0;
+  const int wst = std::max(A.S.NB * 5, tab);
+  const size_t lds = (size_t)(kWave / G) * wst * tsz;
   A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;
-  const size_t phi_b = A.lds_phi ? (lds + 7) / 8 * 8 : 0;
+  A.een_tab = (A.lds_phi && tab > 0) ? 1 : 0;
+  A.lds_wst = A.een_tab ? wst : A.S.NB * 5;
+  const size_t phi_b = A.lds_phi ? ((size_t)(kWave / G) * A.lds_wst * tsz + 7) / 8 * 8 : 0;
   // eval_y's determinant values and alpha inverses staged in LDS after the
   // basis values, when the two together stay within 48 KB per block and
   // still let every block of the launch be resident at once (160 KB of LDS
@@ -1179,6 +1188,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
   if (const char *e = getenv("QMCW_DMC_UNFUSED")) h->dmc_unfused = atoi(e) != 0;
   if (const char *e = getenv("QMCW_HOST_SPLIT_JOIN")) h->host_split_join = atoi(e) != 0;
   if (const char *e = getenv("QMCW_NO_GPROJ_MFMA")) h->no_gproj_mfma = atoi(e) != 0;
+  if (const char *e = getenv("QMCW_NO_EEN_TAB")) h->no_een_tab = atoi(e) != 0;
   {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) == hipSuccess &&

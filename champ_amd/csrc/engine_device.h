@@ -375,6 +375,179 @@ __device__ __forceinline__ T een_horner(const T *gam, T U, T B) {
   return f;
 }
 
+// The e-e-n polynomial of a fixed A (the moved electron at a centre) with
+// its first and second A derivatives, each as a bivariate polynomial in
+// (U, B) laid out as een_gamma's table: g[0..NG) the value, g[NG..2NG) d/dA,
+// g[2NG..3NG) d2/dA2 (NG = een_gi<ORD>(ORD, 0) + 1).
+template <typename T, int ORD>
+__device__ __forceinline__ void een_gamma3(T A, cptr<double> c, T *g) {
+  constexpr int NG = een_gi<ORD>(ORD, 0) + 1;
+  T ap[ORD + 1];
+  ap[0] = T(1);
+#pragma unroll
+  for (int q = 1; q <= ORD; q++) ap[q] = ap[q - 1] * A;
+  T r[3 * NG];
+#pragma unroll
+  for (int q = 0; q < 3 * NG; q++) r[q] = T(0);
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k <= ORD; k++)
+#pragma unroll
+    for (int m = 0; 2 * m <= ORD - k; m++)
+#pragma unroll
+      for (int l = 0; l <= ORD - k - 2 * m; l++)
+        if (QMCW_EEN_TERM(k, l, m)) {
+          const T ct = T(c[idx]);
+          idx++;
+          const int lm = l + m, a = een_gi<ORD>(k, m), b = een_gi<ORD>(k, lm);
+          r[a] += ct * ap[lm];
+          r[b] += ct * ap[m];
+          if (lm >= 1) r[NG + a] += ct * T(lm) * ap[lm >= 1 ? lm - 1 : 0];
+          if (m >= 1) r[NG + b] += ct * T(m) * ap[m >= 1 ? m - 1 : 0];
+          if (lm >= 2) r[2 * NG + a] += ct * T(lm * (lm - 1)) * ap[lm >= 2 ? lm - 2 : 0];
+          if (m >= 2) r[2 * NG + b] += ct * T(m * (m - 1)) * ap[m >= 2 ? m - 2 : 0];
+        }
+#pragma unroll
+  for (int q = 0; q < 3 * NG; q++) g[q] = r[q];
+}
+
+// T values per centre of the move's e-e-n tables: the three tables of
+// een_gamma3, then dA/dr, d2A/dr2 and r of the moved electron
+template <int ORD>
+__host__ __device__ constexpr int een_tab_stride() { return 3 * (een_gi<ORD>(ORD, 0) + 1) + 4; }
+
+// The moved electron's tables at its new position (distances rn to the
+// centres), one centre per lane
+template <typename T, int ORD, int G>
+__device__ __forceinline__ void een_tables(const DevSys &S, const double *rn, T *tab, int lane) {
+  constexpr int NG = een_gi<ORD>(ORD, 0) + 1, TS = een_tab_stride<ORD>();
+  const T k = T(S.scalek);
+  for (int ic = lane; ic < S.M; ic += G) {
+    T A, Ad1, Ad2;
+    scaled<T>(S.isc, k, (T)rn[ic], A, Ad1, Ad2);
+    A = T(1) - k * A;
+    T *tb = tab + ic * TS;
+    een_gamma3<T, ORD>(A, cst(S.c) + cst(S.iwctype)[ic] * S.nterms, tb);
+    tb[3 * NG] = -k * Ad1;
+    tb[3 * NG + 1] = -k * Ad2;
+    tb[3 * NG + 2] = (T)rn[ic];
+  }
+}
+
+// jas_pair for a pair of the moved electron m (at the position of the
+// tables tab, its vectors to the centres rvm_c) and another electron o
+// (distances from po): the e-e-n polynomial and its derivatives come from
+// Horner sums over the tables instead of the term-by-term sum.  rvmo = r_m
+// - r_o.  Output in jas_pair's convention with electron i = m when m_hi,
+// else i = o.
+template <typename T, int ORD, typename PO>
+__device__ void jas_pair_tab(const DevSys &S, bool parallel_spin, T rij, const T *rvmo, const T *tab,
+                             const double *rvm_c, const PO &po, bool m_hi, PairOut<T> &out) {
+  constexpr int NG = een_gi<ORD>(ORD, 0) + 1, TS = een_tab_stride<ORD>();
+  const T k = T(S.scalek);
+  T sspinn = T(1);
+  int isb = 0;
+  double asymp_b = S.asymp_jasb0;
+  if (parallel_spin) {
+    if (S.nspin2b == 2) isb = 1;
+    else if (S.nocuspb == 0) sspinn = T(0.5);
+    asymp_b = S.asymp_jasb1;
+  }
+  const double *b = S.b + isb * (S.nordb + 1);
+  T u, d1, d2;
+  scaled<T>(S.isc, k, rij, u, d1, d2);
+  T bot = T(1) + T(b[1]) * u;
+  T f = sspinn * T(b[0]) * u / bot;
+  T fu = sspinn * T(b[0]) / (bot * bot);
+  T fuu = T(-2) * fu * T(b[1]) / bot;
+  f -= T(asymp_b);
+  T up = u, upm = T(1);
+  for (int iord = 2; iord <= S.nordb; iord++) {
+    T bb = T(b[iord]);
+    fuu += bb * T(iord * (iord - 1)) * upm;
+    fu += bb * T(iord) * up;
+    upm = up;
+    up *= u;
+    f += bb * up;
+  }
+  fuu = fuu * d1 * d1 + fu * d2;
+  fu = fu * d1 / rij;
+  T gm[3], go[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) { gm[q] = fu * rvmo[q]; go[q] = -fu * rvmo[q]; }
+  T lm = fuu + T(2) * fu, lo = lm;
+  if (S.nordc > 1) {
+    T U, Ud1, Ud2;
+    scaled<T>(S.isc, k, rij, U, Ud1, Ud2);
+    U = T(1) - k * U;
+    Ud1 = -k * Ud1;
+    Ud2 = -k * Ud2;
+    T uq[ORD + 1];
+    uq[0] = T(1);
+#pragma unroll
+    for (int q = 1; q <= ORD; q++) uq[q] = uq[q - 1] * U;
+    for (int ic = 0; ic < S.M; ic++) {
+      const T *tb = tab + ic * TS;
+      double rod, rvo[3];
+      po.get(S, ic, rod, rvo);
+      const T ro = (T)rod;
+      T B, Bd1, Bd2;
+      scaled<T>(S.isc, k, ro, B, Bd1, Bd2);
+      B = T(1) - k * B; Bd1 = -k * Bd1; Bd2 = -k * Bd2;
+      // rows of the tables in B (with the first two B derivatives of the
+      // value table), then the sums over the powers of U
+      T F = 0, FU = 0, FUU = 0, FB = 0, FBB = 0, FUB = 0, FA = 0, FAA = 0, FUA = 0;
+#pragma unroll
+      for (int kk = 0; kk <= ORD; kk++) {
+        T r = 0, rb = 0, rbb = 0, ra = 0, raa = 0;
+#pragma unroll
+        for (int n = ORD - kk; n >= 0; n--) {
+          rbb = rbb * B + T(2) * rb;
+          rb = rb * B + r;
+          r = r * B + tb[een_gi<ORD>(kk, n)];
+          ra = ra * B + tb[NG + een_gi<ORD>(kk, n)];
+          raa = raa * B + tb[2 * NG + een_gi<ORD>(kk, n)];
+        }
+        F += r * uq[kk];
+        FB += rb * uq[kk];
+        FBB += rbb * uq[kk];
+        FA += ra * uq[kk];
+        FAA += raa * uq[kk];
+        if (kk >= 1) {
+          FU += T(kk) * r * uq[kk - 1];
+          FUB += T(kk) * rb * uq[kk - 1];
+          FUA += T(kk) * ra * uq[kk - 1];
+        }
+        if (kk >= 2) FUU += T(kk * (kk - 1)) * r * uq[kk - 2];
+      }
+      const T Ad1 = tb[3 * NG], Ad2 = tb[3 * NG + 1], rm = tb[3 * NG + 2];
+      // chain rule to unscaled distances (A: the moved electron, B: the other)
+      const T gU = FU * Ud1 / rij, gUU = FUU * Ud1 * Ud1 + FU * Ud2;
+      const T gA = FA * Ad1 / rm, gAA = FAA * Ad1 * Ad1 + FA * Ad2;
+      const T gB = FB * Bd1 / ro, gBB = FBB * Bd1 * Bd1 + FB * Bd2;
+      const T gUA = FUA * Ud1 * Ad1, gUB = FUB * Ud1 * Bd1;
+      f += F;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        gm[q] += gA * (T)rvm_c[3 * ic + q] + gU * rvmo[q];
+        go[q] += gB * (T)rvo[q] - gU * rvmo[q];
+      }
+      const T cM = (rij * rij + rm * rm - ro * ro) / (rij * rm);
+      const T cO = (rij * rij + ro * ro - rm * rm) / (rij * ro);
+      lm += T(2) * (gU + gA) + gUU + gAA + gUA * cM;
+      lo += T(2) * (gU + gB) + gUU + gBB + gUB * cO;
+    }
+  }
+  out.f = f;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    out.gi[q] = m_hi ? gm[q] : go[q];
+    out.gj[q] = m_hi ? go[q] : gm[q];
+  }
+  out.li = m_hi ? lm : lo;
+  out.lj = m_hi ? lo : lm;
+}
+
 // e-e Pade + polynomial of a pair at scaled distance u
 template <typename T>
 __device__ __forceinline__ T jas_b_val(const DevSys &S, bool parallel_spin, T u) {

@@ -22,6 +22,9 @@ struct KArgs {
   int derivs;        // compute Jastrow/NLPP parameter derivative pieces
   int full_orb;      // evaluate all NO orbitals (orbital-rotation derivatives)
   int lds_phi;       // k_ratio_1e keeps the moved electron's basis values in LDS
+  int lds_wst;       // T values per walker of that LDS region (at least NB * 5)
+  int een_tab;       // k_ratio_1e: the moved electron's e-e-n tables in the same region
+                     // during the Jastrow row (een_tables, jas_pair_tab)
   int lds_det;       // k_full_det keeps a chunk's basis values in LDS
   int keep_state;    // the full evaluation takes the rows the committed moves kept in the state
                      // (distances, Jastrow pairs and, without full_orb, the orbitals)
@@ -1344,6 +1347,14 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   double *fsn = mv + ML.fsn, *fijn_i = mv + ML.fijn_i, *fijn_j = mv + ML.fijn_j,
          *lapn = mv + ML.lapn_i, *fjn = mv + ML.fjn;
   double dsum = 0.0, gix = 0, giy = 0, giz = 0, lapi = 0;
+  // the e-e-n tables of the new position, one per centre, in the LDS region
+  // the basis values take after the Jastrow row
+  extern __shared__ double dyn_lds[];
+  T *etab = reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * A.lds_wst;
+  if (A.een_tab) {
+    een_tables<T, ORD, G>(S, ren, etab, lane);
+    __syncthreads();
+  }
   for (int j = lane; j < N; j += G) {
     if (j == ie) {  // e-n diagonal term, on the lane the pair list leaves free
       T f, g[3], lp;
@@ -1368,8 +1379,15 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     // the other electron's distances formed from its position (see EnFromX)
     const EnRows pI{rI, rvI};
     const EnFromX pJ{x[3 * j], x[3 * j + 1], x[3 * j + 2]};
-    if (hi == ie) jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pI, pJ, po);
-    else jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pJ, pI, po);
+    if (A.een_tab) {
+      const T rvmo[3] = {hi == ie ? rvij[0] : -rvij[0], hi == ie ? rvij[1] : -rvij[1],
+                         hi == ie ? rvij[2] : -rvij[2]};
+      jas_pair_tab<T, ORD>(S, par, (T)ree[j], rvmo, etab, rvI, pJ, hi == ie, po);
+    } else if (hi == ie) {
+      jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pI, pJ, po);
+    } else {
+      jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pJ, pI, po);
+    }
     // old pair values before the stores to the pending buffers below
     const double f_old = fso[hi * N + lo];
     const double gi_old[3] = {fijo[3 * (ie * N + j)], fijo[3 * (ie * N + j) + 1], fijo[3 * (ie * N + j) + 2]};
@@ -1407,11 +1425,10 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   // ---- orbitals of the new position (occupied set of the spin)
   // basis values of the new position: LDS when the launch reserved it (the
   // two calls keep the LDS pointer visible as such to the compiler)
-  extern __shared__ double dyn_lds[];
   double *orbe = mv + ML.orbe;
+  if (A.een_tab) __syncthreads();      // the tables' region takes the basis values
   if (A.lds_phi)
-    orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * S.NB * 5,
-                      orbe, lane);
+    orbitals_1e<T, G>(A, s, nocc, ren, rven, etab, orbe, lane);
   else
     orbitals_1e<T, G>(A, s, nocc, ren, rven, reinterpret_cast<T *>(ss + SL.phi), orbe, lane);
   __syncthreads();

@@ -96,3 +96,32 @@ def test_single_determinant_move_equals_the_general_path(monkeypatch, mode):
         e.close()
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("tables", ["1", "0"])
+def test_chain98_one_electron_ratios_match_oracle(monkeypatch, tables):
+    """One-electron moves of the chain against the oracle's ratio_1e: the
+    move's e-e-n terms of the 97 pairs x 50 centres through the moved
+    electron's per-centre tables (the default) and summed term by term
+    (QMCW_NO_EEN_TAB=1); ratios, gradients and laplacians to 1e-9."""
+    monkeypatch.setenv("QMCW_NO_EEN_TAB", "0" if tables == "1" else "1")
+    W = 4
+    x = system.random_walkers(S, W, seed=45)
+    e = Engine(S, W)
+    e.set_group_width(64)
+    e.set_walkers(x)
+    e.psi_at()
+    O = oracle.Oracle(S)
+    ws = [O.L.orc_walker_create(O.h) for _ in range(W)]
+    for w in range(W):
+        O.eval_full(x[w], walker=ws[w])
+    rs = np.random.default_rng(7)
+    for ie in (0, 37, 61, 97):
+        xn = x[:, ie, :] + rs.normal(0, 0.3, (W, 3))
+        g = e.move_1e(ie, xn)
+        ref = [O.ratio_1e(ie, xn[w], walker=ws[w]) for w in range(W)]
+        assert _rel(g["ratio"], [r["ratio"] for r in ref], 1e-12) < 1e-9
+        assert _rel(g["grad"], [r["grad"] for r in ref], 1e-8) < 1e-9
+        assert _rel(g["lap"], [r["lap"] for r in ref], 1e-8) < 1e-8
+    for w in ws:
+        O.L.orc_walker_destroy(w)

@@ -3,7 +3,8 @@ default ones do: global scratch instead of LDS staging (basis values, the
 pseudopotential point scratch, eval_y's inputs), electron chunks in global
 scratch in the full evaluation, the one-electron move as three launches, and
 full evaluations that recompute the occupied orbitals instead of keeping the
-rows the moves committed.
+rows the moves committed, and the move's e-e-n terms summed term by term
+per pair instead of through the moved electron's per-centre tables.
 Large systems take some of these paths by size; here they are forced on the
 C2-like system through the engine's environment switches, read when a
 handle is created."""
@@ -41,7 +42,9 @@ def env(monkeypatch):
     {"QMCW_SPLIT_MOVE": "1"},
     {"QMCW_SPLIT_MOVE": "1", "QMCW_NO_LDS": "1"},
     {"QMCW_NO_KEEP_ORB": "1"},
-], ids=["no_lds", "det_chunks", "no_lds_y", "split_move", "split_no_lds", "recompute_orbitals"])
+    {"QMCW_NO_EEN_TAB": "1"},
+], ids=["no_lds", "det_chunks", "no_lds_y", "split_move", "split_no_lds", "recompute_orbitals",
+        "een_per_pair"])
 def test_alternative_paths_match_oracle(env, switches):
     env(**switches)
     W = 17
