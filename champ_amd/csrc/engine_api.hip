@@ -683,7 +683,7 @@ static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, dou
   const size_t tsz = h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float);
   const int ordk = std::max(2, A.S.nordc);
   const int tab = (A.S.nordc > 1 && !h->split_move && !h->no_een_tab)
-                      ? A.S.M * (3 * ((ordk + 1) * (ordk + 2) / 2) + 4) : 0;
+                      ? A.S.M * (3 * ((ordk + 1) * (ordk + 2) / 2) + 8) : 0;
   const int wst = std::max(A.S.NB * 5, tab);
   const size_t lds = (size_t)(kWave / G) * wst * tsz;
   A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;

@@ -412,14 +412,16 @@ __device__ __forceinline__ void een_gamma3(T A, cptr<double> c, T *g) {
 }
 
 // T values per centre of the move's e-e-n tables: the three tables of
-// een_gamma3, then dA/dr, d2A/dr2 and r of the moved electron
+// een_gamma3, then dA/dr, d2A/dr2, r, 1/r and the vector to the centre of
+// the moved electron
 template <int ORD>
-__host__ __device__ constexpr int een_tab_stride() { return 3 * (een_gi<ORD>(ORD, 0) + 1) + 4; }
+__host__ __device__ constexpr int een_tab_stride() { return 3 * (een_gi<ORD>(ORD, 0) + 1) + 8; }
 
-// The moved electron's tables at its new position (distances rn to the
-// centres), one centre per lane
+// The moved electron's tables at its new position (distances rn and vectors
+// rvn to the centres), one centre per lane
 template <typename T, int ORD, int G>
-__device__ __forceinline__ void een_tables(const DevSys &S, const double *rn, T *tab, int lane) {
+__device__ __forceinline__ void een_tables(const DevSys &S, const double *rn, const double *rvn, T *tab,
+                                           int lane) {
   constexpr int NG = een_gi<ORD>(ORD, 0) + 1, TS = een_tab_stride<ORD>();
   const T k = T(S.scalek);
   for (int ic = lane; ic < S.M; ic += G) {
@@ -431,18 +433,21 @@ __device__ __forceinline__ void een_tables(const DevSys &S, const double *rn, T 
     tb[3 * NG] = -k * Ad1;
     tb[3 * NG + 1] = -k * Ad2;
     tb[3 * NG + 2] = (T)rn[ic];
+    tb[3 * NG + 3] = T(1) / (T)rn[ic];
+#pragma unroll
+    for (int q = 0; q < 3; q++) tb[3 * NG + 4 + q] = (T)rvn[3 * ic + q];
   }
 }
 
 // jas_pair for a pair of the moved electron m (at the position of the
-// tables tab, its vectors to the centres rvm_c) and another electron o
+// tables tab) and another electron o
 // (distances from po): the e-e-n polynomial and its derivatives come from
 // Horner sums over the tables instead of the term-by-term sum.  rvmo = r_m
 // - r_o.  Output in jas_pair's convention with electron i = m when m_hi,
 // else i = o.
 template <typename T, int ORD, typename PO>
 __device__ void jas_pair_tab(const DevSys &S, bool parallel_spin, T rij, const T *rvmo, const T *tab,
-                             const double *rvm_c, const PO &po, bool m_hi, PairOut<T> &out) {
+                             const PO &po, bool m_hi, PairOut<T> &out) {
   constexpr int NG = een_gi<ORD>(ORD, 0) + 1, TS = een_tab_stride<ORD>();
   const T k = T(S.scalek);
   T sspinn = T(1);
@@ -486,6 +491,7 @@ __device__ void jas_pair_tab(const DevSys &S, bool parallel_spin, T rij, const T
     uq[0] = T(1);
 #pragma unroll
     for (int q = 1; q <= ORD; q++) uq[q] = uq[q - 1] * U;
+    const T irij = T(1) / rij, rij2 = rij * rij;
     for (int ic = 0; ic < S.M; ic++) {
       const T *tb = tab + ic * TS;
       double rod, rvo[3];
@@ -520,20 +526,21 @@ __device__ void jas_pair_tab(const DevSys &S, bool parallel_spin, T rij, const T
         }
         if (kk >= 2) FUU += T(kk * (kk - 1)) * r * uq[kk - 2];
       }
-      const T Ad1 = tb[3 * NG], Ad2 = tb[3 * NG + 1], rm = tb[3 * NG + 2];
+      const T Ad1 = tb[3 * NG], Ad2 = tb[3 * NG + 1], rm = tb[3 * NG + 2], irm = tb[3 * NG + 3];
+      const T iro = T(1) / ro;
       // chain rule to unscaled distances (A: the moved electron, B: the other)
-      const T gU = FU * Ud1 / rij, gUU = FUU * Ud1 * Ud1 + FU * Ud2;
-      const T gA = FA * Ad1 / rm, gAA = FAA * Ad1 * Ad1 + FA * Ad2;
-      const T gB = FB * Bd1 / ro, gBB = FBB * Bd1 * Bd1 + FB * Bd2;
+      const T gU = FU * Ud1 * irij, gUU = FUU * Ud1 * Ud1 + FU * Ud2;
+      const T gA = FA * Ad1 * irm, gAA = FAA * Ad1 * Ad1 + FA * Ad2;
+      const T gB = FB * Bd1 * iro, gBB = FBB * Bd1 * Bd1 + FB * Bd2;
       const T gUA = FUA * Ud1 * Ad1, gUB = FUB * Ud1 * Bd1;
       f += F;
 #pragma unroll
       for (int q = 0; q < 3; q++) {
-        gm[q] += gA * (T)rvm_c[3 * ic + q] + gU * rvmo[q];
+        gm[q] += gA * tb[3 * NG + 4 + q] + gU * rvmo[q];
         go[q] += gB * (T)rvo[q] - gU * rvmo[q];
       }
-      const T cM = (rij * rij + rm * rm - ro * ro) / (rij * rm);
-      const T cO = (rij * rij + ro * ro - rm * rm) / (rij * ro);
+      const T cM = (rij2 + rm * rm - ro * ro) * (irij * irm);
+      const T cO = (rij2 + ro * ro - rm * rm) * (irij * iro);
       lm += T(2) * (gU + gA) + gUU + gAA + gUA * cM;
       lo += T(2) * (gU + gB) + gUU + gBB + gUB * cO;
     }

@@ -1352,7 +1352,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
   extern __shared__ double dyn_lds[];
   T *etab = reinterpret_cast<T *>(dyn_lds) + (int64_t)grp * A.lds_wst;
   if (A.een_tab) {
-    een_tables<T, ORD, G>(S, ren, etab, lane);
+    een_tables<T, ORD, G>(S, ren, rven, etab, lane);
     __syncthreads();
   }
   for (int j = lane; j < N; j += G) {
@@ -1382,7 +1382,7 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     if (A.een_tab) {
       const T rvmo[3] = {hi == ie ? rvij[0] : -rvij[0], hi == ie ? rvij[1] : -rvij[1],
                          hi == ie ? rvij[2] : -rvij[2]};
-      jas_pair_tab<T, ORD>(S, par, (T)ree[j], rvmo, etab, rvI, pJ, hi == ie, po);
+      jas_pair_tab<T, ORD>(S, par, (T)ree[j], rvmo, etab, pJ, hi == ie, po);
     } else if (hi == ie) {
       jas_pair_g<T, ORD>(S, par, (T)ree[j], rvij, pI, pJ, po);
     } else {
