@@ -297,9 +297,24 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
         e.sync()
     dt = time.perf_counter() - t0
     agg = sum(W for _, W, _, _ in engines) * steps / dt
+    # the same systems as one batch: every kernel of the step once over the
+    # walkers of all systems (the first system's lane-group widths)
+    from champ_amd.engine import Batch
+    b = Batch([e for _, _, e, _ in engines])
+    b.vmc_device(max(1, warmup), delta)
+    for _, _, e, _ in engines:
+        e.sync()
+    t0 = time.perf_counter()
+    b.vmc_device(steps, delta)
+    for _, _, e, _ in engines:
+        e.sync()
+    dt1 = time.perf_counter() - t0
+    agg1 = sum(W for _, W, _, _ in engines) * steps / dt1
+    b.close()
     for _, _, e, _ in engines:
         e.close()
-    return dict(aggregate=agg, per_system=per, seconds=dt)
+    return dict(aggregate=agg, per_system=per, seconds=dt, aggregate_one_launch=agg1,
+                seconds_one_launch=dt1)
 
 
 def cpu_baseline_dmc(s, seconds=10.0, nthreads=None, tau=0.01):
@@ -491,12 +506,15 @@ def main():
     if args.batch:
         prec = "mixed" if args.precision == "both" else args.precision
         b = run_batch(args.batch, args.steps, args.warmup, prec, args.delta)
-        print(json.dumps({"metric": "walker-moves/s (multi-system batch)", "value": b["aggregate"],
+        print(json.dumps({"metric": "walker-moves/s (multi-system batch)",
+                          "value": max(b["aggregate"], b["aggregate_one_launch"]),
                           "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
                           "warmup": args.warmup, "higher_is_better": True, "dtype":
                           "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
-                          "config": {"workload": f"{args.batch} synthetic molecules, one stream each",
+                          "config": {"workload": f"{args.batch} synthetic molecules",
                                      "precision": prec},
+                          "streams_walker_moves_per_s": b["aggregate"],
+                          "one_launch_walker_moves_per_s": b["aggregate_one_launch"],
                           "per_system": b["per_system"]}), flush=True)
         return
     from champ_amd import rng, system

@@ -150,6 +150,10 @@ SIGNATURES = {
     "qmcw_vmc_run": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, _pd,
                                C.POINTER(C.c_int64)]),
     "qmcw_vmc_run_device": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double]),
+    "qmcw_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int32, C.POINTER(_vp)]),
+    "qmcw_batch_vmc_run": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double]),
+    "qmcw_batch_destroy": (C.c_int, [_vp]),
+    "qmcw_batch_last_error": (C.c_char_p, [_vp]),
     "qmcw_set_vmc_mover": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double]),
     "qmcw_set_reinversion": (C.c_int, [_vp, C.c_int32]),
     "qmcw_reinvert": (C.c_int, [_vp]),

@@ -559,7 +559,7 @@ static void launch_derivs_asm(qmcw_handle *h, const KArgs &A) {
 // the basis values the launch may keep there, when a block's share stays
 // within 64 KB.  Small determinants keep the global path, which measured
 // faster for the 4 x 4 matrices of the C2-like system.
-static void det_lds_inv(qmcw_handle *h, KArgs &A, size_t &smem) {
+static void det_lds_inv(const qmcw_handle *h, KArgs &A, size_t &smem) {
   A.lds_inv = 0;
   A.lds_inv_off = 0;
   const int nsm = std::max(A.S.ns[0], A.S.ns[1]);
@@ -574,14 +574,20 @@ static void det_lds_inv(qmcw_handle *h, KArgs &A, size_t &smem) {
   smem = tot;
 }
 
-static void launch_eval(qmcw_handle *h, bool derivs) {
+// The arguments of the full evaluation's kernels for the handle's current
+// state (keep: take the rows the committed moves kept): jas for k_full_jas
+// (and k_deriv_jas), det for k_full_det, k_gproj_mfma and k_full_fin (and
+// k_derivs), nlpp for k_full_nlpp (and k_deriv_nlpp), with their dynamic LDS
+struct EvalArgs {
+  KArgs jas, det, nlpp;
+  size_t det_smem = 0, nlpp_smem = 0, dnlpp_smem = 0;
+};
+static void eval_args(const qmcw_handle *h, bool derivs, bool keep, EvalArgs &e) {
   KArgs A = h->A;
   A.full_orb = derivs ? 1 : 0;
   // (the derivatives need all orbitals of every electron, the moves keep
   // only the occupied ones: k_full_det recomputes them under full_orb)
-  A.keep_state = (h->orb_valid && !h->no_keep_orb) ? 1 : 0;
-  h->orb_valid = 1;
-  h->moves_since_inv = 0;
+  A.keep_state = keep ? 1 : 0;
   // electrons per basis chunk of k_full_det: a wavefront's basis values
   // (64/G walkers) stay near 16 KB so that they are read back from L2
   const int gd = h->Gk[KC_DET];
@@ -600,11 +606,10 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     det_smem = per_e;
   }
   det_lds_inv(h, A, det_smem);
-  const OrdKernels &K = ord_kernels(A.S.nordc);
-  const LaunchCtx cj = make_ctx(h, A, h->Gk[KC_JAS]);
+  e.jas = A;
   // k_full_nlpp's per-lane point scratch (4M distances, N values of U) in
   // LDS when all of the launch's blocks stay resident with it
-  LaunchCtx cn = make_ctx(h, A, h->Gk[KC_NLPP]);
+  e.nlpp = A;
   {
     const int M = A.S.M, N = A.S.N;
     const size_t tsz = h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float);
@@ -613,17 +618,36 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     const size_t bpc = ((size_t)h->Wl / (kWave / h->Gk[KC_NLPP]) + h->ncu - 1) / h->ncu;
     const size_t cap = std::min<size_t>(48 * 1024, (160 * 1024) / std::max<size_t>(1, bpc));
     if (h->use_lds && blk <= cap) {
-      cn.A.lds_nlpp = per_lane;
-      cn.smem_nlpp = blk;
+      e.nlpp.lds_nlpp = per_lane;
+      e.nlpp_smem = blk;
     }
     // k_deriv_nlpp: the same plus N doubles per lane
     const int per_lane_d = per_lane + N;
     const size_t blk_d = (size_t)kWave * per_lane_d * sizeof(double);
     if (derivs && h->use_lds && blk_d <= cap) {
-      cn.A.lds_dnlpp = per_lane_d;
-      cn.smem_dnlpp = blk_d;
+      e.nlpp.lds_dnlpp = per_lane_d;
+      e.dnlpp_smem = blk_d;
     }
   }
+  // the pseudopotential projection on FP64 MFMA, rows = (walker, electron)
+  A.gproj_mfma = (A.S.nloc > 0 && !h->no_gproj_mfma) ? 1 : 0;
+  e.det = A;
+  e.det_smem = det_smem;
+}
+
+static void launch_eval(qmcw_handle *h, bool derivs) {
+  EvalArgs e;
+  eval_args(h, derivs, h->orb_valid && !h->no_keep_orb, e);
+  h->orb_valid = 1;
+  h->moves_since_inv = 0;
+  const int gd = h->Gk[KC_DET];
+  const OrdKernels &K = ord_kernels(h->A.S.nordc);
+  const LaunchCtx cj = make_ctx(h, e.jas, h->Gk[KC_JAS]);
+  LaunchCtx cn = make_ctx(h, e.nlpp, h->Gk[KC_NLPP]);
+  cn.smem_nlpp = e.nlpp_smem;
+  cn.smem_dnlpp = e.dnlpp_smem;
+  const KArgs &A = e.det;
+  const size_t det_smem = e.det_smem;
   prof_begin(h, QMCW_PROF_FULL_JAS);
   K.full_jas(cj);
   prof_end(h);
@@ -632,8 +656,6 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     K.deriv_jas(cj);
     prof_end(h);
   }
-  // the pseudopotential projection on FP64 MFMA, rows = (walker, electron)
-  A.gproj_mfma = (A.S.nloc > 0 && !h->no_gproj_mfma) ? 1 : 0;
   prof_begin(h, QMCW_PROF_FULL_DET);
   DISPATCH_GW(gd, launch_det, h, A, det_smem);
   if (A.gproj_mfma)
@@ -673,23 +695,24 @@ static void launch_move_bc(qmcw_handle *h, const KArgs &A, size_t smem_b, size_t
   hipLaunchKernelGGL((k_move_c<G>), grid, block, smem_c, h->stream, A, ie, mode, delta, fbias, out,
                      nacc);
 }
-static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
-                         double fbias, double *out, long long *nacc = nullptr) {
-  const int G = h->Gk[KC_MOVE];
+// The one-electron move's arguments at lane-group width G: LDS for the
+// moved electron's basis values (phi_b bytes) and, after them, eval_y's
+// inputs (smem: the whole dynamic LDS of the single-kernel move)
+static KArgs move_args(const qmcw_handle *h, int G, bool split, size_t &phi_b, size_t &smem) {
   KArgs A = h->A;
   // the moved electron's basis values in LDS when they fit (<= 48 KB per
   // block); in the single-kernel move the same region first holds the
   // e-e-n tables of the Jastrow row (one per centre)
   const size_t tsz = h->precision == QMCW_FP64 ? sizeof(double) : sizeof(float);
   const int ordk = std::max(2, A.S.nordc);
-  const int tab = (A.S.nordc > 1 && !h->split_move && !h->no_een_tab)
+  const int tab = (A.S.nordc > 1 && !split && !h->no_een_tab)
                       ? A.S.M * (3 * ((ordk + 1) * (ordk + 2) / 2) + 8) : 0;
   const int wst = std::max(A.S.NB * 5, tab);
   const size_t lds = (size_t)(kWave / G) * wst * tsz;
   A.lds_phi = (h->use_lds && lds <= 48 * 1024) ? 1 : 0;
   A.een_tab = (A.lds_phi && tab > 0) ? 1 : 0;
   A.lds_wst = A.een_tab ? wst : A.S.NB * 5;
-  const size_t phi_b = A.lds_phi ? ((size_t)(kWave / G) * A.lds_wst * tsz + 7) / 8 * 8 : 0;
+  phi_b = A.lds_phi ? ((size_t)(kWave / G) * A.lds_wst * tsz + 7) / 8 * 8 : 0;
   // eval_y's determinant values and alpha inverses staged in LDS after the
   // basis values, when the two together stay within 48 KB per block and
   // still let every block of the launch be resident at once (160 KB of LDS
@@ -701,8 +724,17 @@ static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, dou
   const bool ly = h->use_lds && h->lds_y && phi_b + y_b <= lds_cap;
   A.lds_y = ly ? y_lds_doubles(A.S.ndetu[0], A.S.ndetu[1], A.S.kmax2) : 0;
   A.lds_y_off = (int)(phi_b / 8);
+  smem = phi_b + (ly ? y_b : 0);
+  return A;
+}
+
+static void launch_ratio(qmcw_handle *h, int ie, int mode, const double *xn, double delta,
+                         double fbias, double *out, long long *nacc = nullptr) {
+  const int G = h->Gk[KC_MOVE];
+  size_t phi_b = 0, smem = 0;
+  const KArgs A = move_args(h, G, h->split_move != 0, phi_b, smem);
   LaunchCtx c = make_ctx(h, A, G);
-  c.smem_ratio = phi_b + (ly ? y_b : 0);
+  c.smem_ratio = smem;
   prof_begin(h, QMCW_PROF_RATIO);
   if (h->split_move) {
     ord_kernels(A.S.nordc).move_a(c, ie, mode, xn, delta, fbias);
@@ -1597,6 +1629,248 @@ extern "C" int qmcw_set_vmc_mover(qmcw_handle *h, int32_t imetro, double deltar,
 extern "C" int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta, double fbias) {
   if (!h || nstep < 0 || delta <= 0) return QMCW_ERR_ARG;
   return vmc_steps(h, nstep, delta, fbias, false);
+}
+
+// ---------------------------------------------------------------------------
+// Batches: the VMC step of several handles (systems of uneven electron and
+// basis counts) in one launch per kernel (engine_batch.h).  Argument sets:
+// 0 the first full evaluation (each handle's kept rows as it left them),
+// 1 the steps.  Kernels per set: move, jas, det (also gproj, fin), nlpp,
+// rotqua; block offsets per kernel: move, jas, det, nlpp, rotqua, gproj.
+enum { KB_MOVE, KB_JAS, KB_DET, KB_NLPP, KB_ROT, KB_NARGS, KB_GPROJ = KB_NARGS, KB_NOFF };
+
+struct qmcw_batch {
+  std::vector<qmcw_handle *> hs;
+  hipStream_t stream = nullptr;              // the first handle's stream
+  hipEvent_t done = nullptr;                 // recorded after the batch's work
+  int n = 0, maxN = 0, precision = 0, nordc = 0;
+  bool any_nloc = false, any_gproj = false;
+  bool sweep = false;                        // QMCW_BATCH_SWEEP=1: each system's sweep in one launch
+  KArgs *dA[2] = {nullptr, nullptr};         // [KB_NARGS][n] per set
+  int *dboff = nullptr;                      // [KB_NOFF][n + 1]
+  int *dnw = nullptr;                        // [n]
+  long long **dnacc = nullptr;               // [n]
+  int nblk[KB_NOFF] = {0, 0, 0, 0, 0, 0};
+  size_t smem_move = 0, smem_det = 0, smem_nlpp = 0;
+  std::string err;
+};
+
+static int batch_fail(qmcw_batch *b, int code, const std::string &msg) {
+  if (b) b->err = msg;
+  g_lib_err = msg;
+  return code;
+}
+
+#define BHIPCHK(b, call)                                                             \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return batch_fail(b, QMCW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// the argument set `set` of every handle (keep: each handle's kept rows,
+// else the rows the moves keep during the steps) and, for set 1, the block
+// offsets, walker counts and counters; synchronous copies
+static int batch_upload(qmcw_batch *b, int set, bool first) {
+  const int n = b->n;
+  std::vector<KArgs> A((size_t)KB_NARGS * n);
+  std::vector<int> boff((size_t)KB_NOFF * (n + 1), 0), nw(n);
+  std::vector<long long *> nacc(n);
+  for (int s = 0; s < n; s++) {
+    qmcw_handle *h = b->hs[s];
+    size_t phi_b = 0, sm = 0;
+    A[(size_t)KB_MOVE * n + s] = move_args(h, h->Gk[KC_MOVE], false, phi_b, sm);
+    b->smem_move = std::max(b->smem_move, sm);
+    EvalArgs e;
+    const bool keep = first ? (h->orb_valid != 0) : true;
+    eval_args(h, false, keep && !h->no_keep_orb, e);
+    A[(size_t)KB_JAS * n + s] = e.jas;
+    A[(size_t)KB_DET * n + s] = e.det;
+    A[(size_t)KB_NLPP * n + s] = e.nlpp;
+    A[(size_t)KB_ROT * n + s] = h->A;
+    b->smem_det = std::max(b->smem_det, e.det_smem);
+    b->smem_nlpp = std::max(b->smem_nlpp, e.nlpp_smem);
+    const int Wl = h->Wl;
+    const int gx = (int)(((int64_t)Wl * std::max(h->A.S.ns[0], h->A.S.ns[1]) + 63) / 64);
+    const int gy = (h->A.S.NB + 63) / 64;
+    const int cnt[KB_NOFF] = {Wl / (kWave / h->Gk[KC_MOVE]), Wl / (kWave / h->Gk[KC_JAS]),
+                              Wl / (kWave / h->Gk[KC_DET]), Wl / (kWave / h->Gk[KC_NLPP]),
+                              (Wl + kWave - 1) / kWave, e.det.gproj_mfma ? gx * gy * 2 : 0};
+    for (int k = 0; k < KB_NOFF; k++)
+      boff[(size_t)k * (n + 1) + s + 1] = boff[(size_t)k * (n + 1) + s] + cnt[k];
+    nw[s] = Wl;
+    nacc[s] = h->nacc;
+  }
+  BHIPCHK(b, hipMemcpy(b->dA[set], A.data(), A.size() * sizeof(KArgs), hipMemcpyHostToDevice));
+  if (set == 1) {
+    for (int k = 0; k < KB_NOFF; k++) b->nblk[k] = boff[(size_t)k * (n + 1) + n];
+    BHIPCHK(b, hipMemcpy(b->dboff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice));
+    BHIPCHK(b, hipMemcpy(b->dnw, nw.data(), nw.size() * sizeof(int), hipMemcpyHostToDevice));
+    BHIPCHK(b, hipMemcpy(b->dnacc, nacc.data(), nacc.size() * sizeof(long long *), hipMemcpyHostToDevice));
+  }
+  return QMCW_OK;
+}
+
+static BatchArgs batch_args(const qmcw_batch *b, int set, int kargs, int koff) {
+  BatchArgs B;
+  B.A = b->dA[set] + (size_t)kargs * b->n;
+  B.boff = b->dboff + (size_t)koff * (b->n + 1);
+  B.nw = b->dnw;
+  B.nacc = b->dnacc;
+  B.n = b->n;
+  return B;
+}
+
+template <int G>
+static void batch_det_fin(const qmcw_batch *b, const BatchArgs &B, bool fin) {
+  const dim3 grid(b->nblk[KB_DET]), block(kWave);
+  if (fin)
+    hipLaunchKernelGGL((k_full_fin_batch<G>), grid, block, 0, b->stream, B);
+  else if (b->precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_full_det_batch<double, G>), grid, block, b->smem_det, b->stream, B);
+  else
+    hipLaunchKernelGGL((k_full_det_batch<float, G>), grid, block, b->smem_det, b->stream, B);
+}
+
+// the full evaluation of every handle with argument set `set`
+static void batch_eval(const qmcw_batch *b, int set) {
+  const OrdKernels &K = ord_kernels(b->nordc);
+  const qmcw_handle *h0 = b->hs[0];
+  BatchCtx c;
+  c.stream = b->stream;
+  c.precision = b->precision;
+  c.B = batch_args(b, set, KB_JAS, KB_JAS);
+  c.G = h0->Gk[KC_JAS];
+  c.nblocks = b->nblk[KB_JAS];
+  c.smem = 0;
+  K.full_jas_batch(c);
+  const BatchArgs Bd = batch_args(b, set, KB_DET, KB_DET);
+  DISPATCH_GW(h0->Gk[KC_DET], batch_det_fin, b, Bd, false);
+  if (b->any_gproj && b->nblk[KB_GPROJ] > 0)
+    hipLaunchKernelGGL(k_gproj_mfma_batch, dim3(b->nblk[KB_GPROJ]), dim3(kWave), 0, b->stream,
+                       batch_args(b, set, KB_DET, KB_GPROJ));
+  if (b->any_nloc) {
+    c.B = batch_args(b, set, KB_NLPP, KB_NLPP);
+    c.G = h0->Gk[KC_NLPP];
+    c.nblocks = b->nblk[KB_NLPP];
+    c.smem = b->smem_nlpp;
+    K.full_nlpp_batch(c);
+  }
+  DISPATCH_GW(h0->Gk[KC_DET], batch_det_fin, b, Bd, true);
+}
+
+static void batch_step(const qmcw_batch *b, double delta, double fbias) {
+  const OrdKernels &K = ord_kernels(b->nordc);
+  if (b->any_nloc)
+    hipLaunchKernelGGL(k_rotqua_batch, dim3(b->nblk[KB_ROT]), dim3(kWave), 0, b->stream,
+                       batch_args(b, 1, KB_ROT, KB_ROT));
+  BatchCtx c;
+  c.stream = b->stream;
+  c.precision = b->precision;
+  c.B = batch_args(b, 1, KB_MOVE, KB_MOVE);
+  c.G = b->hs[0]->Gk[KC_MOVE];      // every handle's, after qmcw_batch_create
+  c.nblocks = b->nblk[KB_MOVE];
+  c.smem = b->smem_move;
+  if (b->sweep)
+    K.sweep_batch(c, delta, fbias);
+  else
+    for (int ie = 0; ie < b->maxN; ie++) K.ratio_batch(c, ie, delta, fbias);
+  batch_eval(b, 1);
+}
+
+extern "C" int qmcw_batch_destroy(qmcw_batch *b) {
+  if (!b) return QMCW_ERR_ARG;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (int set = 0; set < 2; set++)
+    if (b->dA[set]) (void)hipFree(b->dA[set]);
+  if (b->dboff) (void)hipFree(b->dboff);
+  if (b->dnw) (void)hipFree(b->dnw);
+  if (b->dnacc) (void)hipFree(b->dnacc);
+  if (b->done) (void)hipEventDestroy(b->done);
+  delete b;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_batch_create(qmcw_handle *const *hs, int32_t n, qmcw_batch **out) {
+  if (!hs || n <= 0 || !out) return fail(nullptr, QMCW_ERR_ARG, "invalid arguments");
+  *out = nullptr;
+  for (int s = 0; s < n; s++) {
+    const qmcw_handle *h = hs[s];
+    if (!h) return fail(nullptr, QMCW_ERR_ARG, "null handle in the batch");
+    if (h->precision != hs[0]->precision)
+      return fail(nullptr, QMCW_ERR_ARG, "the handles of a batch must share the precision");
+    if (&ord_kernels(h->A.S.nordc) != &ord_kernels(hs[0]->A.S.nordc))
+      return fail(nullptr, QMCW_ERR_ARG, "the handles of a batch must share the e-e-n order (nordc)");
+    if (h->sample_derivs || h->reinv_every > 0)
+      return fail(nullptr, QMCW_ERR_ARG,
+                  "a batch runs plain VMC steps: no derivative sampling or re-inversion interval");
+    for (int t = 0; t < s; t++)
+      if (hs[t] == h) return fail(nullptr, QMCW_ERR_ARG, "a handle appears twice in the batch");
+  }
+  qmcw_batch *b = new qmcw_batch;
+  b->n = n;
+  b->hs.assign(hs, hs + n);
+  b->stream = hs[0]->stream;
+  b->precision = hs[0]->precision;
+  b->nordc = hs[0]->A.S.nordc;
+  if (const char *e = getenv("QMCW_BATCH_SWEEP")) b->sweep = atoi(e) != 0;
+  // one lane-group width per kernel class for the whole batch: that of the
+  // handle with the most work per step (walkers x electrons^2), given to
+  // every handle
+  int dom = 0;
+  for (int s = 1; s < n; s++) {
+    const double ws = (double)hs[s]->Wl * hs[s]->A.S.N * hs[s]->A.S.N;
+    const double wd = (double)hs[dom]->Wl * hs[dom]->A.S.N * hs[dom]->A.S.N;
+    if (ws > wd) dom = s;
+  }
+  int Gd[4];
+  for (int k = 0; k < 4; k++) Gd[k] = hs[dom]->Gk[k];
+  for (qmcw_handle *h : b->hs) {
+    for (int k = 0; k < 4; k++) h->Gk[k] = Gd[k];
+    h->G = h->Gk[KC_MOVE];
+    drop_graph(h);
+    b->maxN = std::max(b->maxN, h->A.S.N);
+    b->any_nloc = b->any_nloc || h->A.S.nloc > 0;
+    b->any_gproj = b->any_gproj || (h->A.S.nloc > 0 && !h->no_gproj_mfma);
+  }
+  const size_t na = (size_t)KB_NARGS * n * sizeof(KArgs);
+  if (hipMalloc(&b->dA[0], na) != hipSuccess || hipMalloc(&b->dA[1], na) != hipSuccess ||
+      hipMalloc(&b->dboff, (size_t)KB_NOFF * (n + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->dnw, (size_t)n * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->dnacc, (size_t)n * sizeof(long long *)) != hipSuccess ||
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    qmcw_batch_destroy(b);
+    return fail(nullptr, QMCW_ERR_NOMEM, "batch allocation");
+  }
+  *out = b;
+  return QMCW_OK;
+}
+
+extern "C" int qmcw_batch_vmc_run(qmcw_batch *b, int32_t nstep, double delta, double fbias) {
+  if (!b || nstep < 0 || delta <= 0) return QMCW_ERR_ARG;
+  // the arguments are copied synchronously: wait for the handles' own work
+  for (qmcw_handle *h : b->hs) BHIPCHK(b, hipStreamSynchronize(h->stream));
+  int st = batch_upload(b, 0, true);
+  if (st) return st;
+  st = batch_upload(b, 1, false);
+  if (st) return st;
+  // the first full evaluation (as qmcw_vmc_run starts), then the steps
+  batch_eval(b, 0);
+  for (qmcw_handle *h : b->hs) {
+    h->orb_valid = 1;
+    h->moves_since_inv = 0;
+  }
+  for (int is = 0; is < nstep; is++) batch_step(b, delta, fbias);
+  BHIPCHK(b, hipGetLastError());
+  // the handles' later work follows the batch's
+  BHIPCHK(b, hipEventRecord(b->done, b->stream));
+  for (qmcw_handle *h : b->hs)
+    if (h->stream != b->stream) BHIPCHK(b, hipStreamWaitEvent(h->stream, b->done, 0));
+  return QMCW_OK;
+}
+
+extern "C" const char *qmcw_batch_last_error(const qmcw_batch *b) {
+  return b ? b->err.c_str() : g_lib_err.c_str();
 }
 
 extern "C" int qmcw_profile(qmcw_handle *h, int32_t enable) {

@@ -76,6 +76,16 @@ namespace qmcw {
 // offset (past ML.misc + ns + nocc) of the Jastrow sums handed between the move kernels
 constexpr int kMvJas = 16;
 
+// the group's lane, group and walker in block bid of the launch (a kernel's
+// own blockIdx.x, or the block's index within its system in a batch launch)
+#define QMCW_GROUP_SETUP_B(bid)                                  \
+  const int lane = threadIdx.x % G;                              \
+  const int grp = threadIdx.x / G;                               \
+  const int w = (bid) * (kWave / G) + grp;                       \
+  const DevSys &S = A.S;                                         \
+  double *ws = A.st + (int64_t)w * A.L.stride;                   \
+  double *ss = A.sc + (int64_t)w * A.SL.stride;                  \
+  (void)ss;
 #define QMCW_GROUP_SETUP                                         \
   const int lane = threadIdx.x % G;                              \
   const int grp = threadIdx.x / G;                               \
@@ -491,8 +501,8 @@ __device__ void group_g(const DevSys &S, int s, const double *ai,
 // phs[50] pe_en, [51] pe_ee, [52] ln J, [53] lap ln J, [54] lap ln D,
 // [55] local pseudopotential, [56] nonlocal pseudopotential (SLayout::phs).
 template <typename T, int G, int ORD>
-__global__ QMCW_HEAVY void k_full_jas(KArgs A) {
-  QMCW_GROUP_SETUP
+__device__ __forceinline__ void full_jas_b(const KArgs &A, int bid_) {
+  QMCW_GROUP_SETUP_B(bid_)
   const int N = S.N, M = S.M, NO = S.NO;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -603,11 +613,14 @@ __global__ QMCW_HEAVY void k_full_jas(KArgs A) {
     red[50] = pe_en; red[51] = pe_ee; red[52] = fsum; red[53] = d2j;
   }
 }
+template <typename T, int G, int ORD>
+__global__ QMCW_HEAVY void k_full_jas(KArgs A) {
+  full_jas_b<T, G, ORD>(A, blockIdx.x);
+}
 
 template <typename T, int G>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : QMCW_WPE, 8)))
-void k_full_det(KArgs A) {
-  QMCW_GROUP_SETUP
+__device__ __forceinline__ void full_det_b(const KArgs &A, int bid_) {
+  QMCW_GROUP_SETUP_B(bid_)
   const int N = S.N, M = S.M, NO = S.NO;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -744,6 +757,11 @@ void k_full_det(KArgs A) {
   // (k_gproj_mfma instead, after this kernel, for large determinants)
   if (S.nloc > 0 && !A.gproj_mfma) group_gproj<G>(S, ss + SL.gproj, ss + SL.g[0], ss + SL.g[1], A.bf16, lane);
 }
+template <typename T, int G>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : QMCW_WPE, 8)))
+void k_full_det(KArgs A) {
+  full_det_b<T, G>(A, blockIdx.x);
+}
 
 #ifndef QMCW_ORD   // launched from the host translation unit only
 typedef double gp_d4 __attribute__((ext_vector_type(4)));
@@ -759,11 +777,10 @@ typedef double gp_d4 __attribute__((ext_vector_type(4)));
 // (the B operand) are shared by every walker.  One wavefront per block.
 // Each sum runs in the MFMA's order, not group_gproj's, and is the same
 // whatever rows share its wavefront.
-__global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A, int nw) {
+__device__ __forceinline__ void gproj_mfma_b(const KArgs &A, int nw, int bx, int cb, int sp) {
   const DevSys &S = A.S;
-  const int cb = blockIdx.y, sp = blockIdx.z;
   const int ns = S.ns[sp], nocc = S.nocc[sp], NB = S.NB, off = sp ? S.nup : 0;
-  const int64_t rows = (int64_t)nw * ns, row0 = (int64_t)blockIdx.x * 64;
+  const int64_t rows = (int64_t)nw * ns, row0 = (int64_t)bx * 64;
   if (ns == 0 || row0 >= rows) return;
   const int64_t stride = A.SL.stride;
   const double *__restrict__ co = S.coef_occ[sp];        // coef(b, occ_j) at co[b * nocc + j]
@@ -819,6 +836,9 @@ __global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A, int nw) {
         }
       }
     }
+}
+__global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A, int nw) {
+  gproj_mfma_b(A, nw, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 #endif
 
@@ -944,11 +964,9 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
   return vpsp;
 }
 
-// FP32 instances are held to 128 VGPRs (four waves per SIMD)
 template <typename T, int G, int ORD>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : QMCW_WPE, 8)))
-void k_full_nlpp(KArgs A) {
-  QMCW_GROUP_SETUP
+__device__ __forceinline__ void full_nlpp_b(const KArgs &A, int bid_) {
+  QMCW_GROUP_SETUP_B(bid_)
   const int N = S.N, M = S.M, NO = S.NO;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -1052,6 +1070,12 @@ void k_full_nlpp(KArgs A) {
   }
   if (lane == 0) { red[55] = ploc; red[56] = vpsp; }
 }
+// FP32 instances are held to 128 VGPRs (four waves per SIMD)
+template <typename T, int G, int ORD>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : QMCW_WPE, 8)))
+void k_full_nlpp(KArgs A) {
+  full_nlpp_b<T, G, ORD>(A, blockIdx.x);
+}
 
 // T-moves: Psi ratios of moving electron ie to points of the walker's
 // current state, the linear form of the pseudopotential evaluation
@@ -1117,8 +1141,8 @@ void k_tm_ratios(KArgs A, int ie, int cw, const double *pts, const int *cnt, dou
 }
 
 template <int G>
-__global__ __launch_bounds__(64) void k_full_fin(KArgs A) {
-  QMCW_GROUP_SETUP
+__device__ __forceinline__ void full_fin_b(const KArgs &A, int bid_) {
+  QMCW_GROUP_SETUP_B(bid_)
   const int N = S.N, M = S.M, NO = S.NO;
   const WLayout &L = A.L;
   const SLayout &SL = A.SL;
@@ -1160,6 +1184,10 @@ __global__ __launch_bounds__(64) void k_full_fin(KArgs A) {
     if (!isfinite(eloc)) fl |= 2;
     A.flags[w] |= fl;           // k_full_jas cleared the word; k_full_nlpp may have set bit 4
   }
+}
+template <int G>
+__global__ __launch_bounds__(64) void k_full_fin(KArgs A) {
+  full_fin_b<G>(A, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1282,11 +1310,10 @@ __device__ __forceinline__ void orbitals_1e(const KArgs &A, int s, int nocc, con
 // jastrow4e + orbitals + dete_update + eval_grad into the pending buffers.
 // out[w][8]: ratio, ln|psid_new|, sign, psij_new, grad(3), lap
 template <typename T, int G, int ORD>
-__global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
-                                                 const double *xnew_in,
-                                                 double delta, double fbias,
-                                                 double *out, long long *nacc) {
-  QMCW_GROUP_SETUP
+__device__ __forceinline__ void ratio_1e_b(const KArgs &A, int bid_, int ie, int mode,
+                                           const double *xnew_in, double delta, double fbias,
+                                           double *out, long long *nacc) {
+  QMCW_GROUP_SETUP_B(bid_)
   // mode 4: commit the given move of the walkers that accepted it; the
   // others have nothing to do (a group leaves as a whole)
   if (mode == 4 && !A.acc_in[w]) return;
@@ -1515,6 +1542,11 @@ __global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode,
     __syncthreads();
     if (s_acc[grp]) group_commit<G>(A, ws, mv, ie, lane);
   }
+}
+template <typename T, int G, int ORD>
+__global__ QMCW_HEAVY void k_ratio_1e(KArgs A, int ie, int mode, const double *xnew_in,
+                                      double delta, double fbias, double *out, long long *nacc) {
+  ratio_1e_b<T, G, ORD>(A, blockIdx.x, ie, mode, xnew_in, delta, fbias, out, nacc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1999,8 +2031,8 @@ __global__ __launch_bounds__(64) void k_grad_1e(KArgs A, int ie, double *out) {
 
 // random rotation of each walker's quadrature grid (qmc/vmc/rotqua.f90:621-682)
 #ifndef QMCW_ORD   // launched from the host translation unit only
-__global__ void k_rotqua(KArgs A) {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rotqua_b(const KArgs &A, int bid_) {
+  int w = bid_ * kWave + threadIdx.x;
   if (w >= A.Wp) return;
   const DevSys &S = A.S;
   int *rs = A.rng + 4 * w;
@@ -2017,6 +2049,7 @@ __global__ void k_rotqua(KArgs A) {
   }
   rs[0] = st4[0]; rs[1] = st4[1]; rs[2] = st4[2]; rs[3] = st4[3];
 }
+__global__ __launch_bounds__(64) void k_rotqua(KArgs A) { rotqua_b(A, blockIdx.x); }
 #endif
 
 }  // namespace qmcw

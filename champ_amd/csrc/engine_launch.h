@@ -4,7 +4,7 @@
 // exports a table of these launchers; the host picks the table of the
 // system's nordc, so every kernel carries only the registers of its order.
 #pragma once
-#include "engine_kernels.h"
+#include "engine_batch.h"
 
 namespace qmcw {
 
@@ -20,6 +20,17 @@ struct LaunchCtx {
   size_t smem_dnlpp;    // dynamic LDS of k_deriv_nlpp (0 when A.lds_dnlpp == 0)
 };
 
+// What a batch launcher needs (engine_batch.h): the device arrays of the
+// systems' arguments and block offsets, and the grid
+struct BatchCtx {
+  BatchArgs B;
+  hipStream_t stream;
+  int G;                // lane-group width of the kernel class
+  int nblocks;          // blocks of all systems
+  int precision;
+  size_t smem;          // dynamic LDS: the largest any system's blocks need
+};
+
 struct OrdKernels {
   void (*full_jas)(const LaunchCtx &);
   void (*full_nlpp)(const LaunchCtx &);
@@ -30,6 +41,10 @@ struct OrdKernels {
   void (*move_a)(const LaunchCtx &, int ie, int mode, const double *xn, double delta, double fbias);
   void (*tm_ratios)(const LaunchCtx &, int ie, int cw, const double *pts, const int *cnt,
                     double *out, double *stale);
+  void (*ratio_batch)(const BatchCtx &, int ie, double delta, double fbias);
+  void (*sweep_batch)(const BatchCtx &, double delta, double fbias);
+  void (*full_jas_batch)(const BatchCtx &);
+  void (*full_nlpp_batch)(const BatchCtx &);
 };
 
 extern const OrdKernels kOrdKernels2, kOrdKernels3, kOrdKernels4, kOrdKernels5, kOrdKernels6,

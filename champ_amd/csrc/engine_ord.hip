@@ -99,11 +99,63 @@ void tm_ratios(const LaunchCtx &c, int ie, int cw, const double *pts, const int 
 void deriv_jas(const LaunchCtx &c) { QMCW_BY_G(deriv_jas_g, c); }
 void deriv_nlpp(const LaunchCtx &c) { QMCW_BY_G(deriv_nlpp_g, c); }
 
+#define QMCW_BY_GB(fn, c, ...)                 \
+  switch ((c).G) {                             \
+    case 8: fn<8>(c, ##__VA_ARGS__); break;    \
+    case 16: fn<16>(c, ##__VA_ARGS__); break;  \
+    case 32: fn<32>(c, ##__VA_ARGS__); break;  \
+    default: fn<64>(c, ##__VA_ARGS__); break;  \
+  }
+template <int G>
+void ratio_batch_g(const BatchCtx &c, int ie, double delta, double fbias) {
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_ratio_1e_batch<double, G, ORD>), dim3(c.nblocks), dim3(kWave), c.smem,
+                       c.stream, c.B, ie, delta, fbias);
+  else
+    hipLaunchKernelGGL((k_ratio_1e_batch<float, G, ORD>), dim3(c.nblocks), dim3(kWave), c.smem,
+                       c.stream, c.B, ie, delta, fbias);
+}
+template <int G>
+void sweep_batch_g(const BatchCtx &c, double delta, double fbias) {
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_sweep_1e_batch<double, G, ORD>), dim3(c.nblocks), dim3(kWave), c.smem,
+                       c.stream, c.B, delta, fbias);
+  else
+    hipLaunchKernelGGL((k_sweep_1e_batch<float, G, ORD>), dim3(c.nblocks), dim3(kWave), c.smem,
+                       c.stream, c.B, delta, fbias);
+}
+template <int G>
+void full_jas_batch_g(const BatchCtx &c) {
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_full_jas_batch<double, G, ORD>), dim3(c.nblocks), dim3(kWave), 0, c.stream, c.B);
+  else
+    hipLaunchKernelGGL((k_full_jas_batch<float, G, ORD>), dim3(c.nblocks), dim3(kWave), 0, c.stream, c.B);
+}
+template <int G>
+void full_nlpp_batch_g(const BatchCtx &c) {
+  if (c.precision == QMCW_FP64)
+    hipLaunchKernelGGL((k_full_nlpp_batch<double, G, ORD>), dim3(c.nblocks), dim3(kWave), c.smem,
+                       c.stream, c.B);
+  else
+    hipLaunchKernelGGL((k_full_nlpp_batch<float, G, ORD>), dim3(c.nblocks), dim3(kWave), c.smem,
+                       c.stream, c.B);
+}
+void ratio_batch(const BatchCtx &c, int ie, double delta, double fbias) {
+  QMCW_BY_GB(ratio_batch_g, c, ie, delta, fbias);
+}
+void sweep_batch(const BatchCtx &c, double delta, double fbias) {
+  QMCW_BY_GB(sweep_batch_g, c, delta, fbias);
+}
+void full_jas_batch(const BatchCtx &c) { QMCW_BY_GB(full_jas_batch_g, c); }
+void full_nlpp_batch(const BatchCtx &c) { QMCW_BY_GB(full_nlpp_batch_g, c); }
+
 }  // namespace
 
 #ifndef __HIP_DEVICE_COMPILE__   // a host table of host launchers
 extern const OrdKernels QMCW_CAT(kOrdKernels, QMCW_ORD) = {full_jas, full_nlpp, ratio, deriv_jas,
-                                                            deriv_nlpp, move_a, tm_ratios};
+                                                            deriv_nlpp, move_a, tm_ratios,
+                                                            ratio_batch, sweep_batch,
+                                                            full_jas_batch, full_nlpp_batch};
 #endif
 
 }  // namespace qmcw

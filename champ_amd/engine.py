@@ -255,3 +255,39 @@ class Engine:
 
     def sync(self):
         self._chk(self.lib.qmcw_sync(self.h))
+
+
+class Batch:
+    """Several engines (systems of uneven electron and basis counts) whose
+    VMC steps run as one launch per kernel (qmcw_batch_*).  The engines
+    must share the precision and the e-e-n order; they take the lane-group
+    widths of the engine with the most work per step.  Close the batch
+    before the engines."""
+
+    def __init__(self, engines):
+        self.lib = abi.load_library()
+        self.engines = list(engines)
+        hs = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        b = C.c_void_p()
+        st = self.lib.qmcw_batch_create(hs, len(self.engines), C.byref(b))
+        if st != 0:
+            raise QmcwError(st, self.lib.qmcw_batch_last_error(None).decode())
+        self.b = b
+
+    def vmc_device(self, nstep: int, delta: float, fbias: float = 1.0):
+        """nstep VMC steps of every engine (a full evaluation first), as each
+        engine's vmc_device would run them alone."""
+        st = self.lib.qmcw_batch_vmc_run(self.b, int(nstep), float(delta), float(fbias))
+        if st != 0:
+            raise QmcwError(st, self.lib.qmcw_batch_last_error(self.b).decode())
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.qmcw_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -282,6 +282,25 @@ int qmcw_vmc_run(qmcw_handle *h, int32_t nstep, double delta, double fbias,
 int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta,
                         double fbias);
 
+/* Batches of systems.  No counterpart in the reference, which runs one
+   system per process; the batching of the engine's own design: several
+   handles (systems of uneven electron and basis counts, each with its own
+   walkers, streams and state) advance together, every kernel of the VMC
+   step running once over the walkers of all of them.  The handles must
+   share the precision and the e-e-n order (nordc) and sample no derivatives
+   and set no re-inversion interval; the batch gives them all the
+   lane-group widths of the handle with the most work per step (walkers x
+   electrons^2).  qmcw_batch_vmc_run does what qmcw_vmc_run_device
+   does for each handle (a full evaluation, then nstep steps with each
+   handle's mover, walkers and random streams; the same walk as the handle
+   alone); the handles' later calls are ordered after it.  The batch holds
+   pointers to the handles: destroy it before them. */
+typedef struct qmcw_batch qmcw_batch;
+int qmcw_batch_create(qmcw_handle *const *hs, int32_t n, qmcw_batch **out);
+int qmcw_batch_vmc_run(qmcw_batch *b, int32_t nstep, double delta, double fbias);
+int qmcw_batch_destroy(qmcw_batch *b);
+const char *qmcw_batch_last_error(const qmcw_batch *b);
+
 /* Parameters of one DMC generation (qmc/rundmc_mod.f90:97-330).  The
    defaults in brackets are CHAMP's (qmc/control_mod.f90:11-16) and those of
    its example inputs (idmc 2, itau_eff 1, iacc_rej 1, icross 1). */

@@ -199,6 +199,30 @@ module champ_amd_mod
       real(c_double), value :: delta, fbias
       integer(c_int) :: st
     end function
+    function qmcw_batch_create(hs, n, b) bind(C, name="qmcw_batch_create") result(st)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), intent(in) :: hs(*)
+      integer(c_int32_t), value :: n
+      type(c_ptr), intent(out) :: b
+      integer(c_int) :: st
+    end function
+    function qmcw_batch_vmc_run(b, nstep, delta, fbias) bind(C, name="qmcw_batch_vmc_run") result(st)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: nstep
+      real(c_double), value :: delta, fbias
+      integer(c_int) :: st
+    end function
+    function qmcw_batch_destroy(b) bind(C, name="qmcw_batch_destroy") result(st)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: b
+      integer(c_int) :: st
+    end function
+    function qmcw_batch_last_error(b) bind(C, name="qmcw_batch_last_error") result(msg)
+      import :: c_ptr
+      type(c_ptr), value :: b
+      type(c_ptr) :: msg
+    end function
     function qmcw_get_eloc(h, eloc) bind(C, name="qmcw_get_eloc") result(st)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h

@@ -1,15 +1,16 @@
 """Uneven multi-system batch (the 4-128-electron, 10-800-function molecules
-of the batch workload) run concurrently: one handle and HIP stream per
-system, every system's VMC steps queued before any is awaited.  Each system
-must reproduce the oracle's walk bit for bit (positions, streams) with local
-energies to 1e-9, and the largest (128 electrons, 23 centres, 805 GTOs)
-also its full evaluation."""
+of the batch workload), run two ways: one handle and HIP stream per system
+with every system's VMC steps queued before any is awaited, and as one
+batch (qmcw_batch_*) whose VMC step runs each kernel once over the walkers
+of all systems.  Each system must reproduce the oracle's walk bit for bit
+(positions, streams) with local energies to 1e-9, and the largest (128
+electrons, 23 centres, 805 GTOs) also its full evaluation."""
 import numpy as np
 import pytest
 
 import oracle
 from champ_amd import rng, system
-from champ_amd.engine import Engine
+from champ_amd.engine import Batch, Engine
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +44,43 @@ def test_concurrent_uneven_systems_match_oracle():
         e.vmc_device(nst, delta)
     for s, x, seeds, e in runs:
         e.sync()
+    for s, x, seeds, e in runs:
+        o = oracle.Oracle(s).vmc_run(x, seeds, nst, delta, nthreads=8)
+        assert np.array_equal(e.get_walkers(), o["x"]), s.name
+        assert np.array_equal(e.get_rng(), o["rng"]), s.name
+        assert _rel(e.get_eloc(), o["eloc"][-1], 1e-8) < 1e-9, s.name
+        e.close()
+
+
+@pytest.mark.parametrize("sweep", ["0", "1"], ids=["launch_per_electron", "sweep_per_launch"])
+def test_one_launch_batch_matches_oracle(monkeypatch, sweep):
+    """Four systems of 4 to 128 electrons in one batch: Gaussian bases with
+    pseudopotentials and the all-electron Be atom with its Slater basis and
+    no pseudopotential; two VMC steps (each a sweep of up to 128
+    one-electron launches over all systems, then the full evaluation)
+    against the oracle's walk of each system, at W = 3, 5, 2 and 4; the
+    moves either one launch per electron index over all systems, or each
+    system's whole sweep in one launch (QMCW_BATCH_SWEEP=1)."""
+    monkeypatch.setenv("QMCW_BATCH_SWEEP", sweep)
+    systems = [system.random_molecule(311, nelec=7, ncent=2),
+               system.random_molecule(312, nelec=48, ncent=10),
+               system.be_atom(1),
+               system.random_molecule(314, nelec=128, ncent=23, zeff=6.0, big_basis=True)]
+    Ws = [3, 5, 2, 4]
+    nst, delta = 2, 0.3
+    runs = []
+    for k, (s, W) in enumerate(zip(systems, Ws)):
+        x = system.random_walkers(s, W, seed=70 + k)
+        seeds = np.asarray(rng.walker_seeds("1837465927472523", W, stride=(1 << 20) + k), np.int32)
+        e = Engine(s, W)
+        e.set_walkers(x)
+        e.set_rng(seeds)
+        runs.append((s, x, seeds, e))
+    b = Batch([r[3] for r in runs])
+    b.vmc_device(nst, delta)
+    for s, x, seeds, e in runs:
+        e.sync()
+    b.close()
     for s, x, seeds, e in runs:
         o = oracle.Oracle(s).vmc_run(x, seeds, nst, delta, nthreads=8)
         assert np.array_equal(e.get_walkers(), o["x"]), s.name
