@@ -1633,11 +1633,15 @@ extern "C" int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta, 
 
 // ---------------------------------------------------------------------------
 // Batches: the VMC step of several handles (systems of uneven electron and
-// basis counts) in one launch per kernel (engine_batch.h).  Argument sets:
-// 0 the first full evaluation (each handle's kept rows as it left them),
-// 1 the steps.  Kernels per set: move, jas, det (also gproj, fin), nlpp,
-// rotqua; block offsets per kernel: move, jas, det, nlpp, rotqua, gproj.
+// basis counts) in one launch per kernel and lane-group width
+// (engine_batch.h).  Argument sets: 0 the first full evaluation (each
+// handle's kept rows as it left them), 1 the steps.  Kernels per set: move,
+// jas, det (also gproj, fin), nlpp, rotqua; block offsets per kernel (move,
+// jas, det, nlpp, rotqua, gproj) and width (8, 16, 32, 64: the systems of
+// each width in their own launch; rotqua and gproj use the first).
 enum { KB_MOVE, KB_JAS, KB_DET, KB_NLPP, KB_ROT, KB_NARGS, KB_GPROJ = KB_NARGS, KB_NOFF };
+constexpr int kBatchWidths[4] = {8, 16, 32, 64};
+static int width_index(int G) { return G == 8 ? 0 : (G == 16 ? 1 : (G == 32 ? 2 : 3)); }
 
 struct qmcw_batch {
   std::vector<qmcw_handle *> hs;
@@ -1647,10 +1651,10 @@ struct qmcw_batch {
   bool any_nloc = false, any_gproj = false;
   bool sweep = false;                        // QMCW_BATCH_SWEEP=1: each system's sweep in one launch
   KArgs *dA[2] = {nullptr, nullptr};         // [KB_NARGS][n] per set
-  int *dboff = nullptr;                      // [KB_NOFF][n + 1]
+  int *dboff = nullptr;                      // [KB_NOFF][4][n + 1]
   int *dnw = nullptr;                        // [n]
   long long **dnacc = nullptr;               // [n]
-  int nblk[KB_NOFF] = {0, 0, 0, 0, 0, 0};
+  int nblk[KB_NOFF][4] = {};
   size_t smem_move = 0, smem_det = 0, smem_nlpp = 0;
   std::string err;
 };
@@ -1674,7 +1678,7 @@ static int batch_fail(qmcw_batch *b, int code, const std::string &msg) {
 static int batch_upload(qmcw_batch *b, int set, bool first) {
   const int n = b->n;
   std::vector<KArgs> A((size_t)KB_NARGS * n);
-  std::vector<int> boff((size_t)KB_NOFF * (n + 1), 0), nw(n);
+  std::vector<int> boff((size_t)KB_NOFF * 4 * (n + 1), 0), nw(n);
   std::vector<long long *> nacc(n);
   for (int s = 0; s < n; s++) {
     qmcw_handle *h = b->hs[s];
@@ -1696,14 +1700,20 @@ static int batch_upload(qmcw_batch *b, int set, bool first) {
     const int cnt[KB_NOFF] = {Wl / (kWave / h->Gk[KC_MOVE]), Wl / (kWave / h->Gk[KC_JAS]),
                               Wl / (kWave / h->Gk[KC_DET]), Wl / (kWave / h->Gk[KC_NLPP]),
                               (Wl + kWave - 1) / kWave, e.det.gproj_mfma ? gx * gy * 2 : 0};
+    const int wi[KB_NOFF] = {width_index(h->Gk[KC_MOVE]), width_index(h->Gk[KC_JAS]),
+                             width_index(h->Gk[KC_DET]), width_index(h->Gk[KC_NLPP]), 0, 0};
     for (int k = 0; k < KB_NOFF; k++)
-      boff[(size_t)k * (n + 1) + s + 1] = boff[(size_t)k * (n + 1) + s] + cnt[k];
+      for (int g = 0; g < 4; g++) {
+        int *o = boff.data() + ((size_t)k * 4 + g) * (n + 1);
+        o[s + 1] = o[s] + (g == wi[k] ? cnt[k] : 0);
+      }
     nw[s] = Wl;
     nacc[s] = h->nacc;
   }
   BHIPCHK(b, hipMemcpy(b->dA[set], A.data(), A.size() * sizeof(KArgs), hipMemcpyHostToDevice));
   if (set == 1) {
-    for (int k = 0; k < KB_NOFF; k++) b->nblk[k] = boff[(size_t)k * (n + 1) + n];
+    for (int k = 0; k < KB_NOFF; k++)
+      for (int g = 0; g < 4; g++) b->nblk[k][g] = boff[((size_t)k * 4 + g) * (n + 1) + n];
     BHIPCHK(b, hipMemcpy(b->dboff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice));
     BHIPCHK(b, hipMemcpy(b->dnw, nw.data(), nw.size() * sizeof(int), hipMemcpyHostToDevice));
     BHIPCHK(b, hipMemcpy(b->dnacc, nacc.data(), nacc.size() * sizeof(long long *), hipMemcpyHostToDevice));
@@ -1711,10 +1721,10 @@ static int batch_upload(qmcw_batch *b, int set, bool first) {
   return QMCW_OK;
 }
 
-static BatchArgs batch_args(const qmcw_batch *b, int set, int kargs, int koff) {
+static BatchArgs batch_args(const qmcw_batch *b, int set, int kargs, int koff, int g = 0) {
   BatchArgs B;
   B.A = b->dA[set] + (size_t)kargs * b->n;
-  B.boff = b->dboff + (size_t)koff * (b->n + 1);
+  B.boff = b->dboff + ((size_t)koff * 4 + g) * (b->n + 1);
   B.nw = b->dnw;
   B.nacc = b->dnacc;
   B.n = b->n;
@@ -1723,7 +1733,7 @@ static BatchArgs batch_args(const qmcw_batch *b, int set, int kargs, int koff) {
 
 template <int G>
 static void batch_det_fin(const qmcw_batch *b, const BatchArgs &B, bool fin) {
-  const dim3 grid(b->nblk[KB_DET]), block(kWave);
+  const dim3 grid(b->nblk[KB_DET][width_index(G)]), block(kWave);
   if (fin)
     hipLaunchKernelGGL((k_full_fin_batch<G>), grid, block, 0, b->stream, B);
   else if (b->precision == QMCW_FP64)
@@ -1735,46 +1745,69 @@ static void batch_det_fin(const qmcw_batch *b, const BatchArgs &B, bool fin) {
 // the full evaluation of every handle with argument set `set`
 static void batch_eval(const qmcw_batch *b, int set) {
   const OrdKernels &K = ord_kernels(b->nordc);
-  const qmcw_handle *h0 = b->hs[0];
   BatchCtx c;
   c.stream = b->stream;
   c.precision = b->precision;
-  c.B = batch_args(b, set, KB_JAS, KB_JAS);
-  c.G = h0->Gk[KC_JAS];
-  c.nblocks = b->nblk[KB_JAS];
   c.smem = 0;
-  K.full_jas_batch(c);
-  const BatchArgs Bd = batch_args(b, set, KB_DET, KB_DET);
-  DISPATCH_GW(h0->Gk[KC_DET], batch_det_fin, b, Bd, false);
-  if (b->any_gproj && b->nblk[KB_GPROJ] > 0)
-    hipLaunchKernelGGL(k_gproj_mfma_batch, dim3(b->nblk[KB_GPROJ]), dim3(kWave), 0, b->stream,
+  for (int g = 0; g < 4; g++)
+    if (b->nblk[KB_JAS][g] > 0) {
+      c.B = batch_args(b, set, KB_JAS, KB_JAS, g);
+      c.G = kBatchWidths[g];
+      c.nblocks = b->nblk[KB_JAS][g];
+      K.full_jas_batch(c);
+    }
+  for (int g = 0; g < 4; g++)
+    if (b->nblk[KB_DET][g] > 0) {
+      const BatchArgs Bd = batch_args(b, set, KB_DET, KB_DET, g);
+      DISPATCH_GW(kBatchWidths[g], batch_det_fin, b, Bd, false);
+    }
+  if (b->any_gproj && b->nblk[KB_GPROJ][0] > 0)
+    hipLaunchKernelGGL(k_gproj_mfma_batch, dim3(b->nblk[KB_GPROJ][0]), dim3(kWave), 0, b->stream,
                        batch_args(b, set, KB_DET, KB_GPROJ));
   if (b->any_nloc) {
-    c.B = batch_args(b, set, KB_NLPP, KB_NLPP);
-    c.G = h0->Gk[KC_NLPP];
-    c.nblocks = b->nblk[KB_NLPP];
     c.smem = b->smem_nlpp;
-    K.full_nlpp_batch(c);
+    for (int g = 0; g < 4; g++)
+      if (b->nblk[KB_NLPP][g] > 0) {
+        c.B = batch_args(b, set, KB_NLPP, KB_NLPP, g);
+        c.G = kBatchWidths[g];
+        c.nblocks = b->nblk[KB_NLPP][g];
+        K.full_nlpp_batch(c);
+      }
   }
-  DISPATCH_GW(h0->Gk[KC_DET], batch_det_fin, b, Bd, true);
+  for (int g = 0; g < 4; g++)
+    if (b->nblk[KB_DET][g] > 0) {
+      const BatchArgs Bd = batch_args(b, set, KB_DET, KB_DET, g);
+      DISPATCH_GW(kBatchWidths[g], batch_det_fin, b, Bd, true);
+    }
 }
 
 static void batch_step(const qmcw_batch *b, double delta, double fbias) {
   const OrdKernels &K = ord_kernels(b->nordc);
   if (b->any_nloc)
-    hipLaunchKernelGGL(k_rotqua_batch, dim3(b->nblk[KB_ROT]), dim3(kWave), 0, b->stream,
+    hipLaunchKernelGGL(k_rotqua_batch, dim3(b->nblk[KB_ROT][0]), dim3(kWave), 0, b->stream,
                        batch_args(b, 1, KB_ROT, KB_ROT));
   BatchCtx c;
   c.stream = b->stream;
   c.precision = b->precision;
-  c.B = batch_args(b, 1, KB_MOVE, KB_MOVE);
-  c.G = b->hs[0]->Gk[KC_MOVE];      // every handle's, after qmcw_batch_create
-  c.nblocks = b->nblk[KB_MOVE];
   c.smem = b->smem_move;
-  if (b->sweep)
-    K.sweep_batch(c, delta, fbias);
-  else
-    for (int ie = 0; ie < b->maxN; ie++) K.ratio_batch(c, ie, delta, fbias);
+  if (b->sweep) {
+    for (int g = 0; g < 4; g++)
+      if (b->nblk[KB_MOVE][g] > 0) {
+        c.B = batch_args(b, 1, KB_MOVE, KB_MOVE, g);
+        c.G = kBatchWidths[g];
+        c.nblocks = b->nblk[KB_MOVE][g];
+        K.sweep_batch(c, delta, fbias);
+      }
+  } else {
+    for (int ie = 0; ie < b->maxN; ie++)
+      for (int g = 0; g < 4; g++)
+        if (b->nblk[KB_MOVE][g] > 0) {
+          c.B = batch_args(b, 1, KB_MOVE, KB_MOVE, g);
+          c.G = kBatchWidths[g];
+          c.nblocks = b->nblk[KB_MOVE][g];
+          K.ratio_batch(c, ie, delta, fbias);
+        }
+  }
   batch_eval(b, 1);
 }
 
@@ -1814,28 +1847,16 @@ extern "C" int qmcw_batch_create(qmcw_handle *const *hs, int32_t n, qmcw_batch *
   b->precision = hs[0]->precision;
   b->nordc = hs[0]->A.S.nordc;
   if (const char *e = getenv("QMCW_BATCH_SWEEP")) b->sweep = atoi(e) != 0;
-  // one lane-group width per kernel class for the whole batch: that of the
-  // handle with the most work per step (walkers x electrons^2), given to
-  // every handle
-  int dom = 0;
-  for (int s = 1; s < n; s++) {
-    const double ws = (double)hs[s]->Wl * hs[s]->A.S.N * hs[s]->A.S.N;
-    const double wd = (double)hs[dom]->Wl * hs[dom]->A.S.N * hs[dom]->A.S.N;
-    if (ws > wd) dom = s;
-  }
-  int Gd[4];
-  for (int k = 0; k < 4; k++) Gd[k] = hs[dom]->Gk[k];
+  // each handle keeps its lane-group widths: the systems of each width run
+  // in a launch of their own
   for (qmcw_handle *h : b->hs) {
-    for (int k = 0; k < 4; k++) h->Gk[k] = Gd[k];
-    h->G = h->Gk[KC_MOVE];
-    drop_graph(h);
     b->maxN = std::max(b->maxN, h->A.S.N);
     b->any_nloc = b->any_nloc || h->A.S.nloc > 0;
     b->any_gproj = b->any_gproj || (h->A.S.nloc > 0 && !h->no_gproj_mfma);
   }
   const size_t na = (size_t)KB_NARGS * n * sizeof(KArgs);
   if (hipMalloc(&b->dA[0], na) != hipSuccess || hipMalloc(&b->dA[1], na) != hipSuccess ||
-      hipMalloc(&b->dboff, (size_t)KB_NOFF * (n + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->dboff, (size_t)KB_NOFF * 4 * (n + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc(&b->dnw, (size_t)n * sizeof(int)) != hipSuccess ||
       hipMalloc(&b->dnacc, (size_t)n * sizeof(long long *)) != hipSuccess ||
       hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {

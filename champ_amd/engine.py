@@ -259,10 +259,9 @@ class Engine:
 
 class Batch:
     """Several engines (systems of uneven electron and basis counts) whose
-    VMC steps run as one launch per kernel (qmcw_batch_*).  The engines
-    must share the precision and the e-e-n order; they take the lane-group
-    widths of the engine with the most work per step.  Close the batch
-    before the engines."""
+    VMC steps run as one launch per kernel and lane-group width
+    (qmcw_batch_*).  The engines must share the precision and the e-e-n
+    order; each keeps its own widths.  Close the batch before the engines."""
 
     def __init__(self, engines):
         self.lib = abi.load_library()

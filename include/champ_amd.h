@@ -286,11 +286,10 @@ int qmcw_vmc_run_device(qmcw_handle *h, int32_t nstep, double delta,
    system per process; the batching of the engine's own design: several
    handles (systems of uneven electron and basis counts, each with its own
    walkers, streams and state) advance together, every kernel of the VMC
-   step running once over the walkers of all of them.  The handles must
+   step running once over the walkers of all of them (once per lane-group
+   width among them: each handle keeps its own widths).  The handles must
    share the precision and the e-e-n order (nordc) and sample no derivatives
-   and set no re-inversion interval; the batch gives them all the
-   lane-group widths of the handle with the most work per step (walkers x
-   electrons^2).  qmcw_batch_vmc_run does what qmcw_vmc_run_device
+   and set no re-inversion interval.  qmcw_batch_vmc_run does what qmcw_vmc_run_device
    does for each handle (a full evaluation, then nstep steps with each
    handle's mover, walkers and random streams; the same walk as the handle
    alone); the handles' later calls are ordered after it.  The batch holds
