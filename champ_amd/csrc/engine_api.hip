@@ -628,6 +628,8 @@ static void eval_args(const qmcw_handle *h, bool derivs, bool keep, EvalArgs &e)
       e.nlpp.lds_dnlpp = per_lane_d;
       e.dnlpp_smem = blk_d;
     }
+    // psp_nonloc_orb's contraction on FP64 MFMA after k_deriv_nlpp
+    e.nlpp.pnl_mfma = h->no_gproj_mfma ? 0 : 1;
   }
   // the pseudopotential projection on FP64 MFMA, rows = (walker, electron)
   A.gproj_mfma = (A.S.nloc > 0 && !h->no_gproj_mfma) ? 1 : 0;
@@ -671,6 +673,10 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
     if (derivs) {
       prof_begin(h, QMCW_PROF_DERIV_NLPP);
       K.deriv_nlpp(cn);
+      if (cn.A.pnl_mfma)
+        hipLaunchKernelGGL(k_pnl_mfma,
+                           dim3((unsigned)(((int64_t)h->Wl * A.S.N + 63) / 64), (A.S.NO + 63) / 64),
+                           dim3(64), 0, h->stream, cn.A, h->Wl);
       prof_end(h);
     }
   }

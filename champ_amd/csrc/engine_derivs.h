@@ -508,7 +508,8 @@ __global__ QMCW_HEAVY void k_deriv_nlpp(KArgs A) {
   }
   __syncthreads();
   double *pnl = ss + DL.pnl;
-  for (int t = lane; t < N * NO; t += G) {
+  // (k_pnl_mfma forms pnl after this kernel when A.pnl_mfma)
+  for (int t = A.pnl_mfma ? N * NO : lane; t < N * NO; t += G) {
     const int i = t / NO, q = t % NO;
     const double *row = pbc + (int64_t)i * M * NB;
     double a = 0.0;

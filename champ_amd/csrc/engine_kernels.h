@@ -35,6 +35,7 @@ struct KArgs {
   int lds_dnlpp;     // doubles per lane of k_deriv_nlpp's point scratch in LDS (0: global)
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
+  int pnl_mfma;      // k_deriv_nlpp leaves psp_nonloc_orb's contraction to k_pnl_mfma
   int gproj_mfma;    // k_full_det leaves the pseudopotential projection out (k_gproj_mfma
                      // forms it for large determinants; a re-inversion needs none)
   int lds_inv;       // k_full_det: doubles per group of LDS for A and the orbital values (0: global)
@@ -839,6 +840,67 @@ __device__ __forceinline__ void gproj_mfma_b(const KArgs &A, int nw, int bx, int
 }
 __global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A, int nw) {
   gproj_mfma_b(A, nw, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// psp_nonloc_orb of the derivatives on FP64 MFMA: pnl[i][q] = sum_b
+// coef(b, q) pbc_i[b] for every walker w < nw and electron i, one GEMM
+// [nw*N x NB] x [NB x NO] with rows = (walker, electron) pairs in walker
+// order (pbc_i: the electron's weighted basis sums, in the row of centre 0
+// of its pairs) and the coefficients shared by every walker; block
+// blockIdx.x takes 64 rows and blockIdx.y 64 orbitals, in the tiles of
+// k_gproj_mfma.
+__global__ __launch_bounds__(64) void k_pnl_mfma(KArgs A, int nw) {
+  const DevSys &S = A.S;
+  const int N = S.N, M = S.M, NB = S.NB, NO = S.NO;
+  const int64_t rows = (int64_t)nw * N, row0 = (int64_t)blockIdx.x * 64;
+  if (row0 >= rows) return;
+  const int64_t stride = A.SL.stride;
+  const cptr<double> co = cst(S.coefT);                  // coef(b, q) at co[b * NO + q]
+  const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
+  const int c0 = blockIdx.y * 64;
+  const double *__restrict__ ga[4];
+  bool va[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int64_t rho = row0 + 16 * t + r;
+    va[t] = rho < rows;
+    const int64_t w = va[t] ? rho / N : 0, i = va[t] ? rho % N : 0;
+    ga[t] = A.sc + w * stride + A.DL.pbc + i * M * NB;
+  }
+  gp_d4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[t][u] = gp_d4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < NB; k0 += 4) {
+    const int kk = k0 + kq;
+    double a[4], bb[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      a[t] = (va[t] && kk < NB) ? ga[t][kk] : 0.0;
+      const int q = c0 + 16 * t + r;
+      bb[t] = (q < NO && kk < NB) ? co[(int64_t)kk * NO + q] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bb[u], acc[t][u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int q4 = 0; q4 < 4; q4++) {
+      const int64_t rho = row0 + 16 * t + kq + 4 * q4;
+      if (rho >= rows) continue;
+      const int64_t w = rho / N, i = rho % N;
+      double *__restrict__ o = A.sc + w * stride + A.DL.pnl + i * NO;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = c0 + 16 * u + r;
+        if (q < NO) o[q] = acc[t][u][q4];
+      }
+    }
 }
 #endif
 
