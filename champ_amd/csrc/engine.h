@@ -159,6 +159,7 @@ struct DLayout {
   int64_t dy[2], dg[2];      // [nocc][ns]
   int64_t aux4;              // [nocc][ns]
   int64_t misc;              // [8]
+  int64_t phiall;            // [N][NB][5] every electron's basis values (FP64), for k_orb_mfma
 };
 
 // Pending one-electron move buffers inside SLayout::mv

@@ -631,8 +631,11 @@ static void eval_args(const qmcw_handle *h, bool derivs, bool keep, EvalArgs &e)
     // psp_nonloc_orb's contraction on FP64 MFMA after k_deriv_nlpp
     e.nlpp.pnl_mfma = h->no_gproj_mfma ? 0 : 1;
   }
-  // the pseudopotential projection on FP64 MFMA, rows = (walker, electron)
+  // the pseudopotential projection on FP64 MFMA, rows = (walker, electron),
+  // and under full_orb the contraction of every orbital, rows = (walker,
+  // electron, component)
   A.gproj_mfma = (A.S.nloc > 0 && !h->no_gproj_mfma) ? 1 : 0;
+  A.orb_mfma = (derivs && A.S.nparm > 0 && !h->no_gproj_mfma) ? 1 : 0;
   e.det = A;
   e.det_smem = det_smem;
 }
@@ -660,6 +663,10 @@ static void launch_eval(qmcw_handle *h, bool derivs) {
   }
   prof_begin(h, QMCW_PROF_FULL_DET);
   DISPATCH_GW(gd, launch_det, h, A, det_smem);
+  if (A.orb_mfma)
+    hipLaunchKernelGGL(k_orb_mfma,
+                       dim3((unsigned)(((int64_t)h->Wl * A.S.N * 5 + 63) / 64), (A.S.NO + 63) / 64),
+                       dim3(64), 0, h->stream, A, h->Wl);
   if (A.gproj_mfma)
     hipLaunchKernelGGL(k_gproj_mfma,
                        dim3((unsigned)(((int64_t)h->Wl * std::max(A.S.ns[0], A.S.ns[1]) + 63) / 64),
@@ -1184,6 +1191,7 @@ extern "C" int qmcw_create(const qmcw_system *s, int32_t nwalk, int32_t precisio
     DL.aib = take((int64_t)nsm * NO);
     DL.aux4 = take(noccm * nsm);
     DL.misc = take(8);
+    DL.phiall = take((int64_t)N * NB * 5);
   }
   SL.stride = (o + 7) / 8 * 8;
 #undef UP

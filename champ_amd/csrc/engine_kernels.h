@@ -36,6 +36,8 @@ struct KArgs {
   int bf16;          // orbital-contraction inputs rounded to bfloat16
   int chunk;         // electrons per basis chunk in the full evaluation
   int pnl_mfma;      // k_deriv_nlpp leaves psp_nonloc_orb's contraction to k_pnl_mfma
+  int orb_mfma;      // k_full_det (full_orb) leaves the contraction of all orbitals to
+                     // k_orb_mfma, writing every electron's basis values to DL.phiall
   int gproj_mfma;    // k_full_det leaves the pseudopotential projection out (k_gproj_mfma
                      // forms it for large determinants; a re-inversion needs none)
   int lds_inv;       // k_full_det: doubles per group of LDS for A and the orbital values (0: global)
@@ -353,7 +355,11 @@ __device__ __forceinline__ void full_orbitals(const KArgs &A, double *ws, double
       if (j >= S.nocc[s]) { for (int c = 0; c < 5; c++) o[c] = 0.0; continue; }
       contract5<T>(cst(S.coef_occ[s]), S.nocc[s], S.NB, j, phi + (int64_t)ii * S.NB * 5, o);
     }
-    if (A.full_orb) {
+    if (A.full_orb && A.orb_mfma) {
+      // the basis values of the chunk, for k_orb_mfma
+      double *pa = ss + A.DL.phiall + (int64_t)i0 * S.NB * 5;
+      for (int t = lane; t < nch * S.NB * 5; t += G) pa[t] = (double)phi[t];
+    } else if (A.full_orb) {
       for (int t = lane; t < nch * NO; t += G) {
         const int ii = t / NO, lab = t % NO;
         const int i = i0 + ii;
@@ -840,6 +846,68 @@ __device__ __forceinline__ void gproj_mfma_b(const KArgs &A, int nw, int bx, int
 }
 __global__ __launch_bounds__(64) void k_gproj_mfma(KArgs A, int nw) {
   gproj_mfma_b(A, nw, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Every orbital of every electron (value, gradient, laplacian) on FP64
+// MFMA for the derivatives: orbf[i][q][c] = sum_b coef(b, q) phi_i[b][c] for
+// every walker w < nw, one GEMM [nw*N*5 x NB] x [NB x NO] whose rows are the
+// (walker, electron, component) triples in walker order and whose B operand
+// (the coefficients) every walker shares; block blockIdx.x takes 64 rows
+// and blockIdx.y 64 orbitals, in the tiles of k_gproj_mfma.  The sums run in
+// the MFMA's order, not contract5's.
+__global__ __launch_bounds__(64) void k_orb_mfma(KArgs A, int nw) {
+  const DevSys &S = A.S;
+  const int N = S.N, NB = S.NB, NO = S.NO;
+  const int64_t rows = (int64_t)nw * N * 5, row0 = (int64_t)blockIdx.x * 64;
+  if (row0 >= rows) return;
+  const int64_t stride = A.SL.stride;
+  const cptr<double> co = cst(S.coefT);                  // coef(b, q) at co[b * NO + q]
+  const int lane = threadIdx.x, r = lane & 15, kq = lane >> 4;
+  const int c0 = blockIdx.y * 64;
+  // A(row, b) of the row (w, i, c) at phiall_w[(i * NB + b) * 5 + c]
+  const double *__restrict__ ga[4];
+  bool va[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int64_t rho = row0 + 16 * t + r;
+    va[t] = rho < rows;
+    const int64_t w = va[t] ? rho / (N * 5) : 0, ic = va[t] ? rho % (N * 5) : 0;
+    ga[t] = A.sc + w * stride + A.DL.phiall + (ic / 5) * NB * 5 + ic % 5;
+  }
+  gp_d4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[t][u] = gp_d4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < NB; k0 += 4) {
+    const int kk = k0 + kq;
+    double a[4], bb[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      a[t] = (va[t] && kk < NB) ? ga[t][(int64_t)kk * 5] : 0.0;
+      const int q = c0 + 16 * t + r;
+      bb[t] = (q < NO && kk < NB) ? co[(int64_t)kk * NO + q] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bb[u], acc[t][u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int q4 = 0; q4 < 4; q4++) {
+      const int64_t rho = row0 + 16 * t + kq + 4 * q4;
+      if (rho >= rows) continue;
+      const int64_t w = rho / (N * 5), ic = rho % (N * 5), i = ic / 5, c = ic % 5;
+      double *__restrict__ o = A.sc + w * stride + A.SL.orb + i * NO * 5 + c;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = c0 + 16 * u + r;
+        if (q < NO) o[(int64_t)q * 5] = acc[t][u][q4];
+      }
+    }
 }
 
 // psp_nonloc_orb of the derivatives on FP64 MFMA: pnl[i][q] = sum_b

@@ -48,8 +48,13 @@ def _derivs_system(name):
     return s
 
 
-@pytest.mark.parametrize("name", ["c2", "be_jastrow", "c2_ktt", "c2_bfd"])
-def test_derivs_match_oracle(name):
+@pytest.mark.parametrize("name,valu", [("c2", "0"), ("be_jastrow", "0"), ("c2_ktt", "0"),
+                                       ("c2_bfd", "0"), ("c2", "1"), ("c2_bfd", "1")])
+def test_derivs_match_oracle(monkeypatch, name, valu):
+    """The derivatives against the oracle; valu = 1 keeps the contractions
+    of every orbital and of psp_nonloc_orb on per-walker VALU loops
+    (QMCW_NO_GPROJ_MFMA=1) instead of the batched MFMA GEMMs."""
+    monkeypatch.setenv("QMCW_NO_GPROJ_MFMA", valu)
     s = _derivs_system(name)
     assert s.nparm > 0
     W = 13
