@@ -131,7 +131,7 @@ struct SLayout {
   int64_t qd;                // [G][4M] quadrature-point distances
   int64_t vnl;               // [N*M][QMCW_MAX_LPOT+1] nonlocal channel values
   int64_t gproj;             // [N][NB] G(:,i)^T coef of the occupied orbitals
-  int64_t bt;                // [N][M] e-e-n variable 1 - k scaled(r_j,ic) (as the kernel's T)
+  int64_t bt;                // [M][N] e-e-n variable 1 - k scaled(r_j,ic) (as the kernel's T)
   int64_t uq;                // [64][N] per-lane e-e-n variable of the pairs of a point
   int64_t plist;             // [N*M] ints: (electron, centre) pairs in pseudopotential range
   int64_t prot;              // [N*M][10] DMC: axes of each in-range pair's grid rotation, then

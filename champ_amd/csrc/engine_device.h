@@ -578,9 +578,13 @@ __device__ __forceinline__ T jas_b_val(const DevSys &S, bool parallel_spin, T u)
 
 // Jastrow value of electron i placed at xn (distances rnew to the centres),
 // with every other electron where it is: its e-n terms plus the e-e and
-// e-e-n terms of all pairs (i, j).  bt[j * M + ic] = 1 - k scaled(r_j,ic)
-// of the current configuration; uq is per-lane scratch of N values.
-template <typename T, int ORD, typename RA, typename UA>
+// e-e-n terms of all pairs (i, j).  bt[ic * N + j] = 1 - k scaled(r_j,ic)
+// of the current configuration; uq is per-lane scratch of N values.  With
+// PAIR the centres go two at a time, so that each U of the other electrons
+// is read once for both (each centre's sum keeps its own order; a kernel
+// instance of its own, for many centres: faster on the 50-centre chain,
+// slower on the 2-centre dimer, whose registers it crowds).
+template <typename T, int ORD, bool PAIR = false, typename RA, typename UA>
 __device__ __forceinline__ T jas_point_val(const DevSys &S, int i, const double *xn, const double *x,
                                            RA rnew, const T *bt, UA uq) {
   const int N = S.N, M = S.M;
@@ -598,15 +602,35 @@ __device__ __forceinline__ T jas_point_val(const DevSys &S, int i, const double 
   }
   if (S.nordc <= 1) return f;
   constexpr int NG = een_gi<ORD>(ORD, 0) + 1;
-  for (int ic = 0; ic < M; ic++) {
+  int ic = 0;
+  for (; PAIR && ic + 1 < M; ic += 2) {
+    T A0, A1, t1, t2;
+    scaled<T>(S.isc, k, (T)rnew[ic], A0, t1, t2);
+    scaled<T>(S.isc, k, (T)rnew[ic + 1], A1, t1, t2);
+    T gam0[NG], gam1[NG];
+    een_gamma<T, ORD>(T(1) - k * A0, cst(S.c) + cst(S.iwctype)[ic] * S.nterms, gam0);
+    een_gamma<T, ORD>(T(1) - k * A1, cst(S.c) + cst(S.iwctype)[ic + 1] * S.nterms, gam1);
+    const T *b0 = bt + ic * N, *b1 = b0 + N;
+    T fc0 = T(0), fc1 = T(0);
+    for (int jj = 0; jj < N; jj++) {
+      if (jj == i) continue;
+      const T u = uq[jj];
+      fc0 += een_horner<T, ORD>(gam0, u, b0[jj]);
+      fc1 += een_horner<T, ORD>(gam1, u, b1[jj]);
+    }
+    f += fc0;
+    f += fc1;
+  }
+  for (; ic < M; ic++) {
     T A, t1, t2;
     scaled<T>(S.isc, k, (T)rnew[ic], A, t1, t2);
     T gam[NG];
     een_gamma<T, ORD>(T(1) - k * A, cst(S.c) + cst(S.iwctype)[ic] * S.nterms, gam);
+    const T *b0 = bt + ic * N;
     T fc = T(0);
     for (int jj = 0; jj < N; jj++) {
       if (jj == i) continue;
-      fc += een_horner<T, ORD>(gam, uq[jj], bt[jj * M + ic]);
+      fc += een_horner<T, ORD>(gam, uq[jj], b0[jj]);
     }
     f += fc;
   }

@@ -1013,7 +1013,7 @@ __device__ __forceinline__ void rot_axes(int *st4, double *ax) {
 // taken by this lane: sum of w exp(dJ) v P_l (Psi_D ratio).  rnew: 4M
 // doubles and uq: N values of per-lane scratch (LDS or global), stored
 // lane-interleaved.
-template <typename T, int G, int ORD>
+template <typename T, int G, int ORD, bool PAIR>
 __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, const double *ss,
                                               int npair, const int *plist, LaneArr<double> rnew,
                                               LaneArr<T> uq, double *tqw, int lane) {
@@ -1075,7 +1075,7 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
       for (int jj = 0; jj < N; jj++)
         if (jj != i) fold += fso[jj < i ? i * N + jj : jj * N + i];
     }
-    const double delta = (double)jas_point_val<T, ORD>(S, i, xn, x, rnew, bt, uq) - fold;
+    const double delta = (double)jas_point_val<T, ORD, PAIR>(S, i, xn, x, rnew, bt, uq) - fold;
     double vpot = 0.0;
     for (int l = 1; l <= cst(S.npotd)[ict]; l++)
       if (l != cst(S.lpotp1)[ict]) vpot += v[l - 1] * legendre_w(l, costh);
@@ -1094,7 +1094,7 @@ __device__ __forceinline__ double nlpp_points(const KArgs &A, const double *ws, 
   return vpsp;
 }
 
-template <typename T, int G, int ORD>
+template <typename T, int G, int ORD, bool PAIR = false>
 __device__ __forceinline__ void full_nlpp_b(const KArgs &A, int bid_) {
   QMCW_GROUP_SETUP_B(bid_)
   const int N = S.N, M = S.M, NO = S.NO;
@@ -1127,10 +1127,11 @@ __device__ __forceinline__ void full_nlpp_b(const KArgs &A, int bid_) {
         v[l - 1] = vl;
       }
       v[QMCW_MAX_LPOT] = inr ? 1.0 : 0.0;
-      // e-e-n variable of the other electrons at their current positions
+      // e-e-n variable of the other electrons at their current positions,
+      // centre by centre (bt[ic * N + j])
       T bu, t1, t2;
       scaled<T>(S.isc, (T)S.scalek, (T)ren[p], bu, t1, t2);
-      bt[p] = T(1) - (T)S.scalek * bu;
+      bt[ic * N + p / M] = T(1) - (T)S.scalek * bu;
     }
     ploc = gsum<G>(ploc);
     // the pair sum of the Jastrow of each electron at its current position
@@ -1189,22 +1190,23 @@ __device__ __forceinline__ void full_nlpp_b(const KArgs &A, int bid_) {
       extern __shared__ double dyn_lds[];
       const LaneArr<double> rq{dyn_lds + threadIdx.x, kWave};
       const LaneArr<T> uq{reinterpret_cast<T *>(dyn_lds + 4 * M * kWave) + threadIdx.x, kWave};
-      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, rq, uq, tqw, lane);
+      vpsp = nlpp_points<T, G, ORD, PAIR>(A, ws, ss, npair, plist, rq, uq, tqw, lane);
     } else {
       // the walker's G lanes interleaved in its scratch
       const LaneArr<double> rq{ss + SL.qd + lane, G};
       const LaneArr<T> uq{reinterpret_cast<T *>(ss + SL.uq) + lane, G};
-      vpsp = nlpp_points<T, G, ORD>(A, ws, ss, npair, plist, rq, uq, tqw, lane);
+      vpsp = nlpp_points<T, G, ORD, PAIR>(A, ws, ss, npair, plist, rq, uq, tqw, lane);
     }
     vpsp = gsum<G>(vpsp);
   }
   if (lane == 0) { red[55] = ploc; red[56] = vpsp; }
 }
-// FP32 instances are held to 128 VGPRs (four waves per SIMD)
-template <typename T, int G, int ORD>
+// FP32 instances are held to 128 VGPRs (four waves per SIMD); PAIR: the
+// point Jastrow two centres at a time (systems of many centres)
+template <typename T, int G, int ORD, bool PAIR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : QMCW_WPE, 8)))
 void k_full_nlpp(KArgs A) {
-  full_nlpp_b<T, G, ORD>(A, blockIdx.x);
+  full_nlpp_b<T, G, ORD, PAIR>(A, blockIdx.x);
 }
 
 // T-moves: Psi ratios of moving electron ie to points of the walker's
@@ -1235,7 +1237,7 @@ void k_tm_ratios(KArgs A, int ie, int cw, const double *pts, const int *cnt, dou
     for (int p = lane; p < N * M; p += G) {
       T bu, t1, t2;
       scaled<T>(S.isc, (T)S.scalek, (T)ren[p], bu, t1, t2);
-      bt[p] = T(1) - (T)S.scalek * bu;
+      bt[(p % M) * N + p / M] = T(1) - (T)S.scalek * bu;
     }
     __syncthreads();
     if (lane == 0) stale[w] = 0.0;

@@ -25,13 +25,24 @@ void full_jas_g(const LaunchCtx &c) {
   else
     hipLaunchKernelGGL((k_full_jas<float, G, ORD>), grid, block, 0, c.stream, c.A);
 }
+// systems of this many centres or more take the instance whose point
+// Jastrow goes two centres at a time
+constexpr int kPairCentres = 8;
 template <int G>
 void full_nlpp_g(const LaunchCtx &c) {
   const dim3 grid(c.Wl / (kWave / G)), block(kWave);
-  if (c.precision == QMCW_FP64)
-    hipLaunchKernelGGL((k_full_nlpp<double, G, ORD>), grid, block, c.smem_nlpp, c.stream, c.A);
-  else
-    hipLaunchKernelGGL((k_full_nlpp<float, G, ORD>), grid, block, c.smem_nlpp, c.stream, c.A);
+  const bool pair = c.A.S.M >= kPairCentres;
+  if (c.precision == QMCW_FP64) {
+    if (pair)
+      hipLaunchKernelGGL((k_full_nlpp<double, G, ORD, true>), grid, block, c.smem_nlpp, c.stream, c.A);
+    else
+      hipLaunchKernelGGL((k_full_nlpp<double, G, ORD, false>), grid, block, c.smem_nlpp, c.stream, c.A);
+  } else {
+    if (pair)
+      hipLaunchKernelGGL((k_full_nlpp<float, G, ORD, true>), grid, block, c.smem_nlpp, c.stream, c.A);
+    else
+      hipLaunchKernelGGL((k_full_nlpp<float, G, ORD, false>), grid, block, c.smem_nlpp, c.stream, c.A);
+  }
 }
 template <int G>
 void ratio_g(const LaunchCtx &c, int ie, int mode, const double *xn, double delta, double fbias,
