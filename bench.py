@@ -262,12 +262,20 @@ def accum_roofline(dres, W):
             "frac": ach / PEAK_FP64_TFLOPS}
 
 
-def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
+def run_batch(nsys, steps, warmup, precision, delta, seed0=1000, derivs_half=True,
+              deriv_bytes=2 << 30):
     """Multi-system batch: nsys synthetic molecules (4-128 electrons, 1-24
     nuclei, 10-800 GTOs), 256-4096 walkers each, one handle (and HIP stream)
-    per system.  Each system is timed alone, then all together: every
-    handle's steps are queued before any is awaited, so the systems run
-    concurrently on their own streams."""
+    per system.  With derivs_half every second system also forms the linear
+    method's derivatives and product sums each step (its Jastrow set and
+    orbital rotations); such a system's walker count is capped so that its
+    derivative scratch (the per-pair basis rows of the nonlocal orbital
+    derivative, N M NB doubles per walker, and the orbital values and
+    derivatives, 5 N NB) stays within deriv_bytes, never below 256.  Each
+    system is timed alone, then all together: every handle's steps are
+    queued before any is awaited, so the systems run concurrently on their
+    own streams.  The one-launch batch runs plain VMC steps, so it is timed
+    with the derivatives switched off."""
     from champ_amd import rng, system
     from champ_amd.engine import Engine
     rs = np.random.default_rng(seed0)
@@ -275,13 +283,21 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
     for k in range(nsys):
         s = system.random_molecule(seed0 + k)
         W = int(rs.integers(256, 4097))
+        der = derivs_half and k % 2 == 1
+        if der:
+            system.molecule_derivs(s)
+            per = 8 * (s.nelec * s.ncent * s.nbasis + 5 * s.nelec * s.nbasis)
+            W = min(W, max(256, deriv_bytes // per))
         e = Engine(s, W, precision)
         e.set_walkers(system.random_walkers(s, W, seed=seed0 + k))
         e.set_rng(rng.walker_seeds("1837465927472523", W, stride=(1 << 20) + k))
         g = e.autotune()
-        engines.append((s, W, e, g))
+        if der:
+            e.accum_reset()
+            e.sample_derivs(2)
+        engines.append((s, W, e, g, der))
     per = []
-    for s, W, e, g in engines:
+    for s, W, e, g, der in engines:
         e.vmc_device(max(1, warmup), delta)
         e.sync()
         t0 = time.perf_counter()
@@ -289,32 +305,36 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000):
         e.sync()
         dt = time.perf_counter() - t0
         per.append(dict(nelec=int(s.nelec), ncent=int(s.ncent), nbasis=int(s.nbasis), walkers=W,
-                        G=g, walker_moves_per_s=W * steps / dt))
+                        G=g, derivs=der, nparm=int(s.nparm) if der else 0,
+                        walker_moves_per_s=W * steps / dt))
     t0 = time.perf_counter()
-    for s, W, e, g in engines:
+    for s, W, e, g, der in engines:
         e.vmc_device(steps, delta)
-    for s, W, e, g in engines:
+    for s, W, e, g, der in engines:
         e.sync()
     dt = time.perf_counter() - t0
-    agg = sum(W for _, W, _, _ in engines) * steps / dt
+    agg = sum(W for _, W, _, _, _ in engines) * steps / dt
     # the same systems as one batch: every kernel of the step once over the
-    # walkers of all systems (the first system's lane-group widths)
+    # walkers of all systems (each system keeps its own lane-group widths)
     from champ_amd.engine import Batch
-    b = Batch([e for _, _, e, _ in engines])
+    for _, _, e, _, der in engines:
+        if der:
+            e.sample_derivs(0)
+    b = Batch([e for _, _, e, _, _ in engines])
     b.vmc_device(max(1, warmup), delta)
-    for _, _, e, _ in engines:
+    for _, _, e, _, _ in engines:
         e.sync()
     t0 = time.perf_counter()
     b.vmc_device(steps, delta)
-    for _, _, e, _ in engines:
+    for _, _, e, _, _ in engines:
         e.sync()
     dt1 = time.perf_counter() - t0
-    agg1 = sum(W for _, W, _, _ in engines) * steps / dt1
+    agg1 = sum(W for _, W, _, _, _ in engines) * steps / dt1
     b.close()
-    for _, _, e, _ in engines:
+    for _, _, e, _, _ in engines:
         e.close()
     return dict(aggregate=agg, per_system=per, seconds=dt, aggregate_one_launch=agg1,
-                seconds_one_launch=dt1)
+                seconds_one_launch=dt1, derivs_half=derivs_half)
 
 
 def cpu_baseline_dmc(s, seconds=10.0, nthreads=None, tau=0.01):
@@ -505,16 +525,20 @@ def main():
         return
     if args.batch:
         prec = "mixed" if args.precision == "both" else args.precision
-        b = run_batch(args.batch, args.steps, args.warmup, prec, args.delta)
+        b = run_batch(args.batch, args.steps, args.warmup, prec, args.delta,
+                      derivs_half=not args.no_derivs)
         print(json.dumps({"metric": "walker-moves/s (multi-system batch)",
-                          "value": max(b["aggregate"], b["aggregate_one_launch"]),
+                          "value": b["aggregate"],
                           "unit": "walker-moves/s", "n_gpus": 1, "steps": args.steps,
                           "warmup": args.warmup, "higher_is_better": True, "dtype":
                           "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
                           "config": {"workload": f"{args.batch} synthetic molecules",
-                                     "precision": prec},
+                                     "precision": prec,
+                                     "derivatives": "every second system" if b["derivs_half"]
+                                     else "none"},
                           "streams_walker_moves_per_s": b["aggregate"],
                           "one_launch_walker_moves_per_s": b["aggregate_one_launch"],
+                          "one_launch_derivatives": "none",
                           "per_system": b["per_system"]}), flush=True)
         return
     from champ_amd import rng, system

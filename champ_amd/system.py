@@ -1162,6 +1162,19 @@ def random_molecule(seed: int, nelec: int | None = None, ncent: int | None = Non
                   name=f"mol{seed}", **barr)
 
 
+def molecule_derivs(s: System) -> System:
+    """The linear method's parameters of a single-determinant molecule: the
+    Jastrow set CHAMP varies (jastrow_opt_list, a1 included) and the
+    rotations from the four highest up-spin occupied orbitals into the
+    virtual orbitals the MO set holds (four at most for random_molecule)."""
+    ja, jb, jc = jastrow_opt_list(s, vary_a1=True)
+    s.jas_a, s.jas_b, s.jas_c = ja, jb, jc
+    nup, norb = int(s.nup), int(np.asarray(s.coef).shape[0])
+    s.orb_parm = [(i, a, 0) for i in range(max(0, nup - 4), nup) for a in range(nup, norb)]
+    s.name = f"{s.name}_derivs"
+    return s
+
+
 def random_walkers(sysm: System, nw: int, seed: int = 7, sigma: float = 1.0):
     """Electrons Gaussian (sigma bohr) around the nuclei, assigned round-robin."""
     rng = np.random.default_rng(seed)

@@ -35,6 +35,10 @@ def _derivs_system(name):
         s.a4 = np.array([p["a"]]); s.b = np.array([p["b"]])
         s.c = system.cusp_exact(np.array([p["c"]]), s.nordc)
         return s
+    if name == "mol":
+        # a 39-electron random molecule of seven centres (Z_eff 1, 4, 6 with
+        # pseudopotentials): its Jastrow set and 16 orbital rotations
+        return system.molecule_derivs(system.random_molecule(1011))
     if name == "c2_bfd":
         # the pseudopotential C2 example (tabulated radial basis, 6-point
         # quadrature): the 24 Jastrow parameters its optimisation section varies
@@ -49,7 +53,8 @@ def _derivs_system(name):
 
 
 @pytest.mark.parametrize("name,valu", [("c2", "0"), ("be_jastrow", "0"), ("c2_ktt", "0"),
-                                       ("c2_bfd", "0"), ("c2", "1"), ("c2_bfd", "1")])
+                                       ("c2_bfd", "0"), ("mol", "0"), ("c2", "1"),
+                                       ("c2_bfd", "1"), ("mol", "1")])
 def test_derivs_match_oracle(monkeypatch, name, valu):
     """The derivatives against the oracle; valu = 1 keeps the contractions
     of every orbital and of psp_nonloc_orb on per-walker VALU loops

@@ -431,8 +431,14 @@ def test_nonlocal_quadrature_exact_for_sp_orbitals():
 
 
 # ---------------------------------------------------------------- derivatives
-def test_parameter_derivatives_fd():
-    s = system.c2_dimer(2, derivs=True)
+@pytest.mark.parametrize("name", ["c2", "mol"])
+def test_parameter_derivatives_fd(name):
+    """name = mol: a 39-electron random molecule of seven centres (Z_eff 1,
+    4 and 6, pseudopotentials) with its Jastrow set and 16 orbital rotations"""
+    if name == "c2":
+        s = system.c2_dimer(2, derivs=True)
+    else:
+        s = system.molecule_derivs(system.random_molecule(1011))
     O = oracle.Oracle(s)
     x = system.random_walkers(s, 1, seed=17)[0]
     r0 = O.eval_full(x)
@@ -473,7 +479,7 @@ def test_parameter_derivatives_fd():
         checks.append((p, sp, sm)); p += 1
     assert p == len(dpsi)
     rs = np.random.default_rng(0)
-    sel = sorted(set(rs.choice(len(checks), 60, replace=False).tolist()) |
+    sel = sorted(set(rs.choice(len(checks), min(60, len(checks)), replace=False).tolist()) |
                  set(range(len(s.csf_parm) + 3)))
     for k in sel:
         p, sp, sm = checks[k]
