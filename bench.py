@@ -366,10 +366,13 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0, re
     pseudopotential, 34 H, 424 GTOs, one determinant): W0 walkers, capacity
     2 W0, time step tau, under the rundmc driver (population control),
     started, as a DMC run starts from VMC configurations, from `equil` VMC
-    sweeps of the random initial walkers (step 0.3 bohr).  The synthetic
-    wavefunction (random orthonormal MOs) is far from the ground state, so
-    its DMC population drifts over hundreds of generations; the timed
-    generations are few and their population range is reported."""
+    sweeps of the random initial walkers (step 0.3 bohr).  The `warmup`
+    generations are an equilibration run: after them etrial is reset to
+    eest and the averages restart (dmc_mod.f90:386-403 with
+    l_reset_etrial).  The synthetic wavefunction (random orthonormal MOs)
+    is far from the ground state, so its DMC energy keeps relaxing and the
+    population drifts over hundreds of generations; the population range
+    of the timed generations is reported."""
     from champ_amd import rng, system
     from champ_amd.dmc import DmcDriver
     from champ_amd.engine import Engine
@@ -388,6 +391,7 @@ def run_dmc_chain(W0, steps, warmup, precision, tau=0.01, equil=20, tmoves=0, re
     e0 = float(e.get_eloc()[:W0].mean())
     drv = DmcDriver(e, s.nelec, W0, tau=tau, etrial=e0, tmoves=tmoves)
     drv.run(max(1, warmup))
+    drv.end_equilibration(reset_etrial=True)
     e.sync()
     t0 = time.perf_counter()
     h = drv.run(steps)
@@ -509,7 +513,8 @@ def main():
                "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
                "config": {"workload": f"DMC, 98-electron chain (16 Z_eff=4 + 34 H, 424 GTOs), "
                                       f"4096 walkers (target), tau 0.01, {args.steps} timed generations "
-                                      f"from 20 VMC sweeps, re-inversion every {args.reinversion} moves"
+                                      f"from 20 VMC sweeps and {args.warmup} equilibration generations (etrial reset), "
+                                      f"re-inversion every {args.reinversion} moves"
                                       + (", T-moves" if args.tmoves else ""),
                           "tmoves": args.tmoves, "reinversion_every": args.reinversion,
                           "precision": prec, "group_widths": d["G"]},

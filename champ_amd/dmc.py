@@ -70,6 +70,21 @@ class DmcDriver:
                                  acc=st.nacc / max(st.ntry, 1), pop_ctrl=pop_ctrl))
         return st
 
+    def end_equilibration(self, reset_etrial: bool = True):
+        """End of an equilibration run (qmc/dmc_mod.f90:380-403): with
+        reset_etrial, etrial becomes eest and every population-control
+        factor ff is multiplied by exp((eest - etrial) taueff), taueff the
+        weighted average of the run; then the running averages are cleared
+        (stats%clear), so that eest follows the generations after this
+        call only, and the esigma schedule starts again as rundmc's step
+        counter does.  eest keeps its value until the next generation."""
+        if reset_etrial and self.sw > 0:
+            dff = math.exp((self.eest - self.etrial) * (self.swt / self.sw))
+            self.ff = [f * dff for f in self.ff]
+            self.etrial = self.eest
+        self.sw = self.swe = self.swe2 = self.swt = 0.0
+        self.is_ = 0
+
     def run(self, ngen: int):
         for _ in range(ngen):
             self.generation()

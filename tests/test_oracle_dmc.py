@@ -424,3 +424,37 @@ def test_dmc_recovers_the_exact_helium_energy():
     blocks = e.reshape(20, 20).mean(axis=1)
     est, err = blocks.mean(), blocks.std() / np.sqrt(len(blocks))
     assert abs(est + 2.903724) < max(4 * err, 0.005), (est, err)
+
+
+def test_end_of_equilibration_resets_etrial_and_clears_the_averages():
+    """CHAMP's end of an equilibration run (qmc/dmc_mod.f90:386-403) with
+    l_reset_etrial: etrial takes eest, every population-control factor is
+    multiplied by exp((eest - etrial) taueff), and the averages restart, so
+    the next generation's eest is that generation's weighted E_L and its
+    pop_ctrl has no energy factor left: (wtarg / wtot)^(1 / ngen_pop_ctrl)."""
+    import math
+    from champ_amd.dmc import DmcDriver
+    s = he_atom()
+    W0, cap = 200, 500
+    x = system.random_walkers(s, W0, seed=8, sigma=0.6)
+    d = oracle.OracleDMC(s, cap)
+    assert d.set(x, rng.walker_seeds("1837465927472523", W0)) == 0
+    brng = np.array(rng.seed_limbs("0000111122223333"), np.int32)
+    brng[3] |= 1
+    drv = DmcDriver(OracleStepper(d, brng), 2, W0, tau=0.01, etrial=-2.5)
+    drv.run(30)
+    eest, etrial, taueff = drv.eest, drv.etrial, drv.swt / drv.sw
+    ff0, wtot = list(drv.ff), drv.wtot
+    assert abs(eest - etrial) > 1e-3
+    drv.end_equilibration(reset_etrial=True)
+    dff = math.exp((eest - etrial) * taueff)
+    assert drv.etrial == eest
+    assert np.allclose(drv.ff, [f * dff for f in ff0], rtol=1e-15, atol=0)
+    assert drv.sw == drv.swe == drv.swt == 0.0 and drv.is_ == 0
+    h = drv.run(1)[-1]
+    assert abs(h["pop_ctrl"] - (W0 / wtot) ** (1.0 / drv.ngen_pop_ctrl)) < 1e-14
+    assert abs(drv.eest - h["eave"]) < 1e-12 * abs(h["eave"])
+    # without the reset only the averages restart
+    etr = drv.etrial
+    drv.end_equilibration(reset_etrial=False)
+    assert drv.etrial == etr and drv.sw == 0.0
