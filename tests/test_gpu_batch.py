@@ -87,3 +87,48 @@ def test_one_launch_batch_matches_oracle(monkeypatch, sweep):
         assert np.array_equal(e.get_rng(), o["rng"]), s.name
         assert _rel(e.get_eloc(), o["eloc"][-1], 1e-8) < 1e-9, s.name
         e.close()
+
+
+def test_concurrent_systems_with_derivatives_on_half():
+    """The batch workload's mix: concurrent handles, the derivatives and
+    the linear method's product sums on for every second system (its
+    Jastrow set and orbital rotations).  Sampling leaves each walk as the
+    oracle's, bit for bit; the derivative systems' last-step d ln Psi/dp
+    match the oracle's at the final positions and their dE_L/dp a fresh
+    evaluation of the same state."""
+    systems = [system.random_molecule(311, nelec=9, ncent=3),
+               system.molecule_derivs(system.random_molecule(1011)),
+               system.random_molecule(313, nelec=40, ncent=6),
+               system.molecule_derivs(system.random_molecule(314, nelec=30, ncent=5))]
+    W, nst, delta = 3, 2, 0.3
+    runs = []
+    for k, s in enumerate(systems):
+        x = system.random_walkers(s, W, seed=60 + k)
+        seeds = np.asarray(rng.walker_seeds("1837465927472523", W, stride=(1 << 21) + k), np.int32)
+        e = Engine(s, W)
+        e.set_walkers(x)
+        e.set_rng(seeds)
+        if s.nparm:
+            e.accum_reset()
+            e.sample_derivs(2)
+        runs.append((s, x, seeds, e))
+    for s, x, seeds, e in runs:
+        e.vmc_device(nst, delta)
+    for s, x, seeds, e in runs:
+        e.sync()
+    for s, x, seeds, e in runs:
+        o = oracle.Oracle(s).vmc_run(x, seeds, nst, delta, nthreads=8)
+        assert np.array_equal(e.get_walkers(), o["x"]), s.name
+        assert np.array_equal(e.get_rng(), o["rng"]), s.name
+        assert _rel(e.get_eloc(), o["eloc"][-1], 1e-8) < 1e-9, s.name
+        if s.nparm:
+            dp, de = e.get_derivs()
+            r = oracle.Oracle(s).eval_batch(o["x"], derivs=True)
+            assert r["status"] == 0
+            assert _rel(dp, r["dpsi"], 1e-9) < 1e-9, s.name
+            # dE_L/dp was formed on the last step's rotated quadrature grid:
+            # a fresh derivative evaluation of the same state gives it again
+            dp2, de2 = e.derivs()
+            assert np.array_equal(dp, dp2) and np.array_equal(de, de2), s.name
+            assert e.accum_get()["n"] == W * nst
+        e.close()
