@@ -266,7 +266,7 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000, derivs_half=Tru
               deriv_bytes=2 << 30):
     """Multi-system batch: nsys synthetic molecules (4-128 electrons, 1-24
     nuclei, 10-800 GTOs), 256-4096 walkers each, one handle (and HIP stream)
-    per system.  With derivs_half every second system also forms the linear
+    per system.  With derivs_half half of the systems also form the linear
     method's derivatives and product sums each step (its Jastrow set and
     orbital rotations); such a system's walker count is capped so that its
     derivative scratch (the per-pair basis rows of the nonlocal orbital
@@ -283,7 +283,10 @@ def run_batch(nsys, steps, warmup, precision, delta, seed0=1000, derivs_half=Tru
     for k in range(nsys):
         s = system.random_molecule(seed0 + k)
         W = int(rs.integers(256, 4097))
-        der = derivs_half and k % 2 == 1
+        # half of the systems; consecutive handles' streams share the GPU's
+        # few hardware queues in turn, so the derivative systems alternate
+        # with the others along every queue
+        der = derivs_half and (k + k // 4) % 2 == 1
         if der:
             system.molecule_derivs(s)
             per = 8 * (s.nelec * s.ncent * s.nbasis + 5 * s.nelec * s.nbasis)
@@ -539,7 +542,7 @@ def main():
                           "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
                           "config": {"workload": f"{args.batch} synthetic molecules",
                                      "precision": prec,
-                                     "derivatives": "every second system" if b["derivs_half"]
+                                     "derivatives": "half of the systems" if b["derivs_half"]
                                      else "none"},
                           "streams_walker_moves_per_s": b["aggregate"],
                           "one_launch_walker_moves_per_s": b["aggregate_one_launch"],
